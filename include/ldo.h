@@ -1,0 +1,228 @@
+/*
+ * ldo.h - C ABI of libldo_hip.so, the MI355X (gfx950) lattice Monte Carlo
+ * sampler for the LatticeDNAOrigami domain-resolution DNA-origami model.
+ *
+ * The reference has no FFI; its hot path sits behind the C++ simulation
+ * loop contract (include/LatticeDNAOrigami/simulation.hpp:77-81,
+ * src/simulation.cpp:568-653) driven one replica per MPI rank by the
+ * constant-T, PTMC and PTMWUS drivers.  This ABI replaces that contract:
+ * every replica of a run lives on one GPU, and each entry point below names
+ * the reference interface it stands in for.
+ *
+ * Conventions
+ *   - every function returns an int status (LDO_OK on success) and leaves a
+ *     message for ldo_last_error();
+ *   - the caller owns every buffer it passes in; the library copies inputs
+ *     and writes outputs into caller memory;
+ *   - calls are synchronous (the device stream is synchronised before
+ *     return) and one host thread drives one ldo_ctx.
+ *   - identities, positions and orientations follow the reference's system
+ *     JSON (src/files.cpp:36-104): chain 0 is the scaffold, staple chain
+ *     identities are 1-based, domain identities are signed integers and a
+ *     staple domain binds the scaffold domain of opposite sign.
+ */
+#ifndef LDO_H
+#define LDO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; the reference's C++ exceptions (utility.hpp:30-58). */
+#define LDO_OK 0
+#define LDO_E_CONSTRAINT 1 /* utility::OrigamiMisuse */
+#define LDO_E_MISUSE 2     /* utility::SimulationMisuse */
+#define LDO_E_NOTIMPL 3    /* utility::NotImplemented */
+#define LDO_E_IO 4         /* utility::FileError */
+#define LDO_E_HIP 5        /* HIP runtime failure */
+#define LDO_E_ARG 6        /* argument outside the documented range */
+
+/* Option values (the reference's string options, src/parser.cpp:39-57). */
+enum ldo_domain_type { LDO_HALFTURN = 0, LDO_THREEQUARTERTURN = 1 };
+enum ldo_hybridization_pot {
+    LDO_HYB_NEARESTNEIGHBOUR = 0,
+    LDO_HYB_UNIFORM = 1,
+    LDO_HYB_SPECIFIED = 2
+};
+enum ldo_misbinding_pot { LDO_MISBIND_OPPOSING = 0, LDO_MISBIND_DISALLOWED = 1 };
+enum ldo_stacking_pot { LDO_STACK_CONSTANT = 0, LDO_STACK_SEQUENCESPECIFIC = 1 };
+
+/* Move types of a moveset file (src/simulation.cpp:267-318). */
+enum ldo_movetype {
+    LDO_MOVE_ORIENTATION_ROTATION = 0,  /* orientation_movetype.cpp:30-65 */
+    LDO_MOVE_MET_STAPLE_EXCHANGE = 1,   /* met_movetypes.cpp:192-403 */
+    LDO_MOVE_CB_STAPLE_REGROWTH = 2,    /* cb_movetypes.cpp:292-348 */
+    LDO_MOVE_CTCB_SCAFFOLD_REGROWTH = 3 /* cb_movetypes.cpp:680-747 */
+};
+
+/* Random streams.  PHILOX: a counter-based Philox4x32-10 stream keyed by
+ * (seed, replica) whose counter is (step, draw) so every MC step has its own
+ * sub-stream.  TAPE: each replica consumes a caller-supplied sequence of
+ * recorded draws (ldo_set_tape), in the order the reference's RandomGens
+ * would have produced them (random_gens.cpp:22-43). */
+enum ldo_rng_mode { LDO_RNG_PHILOX = 0, LDO_RNG_TAPE = 1 };
+
+/* Replica exchange (ptmc_simulation.cpp:360-412; us_simulation.cpp:770-864) */
+enum ldo_exchange_mode {
+    LDO_EXCHANGE_NONE = 0,        /* independent constant-T chains */
+    LDO_EXCHANGE_TEMPERATURE = 1, /* ut_parallel_tempering */
+};
+
+/* System and energy model: the data setup_origami() reads
+ * (origami_system.cpp:991-1030) and the OrigamiPotential options
+ * (origami_potential.cpp:952-1012). */
+typedef struct ldo_system_desc {
+    int32_t n_chain_types;        /* identities.size(): scaffold + staple types */
+    const int32_t* chain_lengths; /* [n_chain_types] */
+    const int32_t* identities;    /* flattened, chain-type major */
+    const char* const* sequences; /* [sum chain_lengths] or NULL */
+    const double* enthalpies;     /* "Specified": per staple-domain index, or NULL */
+    const double* entropies;
+    int32_t n_specified;
+    int32_t cyclic;
+    int32_t domain_type;       /* ldo_domain_type */
+    int32_t hybridization_pot; /* ldo_hybridization_pot */
+    int32_t misbinding_pot;    /* ldo_misbinding_pot */
+    int32_t stacking_pot;      /* ldo_stacking_pot */
+    int32_t apply_mean_field_cor;
+    int32_t pad0;
+    double binding_h;    /* K   */
+    double binding_s;    /* k_B */
+    double misbinding_h; /* K   */
+    double misbinding_s; /* k_B */
+    double stacking_ene; /* K   */
+    double cation_M;
+    double staple_M;
+} ldo_system_desc;
+
+/* Run parameters: the moveset file and the .inp keys the hot loop reads
+ * (parser.cpp:108-122, 241-251; simulation.cpp:192-237). */
+typedef struct ldo_run_params {
+    int32_t n_movetypes;
+    const int32_t* movetype_kind;         /* ldo_movetype */
+    const double* movetype_freq;          /* parsed "a/b" fractions */
+    const int32_t* movetype_max_regrowth; /* CTCB max_regrowth, else ignored */
+    int32_t max_total_staples;
+    int32_t max_type_staples;
+    int32_t max_staple_size;
+    int32_t centering_freq;
+    int32_t centering_domain;
+    int32_t constraint_check_freq;
+    uint64_t seed;
+    int32_t rng_mode;      /* ldo_rng_mode */
+    int32_t exchange_mode; /* ldo_exchange_mode */
+    int32_t lanes_per_replica; /* 0: pick automatically; 1 or 64 */
+    int32_t pad0;
+} ldo_run_params;
+
+/* One configuration: the Chain records of origami_system.hpp:46-62. */
+typedef struct ldo_config {
+    int32_t n_chains;
+    int32_t pad0;
+    int32_t* chain_index;    /* [n_chains] unique chain indices */
+    int32_t* chain_identity; /* [n_chains] */
+    int32_t* chain_length;   /* [n_chains] (output; ignored on input) */
+    int32_t* positions;      /* [3 * domains] chain by chain */
+    int32_t* orientations;   /* [3 * domains] */
+} ldo_config;
+
+/* Per-replica observables (origami_system.hpp:99-130, 204-246). */
+typedef struct ldo_obs {
+    double energy;          /* m_energy, kT */
+    double enthalpy;        /* hybridization enthalpy (update_enthalpy_and_entropy) */
+    double entropy;
+    double stacking_energy; /* energy - (enthalpy - entropy) */
+    double temperature;
+    int64_t step;
+    int32_t num_staples;
+    int32_t num_domains;
+    int32_t num_bound_domain_pairs;
+    int32_t num_fully_bound_domain_pairs;
+    int32_t num_misbound_domain_pairs;
+    int32_t num_stacked_domain_pairs;
+    int32_t num_unassigned_domains;
+    int32_t status; /* 0, or LDO_E_CONSTRAINT after a failed constraint check */
+} ldo_obs;
+
+#define LDO_MAX_MOVETYPES 8
+
+typedef struct ldo_counters {
+    int64_t attempts[LDO_MAX_MOVETYPES];
+    int64_t accepts[LDO_MAX_MOVETYPES];
+    /* trial placements onto empty sites / onto unbound occupants (binding
+     * evaluations) and committed domain sets+unsets; the byte model of the
+     * throughput roofline is built from these counted events */
+    int64_t site_empty;
+    int64_t site_bind;
+    int64_t commits;
+    int64_t exchange_attempts;
+    int64_t exchange_accepts;
+} ldo_counters;
+
+/* Create a context holding n_replicas replicas of one system.  Every replica
+ * starts from `init` at temperature temps[r] (n_replicas values).
+ * Replaces setup_origami + GCMCSimulation construction
+ * (origami_system.cpp:991-1030; simulation.cpp:182-258). */
+int ldo_create(const ldo_system_desc* sys,
+               const ldo_run_params* run,
+               const ldo_config* init,
+               int32_t n_replicas,
+               const double* temps,
+               void** out_ctx);
+
+void ldo_destroy(void* ctx);
+
+/* Message of the last failed call on ctx (or of the last failed
+ * ldo_create when ctx is NULL). */
+const char* ldo_last_error(void* ctx);
+
+/* Replace the configuration of one replica (OrigamiSystem::set_config,
+ * origami_system.cpp:327-341). */
+int ldo_set_config(void* ctx, int32_t replica, const ldo_config* cfg);
+
+/* Read one replica's configuration (OrigamiSystem::chains(),
+ * origami_system.cpp:175-191).  cfg arrays must hold cap_chains chains and
+ * cap_domains domains. */
+int ldo_get_config(void* ctx, int32_t replica, ldo_config* cfg,
+                   int32_t cap_chains, int32_t cap_domains);
+
+/* Control variables of a temperature ladder slot (PTGCMCSimulation control
+ * q's, ptmc_simulation.cpp:337-358): temperature, staple chemical potential
+ * multiplier, bias multiplier and stacking multiplier. */
+int ldo_set_ladder(void* ctx, int32_t n, const double* temps,
+                   const double* staple_u_mults, const double* bias_mults,
+                   const double* stacking_mults);
+
+/* Supply a recorded draw sequence for one replica (LDO_RNG_TAPE).  Entry k
+ * is the k-th value the reference's RandomGens returned: a uniform_real as
+ * its IEEE-754 bit pattern, a uniform_int as the integer. */
+int ldo_set_tape(void* ctx, int32_t replica, const uint64_t* tape, int64_t n);
+
+/* Run `steps` MC steps on every replica with no exchange
+ * (GCMCSimulation::simulate, simulation.cpp:568-653). */
+int ldo_sweep(void* ctx, int64_t steps);
+
+/* PTMC: n_swaps rounds of [apply ladder slot -> simulate(exchange_interval)
+ * -> dependent q's -> device exchange] (PTGCMCSimulation::run,
+ * ptmc_simulation.cpp:106-150). */
+int ldo_run(void* ctx, int64_t n_swaps, int64_t exchange_interval);
+
+int ldo_get_observables(void* ctx, int32_t replica, ldo_obs* out);
+int ldo_get_counters(void* ctx, ldo_counters* out);
+
+/* Ladder slot -> replica permutation (m_q_to_repi, written to .swp). */
+int ldo_get_permutation(void* ctx, int32_t* q_to_rep);
+
+/* The HIP stream the sampler's kernels are launched on (hipStream_t). */
+void* ldo_get_stream(void* ctx);
+
+/* Name of the sampler kernel, for matching profiler output. */
+const char* ldo_kernel_name(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LDO_H */

@@ -1,0 +1,192 @@
+"""ctypes mirror of include/ldo.h and the loader of libldo_hip.so.
+
+The structs here are field-for-field copies of the C declarations; the
+library is the product (HIP kernels behind a C ABI).  Loading fails loudly
+when the library has not been built: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libldo_hip.so")
+
+LDO_OK = 0
+LDO_E_CONSTRAINT = 1
+LDO_E_MISUSE = 2
+LDO_E_NOTIMPL = 3
+LDO_E_IO = 4
+LDO_E_HIP = 5
+LDO_E_ARG = 6
+
+HALFTURN, THREEQUARTERTURN = 0, 1
+HYB_NEARESTNEIGHBOUR, HYB_UNIFORM, HYB_SPECIFIED = 0, 1, 2
+MISBIND_OPPOSING, MISBIND_DISALLOWED = 0, 1
+STACK_CONSTANT, STACK_SEQUENCESPECIFIC = 0, 1
+MOVE_ORIENTATION_ROTATION = 0
+MOVE_MET_STAPLE_EXCHANGE = 1
+MOVE_CB_STAPLE_REGROWTH = 2
+MOVE_CTCB_SCAFFOLD_REGROWTH = 3
+RNG_PHILOX, RNG_TAPE = 0, 1
+EXCHANGE_NONE, EXCHANGE_TEMPERATURE = 0, 1
+MAX_MOVETYPES = 8
+
+MOVETYPE_NAMES = {
+    "OrientationRotation": MOVE_ORIENTATION_ROTATION,
+    "MetStapleExchange": MOVE_MET_STAPLE_EXCHANGE,
+    "CBStapleRegrowth": MOVE_CB_STAPLE_REGROWTH,
+    "CTCBScaffoldRegrowth": MOVE_CTCB_SCAFFOLD_REGROWTH,
+}
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [
+        ("n_chain_types", C.c_int32),
+        ("chain_lengths", _i32p),
+        ("identities", _i32p),
+        ("sequences", C.POINTER(C.c_char_p)),
+        ("enthalpies", _f64p),
+        ("entropies", _f64p),
+        ("n_specified", C.c_int32),
+        ("cyclic", C.c_int32),
+        ("domain_type", C.c_int32),
+        ("hybridization_pot", C.c_int32),
+        ("misbinding_pot", C.c_int32),
+        ("stacking_pot", C.c_int32),
+        ("apply_mean_field_cor", C.c_int32),
+        ("pad0", C.c_int32),
+        ("binding_h", C.c_double),
+        ("binding_s", C.c_double),
+        ("misbinding_h", C.c_double),
+        ("misbinding_s", C.c_double),
+        ("stacking_ene", C.c_double),
+        ("cation_M", C.c_double),
+        ("staple_M", C.c_double),
+    ]
+
+
+class RunParams(C.Structure):
+    _fields_ = [
+        ("n_movetypes", C.c_int32),
+        ("movetype_kind", _i32p),
+        ("movetype_freq", _f64p),
+        ("movetype_max_regrowth", _i32p),
+        ("max_total_staples", C.c_int32),
+        ("max_type_staples", C.c_int32),
+        ("max_staple_size", C.c_int32),
+        ("centering_freq", C.c_int32),
+        ("centering_domain", C.c_int32),
+        ("constraint_check_freq", C.c_int32),
+        ("seed", C.c_uint64),
+        ("rng_mode", C.c_int32),
+        ("exchange_mode", C.c_int32),
+        ("lanes_per_replica", C.c_int32),
+        ("pad0", C.c_int32),
+    ]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("n_chains", C.c_int32),
+        ("pad0", C.c_int32),
+        ("chain_index", _i32p),
+        ("chain_identity", _i32p),
+        ("chain_length", _i32p),
+        ("positions", _i32p),
+        ("orientations", _i32p),
+    ]
+
+
+class Obs(C.Structure):
+    _fields_ = [
+        ("energy", C.c_double),
+        ("enthalpy", C.c_double),
+        ("entropy", C.c_double),
+        ("stacking_energy", C.c_double),
+        ("temperature", C.c_double),
+        ("step", C.c_int64),
+        ("num_staples", C.c_int32),
+        ("num_domains", C.c_int32),
+        ("num_bound_domain_pairs", C.c_int32),
+        ("num_fully_bound_domain_pairs", C.c_int32),
+        ("num_misbound_domain_pairs", C.c_int32),
+        ("num_stacked_domain_pairs", C.c_int32),
+        ("num_unassigned_domains", C.c_int32),
+        ("status", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Counters(C.Structure):
+    _fields_ = [
+        ("attempts", C.c_int64 * MAX_MOVETYPES),
+        ("accepts", C.c_int64 * MAX_MOVETYPES),
+        ("site_empty", C.c_int64),
+        ("site_bind", C.c_int64),
+        ("commits", C.c_int64),
+        ("exchange_attempts", C.c_int64),
+        ("exchange_accepts", C.c_int64),
+    ]
+
+
+class LdoError(RuntimeError):
+    """A non-zero status from the sampler (or oracle) C ABI."""
+
+    def __init__(self, code, message):
+        super().__init__(f"status {code}: {message}")
+        self.code = code
+
+
+def declare_abi(lib, prefix):
+    """Attach argtypes/restype for the ldo_* (or oracle_*) entry points."""
+    v = C.c_void_p
+    sig = {
+        "destroy": (None, [v]),
+        "last_error": (C.c_char_p, [v]),
+        "set_ladder": (C.c_int, [v, C.c_int32, _f64p, _f64p, _f64p, _f64p]),
+        "sweep": (C.c_int, [v, C.c_int64]),
+        "run": (C.c_int, [v, C.c_int64, C.c_int64]),
+        "get_observables": (C.c_int, [v, C.c_int32, C.POINTER(Obs)]),
+        "get_counters": (C.c_int, [v, C.POINTER(Counters)]),
+        "get_permutation": (C.c_int, [v, _i32p]),
+        "get_config": (C.c_int, [v, C.c_int32, C.POINTER(Config), C.c_int32, C.c_int32]),
+        "set_config": (C.c_int, [v, C.c_int32, C.POINTER(Config)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, prefix + name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+_LIB = None
+
+
+def load_library(path=None):
+    """Load libldo_hip.so (the product).  Raises if it is not built."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise OSError(
+            f"{p} is missing: build the HIP sampler first (python -c "
+            "'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(p)
+    declare_abi(lib, "ldo_")
+    lib.ldo_create.restype = C.c_int
+    lib.ldo_create.argtypes = [C.POINTER(SystemDesc), C.POINTER(RunParams),
+                               C.POINTER(Config), C.c_int32, _f64p,
+                               C.POINTER(C.c_void_p)]
+    lib.ldo_set_tape.restype = C.c_int
+    lib.ldo_set_tape.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64), C.c_int64]
+    lib.ldo_get_stream.restype = C.c_void_p
+    lib.ldo_get_stream.argtypes = [C.c_void_p]
+    lib.ldo_kernel_name.restype = C.c_char_p
+    lib.ldo_kernel_name.argtypes = []
+    if path is None:
+        _LIB = lib
+    return lib

@@ -1,0 +1,587 @@
+// engine.h - device-side replica engine of the sampler (gfx950).
+//
+// One Eng instance drives one replica: it owns pointers into the replica's
+// block (layout.h) and implements the reference's lattice state updates,
+// FourBody energy and move types on slot-indexed arrays and an
+// open-addressed site hash.  Function names follow the reference's
+// operations; each cites the reference function whose semantics it
+// reproduces (the random draws are made in the same order, so a replica's
+// trajectory is a function of its random stream alone).
+#ifndef LDO_ENGINE_H
+#define LDO_ENGINE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "layout.h"
+
+#define DEV __device__ __forceinline__
+#define DEVNI __device__ __noinline__
+
+// Kernel-wide constants (one copy, passed by value to every launch).
+struct Params {
+    Layout L;
+    int32_t n_replicas;
+    int32_t domain_type;
+    int32_t misbinding_pot;
+    int32_t cyclic;
+    int32_t n_moves;
+    int32_t move_kind[LDO_MAX_MOVES];
+    double move_cum[LDO_MAX_MOVES];
+    int32_t move_maxreg[LDO_MAX_MOVES];
+    int32_t max_total_staples;
+    int32_t max_type_staples;
+    int32_t centering_freq;
+    int32_t centering_domain;
+    int32_t check_freq;
+    int32_t rng_mode;
+    uint64_t seed;
+    int32_t ident_lo;
+    int32_t n_ident;
+    int32_t n_slots; // ladder slots with tables
+    int32_t lanes_per_replica;
+    double fugacity; // reduced fugacity = staple_M (never updated)
+    const int32_t* type_len;   // [ntypes]
+    const int16_t* type_ident; // [ntypes * type_ident_stride] table index of each domain
+    int32_t type_ident_stride;
+    const double* hyb_e;       // [n_slots * n_ident^2]
+    const double* hyb_h;
+    const double* hyb_s;
+    const double* stack;
+    const double* init_e; // [n_slots]
+    const double* init_h;
+    const double* init_s;
+    char* blocks;
+    const uint64_t* tapes;    // concatenated tapes
+    const int64_t* tape_off;  // [n_replicas]
+    const int64_t* tape_len;  // [n_replicas]
+};
+
+// move kinds held in RepScalars::kind (the grow_chain dispatch)
+#define LDO_MOVE_KIND_ORIENT 0
+#define LDO_MOVE_KIND_MET 1
+#define LDO_MOVE_KIND_CB 2
+#define LDO_MOVE_KIND_CTCB 3
+
+// status codes written into RepScalars::status
+#define ST_OK 0
+#define ST_CONSTRAINT 1
+#define ST_MISUSE 2
+#define ST_CAPACITY 7
+#define ST_TAPE 8
+#define ST_COORD 9
+
+struct I3 {
+    int x, y, z;
+};
+
+DEV I3 i3(int x, int y, int z) {
+    I3 r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    return r;
+}
+DEV I3 add3(I3 a, I3 b) { return i3(a.x + b.x, a.y + b.y, a.z + b.z); }
+DEV I3 sub3(I3 a, I3 b) { return i3(a.x - b.x, a.y - b.y, a.z - b.z); }
+DEV I3 neg3(I3 a) { return i3(-a.x, -a.y, -a.z); }
+DEV bool eq3(I3 a, I3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+
+// unit vector of orientation index o (+x,-x,+y,-y,+z,-z)
+DEV I3 unit3(int o) {
+    int a = o >> 1;
+    int s = (o & 1) ? -1 : 1;
+    return i3(a == 0 ? s : 0, a == 1 ? s : 0, a == 2 ? s : 0);
+}
+
+// index of a unit vector (never called on other vectors)
+DEV int unit_index(I3 v) {
+    if (v.x != 0) {
+        return v.x > 0 ? 0 : 1;
+    }
+    if (v.y != 0) {
+        return v.y > 0 ? 2 : 3;
+    }
+    return v.z > 0 ? 4 : 5;
+}
+
+// VectorThree::rotate_half (utility.cpp:68-85)
+DEV I3 rotate_half3(I3 v, I3 axis) {
+    int ax = axis.x < 0 ? -axis.x : axis.x;
+    int ay = axis.y < 0 ? -axis.y : axis.y;
+    int az = axis.z < 0 ? -axis.z : axis.z;
+    if (ax == 1 && ay == 0 && az == 0) {
+        return i3(v.x, -v.y, -v.z);
+    }
+    if (ax == 0 && ay == 1 && az == 0) {
+        return i3(-v.x, v.y, -v.z);
+    }
+    if (ax == 0 && ay == 0 && az == 1) {
+        return i3(-v.x, -v.y, v.z);
+    }
+    return v;
+}
+
+// VectorThree::rotate(axis, -1) (utility.cpp:102-142): the quarter turn the
+// ThreeQuarterTurn twist rule uses.
+DEV I3 rotate_minus_quarter3(I3 v, I3 axis) {
+    int dir = -1;
+    int ax = axis.x < 0 ? -axis.x : axis.x;
+    int ay = axis.y < 0 ? -axis.y : axis.y;
+    int az = axis.z < 0 ? -axis.z : axis.z;
+    if (axis.x != ax || axis.y != ay || axis.z != az) {
+        dir = -dir;
+    }
+    if (ax == 1 && ay == 0 && az == 0) {
+        return i3(v.x, -dir * v.z, dir * v.y);
+    }
+    if (ax == 0 && ay == 1 && az == 0) {
+        return i3(-dir * v.z, v.y, dir * v.x);
+    }
+    if (ax == 0 && ay == 0 && az == 1) {
+        return i3(-dir * v.y, dir * v.x, v.z);
+    }
+    return v;
+}
+
+struct Delta {
+    double e;
+    int st;
+};
+
+struct Eng {
+    const Params* P;
+    int rep;
+    RepScalars* S;
+    DomRec* dom;
+    int16_t* dident;
+    ChainRec* ch;
+    int16_t* order;
+    int16_t* freel;
+    int16_t* tlist;
+    int16_t* tcount;
+    uint64_t* hkey;
+    uint32_t* hval;
+    int16_t* modified;
+    int16_t* assigned;
+    PosOre* prev;
+    PosOre* old;
+    int8_t* seg;
+    int16_t* gp;
+    int16_t* stem;
+    int16_t* inact;
+    int8_t* netseg;
+    int8_t* netds_flag;
+    int8_t* checked;
+    int8_t* netcs;
+    int8_t* regrow;
+    int8_t* dir0;
+    int8_t* dir1;
+    Endpoint* active;
+    Endpoint* active0;
+    int16_t* gps;
+    int16_t* iaes;
+    int16_t* netds;
+    NetFrame* frames;
+    int8_t* part;
+    NetFrame* sframes;
+    const double* t_hyb;
+    const double* t_stack;
+    double init_e;
+    int ns, lmax, nslots, nchain, hmask, hbits, n_ident;
+    // FourBody accumulators (BindingPotential::m_delta_config)
+    double dc_e;
+    int dc_st;
+    bool fb_viol;
+    bool pot_viol;
+    int n_gps, n_iaes, n_netds;
+    bool net_external;
+
+    // ------------------------------------------------------------ wiring
+    DEV void attach(const Params* p, char* blk, int r) {
+        P = p;
+        rep = r;
+        const Layout& L = p->L;
+        S = (RepScalars*)blk;
+        dom = (DomRec*)(blk + L.off_dom);
+        dident = (int16_t*)(blk + L.off_dident);
+        ch = (ChainRec*)(blk + L.off_chain);
+        order = (int16_t*)(blk + L.off_order);
+        freel = (int16_t*)(blk + L.off_free);
+        tlist = (int16_t*)(blk + L.off_tlist);
+        tcount = (int16_t*)(blk + L.off_tcount);
+        hkey = (uint64_t*)(blk + L.off_hkey);
+        hval = (uint32_t*)(blk + L.off_hval);
+        modified = (int16_t*)(blk + L.off_modified);
+        assigned = (int16_t*)(blk + L.off_assigned);
+        prev = (PosOre*)(blk + L.off_prev);
+        old = (PosOre*)(blk + L.off_old);
+        seg = (int8_t*)(blk + L.off_seg);
+        gp = (int16_t*)(blk + L.off_gp);
+        stem = (int16_t*)(blk + L.off_stem);
+        inact = (int16_t*)(blk + L.off_inact);
+        netseg = (int8_t*)(blk + L.off_netseg);
+        netds_flag = (int8_t*)(blk + L.off_netds_flag);
+        checked = (int8_t*)(blk + L.off_checked);
+        netcs = (int8_t*)(blk + L.off_netcs);
+        regrow = (int8_t*)(blk + L.off_regrow);
+        dir0 = (int8_t*)(blk + L.off_dir0);
+        dir1 = (int8_t*)(blk + L.off_dir1);
+        active = (Endpoint*)(blk + L.off_active);
+        active0 = (Endpoint*)(blk + L.off_active0);
+        gps = (int16_t*)(blk + L.off_gps);
+        iaes = (int16_t*)(blk + L.off_iaes);
+        netds = (int16_t*)(blk + L.off_netds);
+        frames = (NetFrame*)(blk + L.off_frames);
+        part = (int8_t*)(blk + L.off_part);
+        sframes = (NetFrame*)(blk + L.off_sframes);
+        ns = L.ns;
+        lmax = L.lmax;
+        nslots = L.nslots;
+        nchain = L.nchain;
+        hbits = L.hbits;
+        hmask = L.hcap - 1;
+        n_ident = p->n_ident;
+        select_tables();
+    }
+
+    DEV void select_tables() {
+        size_t n2 = (size_t)P->n_ident * P->n_ident;
+        t_hyb = P->hyb_e + (size_t)S->temp_slot * n2;
+        t_stack = P->stack + (size_t)S->temp_slot * n2;
+        init_e = P->init_e[S->init_slot];
+    }
+
+    DEV void fail(int code) {
+        if (S->status == ST_OK) {
+            S->status = code;
+        }
+    }
+
+    // ------------------------------------------------------------ rng
+    // Philox4x32-10 (Salmon et al., SC'11); counter (draw, 0, step lo, hi),
+    // key (seed lo, replica)
+    DEV uint64_t philox64() {
+        uint32_t c0 = (uint32_t)S->rng_idx, c1 = 0u;
+        uint32_t c2 = (uint32_t)(uint64_t)S->step, c3 = (uint32_t)((uint64_t)S->step >> 32);
+        uint32_t k0 = (uint32_t)P->seed, k1 = (uint32_t)rep;
+#pragma unroll
+        for (int r = 0; r < 10; r++) {
+            uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+            uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+            uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+            uint32_t n1 = (uint32_t)p1;
+            uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+            uint32_t n3 = (uint32_t)p0;
+            c0 = n0;
+            c1 = n1;
+            c2 = n2;
+            c3 = n3;
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        S->rng_idx++;
+        return (uint64_t)c0 | ((uint64_t)c1 << 32);
+    }
+
+    DEV uint64_t tape_next() {
+        int64_t pos = S->tape_pos;
+        if (pos >= P->tape_len[rep]) {
+            fail(ST_TAPE);
+            return 0;
+        }
+        S->tape_pos = pos + 1;
+        return P->tapes[P->tape_off[rep] + pos];
+    }
+
+    // RandomGens::uniform_real (random_gens.cpp:22-24)
+    DEV double uniform_real() {
+        if (P->rng_mode == 1) {
+            return __longlong_as_double((long long)tape_next());
+        }
+        return (double)(philox64() >> 11) * 0x1.0p-53;
+    }
+
+    // RandomGens::uniform_int (random_gens.cpp:26-43)
+    DEV int uniform_int(int lo, int hi) {
+        if (P->rng_mode == 1) {
+            return (int)(int64_t)tape_next();
+        }
+        uint64_t x = philox64();
+        uint64_t n = (uint64_t)((int64_t)hi - (int64_t)lo + 1);
+        return lo + (int)(((x >> 32) * n) >> 32);
+    }
+
+    // ------------------------------------------------------------ slots
+    DEV int cslot_of(int s) const { return s < ns ? 0 : 1 + (s - ns) / lmax; }
+    DEV int d_of(int s) const { return s < ns ? s : (s - ns) % lmax; }
+    DEV int slot(int cs, int d) const { return cs == 0 ? d : ns + (cs - 1) * lmax + d; }
+    DEV int len_of(int cs) const { return ch[cs].len; }
+
+    // Domain::m_forward_domain / m_backward_domain (origami_system.cpp:427-433, 686-692)
+    DEV int fwd(int s) const {
+        if (s < 0) {
+            return LDO_NONE;
+        }
+        int cs = cslot_of(s);
+        int d = d_of(s);
+        if (d + 1 < ch[cs].len) {
+            return s + 1;
+        }
+        return (cs == 0 && P->cyclic) ? 0 : LDO_NONE;
+    }
+    DEV int bwd(int s) const {
+        if (s < 0) {
+            return LDO_NONE;
+        }
+        int cs = cslot_of(s);
+        int d = d_of(s);
+        if (d > 0) {
+            return s - 1;
+        }
+        return (cs == 0 && P->cyclic) ? ns - 1 : LDO_NONE;
+    }
+    // Domain::operator+ (domain.cpp:9-31)
+    DEV int walk(int s, int incr) const {
+        while (s >= 0 && incr > 0) {
+            s = fwd(s);
+            incr--;
+        }
+        while (s >= 0 && incr < 0) {
+            s = bwd(s);
+            incr++;
+        }
+        return s;
+    }
+
+    DEV I3 pos(int s) const { return i3(dom[s].x, dom[s].y, dom[s].z); }
+    DEV I3 ore(int s) const { return unit3(dom[s].ore); }
+    DEV int st(int s) const { return dom[s].state; }
+    DEV int bd(int s) const { return dom[s].bound; }
+    DEV bool eb(int s) const { return s >= 0 && dom[s].state == OCC_BOUND; }
+    DEV int ident_of(int s) const { return dident[s] + P->ident_lo; }
+    DEV double hyb(int a, int b) const { return t_hyb[(size_t)dident[a] * n_ident + dident[b]]; }
+    DEV double stk(int a, int b) const { return t_stack[(size_t)dident[a] * n_ident + dident[b]]; }
+
+    // ------------------------------------------------------------ site hash
+    DEV uint64_t pkey(I3 p) const {
+        return ((uint64_t)(uint32_t)(p.x + LDO_COORD_OFFSET) << 42) |
+               ((uint64_t)(uint32_t)(p.y + LDO_COORD_OFFSET) << 21) |
+               (uint64_t)(uint32_t)(p.z + LDO_COORD_OFFSET);
+    }
+    DEV int home(uint64_t k) const {
+        return (int)((k * 0x9E3779B97F4A7C15ull) >> (64 - hbits));
+    }
+    DEV int hfind(uint64_t k) const {
+        int i = home(k);
+        while (true) {
+            uint64_t kk = hkey[i];
+            if (kk == k) {
+                return i;
+            }
+            if (kk == LDO_HASH_EMPTY) {
+                return -1;
+            }
+            i = (i + 1) & hmask;
+        }
+    }
+    // OrigamiSystem::position_occupancy (origami_system.cpp:193-202)
+    DEV int occupancy(I3 p) {
+        int i = hfind(pkey(p));
+        return i < 0 ? OCC_UNASSIGNED : (int)((hval[i] >> 16) & 3u);
+    }
+    DEV int unbound_at(I3 p) {
+        int i = hfind(pkey(p));
+        return (int)(int16_t)(hval[i] & 0xFFFFu);
+    }
+    DEV void hput(I3 p, int occ, int s) {
+        if (p.x <= -LDO_COORD_OFFSET + 2 || p.x >= LDO_COORD_OFFSET - 2 ||
+            p.y <= -LDO_COORD_OFFSET + 2 || p.y >= LDO_COORD_OFFSET - 2 ||
+            p.z <= -LDO_COORD_OFFSET + 2 || p.z >= LDO_COORD_OFFSET - 2) {
+            fail(ST_COORD);
+        }
+        uint64_t k = pkey(p);
+        int i = home(k);
+        while (hkey[i] != k && hkey[i] != LDO_HASH_EMPTY) {
+            i = (i + 1) & hmask;
+        }
+        hkey[i] = k;
+        hval[i] = ((uint32_t)occ << 16) | (uint32_t)(uint16_t)(int16_t)s;
+    }
+    // linear-probing deletion with backward shift (Knuth, TAOCP 6.4 Alg. R)
+    DEV void herase(I3 p) {
+        int i = hfind(pkey(p));
+        if (i < 0) {
+            return;
+        }
+        int j = i;
+        while (true) {
+            j = (j + 1) & hmask;
+            uint64_t kj = hkey[j];
+            if (kj == LDO_HASH_EMPTY) {
+                break;
+            }
+            int k = home(kj);
+            bool stay = (i <= j) ? (i < k && k <= j) : (i < k || k <= j);
+            if (stay) {
+                continue;
+            }
+            hkey[i] = kj;
+            hval[i] = hval[j];
+            i = j;
+        }
+        hkey[i] = LDO_HASH_EMPTY;
+    }
+
+    // ------------------------------------------------------------ rules
+    // HalfTurnDomain / ThreeQuarterTurnDomain::check_twist_constraint
+    // (domain.cpp:33-40, 69-78)
+    DEV bool twist_ok(int a, I3 ndr, int b) const {
+        I3 r = (P->domain_type == 0) ? rotate_half3(ore(a), ndr)
+                                     : rotate_minus_quarter3(ore(a), ndr);
+        return eq3(r, ore(b));
+    }
+    // check_kink_constraint (domain.cpp:42-57, 80-97)
+    DEV bool kink_ok(int a, I3 ndr, int b) const {
+        I3 oa = ore(a), ob = ore(b);
+        if (eq3(ndr, neg3(oa))) {
+            return false;
+        }
+        if (eq3(ndr, oa)) {
+            if (P->domain_type == 0) {
+                return !eq3(ob, neg3(oa));
+            }
+            return !(eq3(oa, ob) || eq3(oa, neg3(ob)));
+        }
+        if (eq3(ndr, ob) || eq3(ndr, neg3(ob))) {
+            return false;
+        }
+        return true;
+    }
+    // check_junction_constraint (domain.cpp:59-67, 99-118)
+    DEV bool junction_ok(int j1, int j2, int k1, int k2) const {
+        if (P->domain_type == 0) {
+            return true;
+        }
+        I3 nk = sub3(pos(k2), pos(k1));
+        if (eq3(nk, ore(k1))) {
+            I3 n1 = sub3(pos(j2), pos(j1));
+            if (d_of(j1) > d_of(j2)) {
+                n1 = neg3(n1);
+            }
+            if (!twist_ok(k1, n1, k2)) {
+                return false;
+            }
+        }
+        return true;
+    }
+    // doubly_contiguous (origami_potential.cpp:51-68)
+    DEV bool doubly_contig(int a, int b) const {
+        if (cslot_of(a) != cslot_of(b) || d_of(b) != d_of(a) + 1) {
+            return false;
+        }
+        int ba = bd(a), bb = bd(b);
+        if (cslot_of(ba) != cslot_of(bb)) {
+            return false;
+        }
+        int dd = d_of(bb) - d_of(ba);
+        return dd == 1 || dd == -1;
+    }
+    // check_pair_stacked (origami_potential.cpp:70-83)
+    DEV bool pair_stacked(int a, int b) const {
+        if (d_of(a) > d_of(b)) {
+            int t = a;
+            a = b;
+            b = t;
+        }
+        I3 ndr = sub3(pos(b), pos(a));
+        I3 oa = ore(a);
+        if (!eq3(ndr, oa) && !eq3(ndr, neg3(oa))) {
+            return twist_ok(a, ndr, b);
+        }
+        return false;
+    }
+
+    // ------------------------------------------------------------ FourBody
+    DEVNI void fb_single(int h1, int h2, int h3);
+    DEVNI void fb_double(int h1, int h2, int h3);
+    DEVNI void fb_triply(int h1, int h2, int h3);
+    DEVNI void fb_junction(int j1, int j2, int j3, int j4, int k1, int k2);
+    DEVNI void fb_junction_pair(int a, int b, int j);
+    DEVNI void fb_backward_triplets(int a, int b);
+    DEVNI void fb_forward_triplets(int a, int b);
+    DEVNI void fb_central_triplets(int di, int dj);
+    DEVNI void fb_backward_single_junction(int a, int b);
+    DEVNI void fb_forward_single_junction(int a, int b);
+    DEVNI void fb_central_single_junction(int a, int b);
+    DEVNI void fb_regular_pair(int a, int b, int i);
+    DEVNI void fb_helix_pair(int a, int b, int i, int j);
+    DEVNI void fb_around(int cd, int j);
+    DEVNI void fb_terms(int di, int dj);
+    DEVNI Delta bind_domain(int di);
+
+    // ------------------------------------------------------------ system
+    DEVNI void update_occupancies(int s, I3 p);
+    DEVNI Delta internal_check(int s, I3 p, int o);
+    DEVNI Delta internal_unassign(int s, bool need_delta);
+    DEVNI double check_domain_constraints(int s, I3 p, int o);
+    DEVNI double unassign_domain(int s);
+    DEVNI double set_checked_domain_config(int s, I3 p, int o);
+    DEVNI double set_domain_config(int s, I3 p, int o);
+    DEV void set_domain_orientation(int s, int o) {
+        if (dom[s].state == OCC_BOUND) {
+            S->violated = 1;
+        }
+        else {
+            dom[s].ore = (int8_t)o;
+        }
+    }
+    DEVNI int add_chain_with(int ident, int uid);
+    DEVNI int add_chain(int ident);
+    DEVNI void delete_chain(int cs);
+    DEVNI void center();
+    DEVNI void set_all_domains();
+    DEVNI void check_all_constraints();
+    DEVNI void update_energy();
+    DEVNI void update_enthalpy_and_entropy();
+
+    // ------------------------------------------------------------ moves
+    DEVNI int select_random_domain();
+    DEVNI bool test_acceptance(double p);
+    DEVNI bool staple_is_connector(int cs);
+    DEVNI bool scan_for_scaffold(int dom0, int excl_cs);
+    DEVNI int num_bound_staple_domains(int cs);
+    DEVNI void reset_origami();
+    DEVNI double set_growth_point(int gnew, int gold);
+    DEVNI void grow_staple(int cs, int d_i);
+    DEVNI void grow_chain_seq(int cs, int from, int to);
+    DEVNI bool orientation_move();
+    DEVNI bool insert_staple();
+    DEVNI bool delete_staple();
+    DEVNI void cb_unassign_chain(int cs, int from, int to);
+    DEVNI void cb_select_and_set(int d, int prev_d);
+    DEVNI void cb_setup_regrow_old();
+    DEVNI bool cb_test_acceptance();
+    DEVNI void cb_set_old_growth_point(int gnew, int gold);
+    DEVNI bool cb_staple_move();
+    // constraint points
+    DEVNI void cp_reset();
+    DEVNI void cp_add_active(int s, I3 p, int sg);
+    DEVNI void cp_remove_active(int s);
+    DEVNI void cp_update_endpoints(int s);
+    DEVNI bool cp_endpoint_reached(int s, I3 p);
+    DEVNI bool cp_walks_remain(int s, I3 p);
+    DEVNI void cp_scan_network(int d);
+    DEVNI void cp_find(int first, int count, int dir);
+    DEVNI void ctcb_grow_chain_scaffold(int first, int count, int dir);
+    DEVNI void ctcb_grow_staple_from(int gold);
+    DEVNI bool ctcb_scaffold_move(int max_regrowth);
+    DEVNI bool attempt(int m);
+    DEVNI void simulate(int64_t steps);
+    DEVNI void begin_interval(int slot);
+
+    // move-local scratch for the CTCB segment (kept in registers)
+    int seg_first, seg_count, seg_dir;
+};
+
+#endif

@@ -1,0 +1,1078 @@
+// engine_moves.h - the BASELINE move types and the simulate loop.
+//
+// MCMovetype and subclasses (movetypes.cpp, met_movetypes.cpp,
+// cb_movetypes.cpp, orientation_movetype.cpp) and the constraint-point
+// machinery of conserved-topology regrowth (top_constraint_points.cpp).
+// Recursions of the reference (staple-network scans, connector scans,
+// nested staple growth) run on explicit stacks in the replica block.
+#ifndef LDO_ENGINE_MOVES_H
+#define LDO_ENGINE_MOVES_H
+
+#include "engine.h"
+
+// MCMovetype::select_random_domain (movetypes.cpp:98-113)
+DEVNI int Eng::select_random_domain() {
+    int idx = uniform_int(0, S->num_domains - 1);
+    for (int w = 0; w < S->n_chains; w++) {
+        int cs = order[w];
+        int len = ch[cs].len;
+        if (idx < len) {
+            return slot(cs, idx);
+        }
+        idx -= len;
+    }
+    return LDO_NONE;
+}
+
+// MCMovetype::test_acceptance (movetypes.cpp:142-158)
+DEVNI bool Eng::test_acceptance(double p_ratio) {
+    double p_accept = fmin(1.0, p_ratio) * S->modifier;
+    if (p_accept == 1) {
+        return true;
+    }
+    return p_accept > uniform_real();
+}
+
+// MCMovetype::scan_for_scaffold_domain (movetypes.cpp:191-232) as an
+// explicit depth-first search; part[] holds the participating chains
+DEVNI bool Eng::scan_for_scaffold(int dom0, int excl_cs) {
+    for (int c = 0; c < nchain; c++) {
+        part[c] = 0;
+    }
+    if (excl_cs >= 0) {
+        part[excl_cs] = 1;
+    }
+    int sp = 0;
+    sframes[sp].growth = (int16_t)dom0;
+    sframes[sp].idx = 0;
+    sp++;
+    part[cslot_of(dom0)] = 1;
+    while (sp > 0) {
+        NetFrame& f = sframes[sp - 1];
+        int c = cslot_of(f.growth);
+        if (f.idx >= ch[c].len) {
+            sp--;
+            continue;
+        }
+        int cur = slot(c, f.idx);
+        f.idx++;
+        if (cur == f.growth) {
+            continue;
+        }
+        int b = dom[cur].bound;
+        if (b < 0) {
+            continue;
+        }
+        int bc = cslot_of(b);
+        if (bc == c) {
+            continue;
+        }
+        if (bc == 0) {
+            return true;
+        }
+        if (part[bc]) {
+            continue;
+        }
+        part[bc] = 1;
+        if (sp >= nchain) {
+            fail(ST_CAPACITY);
+            return true;
+        }
+        sframes[sp].growth = (int16_t)b;
+        sframes[sp].idx = 0;
+        sp++;
+    }
+    return false;
+}
+
+// MCMovetype::staple_is_connector (movetypes.cpp:160-177)
+DEVNI bool Eng::staple_is_connector(int cs) {
+    for (int d = 0; d < ch[cs].len; d++) {
+        int s = slot(cs, d);
+        if (dom[s].state != OCC_UNBOUND) {
+            int b = dom[s].bound;
+            if (cslot_of(b) == 0) {
+                continue;
+            }
+            if (!scan_for_scaffold(b, cs)) {
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+// RegrowthMCMovetype::num_bound_staple_domains (movetypes.cpp:395-405)
+DEVNI int Eng::num_bound_staple_domains(int cs) {
+    int n = 0;
+    for (int d = 0; d < ch[cs].len; d++) {
+        int stt = dom[slot(cs, d)].state;
+        if (stt == OCC_BOUND || stt == OCC_MISBOUND) {
+            n++;
+        }
+    }
+    return n;
+}
+
+// MCMovetype::reset_origami (movetypes.cpp:53-86)
+DEVNI void Eng::reset_origami() {
+    for (int k = 0; k < S->n_assigned; k++) {
+        unassign_domain(assigned[k]);
+    }
+    for (int k = 0; k < S->n_added; k++) {
+        delete_chain(S->added[k]);
+        S->current_c_i -= 1;
+    }
+    for (int k = 0; k < S->n_modified; k++) {
+        int s = modified[k];
+        set_checked_domain_config(s, i3(prev[s].x, prev[s].y, prev[s].z), prev[s].ore);
+    }
+    S->violated = 0;
+}
+
+// RegrowthMCMovetype::set_growth_point (movetypes.cpp:322-341)
+DEVNI double Eng::set_growth_point(int gnew, int gold) {
+    double de = set_domain_config(gnew, pos(gold), dom[gold].ore ^ 1);
+    if (S->violated) {
+        S->rejected = 1;
+    }
+    else {
+        assigned[S->n_assigned++] = (int16_t)gnew;
+    }
+    return de;
+}
+
+// RegrowthMCMovetype::grow_staple (movetypes.cpp:343-370)
+DEVNI void Eng::grow_staple(int cs, int d_i) {
+    int len = ch[cs].len;
+    int first = slot(cs, d_i);
+    if (len - d_i > 1) {
+        grow_chain_seq(first, len - d_i, 1);
+    }
+    if (S->rejected) {
+        return;
+    }
+    if (d_i + 1 > 1) {
+        grow_chain_seq(first, d_i + 1, -1);
+    }
+}
+
+// grow_chain of the Metropolis (met_movetypes.cpp:50-67) and CB staple
+// (cb_movetypes.cpp:354-361) moves over the domains first, first+dir, ...
+DEVNI void Eng::grow_chain_seq(int first, int count, int dir) {
+    int prev_d = first;
+    for (int i = 1; i < count; i++) {
+        int d = walk(prev_d, dir);
+        if (S->kind == LDO_MOVE_KIND_MET) {
+            I3 p = add3(pos(prev_d), unit3(uniform_int(0, 5)));
+            int o = uniform_int(0, 5);
+            S->delta_e += set_domain_config(d, p, o);
+            if (S->violated) {
+                S->rejected = 1;
+                break;
+            }
+            assigned[S->n_assigned++] = (int16_t)d;
+        }
+        else {
+            cb_select_and_set(d, prev_d);
+            if (S->rejected) {
+                break;
+            }
+        }
+        prev_d = d;
+    }
+}
+
+// OrientationRotationMCMovetype::internal_attempt_move (orientation_movetype.cpp:30-65)
+DEVNI bool Eng::orientation_move() {
+    bool accepted = false;
+    int d = select_random_domain();
+    int o_new = uniform_int(0, 5);
+    int stt = dom[d].state;
+    if (stt == OCC_BOUND || stt == OCC_MISBOUND) {
+        double de = 0;
+        int o_old = dom[d].ore;
+        int b = dom[d].bound;
+        de += unassign_domain(b);
+        set_domain_orientation(d, o_new);
+        I3 p = pos(d);
+        de += set_domain_config(b, p, o_new ^ 1);
+        if (!S->violated) {
+            accepted = test_acceptance(exp(-de));
+        }
+        if (!accepted) {
+            unassign_domain(b);
+            set_domain_orientation(d, o_old);
+            set_checked_domain_config(b, p, o_old ^ 1);
+        }
+    }
+    else {
+        set_domain_orientation(d, o_new);
+        accepted = true;
+    }
+    return accepted;
+}
+
+// MetStapleExchangeMCMovetype::insert_staple + staple_insertion_accepted
+// (met_movetypes.cpp:303-359, 212-253); exchange multipliers are 1
+DEVNI bool Eng::insert_staple() {
+    int ntypes = P->L.ntypes;
+    int ident = uniform_int(1, ntypes - 1);
+    if (S->num_staples == P->max_total_staples) {
+        return false;
+    }
+    if (tcount[ident] == P->max_type_staples) {
+        return false;
+    }
+    int cs = add_chain(ident);
+    if (cs < 0) {
+        return false;
+    }
+    S->delta_e += init_e;
+    S->added[S->n_added++] = cs;
+    int len = ch[cs].len;
+    int gi = uniform_int(0, len - 1);
+    int gnew = slot(cs, gi);
+    int gold = select_random_domain();
+    int guard = 0;
+    while (cslot_of(gold) == cs) {
+        gold = select_random_domain();
+        if (++guard > (1 << 20) || S->status != ST_OK) {
+            fail(ST_MISUSE);
+            return false;
+        }
+    }
+    S->delta_e += set_growth_point(gnew, gold);
+    if (S->rejected) {
+        return false;
+    }
+    grow_staple(cs, gi);
+    if (S->rejected) {
+        return false;
+    }
+    double boltz = exp(-S->delta_e);
+    int n_new = tcount[ident];
+    double pratio = (double)len / 6.0 / n_new * boltz;
+    pratio *= S->insertion_sites * P->fugacity;
+    pratio /= num_bound_staple_domains(cs);
+    return test_acceptance(pratio);
+}
+
+// MetStapleExchangeMCMovetype::delete_staple + staple_deletion_accepted
+// (met_movetypes.cpp:361-403, 255-301)
+DEVNI bool Eng::delete_staple() {
+    int ntypes = P->L.ntypes;
+    int ident = uniform_int(1, ntypes - 1);
+    int n = tcount[ident];
+    if (n == 0) {
+        S->rejected = 1;
+        return false;
+    }
+    int cs = tlist[ident * P->L.tcap + uniform_int(0, n - 1)];
+    if (staple_is_connector(cs)) {
+        return false;
+    }
+    int n_bd = num_bound_staple_domains(cs);
+    int len = ch[cs].len;
+    for (int d = 0; d < len; d++) {
+        int s = slot(cs, d);
+        prev[s].x = dom[s].x;
+        prev[s].y = dom[s].y;
+        prev[s].z = dom[s].z;
+        prev[s].ore = dom[s].ore;
+        modified[S->n_modified++] = (int16_t)s;
+        S->delta_e += unassign_domain(s);
+    }
+    S->delta_e -= init_e;
+    double boltz = exp(-S->delta_e);
+    int ni = tcount[ident];
+    double pratio = ni * 6.0 / (double)len * boltz;
+    pratio /= P->fugacity * (double)(S->insertion_sites - len);
+    pratio *= n_bd;
+    bool accepted = test_acceptance(pratio);
+    if (accepted) {
+        delete_chain(cs);
+    }
+    return accepted;
+}
+
+// CBMCMovetype::calc_biases + calc_bias + select_and_set_config
+// (cb_movetypes.cpp:58-166, 363-387, 486-539)
+DEVNI void Eng::cb_select_and_set(int d, int prev_d) {
+    I3 pp = pos(prev_d);
+    I3 cpos[6];
+    int core[6];
+    double w[6];
+    int n = 0;
+    for (int v = 0; v < 6; v++) {
+        I3 pn = add3(pp, unit3(v));
+        int occ = occupancy(pn);
+        if (occ == OCC_BOUND || occ == OCC_MISBOUND) {
+            continue;
+        }
+        if (occ == OCC_UNBOUND) {
+            int ub = unbound_at(pn);
+            int on = dom[ub].ore ^ 1;
+            double de = check_domain_constraints(d, pn, on);
+            if (!S->violated) {
+                cpos[n] = pn;
+                core[n] = on;
+                w[n] = exp(-de);
+                n++;
+            }
+            else {
+                S->violated = 0;
+            }
+        }
+        else {
+            cpos[n] = pn;
+            core[n] = -1;
+            double de = check_domain_constraints(d, pn, 0);
+            w[n] = 6 * exp(-de);
+            n++;
+        }
+    }
+    double sum = 0;
+    if (S->kind == LDO_MOVE_KIND_CTCB) {
+        for (int i = 0; i < n; i++) {
+            if (occupancy(cpos[i]) == OCC_UNBOUND) {
+                int occd = unbound_at(cpos[i]);
+                bool same_chain = cslot_of(occd) == cslot_of(d);
+                bool endpoint = cp_endpoint_reached(d, cpos[i]);
+                if (!(same_chain || endpoint)) {
+                    w[i] = 0;
+                }
+            }
+            if (!cp_walks_remain(d, cpos[i])) {
+                w[i] = 0;
+            }
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        sum += w[i];
+    }
+    if (sum == 0) {
+        S->rejected = 1;
+        return;
+    }
+    S->bias *= sum;
+    if (!S->regrow_old) {
+        double u = uniform_real();
+        double cum = 0;
+        int sel = -1;
+        for (int i = 0; i < n; i++) {
+            cum += w[i] / sum;
+            if (u < cum) {
+                sel = i;
+                break;
+            }
+        }
+        I3 p = sel >= 0 ? cpos[sel] : i3(0, 0, 0);
+        int o = sel >= 0 ? core[sel] : -1;
+        if (o < 0) {
+            o = uniform_int(0, 5);
+        }
+        set_checked_domain_config(d, p, o);
+    }
+    else {
+        set_checked_domain_config(d, i3(old[d].x, old[d].y, old[d].z), old[d].ore);
+    }
+    assigned[S->n_assigned++] = (int16_t)d;
+}
+
+// CBMCMovetype::unassign_domains (cb_movetypes.cpp:200-213) over the domains
+// of chain cs with d in [from, to)
+DEVNI void Eng::cb_unassign_chain(int cs, int from, int to) {
+    for (int d = from; d < to; d++) {
+        int s = slot(cs, d);
+        prev[s].x = dom[s].x;
+        prev[s].y = dom[s].y;
+        prev[s].z = dom[s].z;
+        prev[s].ore = dom[s].ore;
+        modified[S->n_modified++] = (int16_t)s;
+        unassign_domain(s);
+    }
+}
+
+// CBMCMovetype::setup_for_regrow_old (cb_movetypes.cpp:215-225)
+DEVNI void Eng::cb_setup_regrow_old() {
+    S->regrow_old = 1;
+    S->new_bias = S->bias;
+    S->new_modifier = S->modifier;
+    S->bias = 1;
+    for (int k = 0; k < S->n_modified; k++) {
+        int s = modified[k];
+        old[s] = prev[s];
+    }
+    S->n_modified = 0;
+    S->n_assigned = 0;
+}
+
+// CBMCMovetype::test_cb_acceptance (cb_movetypes.cpp:182-198)
+DEVNI bool Eng::cb_test_acceptance() {
+    double ratio = S->new_bias / S->bias;
+    if (test_acceptance(ratio)) {
+        reset_origami();
+        return true;
+    }
+    S->n_modified = 0;
+    S->n_assigned = 0;
+    return false;
+}
+
+// CBMCMovetype::set_old_growth_point (cb_movetypes.cpp:168-180)
+DEVNI void Eng::cb_set_old_growth_point(int gnew, int gold) {
+    double de = set_checked_domain_config(gnew, pos(gold), old[gnew].ore);
+    S->bias *= exp(-de);
+    assigned[S->n_assigned++] = (int16_t)gnew;
+}
+
+// CBStapleRegrowthMCMovetype::internal_attempt_move (cb_movetypes.cpp:292-348)
+DEVNI bool Eng::cb_staple_move() {
+    if (S->num_staples == 0) {
+        return false;
+    }
+    int w = uniform_int(1, S->num_staples);
+    int cs = order[w];
+    if (staple_is_connector(cs)) {
+        return false;
+    }
+    int len = ch[cs].len;
+    int gnew_l[8], gold_l[8];
+    int nb = 0;
+    for (int d = 0; d < len; d++) {
+        int s = slot(cs, d);
+        int b = dom[s].bound;
+        if (b >= 0 && cslot_of(b) != cs) {
+            gnew_l[nb] = s;
+            gold_l[nb] = b;
+            nb++;
+        }
+    }
+    if (nb == 0) {
+        fail(ST_CONSTRAINT);
+        return false;
+    }
+    S->bias *= nb;
+    int k = uniform_int(0, nb - 1);
+    cb_unassign_chain(cs, 0, len);
+    double de = set_growth_point(gnew_l[k], gold_l[k]);
+    S->bias *= exp(-de);
+    if (!S->rejected) {
+        grow_staple(cs, d_of(gnew_l[k]));
+    }
+    if (S->rejected) {
+        return false;
+    }
+    S->bias /= num_bound_staple_domains(cs);
+    cb_setup_regrow_old();
+    k = uniform_int(0, nb - 1);
+    cb_unassign_chain(cs, 0, len);
+    cb_set_old_growth_point(gnew_l[k], gold_l[k]);
+    if (!S->rejected) {
+        grow_staple(cs, d_of(gnew_l[k]));
+    }
+    S->modifier = S->new_modifier;
+    return cb_test_acceptance();
+}
+
+// ------------------------------------------------- constraint points
+// Constraintpoints::reset_internal (top_constraint_points.cpp:191-205)
+DEVNI void Eng::cp_reset() {
+    for (int s = 0; s < nslots; s++) {
+        seg[s] = -1;
+        gp[s] = LDO_NONE;
+        stem[s] = LDO_NONE;
+        inact[s] = LDO_NONE;
+        netseg[s] = -1;
+        netds_flag[s] = 0;
+    }
+    for (int c = 0; c < nchain; c++) {
+        checked[c] = 0;
+        netcs[c] = 0;
+        regrow[c] = 0;
+        dir0[c] = 0;
+        dir1[c] = 0;
+    }
+    S->n_active = 0;
+    S->n_active0 = 0;
+    n_netds = 0;
+}
+
+// Constraintpoints::add_active_endpoint (:264-278)
+DEVNI void Eng::cp_add_active(int s, I3 p, int sg) {
+    if (S->n_active >= P->L.ecap) {
+        fail(ST_CAPACITY);
+        return;
+    }
+    Endpoint& e = active[S->n_active++];
+    e.cslot = (int16_t)cslot_of(s);
+    e.seg = (int8_t)sg;
+    e.d = (int8_t)d_of(s);
+    e.x = p.x;
+    e.y = p.y;
+    e.z = p.z;
+}
+
+// Constraintpoints::remove_active_endpoint (:299-320)
+DEVNI void Eng::cp_remove_active(int s) {
+    int sg = seg[s];
+    int cs = cslot_of(s), dd = d_of(s);
+    int k = 0;
+    for (int i = 0; i < S->n_active; i++) {
+        Endpoint e = active[i];
+        if (e.cslot == cs && e.seg == sg && e.d == dd) {
+            continue;
+        }
+        active[k++] = e;
+    }
+    S->n_active = k;
+}
+
+// Constraintpoints::update_endpoints (:340-349)
+DEVNI void Eng::cp_update_endpoints(int s) {
+    cp_remove_active(s);
+    int e = inact[s];
+    if (e >= 0) {
+        cp_add_active(e, pos(s), seg[e]);
+    }
+}
+
+// Constraintpoints::endpoint_reached (:359-372)
+DEVNI bool Eng::cp_endpoint_reached(int s, I3 p) {
+    int sg = seg[s];
+    int cs = cslot_of(s), dd = d_of(s);
+    for (int i = 0; i < S->n_active; i++) {
+        const Endpoint& e = active[i];
+        if (e.cslot == cs && e.seg == sg && e.d == dd && e.x == p.x && e.y == p.y &&
+            e.z == p.z) {
+            return true;
+        }
+    }
+    return false;
+}
+
+// Constraintpoints::walks_remain (:397-442, 571-601).  The ideal-walk count
+// is only tested against zero, which holds exactly when the Manhattan
+// distance exceeds the remaining steps or differs from them in parity
+// (ideal_random_walk.cpp:35-41).  Stem domains look up their segment pair in
+// a map the contiguous move never fills, so they have no walk constraint.
+DEVNI bool Eng::cp_walks_remain(int s, I3 p) {
+    if (stem[s] >= 0) {
+        return true;
+    }
+    int sg = seg[s];
+    int cs = cslot_of(s), dd = d_of(s);
+    int dir = sg == 0 ? dir0[cs] : dir1[cs];
+    for (int i = 0; i < S->n_active; i++) {
+        const Endpoint& e = active[i];
+        if (e.cslot != cs || e.seg != sg) {
+            continue;
+        }
+        int steps;
+        int ed = e.d;
+        if (P->cyclic && cs == 0) {
+            if (dir > 0 && ed < dd) {
+                steps = ns + ed - dd;
+            }
+            else if (dir < 0 && ed > dd) {
+                steps = dd + ns - ed;
+            }
+            else if (dir == 0 || ed == dd) {
+                steps = 0;
+            }
+            else {
+                steps = ed > dd ? ed - dd : dd - ed;
+            }
+        }
+        else {
+            steps = ed > dd ? ed - dd : dd - ed;
+        }
+        int ax = e.x - p.x, ay = e.y - p.y, az = e.z - p.z;
+        int l1 = (ax < 0 ? -ax : ax) + (ay < 0 ? -ay : ay) + (az < 0 ? -az : az);
+        if (l1 > steps || ((steps - l1) & 1)) {
+            return false;
+        }
+    }
+    return true;
+}
+
+// StapleNetwork::scan_network + scan_staple_topology + make_staple_stack
+// (top_constraint_points.cpp:36-44, 78-170).  The recursion into bound
+// staples runs on frames[]; gps/iaes hold (domain, domain) pairs.
+DEVNI void Eng::cp_scan_network(int d0) {
+    for (int i = 0; i < n_netds; i++) {
+        int s = netds[i];
+        netseg[s] = -1;
+        netds_flag[s] = 0;
+        netcs[cslot_of(s)] = 0;
+    }
+    net_external = false;
+    n_gps = 0;
+    n_iaes = 0;
+    n_netds = 0;
+    netcs[0] = 1;
+    int sp = 0;
+    int growth = d0;
+    // enter a recursion level for `growth`
+    {
+        int ci = cslot_of(growth);
+        netcs[ci] = 1;
+        netds[n_netds++] = (int16_t)growth;
+        netds_flag[growth] = 1;
+        int gd = d_of(growth), len = ch[ci].len;
+        netseg[growth] = 0;
+        for (int k = gd + 1; k < len; k++) {
+            netseg[slot(ci, k)] = 0;
+        }
+        for (int k = gd - 1; k >= 0; k--) {
+            netseg[slot(ci, k)] = 1;
+        }
+        frames[sp].growth = (int16_t)growth;
+        frames[sp].idx = 0;
+        sp++;
+    }
+    while (sp > 0) {
+        NetFrame& f = frames[sp - 1];
+        int ci = cslot_of(f.growth);
+        int len = ch[ci].len;
+        int gd = d_of(f.growth);
+        if (f.idx >= len - 1) {
+            sp--;
+            continue;
+        }
+        int k = f.idx++;
+        int dd = (k < len - 1 - gd) ? slot(ci, gd + 1 + k) : slot(ci, gd - 1 - (k - (len - 1 - gd)));
+        netds[n_netds++] = (int16_t)dd;
+        netds_flag[dd] = 1;
+        int b = dom[dd].bound;
+        if (b < 0 || cslot_of(b) == ci) {
+            continue;
+        }
+        int bci = cslot_of(b);
+        if (netcs[bci]) {
+            bool external = false;
+            if (bci == 0) {
+                int rel = (d_of(b) - d_of(seg_first)) * seg_dir;
+                if (P->cyclic) {
+                    rel = ((rel % ns) + ns) % ns;
+                }
+                external = !(rel >= 0 && rel < seg_count);
+            }
+            if (!external) {
+                if (netds_flag[b]) {
+                    iaes[2 * n_iaes] = (int16_t)b;
+                    iaes[2 * n_iaes + 1] = (int16_t)dd;
+                }
+                else {
+                    iaes[2 * n_iaes] = (int16_t)dd;
+                    iaes[2 * n_iaes + 1] = (int16_t)b;
+                }
+                n_iaes++;
+            }
+            if (!net_external && external) {
+                net_external = true;
+            }
+        }
+        else {
+            gps[2 * n_gps] = (int16_t)dd;
+            gps[2 * n_gps + 1] = (int16_t)b;
+            n_gps++;
+            if (sp >= nchain) {
+                fail(ST_CAPACITY);
+                return;
+            }
+            // recurse into the bound staple
+            int cj = bci;
+            netcs[cj] = 1;
+            netds[n_netds++] = (int16_t)b;
+            netds_flag[b] = 1;
+            int bd0 = d_of(b), blen = ch[cj].len;
+            netseg[b] = 0;
+            for (int q = bd0 + 1; q < blen; q++) {
+                netseg[slot(cj, q)] = 0;
+            }
+            for (int q = bd0 - 1; q >= 0; q--) {
+                netseg[slot(cj, q)] = 1;
+            }
+            frames[sp].growth = (int16_t)b;
+            frames[sp].idx = 0;
+            sp++;
+        }
+    }
+}
+
+// Constraintpoints::calculate_constraintpoints + find_growthpoints_endpoints
+// (top_constraint_points.cpp:207-221, 454-569) for the scaffold segment
+// first, first+dir, ... (count domains)
+DEVNI void Eng::cp_find(int first, int count, int dir) {
+    dir0[0] = (int8_t)dir;
+    int d = first;
+    for (int i = 0; i < count; i++) {
+        if (i > 0) {
+            d = walk(d, dir);
+        }
+        seg[d] = 0;
+        int b = dom[d].bound;
+        if (b < 0 || cslot_of(b) == 0 || checked[cslot_of(b)]) {
+            continue;
+        }
+        cp_scan_network(b);
+        gps[2 * n_gps] = (int16_t)d;
+        gps[2 * n_gps + 1] = (int16_t)b;
+        n_gps++;
+        if (net_external) {
+            for (int g = 0; g < n_gps; g++) {
+                int g1 = gps[2 * g];
+                if (cslot_of(g1) == 0) {
+                    cp_add_active(g1, pos(g1), 0);
+                }
+            }
+            for (int a = 0; a < n_iaes; a++) {
+                int a1 = iaes[2 * a], a2 = iaes[2 * a + 1];
+                if (cslot_of(a2) == 0) {
+                    cp_add_active(a1, pos(a2), 0);
+                }
+            }
+        }
+        else {
+            for (int g = 0; g < n_gps; g++) {
+                gp[gps[2 * g]] = gps[2 * g + 1];
+                stem[gps[2 * g + 1]] = gps[2 * g];
+            }
+            for (int a = 0; a < n_iaes; a++) {
+                inact[iaes[2 * a]] = iaes[2 * a + 1];
+            }
+            for (int q = 0; q < n_netds; q++) {
+                int s = netds[q];
+                int c = cslot_of(s);
+                if (c != 0) {
+                    regrow[c] = 1;
+                    dir0[c] = 1;
+                    dir1[c] = -1;
+                }
+                if (seg[s] < 0) {
+                    seg[s] = netseg[s];
+                }
+            }
+        }
+        for (int q = 0; q < n_netds; q++) {
+            checked[cslot_of(netds[q])] = 1;
+        }
+        checked[0] = 1;
+    }
+    for (int i = 0; i < S->n_active; i++) {
+        active0[i] = active[i];
+    }
+    S->n_active0 = S->n_active;
+}
+
+struct GrowFrame {
+    int16_t first, count, i, prev;
+    int16_t gold, gnew;
+    int8_t dir, kind, phase, pad;
+};
+
+// CTCBRegrowthMCMovetype::grow_chain (cb_movetypes.cpp:463-484) and
+// grow_staple_and_update_endpoints (:541-562), interleaved through an
+// explicit stack.  Growing starts with the scaffold segment and descends
+// into staples at growthpoints; any rejection unwinds the whole growth.
+DEVNI void Eng::ctcb_grow_chain_scaffold(int first, int count, int dir) {
+    GrowFrame stk[32];
+    int sp = 0;
+    if (count > 1) {
+        GrowFrame f;
+        f.first = (int16_t)first;
+        f.count = (int16_t)count;
+        f.i = 1;
+        f.prev = (int16_t)first;
+        f.dir = (int8_t)dir;
+        f.kind = 0;
+        f.phase = 0;
+        stk[sp++] = f;
+    }
+    if (gp[first] >= 0) {
+        GrowFrame f;
+        f.gold = (int16_t)first;
+        f.kind = 1;
+        f.phase = 0;
+        stk[sp++] = f;
+    }
+    while (sp > 0 && !S->rejected && S->status == ST_OK) {
+        GrowFrame& F = stk[sp - 1];
+        if (F.kind == 0) {
+            if (F.i >= F.count) {
+                sp--;
+                continue;
+            }
+            int d = walk(F.prev, F.dir);
+            if (seg[d] < 0) {
+                seg[d] = 0; // get_dir's map insertion (top_constraint_points.cpp:179-185)
+            }
+            cb_select_and_set(d, F.prev);
+            if (S->rejected) {
+                break;
+            }
+            cp_update_endpoints(d);
+            F.prev = (int16_t)d;
+            F.i++;
+            if (gp[d] >= 0) {
+                if (sp >= 32) {
+                    fail(ST_CAPACITY);
+                    break;
+                }
+                GrowFrame g;
+                g.gold = (int16_t)d;
+                g.kind = 1;
+                g.phase = 0;
+                stk[sp++] = g;
+            }
+            continue;
+        }
+        // staple growth from growthpoint F.gold
+        if (F.phase == 0) {
+            int gnew = gp[F.gold];
+            int cs = cslot_of(gnew);
+            if (cs == 0) {
+                sp--;
+                continue;
+            }
+            if (S->regrow_old) {
+                cb_set_old_growth_point(gnew, F.gold);
+            }
+            else {
+                double de = set_growth_point(gnew, F.gold);
+                S->bias *= exp(-de);
+            }
+            if (S->rejected) {
+                break;
+            }
+            cp_update_endpoints(gnew);
+            F.gnew = (int16_t)gnew;
+            F.phase = 1;
+            int len = ch[cs].len, di = d_of(gnew);
+            if (len - di > 1) {
+                if (sp >= 32) {
+                    fail(ST_CAPACITY);
+                    break;
+                }
+                GrowFrame l;
+                l.first = (int16_t)gnew;
+                l.count = (int16_t)(len - di);
+                l.i = 1;
+                l.prev = (int16_t)gnew;
+                l.dir = 1;
+                l.kind = 0;
+                l.phase = 0;
+                stk[sp++] = l;
+            }
+            continue;
+        }
+        if (F.phase == 1) {
+            F.phase = 2;
+            int di = d_of(F.gnew);
+            if (di + 1 > 1) {
+                if (sp >= 32) {
+                    fail(ST_CAPACITY);
+                    break;
+                }
+                GrowFrame l;
+                l.first = F.gnew;
+                l.count = (int16_t)(di + 1);
+                l.i = 1;
+                l.prev = F.gnew;
+                l.dir = -1;
+                l.kind = 0;
+                l.phase = 0;
+                stk[sp++] = l;
+            }
+            continue;
+        }
+        sp--;
+    }
+}
+
+// CTCBScaffoldRegrowthMCMovetype::internal_attempt_move (cb_movetypes.cpp:680-747)
+// with CTRegrowthMCMovetype::select_indices (movetypes.cpp:469-510)
+DEVNI bool Eng::ctcb_scaffold_move(int max_regrowth) {
+    S->regrow_old = 0;
+    cp_reset();
+    int first, count, dir;
+    int guard = 0;
+    while (true) {
+        int max_len = ns < max_regrowth ? ns : max_regrowth;
+        int sel = uniform_int(2, max_len);
+        int start = uniform_int(0, ns - 1);
+        dir = uniform_int(0, 1);
+        if (dir == 0) {
+            dir = -1;
+        }
+        int start_s = slot(0, start);
+        int c = start_s;
+        int nf = 0;
+        while (c >= 0 && nf != sel) {
+            nf++;
+            c = walk(c, dir);
+        }
+        int cur_end = c;
+        first = start_s;
+        int nb = 0;
+        int b = walk(start_s, -dir);
+        while (b >= 0 && nf + nb != sel) {
+            first = b;
+            nb++;
+            b = walk(b, -dir);
+        }
+        count = nf + nb;
+        if (count < 2) {
+            if (++guard > 1000 || S->status != ST_OK) {
+                fail(ST_MISUSE);
+                return false;
+            }
+            continue;
+        }
+        if (cur_end >= 0) {
+            cp_add_active(cur_end, pos(cur_end), 0);
+        }
+        break;
+    }
+    seg_first = first;
+    seg_count = count;
+    seg_dir = dir;
+    cp_find(first, count, dir);
+    bool whole_ring = P->cyclic && count == ns;
+    if (!whole_ring) {
+        cp_remove_active(first);
+    }
+    // staples to be regrown, in ascending unique-index order (std::set<int>);
+    // the list borrows the connector-scan frames, which are idle here
+    int16_t* sl = (int16_t*)sframes;
+    int nst = 0;
+    for (int c2 = 1; c2 < nchain; c2++) {
+        if (regrow[c2] && ch[c2].ident >= 0) {
+            int k = nst++;
+            while (k > 0 && ch[sl[k - 1]].uid > ch[c2].uid) {
+                sl[k] = sl[k - 1];
+                k--;
+            }
+            sl[k] = (int16_t)c2;
+        }
+    }
+    for (int phase = 0; phase < 2; phase++) {
+        if (phase == 1) {
+            cb_setup_regrow_old();
+            for (int i = 0; i < S->n_active0; i++) {
+                active[i] = active0[i];
+            }
+            S->n_active = S->n_active0;
+            if (!whole_ring) {
+                cp_remove_active(first);
+            }
+        }
+        int d = first;
+        for (int i = 1; i < count; i++) {
+            d = walk(d, dir);
+            prev[d].x = dom[d].x;
+            prev[d].y = dom[d].y;
+            prev[d].z = dom[d].z;
+            prev[d].ore = dom[d].ore;
+            modified[S->n_modified++] = (int16_t)d;
+            unassign_domain(d);
+        }
+        for (int k = 0; k < nst; k++) {
+            cb_unassign_chain(sl[k], 0, ch[sl[k]].len);
+        }
+        ctcb_grow_chain_scaffold(first, count, dir);
+        if (phase == 0 && S->rejected) {
+            return false;
+        }
+    }
+    S->modifier = 1;
+    return cb_test_acceptance();
+}
+
+// MCMovetype::attempt_move and the reset_internal chain (movetypes.cpp:41-51,
+// 242-250; met_movetypes.cpp:45-48, 139-143; cb_movetypes.cpp:44-50, 457-461)
+DEVNI bool Eng::attempt(int m) {
+    S->n_modified = 0;
+    S->n_assigned = 0;
+    S->n_added = 0;
+    S->rejected = 0;
+    S->modifier = 1;
+    S->delta_e = 0;
+    S->bias = 1;
+    S->new_bias = 1;
+    S->new_modifier = 1;
+    S->regrow_old = 0;
+    S->insertion_sites = S->num_domains;
+    int kind = P->move_kind[m];
+    switch (kind) {
+    case 0:
+        S->kind = LDO_MOVE_KIND_ORIENT;
+        return orientation_move();
+    case 1:
+        S->kind = LDO_MOVE_KIND_MET;
+        if (uniform_real() < 0.5) {
+            return insert_staple();
+        }
+        return delete_staple();
+    case 2:
+        S->kind = LDO_MOVE_KIND_CB;
+        return cb_staple_move();
+    case 3:
+        S->kind = LDO_MOVE_KIND_CTCB;
+        return ctcb_scaffold_move(P->move_maxreg[m]);
+    }
+    fail(ST_MISUSE);
+    return false;
+}
+
+// GCMCSimulation::simulate (simulation.cpp:568-653) and select_movetype
+// (:655-665); output writers are host-side and not part of the step
+DEVNI void Eng::simulate(int64_t steps) {
+    for (int64_t n = 0; n < steps; n++) {
+        if (S->status != ST_OK) {
+            break;
+        }
+        S->step += 1;
+        S->rng_idx = 0;
+        double prob = uniform_real();
+        int m = 0;
+        while (m < P->n_moves && !(prob < P->move_cum[m])) {
+            m++;
+        }
+        if (m == P->n_moves) {
+            m = P->n_moves - 1;
+        }
+        S->attempts[m]++;
+        bool acc = attempt(m);
+        S->accepts[m] += acc ? 1 : 0;
+        if (!acc) {
+            reset_origami();
+        }
+        if (P->centering_freq != 0 && S->step % P->centering_freq == 0) {
+            center();
+        }
+        if (P->check_freq != 0 && S->step % P->check_freq == 0) {
+            check_all_constraints();
+        }
+    }
+}
+
+// TPT/UTPTGCMCSimulation::update_control_qs (ptmc_simulation.cpp:651-663):
+// switch to the tables of ladder slot `slot` and recompute the energy.  A
+// slot met for the first time rebuilds its tables in the reference, which
+// also refreshes the initiation terms (origami_potential.cpp:1019-1042,
+// 1057-1063); on later visits the initiation terms keep their last values.
+DEVNI void Eng::begin_interval(int slot_) {
+    S->temp_slot = slot_;
+    uint64_t bit = 1ull << slot_;
+    if (!(S->visited & bit)) {
+        S->visited |= bit;
+        S->init_slot = slot_;
+    }
+    select_tables();
+    update_energy();
+}
+
+#endif

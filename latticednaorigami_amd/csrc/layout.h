@@ -1,0 +1,129 @@
+// layout.h - device data layout of the sampler (host + device).
+//
+// Every replica owns one fixed-size byte block ("replica block") in HBM that
+// holds its whole lattice state and the scratch of the move in flight.  The
+// block is built from structure-of-arrays sections at offsets computed once
+// on the host (Layout) so the same code addresses it in HBM (one lane per
+// replica) or after staging it into LDS (one wavefront per replica).
+//
+// Domains live in fixed slots: slots [0, ns) are the scaffold, staple chain
+// slot k (1 <= k <= max_total_staples) owns slots ns + (k-1)*lmax .. + lmax.
+// A lattice site is one entry of an open-addressed, linear-probing hash
+// keyed by the packed position; its value holds the occupancy state and, for
+// an unbound site, the slot of the domain on it.
+#ifndef LDO_LAYOUT_H
+#define LDO_LAYOUT_H
+
+#include <stdint.h>
+
+#define LDO_NONE ((int16_t)-1)
+#define LDO_HASH_EMPTY 0xFFFFFFFFFFFFFFFFull
+#define LDO_COORD_BITS 21
+#define LDO_COORD_OFFSET (1 << 20)
+
+enum { OCC_UNASSIGNED = 0, OCC_UNBOUND = 1, OCC_BOUND = 2, OCC_MISBOUND = 3 };
+
+// Domain record, 16 bytes.
+struct DomRec {
+    int32_t x, y, z;
+    int8_t ore;   // index into the six unit vectors, (+x,-x,+y,-y,+z,-z)
+    int8_t state; // OCC_*
+    int16_t bound; // slot of the bound partner or LDO_NONE
+};
+
+// Saved position/orientation (prev/old configuration), 16 bytes.
+struct PosOre {
+    int32_t x, y, z;
+    int32_t ore;
+};
+
+struct ChainRec {
+    int32_t uid;   // unique chain index (the reference's m_c)
+    int16_t ident; // chain identity, -1 when the slot is free
+    int16_t len;
+};
+
+// Active endpoint of the constraint-point machinery, 16 bytes.
+struct Endpoint {
+    int16_t cslot; // chain slot of the endpoint domain's chain
+    int8_t seg;
+    int8_t d;      // domain index within the chain
+    int32_t x, y, z;
+};
+
+struct NetFrame {
+    int16_t growth; // growth domain slot of this recursion level
+    int16_t idx;    // position in that staple's domain stack
+};
+
+#define LDO_MAX_MOVES 8
+
+// Per-replica scalars at the head of a replica block (kept 8-byte aligned).
+struct RepScalars {
+    int64_t step;
+    double energy;
+    double hyb_h;
+    double hyb_s;
+    double stack_e;
+    double modifier;
+    double delta_e;
+    double bias;
+    double new_bias;
+    double new_modifier;
+    int64_t tape_pos;
+    uint64_t visited; // ladder slots whose tables this replica has built
+    int64_t attempts[LDO_MAX_MOVES];
+    int64_t accepts[LDO_MAX_MOVES];
+    int64_t site_empty;
+    int64_t site_bind;
+    int64_t commits;
+    int32_t num_domains;
+    int32_t num_staples;
+    int32_t num_bound;
+    int32_t num_fully;
+    int32_t num_self;
+    int32_t num_unassigned;
+    int32_t num_stacked;
+    int32_t current_c_i;
+    int32_t n_chains; // working order length
+    int32_t status;
+    int32_t violated;
+    int32_t temp_slot; // ladder slot whose tables are in use
+    int32_t init_slot; // ladder slot the initiation terms come from
+    int32_t n_free;
+    int32_t n_modified;
+    int32_t n_assigned;
+    int32_t n_added;
+    int32_t rejected;
+    int32_t regrow_old;
+    int32_t insertion_sites;
+    int32_t kind;
+    int32_t n_active;
+    int32_t n_active0;
+    int32_t rng_idx;
+    int32_t added[4];
+    int32_t pad[2];
+};
+
+// Byte offsets of the sections of a replica block, and sizes.
+struct Layout {
+    int32_t ns;        // scaffold domains
+    int32_t lmax;      // longest staple
+    int32_t smax;      // max_total_staples
+    int32_t nslots;    // ns + smax*lmax
+    int32_t nchain;    // 1 + smax chain slots
+    int32_t ntypes;    // chain identities (scaffold + staple types)
+    int32_t tcap;      // per-type list capacity
+    int32_t hcap;      // hash capacity (power of two)
+    int32_t hbits;
+    int32_t ecap;      // endpoint list capacity
+    int64_t off_dom, off_dident, off_chain, off_order, off_free, off_tlist, off_tcount;
+    int64_t off_hkey, off_hval, off_modified, off_assigned, off_prev, off_old;
+    int64_t off_seg, off_gp, off_stem, off_inact, off_netseg, off_netds_flag;
+    int64_t off_checked, off_netcs, off_regrow, off_dir0, off_dir1;
+    int64_t off_active, off_active0, off_gps, off_iaes, off_netds, off_frames;
+    int64_t off_part, off_sframes;
+    int64_t bytes; // block size (multiple of 256)
+};
+
+#endif

@@ -1,0 +1,89 @@
+"""Python driver over the C ABI of libldo_hip.so.
+
+`Sampler` mirrors the reference's simulation drivers for the hot path: a
+constant-T run is `sweep(steps)` (ConstantTGCMCSimulation::run,
+constant_temp_simulation.hpp:31) and a temperature-exchange run is
+`run(n_swaps, interval)` (PTGCMCSimulation::run, ptmc_simulation.cpp:106-150).
+All replicas live on the GPU; every call is synchronous.
+"""
+import ctypes as C
+
+from . import abi
+from .systems import config_from_abi, config_to_abi, empty_config
+
+
+class Sampler:
+    def __init__(self, system, temps=None, rng_mode=abi.RNG_PHILOX,
+                 exchange_mode=abi.EXCHANGE_NONE, lanes_per_replica=0, lib=None):
+        self.lib = lib if lib is not None else abi.load_library()
+        self.system = system
+        temps = list(temps if temps is not None else (system.temps or [300.0]))
+        self.n = len(temps)
+        sd, rp, cfg, self._keep = system.to_abi(rng_mode, exchange_mode, lanes_per_replica)
+        t = (C.c_double * self.n)(*temps)
+        h = C.c_void_p()
+        rc = self.lib.ldo_create(C.byref(sd), C.byref(rp), C.byref(cfg), self.n, t, C.byref(h))
+        if rc != 0:
+            raise abi.LdoError(rc, self.lib.ldo_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise abi.LdoError(rc, self.lib.ldo_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ldo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sweep(self, steps):
+        self._check(self.lib.ldo_sweep(self.h, steps))
+
+    def run(self, n_swaps, interval):
+        self._check(self.lib.ldo_run(self.h, n_swaps, interval))
+
+    def set_ladder(self, temps, u=None, b=None, s=None):
+        n = len(temps)
+        one = [1.0] * n
+        arrs = [(C.c_double * n)(*v) for v in (temps, u or one, b or one, s or one)]
+        self._check(self.lib.ldo_set_ladder(self.h, n, *arrs))
+
+    def set_tape(self, r, values):
+        buf = (C.c_uint64 * max(1, len(values)))(*values)
+        self._check(self.lib.ldo_set_tape(self.h, r, buf, len(values)))
+
+    def observables(self, r=0):
+        o = abi.Obs()
+        self._check(self.lib.ldo_get_observables(self.h, r, C.byref(o)))
+        return o.as_dict()
+
+    def counters(self):
+        c = abi.Counters()
+        self._check(self.lib.ldo_get_counters(self.h, C.byref(c)))
+        return c
+
+    def permutation(self):
+        p = (C.c_int32 * self.n)()
+        self._check(self.lib.ldo_get_permutation(self.h, p))
+        return list(p)
+
+    def config(self, r=0):
+        cap_c = self.system.max_total_staples + 1
+        cap_d = self.system.max_domains()
+        cfg, keep = empty_config(cap_c, cap_d)
+        self._check(self.lib.ldo_get_config(self.h, r, C.byref(cfg), cap_c, cap_d))
+        return config_from_abi(cfg)
+
+    def set_config(self, r, chains):
+        keep = []
+        cfg = config_to_abi(chains, keep)
+        self._check(self.lib.ldo_set_config(self.h, r, C.byref(cfg)))
+
+    def stream(self):
+        return self.lib.ldo_get_stream(self.h)

@@ -1,0 +1,346 @@
+"""System descriptions: the reference's input files and the synthetic
+BASELINE workloads, lowered to the C ABI structs of include/ldo.h.
+
+A `System` holds what setup_origami() reads from the system JSON and the
+.inp file (origami_system.cpp:991-1030, files.cpp:36-104, parser.cpp:27-449)
+plus the moveset (files.cpp:264-280).  The generators build the synthetic
+systems the benchmark configurations are defined on; their seeds are part of
+the workload definition.
+"""
+import ctypes as C
+import json
+import math
+from dataclasses import dataclass, field
+from fractions import Fraction
+
+import numpy as np
+
+from . import abi
+
+# kcal/mol -> K (for H) and kcal/mol/K -> k_B (for S): 4184 / R
+KCAL_PER_MOL_TO_K = 4184.0 / 8.3144598
+
+UNIT_VECTORS = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+
+
+@dataclass
+class Move:
+    kind: int
+    freq: float
+    max_regrowth: int = 0
+
+
+@dataclass
+class Chain:
+    index: int
+    identity: int
+    positions: list
+    orientations: list
+
+
+@dataclass
+class System:
+    identities: list
+    sequences: list = None
+    cyclic: bool = False
+    domain_type: int = abi.HALFTURN
+    hybridization_pot: int = abi.HYB_NEARESTNEIGHBOUR
+    misbinding_pot: int = abi.MISBIND_OPPOSING
+    stacking_pot: int = abi.STACK_CONSTANT
+    binding_h: float = 0.0
+    binding_s: float = 0.0
+    misbinding_h: float = 0.0
+    misbinding_s: float = 0.0
+    stacking_ene: float = 1.0
+    cation_M: float = 1.0
+    staple_M: float = 1.0
+    enthalpies: list = None
+    entropies: list = None
+    chains: list = field(default_factory=list)
+    moves: list = field(default_factory=list)
+    max_total_staples: int = 999
+    max_type_staples: int = 999
+    max_staple_size: int = 2
+    centering_freq: int = 0
+    centering_domain: int = 0
+    constraint_check_freq: int = 0
+    seed: int = 0
+    name: str = ""
+    temps: list = None
+
+    # ------------------------------------------------------------- sizes
+    @property
+    def n_scaffold(self):
+        return len(self.identities[0])
+
+    def max_domains(self):
+        """Largest number of domains a replica can hold."""
+        longest = max((len(c) for c in self.identities[1:]), default=0)
+        return self.n_scaffold + self.max_total_staples * longest
+
+    # ------------------------------------------------------------- lowering
+    def to_abi(self, rng_mode=abi.RNG_PHILOX, exchange_mode=abi.EXCHANGE_NONE,
+               lanes_per_replica=0):
+        """Return (SystemDesc, RunParams, Config, keepalive)."""
+        keep = []
+
+        def arr(ctype, values):
+            a = (ctype * max(1, len(values)))(*values)
+            keep.append(a)
+            return a
+
+        lengths = [len(c) for c in self.identities]
+        flat = [v for c in self.identities for v in c]
+        sd = abi.SystemDesc()
+        sd.n_chain_types = len(self.identities)
+        sd.chain_lengths = arr(C.c_int32, lengths)
+        sd.identities = arr(C.c_int32, flat)
+        if self.sequences is not None:
+            seqs = [s.encode() for c in self.sequences for s in c]
+            sd.sequences = arr(C.c_char_p, seqs)
+        if self.enthalpies is not None:
+            sd.enthalpies = arr(C.c_double, self.enthalpies)
+            sd.entropies = arr(C.c_double, self.entropies)
+            sd.n_specified = len(self.enthalpies)
+        sd.cyclic = int(self.cyclic)
+        sd.domain_type = self.domain_type
+        sd.hybridization_pot = self.hybridization_pot
+        sd.misbinding_pot = self.misbinding_pot
+        sd.stacking_pot = self.stacking_pot
+        sd.binding_h = self.binding_h
+        sd.binding_s = self.binding_s
+        sd.misbinding_h = self.misbinding_h
+        sd.misbinding_s = self.misbinding_s
+        sd.stacking_ene = self.stacking_ene
+        sd.cation_M = self.cation_M
+        sd.staple_M = self.staple_M
+
+        rp = abi.RunParams()
+        rp.n_movetypes = len(self.moves)
+        rp.movetype_kind = arr(C.c_int32, [m.kind for m in self.moves])
+        rp.movetype_freq = arr(C.c_double, [m.freq for m in self.moves])
+        rp.movetype_max_regrowth = arr(C.c_int32, [m.max_regrowth for m in self.moves])
+        rp.max_total_staples = self.max_total_staples
+        rp.max_type_staples = self.max_type_staples
+        rp.max_staple_size = self.max_staple_size
+        rp.centering_freq = self.centering_freq
+        rp.centering_domain = self.centering_domain
+        rp.constraint_check_freq = self.constraint_check_freq
+        rp.seed = self.seed
+        rp.rng_mode = rng_mode
+        rp.exchange_mode = exchange_mode
+        rp.lanes_per_replica = lanes_per_replica
+
+        cfg = config_to_abi(self.chains, keep)
+        return sd, rp, cfg, keep
+
+
+def config_to_abi(chains, keep):
+    def arr(ctype, values):
+        a = (ctype * max(1, len(values)))(*values)
+        keep.append(a)
+        return a
+
+    cfg = abi.Config()
+    cfg.n_chains = len(chains)
+    cfg.chain_index = arr(C.c_int32, [c.index for c in chains])
+    cfg.chain_identity = arr(C.c_int32, [c.identity for c in chains])
+    cfg.chain_length = arr(C.c_int32, [len(c.positions) for c in chains])
+    cfg.positions = arr(C.c_int32, [v for c in chains for p in c.positions for v in p])
+    cfg.orientations = arr(C.c_int32, [v for c in chains for p in c.orientations for v in p])
+    return cfg
+
+
+def empty_config(cap_chains, cap_domains):
+    keep = []
+    cfg = abi.Config()
+    for name, n in (("chain_index", cap_chains), ("chain_identity", cap_chains),
+                    ("chain_length", cap_chains), ("positions", 3 * cap_domains),
+                    ("orientations", 3 * cap_domains)):
+        a = (C.c_int32 * n)()
+        keep.append(a)
+        setattr(cfg, name, a)
+    return cfg, keep
+
+
+def config_from_abi(cfg):
+    chains = []
+    k = 0
+    for c in range(cfg.n_chains):
+        n = cfg.chain_length[c]
+        pos = [tuple(cfg.positions[3 * (k + d): 3 * (k + d) + 3]) for d in range(n)]
+        ore = [tuple(cfg.orientations[3 * (k + d): 3 * (k + d) + 3]) for d in range(n)]
+        chains.append(Chain(cfg.chain_index[c], cfg.chain_identity[c], pos, ore))
+        k += n
+    return chains
+
+
+# ----------------------------------------------------------- reference files
+def parse_fraction(text):
+    """utility::Fraction (utility.cpp:266-282): "a/b" or a real number."""
+    text = str(text)
+    if "/" in text:
+        num, den = text.split("/", 1)
+        return float(num) / float(den)
+    return float(text)
+
+
+def read_system_json(path_or_dict):
+    """OrigamiInputFile::read_file (files.cpp:36-104)."""
+    if isinstance(path_or_dict, dict):
+        root = path_or_dict
+    else:
+        with open(path_or_dict) as f:
+            root = json.load(f)
+    o = root["origami"]
+    chains = []
+    for ch in o["configurations"][0]["chains"]:
+        chains.append(Chain(int(ch["index"]), int(ch["identity"]),
+                            [tuple(p) for p in ch["positions"]],
+                            [tuple(p) for p in ch["orientations"]]))
+    return dict(identities=o["identities"], sequences=o.get("sequences"),
+                enthalpies=o.get("enthalpies"), entropies=o.get("entropies"),
+                cyclic=bool(o.get("cyclic", False)), chains=chains)
+
+
+def read_moveset_json(path_or_dict):
+    """OrigamiMovetypeFile (files.cpp:264-314)."""
+    if isinstance(path_or_dict, dict):
+        root = path_or_dict
+    else:
+        with open(path_or_dict) as f:
+            root = json.load(f)
+    moves = []
+    for m in root["origami"]["movetypes"]:
+        if m["type"] not in abi.MOVETYPE_NAMES:
+            raise NotImplementedError(f"{m['type']}: movetype not supported")
+        moves.append(Move(abi.MOVETYPE_NAMES[m["type"]], parse_fraction(m["freq"]),
+                          int(m.get("max_regrowth", 0))))
+    return moves
+
+
+# ------------------------------------------------------- synthetic workloads
+def standard_moveset(max_regrowth):
+    """Orientation 1/10, staple exchange 4/10, CB staple 3/10, CTCB 2/10."""
+    return [Move(abi.MOVE_ORIENTATION_ROTATION, 1 / 10),
+            Move(abi.MOVE_MET_STAPLE_EXCHANGE, 4 / 10),
+            Move(abi.MOVE_CB_STAPLE_REGROWTH, 3 / 10),
+            Move(abi.MOVE_CTCB_SCAFFOLD_REGROWTH, 2 / 10, max_regrowth)]
+
+
+def four_domain_system(seed=20261202, max_total_staples=4):
+    """The enumerable plain-method workload: a 4-domain HalfTurn scaffold,
+    staple types [1,2] and [3,4], Uniform hybridization with
+    dH = -120 kcal/mol and dS = -0.33 kcal/mol/K, Opposing misbinding,
+    constant stacking, 1 uM staples at 330 K."""
+    return System(
+        identities=[[-1, -2, -3, -4], [1, 2], [3, 4]],
+        cyclic=False,
+        domain_type=abi.HALFTURN,
+        hybridization_pot=abi.HYB_UNIFORM,
+        misbinding_pot=abi.MISBIND_OPPOSING,
+        binding_h=-120.0 * KCAL_PER_MOL_TO_K,
+        binding_s=-0.33 * KCAL_PER_MOL_TO_K,
+        stacking_ene=-1000.0,
+        cation_M=1.0,
+        staple_M=1e-6,
+        chains=[Chain(0, 0, [(i, 0, 0) for i in range(4)], [(0, 0, 1)] * 4)],
+        moves=standard_moveset(4),
+        max_total_staples=max_total_staples,
+        max_type_staples=4,
+        max_staple_size=2,
+        centering_freq=10000,
+        constraint_check_freq=10000,
+        seed=seed,
+        name="four-domain HalfTurn scaffold, 2 two-domain staple types, Uniform, 330 K",
+        temps=[330.0],
+    )
+
+
+def raster_position(i, width):
+    row, col = divmod(i, width)
+    if row % 2 == 1:
+        col = width - 1 - col
+    return (col, row, 0)
+
+
+def domino_tiling(rows, cols, flips, seed):
+    """Seeded random domino tiling of a rows x cols grid: start from vertical
+    dominoes and apply `flips` random 2x2 rotations."""
+    partner = {}
+    for r in range(0, rows - 1, 2):
+        for c in range(cols):
+            partner[(r, c)] = (r + 1, c)
+            partner[(r + 1, c)] = (r, c)
+    rng = np.random.default_rng(seed)
+    picks = rng.integers(0, [rows - 1, cols - 1], size=(flips, 2))
+    for r, c in picks:
+        a, b, d, e = (r, c), (r, c + 1), (r + 1, c), (r + 1, c + 1)
+        if partner.get(a) == d and partner.get(b) == e:
+            partner[a], partner[b] = b, a
+            partner[d], partner[e] = e, d
+        elif partner.get(a) == b and partner.get(d) == e:
+            partner[a], partner[d] = d, a
+            partner[b], partner[e] = e, b
+    pairs = sorted({tuple(sorted((k, v))) for k, v in partner.items()})
+    return pairs
+
+
+def _revcomp(s):
+    return s[::-1].translate(str.maketrans("ACGT", "TGCA"))
+
+
+def random_sequences(n, length, seed):
+    rng = np.random.default_rng(seed)
+    while True:
+        seqs = ["".join(rng.choice(list("ACGT"), size=length)) for _ in range(n)]
+        rcs = {_revcomp(s) for s in seqs}
+        if len(set(seqs)) == n and not (rcs & set(seqs)):
+            return seqs
+
+
+def raster_system(n_domains=64, width=8, replicas=32, t_lo=300.0, t_hi=360.0,
+                  seed=20261202, tiling_seed=1, sequence_seed=2):
+    """The PTMC workload: a 64-domain HalfTurn scaffold on an 8x8
+    boustrophedon raster in z=0, 32 two-domain staple types from a seeded
+    domino tiling, random 16-nt sequences, NearestNeighbour energies, 0.5 M
+    cation, 1 uM staples, 32 replicas at 300-360 K."""
+    rows = n_domains // width
+    index_of = {}
+    for i in range(n_domains):
+        col, row, _ = raster_position(i, width)
+        index_of[(row, col)] = i
+    pairs = domino_tiling(rows, width, 10000, tiling_seed)
+    staples = []
+    for a, b in pairs:
+        i, j = sorted((index_of[a], index_of[b]))
+        staples.append([i + 1, j + 1])
+    staples.sort()
+    seqs = random_sequences(n_domains, 16, sequence_seed)
+    seq_of = {-(i + 1): s for i, s in enumerate(seqs)}
+    identities = [[-(i + 1) for i in range(n_domains)]] + staples
+    sequences = [seqs] + [[_revcomp(seq_of[-v]) for v in st] for st in staples]
+    temps = [t_lo + (t_hi - t_lo) * k / max(1, replicas - 1) for k in range(replicas)]
+    sysd = System(
+        identities=identities,
+        sequences=sequences,
+        cyclic=False,
+        domain_type=abi.HALFTURN,
+        hybridization_pot=abi.HYB_NEARESTNEIGHBOUR,
+        misbinding_pot=abi.MISBIND_OPPOSING,
+        stacking_ene=-1000.0,
+        cation_M=0.5,
+        staple_M=1e-6,
+        chains=[Chain(0, 0, [raster_position(i, width) for i in range(n_domains)],
+                      [(0, 0, 1)] * n_domains)],
+        moves=standard_moveset(12),
+        max_total_staples=n_domains,
+        max_type_staples=2,
+        max_staple_size=2,
+        centering_freq=0,
+        constraint_check_freq=0,
+        seed=seed,
+        name=f"{n_domains}-domain HalfTurn raster scaffold, {len(staples)} staple types, NN",
+        temps=temps,
+    )
+    return sysd

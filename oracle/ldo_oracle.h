@@ -1,0 +1,90 @@
+/*
+ * ldo_oracle.h - C API of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a single-threaded FP64
+ * restatement of the LatticeDNAOrigami hot path (lattice state, FourBody
+ * energy, the four BASELINE move types, the simulate loop, PTMC exchange and
+ * the exact enumerator).  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker or the CPU baseline;
+ * the product library (libldo_hip.so) never links or calls it.
+ *
+ * It shares the plain data structs of include/ldo.h so that one Python
+ * description of a system drives both the oracle and the HIP sampler.
+ */
+#ifndef LDO_ORACLE_H
+#define LDO_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/ldo.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* rng_kind: 0 = Philox stream identical to the HIP sampler's LDO_RNG_PHILOX;
+ *           1 = std::mt19937_64 with libstdc++ distributions, i.e. the
+ *               reference's RandomGens (random_gens.cpp:12-49). */
+void* oracle_create(const ldo_system_desc* sys, const ldo_run_params* run,
+                    const ldo_config* init, int32_t n_replicas,
+                    const double* temps, int32_t rng_kind);
+void oracle_destroy(void* o);
+const char* oracle_last_error(void* o);
+
+int oracle_set_ladder(void* o, int32_t n, const double* temps,
+                      const double* staple_u_mults, const double* bias_mults,
+                      const double* stacking_mults);
+int oracle_sweep(void* o, int64_t steps);
+int oracle_run(void* o, int64_t n_swaps, int64_t exchange_interval);
+int oracle_get_observables(void* o, int32_t replica, ldo_obs* out);
+int oracle_get_counters(void* o, ldo_counters* out);
+int oracle_get_permutation(void* o, int32_t* q_to_rep);
+int oracle_get_config(void* o, int32_t replica, ldo_config* cfg,
+                      int32_t cap_chains, int32_t cap_domains);
+int oracle_set_config(void* o, int32_t replica, const ldo_config* cfg);
+
+/* Tape recording (mt19937_64 mode): every draw of replica r is appended to
+ * its tape; the tape is what ldo_set_tape replays on the GPU. */
+int oracle_record_tape(void* o, int32_t replica, int32_t on);
+int64_t oracle_tape_length(void* o, int32_t replica);
+int oracle_get_tape(void* o, int32_t replica, uint64_t* out, int64_t cap);
+
+/* Energy tables at a temperature (kT units; origami_potential.cpp:1057-1221) */
+int oracle_energy_tables(void* o, double temp, double stacking_mult,
+                         int32_t ident_i, int32_t ident_j, double* out5);
+
+/* Recompute the energy of replica r's configuration from scratch by the
+ * reference's unassign-all / set-all sequence (origami_system.cpp:808-826) */
+int oracle_recompute_energy(void* o, int32_t replica, double* out_energy);
+
+/* Nearest-neighbour thermodynamics (nearest_neighbour.cpp:34-177) */
+int oracle_nn_hybridization_thermo(const char* seq, double temp, double cation_M,
+                                   double* out_h, double* out_s);
+double oracle_nn_reduced_energy(const char* seq, double temp, double cation_M);
+/* writes the longest contiguous complements, separated by ' ', into out */
+int oracle_nn_longest_contig_complement(const char* seq_i, const char* seq_j,
+                                        char* out, int32_t cap);
+
+/* Ideal random walk count (ideal_random_walk.cpp:14-73), long double sum */
+double oracle_num_walks(int32_t dx, int32_t dy, int32_t dz, int32_t steps);
+
+/* VectorThree rotations (utility.cpp:68-142): v, axis as 3-int arrays */
+int oracle_rotate_half(const int32_t* v, const int32_t* axis, int32_t* out);
+int oracle_rotate(const int32_t* v, const int32_t* axis, int32_t turns,
+                  int32_t* out);
+
+/* Philox4x32-10: ctr[4], key[2] -> out[4] */
+void oracle_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
+
+/* Exact enumeration (enumerate.cpp:19-81, 639-664) for small systems at
+ * temps[0]: fills probabilities of the joint state
+ * (numstaples, numfulldomains, nummisdomains, numstackedpairs) into a dense
+ * array out[ns][nf][nm][nst] with the given extents. */
+int oracle_enumerate(void* o, int32_t max_ns, int32_t max_nf, int32_t max_nm,
+                     int32_t max_nst, double* out, double* out_log_z);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LDO_ORACLE_H */

@@ -1,0 +1,407 @@
+// oracle_api.cpp - extern "C" entry points of the CPU oracle.
+//
+// TEST INFRASTRUCTURE ONLY (see ldo_oracle.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ldo_oracle.h"
+#include "nn_thermo.hpp"
+#include "oracle_core.hpp"
+
+namespace orc {
+
+struct Oracle {
+    ldo_system_desc sys {};
+    std::vector<std::vector<int>> idents;
+    std::vector<std::vector<std::string>> seqs;
+    std::vector<std::unique_ptr<Replica>> reps;
+    std::vector<double> ladder_t, ladder_u, ladder_b, ladder_s;
+    std::vector<int> q_to_rep;
+    int rng_kind {0};
+    uint64_t seed {0};
+    Rng exchange_rng;
+    int64_t swap_counter {0};
+    ldo_counters counters {};
+    std::string err;
+};
+
+static std::vector<ChainSpec> specs_from(const ldo_config* cfg,
+                                         const std::vector<std::vector<int>>& idents) {
+    std::vector<ChainSpec> out;
+    int k = 0;
+    for (int c = 0; c < cfg->n_chains; c++) {
+        ChainSpec s;
+        s.index = cfg->chain_index[c];
+        s.identity = cfg->chain_identity[c];
+        int len = (int)idents.at(s.identity).size();
+        for (int d = 0; d < len; d++, k++) {
+            s.pos.push_back({cfg->positions[3 * k], cfg->positions[3 * k + 1],
+                             cfg->positions[3 * k + 2]});
+            s.ore.push_back({cfg->orientations[3 * k], cfg->orientations[3 * k + 1],
+                             cfg->orientations[3 * k + 2]});
+        }
+        out.push_back(s);
+    }
+    return out;
+}
+
+} // namespace orc
+
+using namespace orc;
+
+static std::string g_create_error;
+
+extern "C" {
+
+void* oracle_create(const ldo_system_desc* sd, const ldo_run_params* run,
+                    const ldo_config* init, int32_t n_replicas, const double* temps,
+                    int32_t rng_kind) {
+    try {
+        auto o = new Oracle;
+        o->sys = *sd;
+        int k = 0;
+        for (int c = 0; c < sd->n_chain_types; c++) {
+            std::vector<int> ids;
+            std::vector<std::string> sq;
+            for (int d = 0; d < sd->chain_lengths[c]; d++, k++) {
+                ids.push_back(sd->identities[k]);
+                sq.push_back(sd->sequences ? std::string(sd->sequences[k]) : std::string());
+            }
+            o->idents.push_back(ids);
+            o->seqs.push_back(sq);
+        }
+        std::vector<MovesetEntry> moves;
+        for (int m = 0; m < run->n_movetypes; m++) {
+            moves.push_back({run->movetype_kind[m], run->movetype_freq[m],
+                             run->movetype_max_regrowth ? run->movetype_max_regrowth[m] : 0});
+        }
+        Limits lim {run->max_total_staples, run->max_type_staples};
+        std::vector<ChainSpec> cfg = specs_from(init, o->idents);
+        o->rng_kind = rng_kind;
+        o->seed = run->seed;
+        for (int r = 0; r < n_replicas; r++) {
+            o->reps.emplace_back(new Replica(*sd, o->idents, o->seqs, cfg, temps[r], moves, lim,
+                                             run->centering_freq, run->centering_domain,
+                                             run->constraint_check_freq, rng_kind, run->seed,
+                                             (uint32_t)r));
+            o->ladder_t.push_back(temps[r]);
+            o->ladder_u.push_back(1.0);
+            o->ladder_b.push_back(1.0);
+            o->ladder_s.push_back(1.0);
+            o->q_to_rep.push_back(r);
+        }
+        o->exchange_rng.init(0, run->seed, 0xFFFFFFFFu);
+        return o;
+    } catch (std::exception& e) {
+        g_create_error = e.what();
+        return nullptr;
+    }
+}
+
+void oracle_destroy(void* o) { delete static_cast<Oracle*>(o); }
+
+const char* oracle_last_error(void* o) {
+    if (o == nullptr) {
+        return g_create_error.c_str();
+    }
+    return static_cast<Oracle*>(o)->err.c_str();
+}
+
+#define ORACLE_TRY(o, ...)                                                                         \
+    Oracle* O = static_cast<Oracle*>(o);                                                           \
+    try {                                                                                          \
+        __VA_ARGS__;                                                                               \
+    } catch (OrigamiError & e) {                                                                   \
+        O->err = e.what();                                                                         \
+        return e.code;                                                                             \
+    } catch (std::exception & e) {                                                                 \
+        O->err = e.what();                                                                         \
+        return LDO_E_MISUSE;                                                                       \
+    }                                                                                              \
+    return LDO_OK;
+
+int oracle_set_ladder(void* o, int32_t n, const double* t, const double* u, const double* b,
+                      const double* s) {
+    ORACLE_TRY(o, {
+        if (n != (int)O->reps.size()) {
+            throw OrigamiError(LDO_E_ARG, "ladder size must equal the replica count");
+        }
+        O->ladder_t.assign(t, t + n);
+        O->ladder_u.assign(u, u + n);
+        O->ladder_b.assign(b, b + n);
+        O->ladder_s.assign(s, s + n);
+    })
+}
+
+int oracle_sweep(void* o, int64_t steps) {
+    ORACLE_TRY(o, {
+        for (auto& r: O->reps) {
+            r->simulate(steps);
+        }
+    })
+}
+
+// PTGCMCSimulation::run / OneDPTGCMCSimulation::attempt_exchange for a
+// ut_parallel_tempering run (ptmc_simulation.cpp:106-161, 255-313, 360-412,
+// 658-663), with every rank's replica held in this process.
+int oracle_run(void* o, int64_t n_swaps, int64_t interval) {
+    ORACLE_TRY(o, {
+        int R = (int)O->reps.size();
+        std::vector<int> slot_of(R);
+        for (int64_t s = 0; s < n_swaps; s++) {
+            int64_t swap_i = ++O->swap_counter;
+            for (int q = 0; q < R; q++) {
+                slot_of[O->q_to_rep[q]] = q;
+            }
+            std::vector<double> dh(R), dst(R);
+            for (int r = 0; r < R; r++) {
+                Replica& rep = *O->reps[r];
+                int q = slot_of[r];
+                double temp = O->ladder_t[q];
+                rep.sys.update_temp(temp, 1.0);
+                rep.sys.update_staple_us(temp, O->ladder_u[q]);
+                rep.simulate(interval);
+                rep.sys.update_enthalpy_and_entropy();
+                dh[r] = rep.sys.hyb_enthalpy;
+                dst[r] = rep.sys.stacking_energy;
+            }
+            Rng& xr = (O->rng_kind == 1) ? O->reps[0]->rng : O->exchange_rng;
+            if (O->rng_kind != 1) {
+                xr.begin_step(swap_i);
+            }
+            int swap_set = (int)(swap_i % 2);
+            for (int i = swap_set; i < R - 1; i += 2) {
+                O->counters.exchange_attempts++;
+                int r1 = O->q_to_rep[i];
+                int r2 = O->q_to_rep[i + 1];
+                double t1 = O->ladder_t[i], t2 = O->ladder_t[i + 1];
+                double sm1 = O->ladder_s[i], sm2 = O->ladder_s[i + 1];
+                std::vector<int> n1 = O->reps[r1]->sys.staple_counts();
+                std::vector<int> n2 = O->reps[r2]->sys.staple_counts();
+                double dbu_dn = 0;
+                for (size_t k = 0; k < n1.size(); k++) {
+                    double u1 = O->reps[r1]->sys.staple_us[k];
+                    double u2 = O->reps[r2]->sys.staple_us[k];
+                    dbu_dn += (u2 / t2 - u1 / t1) * ((double)n2[k] - (double)n1[k]);
+                }
+                double DB = 1 / t2 - 1 / t1;
+                double DH = dh[r2] * t2 - dh[r1] * t1;
+                double Dst = dst[r2] * t2 - dst[r1] * t1;
+                double DBM = sm2 / t2 - sm1 / t1;
+                double DBias = 0.0 * t2 - 0.0 * t1;
+                double p = std::min(1.0, std::exp(DB * (DH + DBias) + DBM * Dst - dbu_dn));
+                bool accept = (p == 1) ? true : (p > xr.uniform_real());
+                if (accept) {
+                    O->counters.exchange_accepts++;
+                    O->q_to_rep[i] = r2;
+                    O->q_to_rep[i + 1] = r1;
+                }
+            }
+        }
+    })
+}
+
+int oracle_get_observables(void* o, int32_t r, ldo_obs* out) {
+    ORACLE_TRY(o, {
+        Replica& rep = *O->reps.at(r);
+        rep.sys.update_enthalpy_and_entropy();
+        out->energy = rep.sys.energy;
+        out->enthalpy = rep.sys.hyb_enthalpy;
+        out->entropy = rep.sys.hyb_entropy;
+        out->stacking_energy = rep.sys.stacking_energy;
+        out->temperature = rep.sys.temp;
+        out->step = rep.step;
+        out->num_staples = rep.sys.num_staples;
+        out->num_domains = rep.sys.num_domains;
+        out->num_bound_domain_pairs = rep.sys.num_bound_pairs;
+        out->num_fully_bound_domain_pairs = rep.sys.num_fully_bound_pairs;
+        out->num_misbound_domain_pairs = rep.sys.num_bound_pairs - rep.sys.num_fully_bound_pairs;
+        out->num_stacked_domain_pairs = rep.sys.num_stacked;
+        out->num_unassigned_domains = rep.sys.num_unassigned;
+        out->status = rep.status;
+    })
+}
+
+int oracle_get_counters(void* o, ldo_counters* out) {
+    ORACLE_TRY(o, {
+        ldo_counters c = O->counters;
+        for (auto& r: O->reps) {
+            for (size_t m = 0; m < r->attempts.size() && m < LDO_MAX_MOVETYPES; m++) {
+                c.attempts[m] += r->attempts[m];
+                c.accepts[m] += r->accepts[m];
+            }
+        }
+        *out = c;
+    })
+}
+
+int oracle_get_permutation(void* o, int32_t* q_to_rep) {
+    ORACLE_TRY(o, {
+        for (size_t i = 0; i < O->q_to_rep.size(); i++) {
+            q_to_rep[i] = O->q_to_rep[i];
+        }
+    })
+}
+
+int oracle_get_config(void* o, int32_t r, ldo_config* cfg, int32_t cap_chains,
+                      int32_t cap_domains) {
+    ORACLE_TRY(o, {
+        auto snap = O->reps.at(r)->sys.snapshot();
+        if ((int)snap.size() > cap_chains) {
+            throw OrigamiError(LDO_E_ARG, "chain capacity too small");
+        }
+        int k = 0;
+        for (size_t c = 0; c < snap.size(); c++) {
+            cfg->chain_index[c] = snap[c].index;
+            cfg->chain_identity[c] = snap[c].identity;
+            cfg->chain_length[c] = (int)snap[c].pos.size();
+            for (size_t d = 0; d < snap[c].pos.size(); d++, k++) {
+                if (k >= cap_domains) {
+                    throw OrigamiError(LDO_E_ARG, "domain capacity too small");
+                }
+                cfg->positions[3 * k] = snap[c].pos[d].x;
+                cfg->positions[3 * k + 1] = snap[c].pos[d].y;
+                cfg->positions[3 * k + 2] = snap[c].pos[d].z;
+                cfg->orientations[3 * k] = snap[c].ore[d].x;
+                cfg->orientations[3 * k + 1] = snap[c].ore[d].y;
+                cfg->orientations[3 * k + 2] = snap[c].ore[d].z;
+            }
+        }
+        cfg->n_chains = (int)snap.size();
+    })
+}
+
+int oracle_set_config(void* o, int32_t r, const ldo_config* cfg) {
+    ORACLE_TRY(o, { O->reps.at(r)->sys.set_config(specs_from(cfg, O->idents)); })
+}
+
+int oracle_record_tape(void* o, int32_t r, int32_t on) {
+    ORACLE_TRY(o, { O->reps.at(r)->rng.set_record(on != 0); })
+}
+
+int64_t oracle_tape_length(void* o, int32_t r) {
+    return (int64_t)static_cast<Oracle*>(o)->reps.at(r)->rng.tape.size();
+}
+
+int oracle_get_tape(void* o, int32_t r, uint64_t* out, int64_t cap) {
+    ORACLE_TRY(o, {
+        auto& t = O->reps.at(r)->rng.tape;
+        if ((int64_t)t.size() > cap) {
+            throw OrigamiError(LDO_E_ARG, "tape capacity too small");
+        }
+        std::memcpy(out, t.data(), t.size() * sizeof(uint64_t));
+    })
+}
+
+int oracle_energy_tables(void* o, double temp, double stacking_mult, int32_t ii, int32_t ij,
+                         double* out5) {
+    ORACLE_TRY(o, {
+        Potential& p = O->reps.at(0)->sys.pot;
+        Tables t = p.build(temp, stacking_mult);
+        size_t k = (size_t)(ii - p.ident_lo) * p.n_ident + (ij - p.ident_lo);
+        out5[0] = t.hyb_e[k];
+        out5[1] = t.hyb_h[k];
+        out5[2] = t.hyb_s[k];
+        out5[3] = t.stack[k];
+        out5[4] = t.init_e;
+    })
+}
+
+int oracle_recompute_energy(void* o, int32_t r, double* out_energy) {
+    ORACLE_TRY(o, {
+        System& s = O->reps.at(r)->sys;
+        s.update_energy();
+        *out_energy = s.energy;
+    })
+}
+
+int oracle_nn_hybridization_thermo(const char* seq, double temp, double cation_M,
+                                   double* out_h, double* out_s) {
+    try {
+        orc_nn::HS hs = orc_nn::reduced_duplex(seq, temp, cation_M);
+        *out_h = hs.h;
+        *out_s = hs.s;
+    } catch (std::exception&) {
+        return LDO_E_ARG;
+    }
+    return LDO_OK;
+}
+
+double oracle_nn_reduced_energy(const char* seq, double temp, double cation_M) {
+    return orc_nn::reduced_duplex_energy(seq, temp, cation_M);
+}
+
+int oracle_nn_longest_contig_complement(const char* a, const char* b, char* out, int32_t cap) {
+    std::vector<std::string> v = orc_nn::longest_contig_complements(a, b);
+    std::string joined;
+    for (size_t i = 0; i < v.size(); i++) {
+        if (i) {
+            joined += ' ';
+        }
+        joined += v[i];
+    }
+    if ((int)joined.size() + 1 > cap) {
+        return LDO_E_ARG;
+    }
+    std::memcpy(out, joined.c_str(), joined.size() + 1);
+    return LDO_OK;
+}
+
+// IdealRandomWalks::num_walks (ideal_random_walk.cpp:14-73): the number of
+// `steps`-step walks on Z^3 between points a displacement (dx,dy,dz) apart,
+// summed as multinomials over the counts of negative x and y steps.
+double oracle_num_walks(int32_t dx, int32_t dy, int32_t dz, int32_t steps) {
+    int d[3] = {std::abs(dx), std::abs(dy), std::abs(dz)};
+    std::sort(d, d + 3);
+    int DX = d[0], DY = d[1], DZ = d[2];
+    int sum = DX + DY + DZ;
+    if (sum > steps || (steps - sum) % 2 != 0) {
+        return 0.0;
+    }
+    int n_minus = (steps - DX - DY - DZ) / 2;
+    int n_plus = (steps - DX - DY + DZ) / 2;
+    auto fact = [](int n) {
+        long double f = 1;
+        for (int i = 2; i <= n; i++) {
+            f *= i;
+        }
+        return f;
+    };
+    long double walks = 0;
+    for (int yb = 0; yb <= n_minus; yb++) {
+        for (int xb = 0; xb <= n_minus - yb; xb++) {
+            if (n_plus - xb - yb < 0) {
+                continue;
+            }
+            walks += fact(steps) / (fact(xb) * fact(xb + DX) * fact(yb) * fact(yb + DY) *
+                                    fact(n_minus - xb - yb) * fact(n_plus - xb - yb));
+        }
+    }
+    return (double)walks;
+}
+
+int oracle_rotate_half(const int32_t* v, const int32_t* axis, int32_t* out) {
+    V3 r = rotate_half({v[0], v[1], v[2]}, {axis[0], axis[1], axis[2]});
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    return LDO_OK;
+}
+
+int oracle_rotate(const int32_t* v, const int32_t* axis, int32_t turns, int32_t* out) {
+    V3 r = rotate_turns({v[0], v[1], v[2]}, {axis[0], axis[1], axis[2]}, turns);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    return LDO_OK;
+}
+
+void oracle_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    philox4x32_10(ctr, key, out);
+}
+
+} // extern "C"

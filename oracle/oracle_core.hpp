@@ -1,0 +1,409 @@
+// oracle_core.hpp - CPU restatement of the LatticeDNAOrigami hot path.
+//
+// TEST INFRASTRUCTURE ONLY (see ldo_oracle.h).  Each piece cites the
+// reference function it follows.  The structure deliberately mirrors the
+// reference (pointer-linked domains, hash maps keyed by lattice site,
+// std::set iteration orders) so that it reproduces the reference's
+// arithmetic and random-draw order, which the HIP sampler is then checked
+// against draw for draw.
+#ifndef ORACLE_CORE_HPP
+#define ORACLE_CORE_HPP
+
+#include <cstdint>
+#include <map>
+#include <random>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "../include/ldo.h"
+
+namespace orc {
+
+// ---------------------------------------------------------------- vectors
+// utility::VectorThree (utility.hpp:66-95, utility.cpp:26-161)
+struct V3 {
+    int x {0}, y {0}, z {0};
+    V3() = default;
+    V3(int a, int b, int c): x {a}, y {b}, z {c} {}
+    V3 operator+(const V3& o) const { return {x + o.x, y + o.y, z + o.z}; }
+    V3 operator-(const V3& o) const { return {x - o.x, y - o.y, z - o.z}; }
+    V3 operator-() const { return {-x, -y, -z}; }
+    bool operator==(const V3& o) const { return x == o.x && y == o.y && z == o.z; }
+    bool operator!=(const V3& o) const { return !(*this == o); }
+    int& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    int get(int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+
+struct V3Hash {
+    size_t operator()(const V3& v) const {
+        uint64_t h = (uint64_t)(uint32_t)v.x * 0x9E3779B97F4A7C15ull;
+        h ^= (uint64_t)(uint32_t)v.y * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
+        h ^= (uint64_t)(uint32_t)v.z * 0x165667B19E3779F9ull + (h << 6) + (h >> 2);
+        return (size_t)h;
+    }
+};
+
+// The six unit vectors in the reference's order (utility.hpp:156-162)
+extern const V3 kUnit[6];
+
+V3 rotate_half(V3 v, V3 axis);           // utility.cpp:68-85
+V3 rotate_turns(V3 v, V3 axis, int turns); // utility.cpp:102-142
+
+// ----------------------------------------------------------------- domains
+enum class Occ { unassigned = 0, unbound = 1, bound = 2, misbound = 3 };
+
+// domainContainer::Domain (domain.hpp:14-52)
+struct Dom {
+    int c;       // unique chain index
+    int c_ident; // chain identity
+    int d;       // index within its chain
+    int d_ident; // domain identity
+    int c_len;
+    Dom* fwd {nullptr};
+    Dom* bwd {nullptr};
+    V3 pos {};
+    V3 ore {};
+    Occ st {Occ::unassigned};
+    Dom* bd {nullptr};
+};
+
+// Domain::operator+ (domain.cpp:9-31)
+Dom* walk(Dom* d, int incr);
+
+// --------------------------------------------------------------------- rng
+// RandomGens (random_gens.cpp:12-49) or the Philox stream of the sampler.
+class Rng {
+  public:
+    void init(int kind, uint64_t seed, uint32_t stream);
+    void begin_step(int64_t step) {
+        m_step = step;
+        m_idx = 0;
+    }
+    double uniform_real();
+    int uniform_int(int lo, int hi);
+    void set_record(bool on) { m_record = on; }
+    std::vector<uint64_t> tape;
+
+  private:
+    uint64_t philox64();
+    int m_kind {0};
+    uint64_t m_seed {0};
+    uint32_t m_stream {0};
+    int64_t m_step {0};
+    uint32_t m_idx {0};
+    bool m_record {false};
+    std::mt19937_64 m_mt;
+};
+
+void philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+// ------------------------------------------------------------------- energy
+struct Delta {
+    double e {0};
+    int stacked {0};
+};
+
+class System;
+
+// Energy tables keyed by domain identity pairs (origami_potential.cpp:1057-1221)
+struct Tables {
+    std::vector<double> hyb_e, hyb_h, hyb_s, stack;
+    double init_h {0}, init_s {0}, init_e {0};
+};
+
+class Potential {
+  public:
+    void setup(const ldo_system_desc& sys, const std::vector<std::vector<int>>& idents,
+               const std::vector<std::vector<std::string>>& seqs);
+    void update_temp(double temp, double stacking_mult); // :1019-1042
+    void set_uncached(double temp) { m_cur = build(temp, 1.0); }
+    double hyb_e(const Dom& a, const Dom& b) const { return m_cur.hyb_e[key(a, b)]; }
+    double hyb_h(const Dom& a, const Dom& b) const { return m_cur.hyb_h[key(a, b)]; }
+    double hyb_s(const Dom& a, const Dom& b) const { return m_cur.hyb_s[key(a, b)]; }
+    double stack(const Dom& a, const Dom& b) const { return m_cur.stack[key(a, b)]; }
+    double init_e() const { return m_cur.init_e; }
+    double init_h() const { return m_cur.init_h; }
+    double init_s() const { return m_cur.init_s; }
+    Tables build(double temp, double stacking_mult) const; // calc_energies
+    int ident_lo {0};
+    int n_ident {0};
+    int domain_type {0};
+    int misbinding {0};
+    bool violated {false};
+
+    Delta bind_domain(Dom& di);                 // :1282-1312
+    Delta check_stacking(Dom& di, Dom& dj);     // :1314-1317
+
+  private:
+    size_t key(const Dom& a, const Dom& b) const {
+        return (size_t)(a.d_ident - ident_lo) * n_ident + (b.d_ident - ident_lo);
+    }
+    ldo_system_desc m_sys {};
+    std::vector<std::vector<int>> m_idents;
+    std::vector<std::vector<std::string>> m_seqs;
+    std::vector<double> m_spec_h, m_spec_s;
+    Tables m_cur;
+    std::map<std::pair<double, double>, Tables> m_cache;
+
+    // JunctionBindingPotential (origami_potential.cpp:131-934)
+    Delta m_dc;
+    bool m_fb_violated {false};
+    Delta fb_bind(Dom& di, Dom& dj);
+    Delta fb_check(Dom& di, Dom& dj);
+    void fb_terms(Dom& di, Dom& dj);
+    void fb_around(Dom* cd, int j);
+    void fb_regular_pair(Dom* a, Dom* b, int i);
+    void fb_helix_pair(Dom* a, Dom* b, int i, int j);
+    void fb_junction_pair(Dom* a, Dom* b, int j);
+    void fb_junction(Dom* j1, Dom* j2, Dom* j3, Dom* j4, Dom* k1, Dom* k2);
+    void fb_backward_triplets(Dom* a, Dom* b);
+    void fb_forward_triplets(Dom* a, Dom* b);
+    void fb_central_triplets(Dom& di, Dom& dj);
+    void fb_backward_single_junction(Dom* a, Dom* b);
+    void fb_forward_single_junction(Dom* a, Dom* b);
+    void fb_central_single_junction(Dom* a, Dom* b);
+    void fb_single(Dom* h1, Dom* h2, Dom* h3);
+    void fb_double(Dom* h1, Dom* h2, Dom* h3);
+    void fb_triply(Dom* h1, Dom* h2, Dom* h3);
+
+  public:
+    bool pair_stacked(Dom* a, Dom* b) const;
+    bool twist_ok(const Dom& a, V3 ndr, const Dom& b) const;
+    bool kink_ok(const Dom& a, V3 ndr, const Dom& b) const;
+    bool junction_ok(const Dom& j1, const Dom& j2, const Dom& j3, const Dom& j4,
+                     const Dom& k1, const Dom& k2) const;
+};
+
+// ------------------------------------------------------------------ system
+struct ChainSpec {
+    int index;
+    int identity;
+    std::vector<V3> pos;
+    std::vector<V3> ore;
+};
+
+class OrigamiError : public std::runtime_error {
+  public:
+    OrigamiError(int code, const std::string& m): std::runtime_error(m), code {code} {}
+    int code;
+};
+
+// OrigamiSystem (origami_system.hpp:66-225)
+class System {
+  public:
+    System(const ldo_system_desc& sys, const std::vector<std::vector<int>>& idents,
+           const std::vector<std::vector<std::string>>& seqs,
+           const std::vector<ChainSpec>& chains, double temp);
+    ~System();
+    System(const System&) = delete;
+    System& operator=(const System&) = delete;
+
+    std::vector<std::vector<int>> identities;
+    bool cyclic;
+    double temp;
+    double staple_M;
+    std::vector<double> staple_us;
+    std::vector<double> reduced_staple_us;
+    double reduced_fugacity;
+    Potential pot;
+
+    std::vector<std::vector<Dom*>> chains;     // m_domains (working order)
+    std::vector<int> chain_indices;            // working -> unique
+    std::vector<int> chain_identities;         // working -> identity
+    std::vector<std::vector<int>> ident_to_index;
+    std::unordered_map<V3, Dom*, V3Hash> pos_to_unbound;
+    std::unordered_map<V3, Occ, V3Hash> pos_occ;
+    int num_domains {0};
+    int num_staples {0};
+    int num_bound_pairs {0};
+    int num_fully_bound_pairs {0};
+    int num_self_bound_pairs {0};
+    int num_unassigned {0};
+    int num_stacked {0};
+    double energy {0};
+    double hyb_enthalpy {0};
+    double hyb_entropy {0};
+    double stacking_energy {0};
+    bool violated {false};
+    int current_c_i {0};
+
+    Occ occupancy(V3 pos) const;
+    Dom* unbound_at(V3 pos) const { return pos_to_unbound.at(pos); }
+    std::vector<Dom*>& chain_by_index(int c_i);
+    Dom* domain(int c_i, int d_i) { return chain_by_index(c_i)[d_i]; }
+    std::vector<int> staple_counts() const;
+
+    double check_domain_constraints(Dom& di, V3 pos, V3 ore); // :343-355
+    double unassign_domain(Dom& di);                          // :375-385
+    int add_chain(int ident);                                 // :387-400
+    int add_chain(int ident, int c_i);                        // :402-443
+    void delete_chain(int c_i);                               // :445-472
+    double set_checked_domain_config(Dom& di, V3 pos, V3 ore);// :478-515
+    double set_domain_config(Dom& di, V3 pos, V3 ore);        // :517-541
+    void set_domain_orientation(Dom& di, V3 ore);             // :543-551
+    void center(int centering_domain);                        // :553-571
+    void set_all_domains();                                   // :573-586
+    void set_all_domains(const std::vector<ChainSpec>& cfg);  // :588-616
+    void set_config(const std::vector<ChainSpec>& cfg);       // :327-341
+    void check_all_constraints();                             // :267-325
+    void check_distance_constraints();                        // :357-373
+    void update_temp(double t, double stacking_mult);         // :618-622
+    void update_staple_us(double t, double mult);             // :624-628
+    void update_energy();                                     // :808-826
+    void update_enthalpy_and_entropy();                       // :204-246
+    std::vector<ChainSpec> snapshot() const;                  // :175-191
+
+  private:
+    void initialize_scaffold(const ChainSpec& c);
+    void initialize_staples(const std::vector<ChainSpec>& c);
+    Delta internal_unassign(Dom& di);                    // :695-739
+    double unassign_bound(Dom& di);                      // :741-753
+    void unassign_unbound(Dom& di);                      // :755-760
+    void update_occupancies(Dom& di, V3 pos);            // :770-806
+    Delta internal_check(Dom& di, V3 pos, V3 ore);       // :828-871
+};
+
+// -------------------------------------------------------------- movetypes
+struct MovesetEntry {
+    int kind;
+    double freq;
+    int max_regrowth;
+};
+
+struct Limits {
+    int max_total_staples;
+    int max_type_staples;
+};
+
+class Replica; // forward
+
+// Constraintpoints + StapleNetwork (top_constraint_points.cpp:14-601)
+class Constraintpoints {
+  public:
+    explicit Constraintpoints(System& s): m_sys {s} {}
+    void reset();
+    void calculate(const std::vector<Dom*>& scaffold_domains, int dir);
+    std::set<int> staples_to_be_regrown() const { return m_regrowth_staples; }
+    bool is_growthpoint(Dom* d) const { return m_growthpoints.count(d) > 0; }
+    bool is_stemdomain(Dom* d) const { return m_stemdomains.count(d) > 0; }
+    void add_active_endpoint(Dom* d, V3 pos);
+    void add_active_endpoint(Dom* d, V3 pos, int seg);
+    void reset_active_endpoints() { m_active = m_initial_active; }
+    void remove_active_endpoint(Dom* d);
+    void update_endpoints(Dom* d);
+    Dom* domain_to_grow(Dom* d) { return m_growthpoints[d]; }
+    bool endpoint_reached(Dom* d, V3 pos);
+    bool walks_remain(Dom* d, V3 pos);
+    int get_dir(Dom* d);
+
+  private:
+    void find_growthpoints_endpoints(const std::vector<Dom*>& sds, int seg);
+    // StapleNetwork state (top_constraint_points.hpp:73-86)
+    void scan_network(Dom* d);
+    void scan_staple_topology(Dom* growth_d);
+    std::vector<Dom*> make_staple_stack(Dom* d, int ci);
+    System& m_sys;
+    std::vector<Dom*> m_net_scaffold_ds;
+    bool m_net_external {false};
+    std::set<int> m_net_cs;
+    std::vector<std::pair<Dom*, Dom*>> m_net_gps;
+    std::vector<Dom*> m_net_ds;
+    std::vector<std::pair<Dom*, Dom*>> m_net_iaes;
+    std::unordered_map<Dom*, int> m_net_segs;
+    std::map<std::pair<int, int>, int> m_net_dirs;
+
+    std::set<int> m_regrowth_staples;
+    std::vector<Dom*> m_d_stack;
+    std::set<int> m_checked_staples;
+    std::unordered_map<Dom*, int> m_segs;
+    std::map<std::pair<int, int>, int> m_dirs;
+    std::unordered_map<Dom*, Dom*> m_growthpoints;
+    std::unordered_map<Dom*, Dom*> m_stemdomains;
+    std::map<std::pair<int, int>, std::vector<std::pair<int, V3>>> m_active;
+    std::map<std::pair<int, int>, std::vector<std::pair<int, V3>>> m_initial_active;
+    std::unordered_map<Dom*, Dom*> m_inactive;
+};
+
+// One replica: system + moves + RNG (the per-rank GCMCSimulation state)
+class Replica {
+  public:
+    Replica(const ldo_system_desc& sys, const std::vector<std::vector<int>>& idents,
+            const std::vector<std::vector<std::string>>& seqs,
+            const std::vector<ChainSpec>& chains, double temp,
+            const std::vector<MovesetEntry>& moves, const Limits& lim,
+            int centering_freq, int centering_domain, int constraint_check_freq,
+            int rng_kind, uint64_t seed, uint32_t stream);
+    System sys;
+    Rng rng;
+    int64_t step {0};
+    int status {0};
+    std::vector<int64_t> attempts, accepts;
+
+    void simulate(int64_t steps); // simulation.cpp:568-653
+
+  private:
+    std::vector<MovesetEntry> m_moves;
+    std::vector<double> m_cum;
+    Limits m_lim;
+    int m_centering_freq, m_centering_domain, m_check_freq;
+
+    // shared move state (MCMovetype / RegrowthMCMovetype / CBMCMovetype)
+    std::vector<std::pair<int, int>> m_modified, m_assigned;
+    std::vector<int> m_added;
+    std::map<std::pair<int, int>, V3> m_prev_pos, m_prev_ore, m_old_pos, m_old_ore;
+    bool m_rejected {false};
+    double m_modifier {1};
+    double m_delta_e {0};
+    long double m_bias {1}, m_new_bias {1};
+    double m_new_modifier {1};
+    bool m_regrow_old {false};
+    int m_insertion_sites {0};
+    int m_kind {0};
+    int m_dir {0};
+    Constraintpoints m_cp;
+
+    bool attempt(int kind, int max_regrowth);
+    void reset_origami();
+    Dom* select_random_domain();
+    bool test_acceptance(long double p);
+    bool staple_is_connector(const std::vector<Dom*>& staple);
+    bool scan_for_scaffold_domain(Dom* d, std::set<int>& participating);
+    std::vector<std::pair<Dom*, Dom*>> find_bound_domains(const std::vector<Dom*>& ch);
+    int num_bound_staple_domains(const std::vector<Dom*>& staple) const;
+    double set_growth_point(Dom& gnew, Dom& gold);
+    void grow_staple(int d_i, const std::vector<Dom*>& chain);
+    void grow_chain(const std::vector<Dom*>& ds);
+
+    bool orientation_move();
+    bool staple_exchange_move();
+    bool insert_staple();
+    bool delete_staple();
+    void met_grow_chain(const std::vector<Dom*>& ds);
+    void met_unassign_domains(const std::vector<Dom*>& ds);
+
+    bool cb_staple_move();
+    void cb_calc_biases(V3 p_prev, Dom& d, std::vector<std::pair<V3, V3>>& cfgs,
+                        std::vector<double>& bf);
+    std::vector<double> cb_calc_bias(const std::vector<double>& bf,
+                                     const std::vector<std::pair<V3, V3>>& cfgs, Dom* d);
+    void cb_select_and_set(size_t i, const std::vector<Dom*>& ds);
+    void cb_unassign_domains(const std::vector<Dom*>& ds);
+    void cb_setup_regrow_old();
+    bool cb_test_acceptance();
+    double cb_set_old_growth_point(Dom& gnew, Dom& gold);
+    void cb_add_external_bias() {}
+
+    bool ctcb_scaffold_move(int max_regrowth);
+    std::vector<Dom*> select_indices(const std::vector<Dom*>& seg, unsigned min_len,
+                                     unsigned max_regrowth);
+    void ctcb_grow_chain(const std::vector<Dom*>& ds);
+    void ctcb_grow_staple_and_update_endpoints(Dom* gold);
+};
+
+} // namespace orc
+
+#endif

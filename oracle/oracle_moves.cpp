@@ -1,0 +1,962 @@
+// oracle_moves.cpp - CPU restatement of the reference's move types, the
+// simulate loop and the constraint-point machinery.
+//
+// TEST INFRASTRUCTURE ONLY (see ldo_oracle.h).
+#include <algorithm>
+#include <cmath>
+
+#include "oracle_core.hpp"
+
+namespace orc {
+
+static std::pair<int, int> key_of(const Dom* d) { return {d->c, d->d}; }
+
+// ------------------------------------------------------- Constraintpoints
+// top_constraint_points.cpp:191-205
+void Constraintpoints::reset() {
+    m_regrowth_staples.clear();
+    m_checked_staples.clear();
+    m_d_stack.clear();
+    m_segs.clear();
+    m_dirs.clear();
+    m_growthpoints.clear();
+    m_stemdomains.clear();
+    m_active.clear();
+    m_initial_active.clear();
+    m_inactive.clear();
+    m_net_scaffold_ds.clear();
+    m_net_dirs.clear();
+}
+
+// top_constraint_points.cpp:179-185 (operator[] inserts 0 for unknown keys)
+int Constraintpoints::get_dir(Dom* d) {
+    int seg = m_segs[d];
+    return m_dirs[{d->c, seg}];
+}
+
+// top_constraint_points.cpp:207-221
+void Constraintpoints::calculate(const std::vector<Dom*>& sds, int dir) {
+    m_net_scaffold_ds = sds;
+    m_dirs[{sds[0]->c, 0}] = dir;
+    find_growthpoints_endpoints(sds, 0);
+    m_initial_active = m_active;
+}
+
+void Constraintpoints::add_active_endpoint(Dom* d, V3 pos) {
+    int seg = m_segs.at(d);
+    m_active[{d->c, seg}].push_back({d->d, pos});
+}
+
+void Constraintpoints::add_active_endpoint(Dom* d, V3 pos, int seg) {
+    m_active[{d->c, seg}].push_back({d->d, pos});
+}
+
+// top_constraint_points.cpp:299-320
+void Constraintpoints::remove_active_endpoint(Dom* d) {
+    int seg = m_segs.at(d);
+    auto& lst = m_active[{d->c, seg}];
+    std::vector<std::pair<int, V3>> kept;
+    for (auto& e: lst) {
+        if (e.first != d->d) {
+            kept.push_back(e);
+        }
+    }
+    lst = kept;
+}
+
+// top_constraint_points.cpp:340-349
+void Constraintpoints::update_endpoints(Dom* d) {
+    remove_active_endpoint(d);
+    auto it = m_inactive.find(d);
+    if (it != m_inactive.end()) {
+        add_active_endpoint(it->second, d->pos);
+    }
+}
+
+// top_constraint_points.cpp:359-372
+bool Constraintpoints::endpoint_reached(Dom* d, V3 pos) {
+    int seg = m_segs.at(d);
+    for (auto& e: m_active[{d->c, seg}]) {
+        if (d->d == e.first && pos == e.second) {
+            return true;
+        }
+    }
+    return false;
+}
+
+// IdealRandomWalks::num_walks(...) == 0 exactly when the Manhattan distance
+// exceeds the steps or differs from them in parity (ideal_random_walk.cpp:35-41);
+// otherwise the sum (:44-67) has a positive term.
+static bool walks_exist(V3 a, V3 b, int steps) {
+    V3 dr = b - a;
+    int s = std::abs(dr.x) + std::abs(dr.y) + std::abs(dr.z);
+    return !(s > steps || (steps - s) % 2 != 0);
+}
+
+// top_constraint_points.cpp:397-442, 571-601.  A stem domain looks its
+// segments up in m_stemd_to_segs, which only the non-contiguous move fills,
+// so for the contiguous move a stem domain has no walk constraint.
+bool Constraintpoints::walks_remain(Dom* d, V3 pos) {
+    if (is_stemdomain(d)) {
+        return true;
+    }
+    int seg = m_segs.at(d);
+    std::pair<int, int> key {d->c, seg};
+    int dir = m_dirs[key];
+    for (auto& e: m_active[key]) {
+        int steps;
+        if (m_sys.cyclic && d->c == 0) {
+            int n = (int)m_sys.chains[0].size();
+            if (dir > 0 && e.first < d->d) {
+                steps = n + e.first - d->d;
+            }
+            else if (dir < 0 && e.first > d->d) {
+                steps = d->d + n - e.first;
+            }
+            else if (dir == 0 || e.first == d->d) {
+                steps = 0;
+            }
+            else {
+                steps = std::abs(e.first - d->d);
+            }
+        }
+        else {
+            steps = std::abs(e.first - d->d);
+        }
+        if (!walks_exist(pos, e.second, steps)) {
+            return false;
+        }
+    }
+    return true;
+}
+
+// StapleNetwork::scan_network (top_constraint_points.cpp:36-44, 69-76).  The
+// per-network direction map is not cleared between scans, as in the
+// reference; its entries are always +1/-1 for segments 0/1.
+void Constraintpoints::scan_network(Dom* d) {
+    m_net_external = false;
+    m_net_cs.clear();
+    m_net_gps.clear();
+    m_net_ds.clear();
+    m_net_iaes.clear();
+    m_net_segs.clear();
+    m_net_cs.insert(0);
+    scan_staple_topology(d);
+}
+
+// top_constraint_points.cpp:135-161
+std::vector<Dom*> Constraintpoints::make_staple_stack(Dom* d, int ci) {
+    std::vector<Dom*> ds;
+    std::vector<Dom*>& staple = m_sys.chain_by_index(ci);
+    m_net_segs[d] = 0;
+    m_net_dirs[{ci, 0}] = 1;
+    for (size_t di = d->d + 1; di < staple.size(); di++) {
+        ds.push_back(staple[di]);
+        m_net_segs[staple[di]] = 0;
+    }
+    m_net_dirs[{ci, 1}] = -1;
+    for (int di = d->d - 1; di >= 0; di--) {
+        ds.push_back(staple[di]);
+        m_net_segs[staple[di]] = 1;
+    }
+    return ds;
+}
+
+// top_constraint_points.cpp:78-133, 163-170 (excluded staples are always
+// empty: the moveset's num_excluded_staples is hard-wired to 0,
+// simulation.cpp:408-410)
+void Constraintpoints::scan_staple_topology(Dom* growth_d) {
+    int ci = growth_d->c;
+    m_net_cs.insert(ci);
+    m_net_ds.push_back(growth_d);
+    std::vector<Dom*> ds = make_staple_stack(growth_d, ci);
+    for (Dom* d: ds) {
+        m_net_ds.push_back(d);
+        Dom* bd = d->bd;
+        if (bd == nullptr || bd->c == ci) {
+            continue;
+        }
+        int bd_ci = bd->c;
+        if (m_net_cs.count(bd_ci) > 0) {
+            bool external = false;
+            if (bd_ci == 0) {
+                external = std::find(m_net_scaffold_ds.begin(), m_net_scaffold_ds.end(), bd) ==
+                           m_net_scaffold_ds.end();
+            }
+            if (!external) {
+                if (std::find(m_net_ds.begin(), m_net_ds.end(), bd) != m_net_ds.end()) {
+                    m_net_iaes.push_back({bd, d});
+                }
+                else {
+                    m_net_iaes.push_back({d, bd});
+                }
+            }
+            if (!m_net_external && external) {
+                m_net_external = true;
+            }
+        }
+        else {
+            m_net_gps.push_back({d, bd});
+            scan_staple_topology(bd);
+        }
+    }
+}
+
+// top_constraint_points.cpp:454-494 plus the add_* helpers (:500-569)
+void Constraintpoints::find_growthpoints_endpoints(const std::vector<Dom*>& sds, int seg) {
+    for (Dom* d: sds) {
+        m_d_stack.push_back(d);
+        m_segs[d] = seg;
+        if (d->bd == nullptr || d->bd->c == d->c || m_checked_staples.count(d->bd->c) > 0) {
+            continue;
+        }
+        Dom* bd = d->bd;
+        scan_network(bd);
+        std::vector<std::pair<Dom*, Dom*>> gps = m_net_gps;
+        gps.push_back({d, bd});
+        if (m_net_external) {
+            for (auto& g: gps) {
+                if (g.first->c == 0) {
+                    add_active_endpoint(g.first, g.first->pos, seg);
+                }
+            }
+            for (auto& ia: m_net_iaes) {
+                if (ia.second->c == 0) {
+                    add_active_endpoint(ia.first, ia.second->pos, seg);
+                }
+            }
+        }
+        else {
+            for (auto& g: gps) {
+                m_growthpoints[g.first] = g.second;
+                m_stemdomains[g.second] = g.first;
+            }
+            for (auto& ia: m_net_iaes) {
+                m_inactive[ia.first] = ia.second;
+            }
+            for (int ci: m_net_cs) {
+                if (ci != 0) {
+                    m_regrowth_staples.insert(ci);
+                }
+            }
+            m_d_stack.insert(m_d_stack.end(), m_net_ds.begin(), m_net_ds.end());
+            m_segs.insert(m_net_segs.begin(), m_net_segs.end());
+            for (auto& kv: m_net_dirs) {
+                m_dirs.insert(kv);
+            }
+        }
+        for (int ci: m_net_cs) {
+            m_checked_staples.insert(ci);
+        }
+    }
+}
+
+// ----------------------------------------------------------------- Replica
+Replica::Replica(const ldo_system_desc& sd, const std::vector<std::vector<int>>& idents,
+                 const std::vector<std::vector<std::string>>& seqs,
+                 const std::vector<ChainSpec>& cfg, double temp,
+                 const std::vector<MovesetEntry>& moves, const Limits& lim,
+                 int centering_freq, int centering_domain, int check_freq, int rng_kind,
+                 uint64_t seed, uint32_t stream):
+        sys {sd, idents, seqs, cfg, temp},
+        m_moves {moves},
+        m_lim {lim},
+        m_centering_freq {centering_freq},
+        m_centering_domain {centering_domain},
+        m_check_freq {check_freq},
+        m_cp {sys} {
+    rng.init(rng_kind, seed, stream);
+    double cum = 0;
+    for (auto& m: moves) {
+        cum += m.freq;
+        m_cum.push_back(cum);
+    }
+    attempts.assign(moves.size(), 0);
+    accepts.assign(moves.size(), 0);
+}
+
+// GCMCSimulation::simulate (simulation.cpp:568-653) without file output
+void Replica::simulate(int64_t steps) {
+    int64_t end = step + steps;
+    while (step < end) {
+        step++;
+        rng.begin_step(step);
+        // select_movetype (simulation.cpp:655-665)
+        double prob = rng.uniform_real();
+        size_t i = 0;
+        for (; i < m_cum.size(); i++) {
+            if (prob < m_cum[i]) {
+                break;
+            }
+        }
+        if (i == m_cum.size()) {
+            i = m_cum.size() - 1;
+        }
+        m_kind = m_moves[i].kind;
+        attempts[i]++;
+        bool accepted = attempt(m_moves[i].kind, m_moves[i].max_regrowth);
+        accepts[i] += accepted;
+        if (!accepted) {
+            reset_origami();
+        }
+        if (m_centering_freq != 0 && step % m_centering_freq == 0) {
+            sys.center(m_centering_domain);
+        }
+        if (m_check_freq != 0 && step % m_check_freq == 0) {
+            try {
+                sys.check_all_constraints();
+            } catch (OrigamiError& e) {
+                status = LDO_E_CONSTRAINT;
+                throw;
+            }
+        }
+    }
+}
+
+// MCMovetype::attempt_move plus the reset_internal chain of each move type
+// (movetypes.cpp:41-51, 242-250; met_movetypes.cpp:45-48, 139-143;
+// cb_movetypes.cpp:44-50, 457-461)
+bool Replica::attempt(int kind, int max_regrowth) {
+    m_modified.clear();
+    m_assigned.clear();
+    m_added.clear();
+    m_prev_pos.clear();
+    m_prev_ore.clear();
+    m_rejected = false;
+    m_modifier = 1;
+    m_delta_e = 0;
+    m_bias = 1;
+    m_new_bias = 1;
+    m_new_modifier = 1;
+    m_regrow_old = false;
+    m_insertion_sites = sys.num_domains;
+    switch (kind) {
+    case LDO_MOVE_ORIENTATION_ROTATION:
+        return orientation_move();
+    case LDO_MOVE_MET_STAPLE_EXCHANGE:
+        return staple_exchange_move();
+    case LDO_MOVE_CB_STAPLE_REGROWTH:
+        return cb_staple_move();
+    case LDO_MOVE_CTCB_SCAFFOLD_REGROWTH:
+        m_cp.reset();
+        return ctcb_scaffold_move(max_regrowth);
+    }
+    throw OrigamiError(LDO_E_NOTIMPL, "movetype not supported");
+}
+
+// MCMovetype::reset_origami (movetypes.cpp:53-86)
+void Replica::reset_origami() {
+    for (auto& k: m_assigned) {
+        sys.unassign_domain(*sys.domain(k.first, k.second));
+    }
+    for (int c: m_added) {
+        sys.delete_chain(c);
+        sys.current_c_i -= 1;
+    }
+    for (auto& k: m_modified) {
+        V3 pos = m_prev_pos[k];
+        V3 ore = m_prev_ore[k];
+        sys.set_checked_domain_config(*sys.domain(k.first, k.second), pos, ore);
+    }
+    sys.violated = false;
+}
+
+// movetypes.cpp:98-113
+Dom* Replica::select_random_domain() {
+    int idx = rng.uniform_int(0, sys.num_domains - 1);
+    int counter = 0;
+    for (auto& ch: sys.chains) {
+        for (Dom* d: ch) {
+            if (counter == idx) {
+                return d;
+            }
+            counter++;
+        }
+    }
+    return nullptr;
+}
+
+// movetypes.cpp:142-158
+bool Replica::test_acceptance(long double p_ratio) {
+    double p_accept = (double)(std::fmin((long double)1, p_ratio) * m_modifier);
+    if (p_accept == 1) {
+        return true;
+    }
+    return p_accept > rng.uniform_real();
+}
+
+// movetypes.cpp:160-177
+bool Replica::staple_is_connector(const std::vector<Dom*>& staple) {
+    for (Dom* d: staple) {
+        if (d->st != Occ::unbound) {
+            Dom* b = d->bd;
+            if (b->c == 0) {
+                continue;
+            }
+            std::set<int> participating {d->c};
+            if (!scan_for_scaffold_domain(b, participating)) {
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+// movetypes.cpp:191-232
+bool Replica::scan_for_scaffold_domain(Dom* dom, std::set<int>& participating) {
+    int c_i = dom->c;
+    participating.insert(c_i);
+    std::vector<Dom*> staple = sys.chain_by_index(c_i);
+    for (Dom* cur: staple) {
+        if (cur == dom) {
+            continue;
+        }
+        Dom* b = cur->bd;
+        if (b != nullptr) {
+            if (b->c == dom->c) {
+                continue;
+            }
+            if (b->c == 0) {
+                return true;
+            }
+            if (participating.count(b->c) > 0) {
+                continue;
+            }
+            if (scan_for_scaffold_domain(b, participating)) {
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+// movetypes.cpp:258-277
+std::vector<std::pair<Dom*, Dom*>> Replica::find_bound_domains(const std::vector<Dom*>& ch) {
+    std::vector<std::pair<Dom*, Dom*>> out;
+    int ci = ch[0]->c;
+    for (Dom* d: ch) {
+        if (d->bd != nullptr && d->bd->c != ci) {
+            out.push_back({d, d->bd});
+        }
+    }
+    if (out.empty()) {
+        throw OrigamiError(LDO_E_CONSTRAINT, "System has unbound staple");
+    }
+    return out;
+}
+
+// movetypes.cpp:395-405
+int Replica::num_bound_staple_domains(const std::vector<Dom*>& staple) const {
+    int n = 0;
+    for (Dom* d: staple) {
+        if (d->st == Occ::bound || d->st == Occ::misbound) {
+            n++;
+        }
+    }
+    return n;
+}
+
+// movetypes.cpp:322-341
+double Replica::set_growth_point(Dom& gnew, Dom& gold) {
+    V3 o_new = -gold.ore;
+    double de = sys.set_domain_config(gnew, gold.pos, o_new);
+    if (sys.violated) {
+        m_rejected = true;
+    }
+    else {
+        m_assigned.push_back(key_of(&gnew));
+    }
+    return de;
+}
+
+// movetypes.cpp:343-370
+void Replica::grow_staple(int d_i, const std::vector<Dom*>& chain) {
+    std::vector<Dom*> three(chain.begin() + d_i, chain.end());
+    if (three.size() > 1) {
+        grow_chain(three);
+    }
+    if (m_rejected) {
+        return;
+    }
+    std::vector<Dom*> five(chain.begin(), chain.begin() + d_i + 1);
+    std::reverse(five.begin(), five.end());
+    if (five.size() > 1) {
+        grow_chain(five);
+    }
+}
+
+// virtual grow_chain dispatch
+void Replica::grow_chain(const std::vector<Dom*>& ds) {
+    switch (m_kind) {
+    case LDO_MOVE_MET_STAPLE_EXCHANGE:
+        met_grow_chain(ds);
+        break;
+    case LDO_MOVE_CB_STAPLE_REGROWTH:
+        for (size_t i = 1; i < ds.size(); i++) {
+            cb_select_and_set(i, ds);
+            if (m_rejected) {
+                break;
+            }
+        }
+        break;
+    case LDO_MOVE_CTCB_SCAFFOLD_REGROWTH:
+        ctcb_grow_chain(ds);
+        break;
+    }
+}
+
+// --------------------------------------------------------- orientation move
+// orientation_movetype.cpp:30-65
+bool Replica::orientation_move() {
+    bool accepted = false;
+    Dom* d = select_random_domain();
+    V3 o_new = kUnit[rng.uniform_int(0, 5)];
+    if (d->st == Occ::bound || d->st == Occ::misbound) {
+        double de = 0;
+        V3 o_old = d->ore;
+        Dom* b = d->bd;
+        de += sys.unassign_domain(*b);
+        sys.set_domain_orientation(*d, o_new);
+        V3 pos = d->pos;
+        de += sys.set_domain_config(*b, pos, -o_new);
+        if (!sys.violated) {
+            accepted = test_acceptance(std::exp(-de));
+        }
+        if (!accepted) {
+            sys.unassign_domain(*b);
+            sys.set_domain_orientation(*d, o_old);
+            sys.set_checked_domain_config(*b, pos, -o_old);
+        }
+    }
+    else {
+        sys.set_domain_orientation(*d, o_new);
+        accepted = true;
+    }
+    return accepted;
+}
+
+// ------------------------------------------------------ staple exchange move
+// met_movetypes.cpp:192-206
+bool Replica::staple_exchange_move() {
+    if (rng.uniform_real() < 0.5) {
+        return insert_staple();
+    }
+    return delete_staple();
+}
+
+// met_movetypes.cpp:50-67
+void Replica::met_grow_chain(const std::vector<Dom*>& ds) {
+    for (size_t i = 1; i < ds.size(); i++) {
+        Dom* d = ds[i];
+        Dom* prev = ds[i - 1];
+        V3 p = prev->pos + kUnit[rng.uniform_int(0, 5)];
+        V3 o = kUnit[rng.uniform_int(0, 5)];
+        m_delta_e += sys.set_domain_config(*d, p, o);
+        if (sys.violated) {
+            m_rejected = true;
+            break;
+        }
+        m_assigned.push_back(key_of(d));
+    }
+}
+
+// met_movetypes.cpp:77-85
+void Replica::met_unassign_domains(const std::vector<Dom*>& ds) {
+    for (Dom* d: ds) {
+        auto k = key_of(d);
+        m_prev_pos[k] = d->pos;
+        m_prev_ore[k] = d->ore;
+        m_modified.push_back(k);
+        m_delta_e += sys.unassign_domain(*d);
+    }
+}
+
+// met_movetypes.cpp:303-359, 212-253 (exchange multipliers are all 1 and the
+// move-level external bias is 0 without bias functions)
+bool Replica::insert_staple() {
+    int ident = rng.uniform_int(1, (int)sys.identities.size() - 1);
+    if (sys.num_staples == m_lim.max_total_staples) {
+        return false;
+    }
+    if ((int)sys.ident_to_index[ident].size() == m_lim.max_type_staples) {
+        return false;
+    }
+    int c_i = sys.add_chain(ident);
+    m_delta_e += sys.pot.init_e();
+    m_added.push_back(c_i);
+    std::vector<Dom*> chain = sys.chains.back();
+    // select_new_growthpoint (movetypes.cpp:372-383)
+    int gi = rng.uniform_int(0, (int)chain.size() - 1);
+    Dom* gnew = chain[gi];
+    Dom* gold = select_random_domain();
+    while (gold->c == gnew->c) {
+        gold = select_random_domain();
+    }
+    m_delta_e += set_growth_point(*gnew, *gold);
+    if (m_rejected) {
+        return false;
+    }
+    grow_staple(gnew->d, chain);
+    if (m_rejected) {
+        return false;
+    }
+    std::vector<Dom*>& staple = sys.chain_by_index(c_i);
+    double boltz = std::exp(-m_delta_e);
+    int n_new = (int)sys.ident_to_index[ident].size();
+    size_t len = sys.identities[ident].size();
+    double pratio = static_cast<double>(len) / 6.0 / n_new * boltz;
+    pratio *= m_insertion_sites * sys.reduced_fugacity;
+    pratio /= num_bound_staple_domains(staple);
+    return test_acceptance(pratio);
+}
+
+// met_movetypes.cpp:361-403, 255-301
+bool Replica::delete_staple() {
+    int ident = rng.uniform_int(1, (int)sys.identities.size() - 1);
+    int n = (int)sys.ident_to_index[ident].size();
+    if (n == 0) {
+        m_rejected = true;
+        return false;
+    }
+    int c_i = sys.ident_to_index[ident][rng.uniform_int(0, n - 1)];
+    std::vector<Dom*> staple = sys.chain_by_index(c_i);
+    if (staple_is_connector(staple)) {
+        return false;
+    }
+    int n_bd = num_bound_staple_domains(staple);
+    met_unassign_domains(staple);
+    m_delta_e -= sys.pot.init_e();
+    double boltz = std::exp(-m_delta_e);
+    int ni = (int)sys.ident_to_index[ident].size();
+    size_t len = sys.identities[ident].size();
+    double pratio = ni * 6.0 / static_cast<double>(len) * boltz;
+    pratio /= sys.reduced_fugacity * (m_insertion_sites - len);
+    pratio *= n_bd;
+    bool accepted = test_acceptance(pratio);
+    if (accepted) {
+        sys.delete_chain(c_i);
+    }
+    return accepted;
+}
+
+// -------------------------------------------------- CB staple regrowth move
+// cb_movetypes.cpp:58-102
+void Replica::cb_calc_biases(V3 p_prev, Dom& d, std::vector<std::pair<V3, V3>>& cfgs,
+                             std::vector<double>& bf) {
+    for (int v = 0; v < 6; v++) {
+        V3 p_new = p_prev + kUnit[v];
+        Occ occ = sys.occupancy(p_new);
+        if (occ == Occ::bound || occ == Occ::misbound) {
+            continue;
+        }
+        if (occ == Occ::unbound) {
+            Dom* ub = sys.unbound_at(p_new);
+            V3 o_new = -ub->ore;
+            double de = sys.check_domain_constraints(d, p_new, o_new);
+            if (!sys.violated) {
+                cfgs.push_back({p_new, o_new});
+                bf.push_back(std::exp(-de));
+            }
+            else {
+                sys.violated = false;
+            }
+        }
+        else {
+            cfgs.push_back({p_new, V3 {0, 0, 0}});
+            double de = sys.check_domain_constraints(d, p_new, V3 {1, 0, 0});
+            bf.push_back(6 * std::exp(-de));
+        }
+    }
+}
+
+// CBStapleRegrowthMCMovetype::calc_bias (cb_movetypes.cpp:363-387) and
+// CTCBRegrowthMCMovetype::calc_bias (:486-539)
+std::vector<double> Replica::cb_calc_bias(const std::vector<double>& bf,
+                                          const std::vector<std::pair<V3, V3>>& cfgs, Dom* d) {
+    std::vector<double> w;
+    if (m_kind == LDO_MOVE_CB_STAPLE_REGROWTH) {
+        double ros = 0;
+        for (double b: bf) {
+            ros += b;
+        }
+        if (ros == 0) {
+            m_rejected = true;
+        }
+        else {
+            m_bias *= ros;
+            for (double b: bf) {
+                w.push_back(b / ros);
+            }
+        }
+        return w;
+    }
+    std::vector<long double> wl(bf.begin(), bf.end());
+    for (size_t i = 0; i < cfgs.size(); i++) {
+        V3 pos = cfgs[i].first;
+        if (sys.occupancy(pos) == Occ::unbound) {
+            Dom* occ = sys.unbound_at(pos);
+            bool same_chain = occ->c == d->c;
+            bool endpoint = m_cp.endpoint_reached(d, pos);
+            if (!(same_chain || endpoint)) {
+                wl[i] = 0;
+            }
+        }
+        if (!m_cp.walks_remain(d, pos)) {
+            wl[i] = 0;
+        }
+    }
+    long double sum = 0;
+    for (long double x: wl) {
+        sum += x;
+    }
+    if (sum == 0) {
+        m_rejected = true;
+    }
+    else {
+        m_bias *= sum;
+        for (size_t i = 0; i < wl.size(); i++) {
+            w.push_back(static_cast<double>(wl[i] / sum));
+        }
+    }
+    return w;
+}
+
+// cb_movetypes.cpp:104-166
+void Replica::cb_select_and_set(size_t i, const std::vector<Dom*>& ds) {
+    Dom* d = ds[i];
+    V3 p_prev = ds[i - 1]->pos;
+    std::vector<std::pair<V3, V3>> cfgs;
+    std::vector<double> bf;
+    cb_calc_biases(p_prev, *d, cfgs, bf);
+    std::vector<double> w = cb_calc_bias(bf, cfgs, d);
+    if (m_rejected) {
+        return;
+    }
+    if (!m_regrow_old) {
+        std::pair<V3, V3> cfg {};
+        double cum = 0;
+        double u = rng.uniform_real();
+        for (size_t k = 0; k < w.size(); k++) {
+            cum += w[k];
+            if (u < cum) {
+                cfg = cfgs[k];
+                break;
+            }
+        }
+        V3 p_new = cfg.first;
+        V3 o_new = cfg.second;
+        if (o_new == V3 {0, 0, 0}) {
+            o_new = kUnit[rng.uniform_int(0, 5)];
+        }
+        sys.set_checked_domain_config(*d, p_new, o_new);
+    }
+    else {
+        auto k = key_of(d);
+        sys.set_checked_domain_config(*d, m_old_pos[k], m_old_ore[k]);
+    }
+    m_assigned.push_back(key_of(d));
+}
+
+// cb_movetypes.cpp:200-213
+void Replica::cb_unassign_domains(const std::vector<Dom*>& ds) {
+    for (Dom* d: ds) {
+        auto k = key_of(d);
+        m_prev_pos[k] = d->pos;
+        m_prev_ore[k] = d->ore;
+        m_modified.push_back(k);
+        sys.unassign_domain(*d);
+    }
+}
+
+// cb_movetypes.cpp:215-225
+void Replica::cb_setup_regrow_old() {
+    m_regrow_old = true;
+    m_new_bias = m_bias;
+    m_new_modifier = m_modifier;
+    m_bias = 1;
+    m_modified.clear();
+    m_assigned.clear();
+    m_old_pos = m_prev_pos;
+    m_old_ore = m_prev_ore;
+}
+
+// cb_movetypes.cpp:168-180
+double Replica::cb_set_old_growth_point(Dom& gnew, Dom& gold) {
+    auto k = key_of(&gnew);
+    V3 o_old = m_old_ore[k];
+    double de = sys.set_checked_domain_config(gnew, gold.pos, o_old);
+    m_bias *= std::exp(-de);
+    m_assigned.push_back(k);
+    return de;
+}
+
+// cb_movetypes.cpp:182-198 (ops/bias resynchronisation is a no-op without
+// bias functions)
+bool Replica::cb_test_acceptance() {
+    long double ratio = m_new_bias / m_bias;
+    if (test_acceptance(ratio)) {
+        reset_origami();
+        return true;
+    }
+    m_modified.clear();
+    m_assigned.clear();
+    return false;
+}
+
+// cb_movetypes.cpp:292-348, 389-404
+bool Replica::cb_staple_move() {
+    if (sys.num_staples == 0) {
+        return false;
+    }
+    int w = rng.uniform_int(1, sys.num_staples);
+    std::vector<Dom*> chain = sys.chains[w];
+    if (staple_is_connector(chain)) {
+        return false;
+    }
+    auto bound = find_bound_domains(chain);
+    m_bias *= bound.size();
+    auto gp = bound[rng.uniform_int(0, (int)bound.size() - 1)];
+    cb_unassign_domains(chain);
+    {
+        double de = set_growth_point(*gp.first, *gp.second);
+        m_bias *= std::exp(-de);
+        if (!m_rejected) {
+            grow_staple(gp.first->d, chain);
+        }
+    }
+    if (m_rejected) {
+        return false;
+    }
+    m_bias /= num_bound_staple_domains(chain);
+    cb_add_external_bias();
+    cb_setup_regrow_old();
+    gp = bound[rng.uniform_int(0, (int)bound.size() - 1)];
+    cb_unassign_domains(chain);
+    cb_set_old_growth_point(*gp.first, *gp.second);
+    if (!m_rejected) {
+        grow_staple(gp.first->d, chain);
+    }
+    m_modifier = m_new_modifier;
+    return cb_test_acceptance();
+}
+
+// ------------------------------------------ CTCB scaffold regrowth move
+// CTRegrowthMCMovetype::select_indices (movetypes.cpp:469-510)
+std::vector<Dom*> Replica::select_indices(const std::vector<Dom*>& seg, unsigned min_len,
+                                          unsigned max_regrowth) {
+    unsigned seg_len = (unsigned)seg.size();
+    unsigned max_len = std::min(seg_len, max_regrowth);
+    unsigned sel_len = (unsigned)rng.uniform_int((int)min_len, (int)max_len);
+    int start = rng.uniform_int(0, (int)seg_len - 1);
+    m_dir = rng.uniform_int(0, 1);
+    if (m_dir == 0) {
+        m_dir = -1;
+    }
+    Dom* cur = seg[start];
+    std::vector<Dom*> ds;
+    while (cur != nullptr && ds.size() != sel_len) {
+        ds.push_back(cur);
+        cur = walk(cur, m_dir);
+    }
+    Dom* back = walk(seg[start], -m_dir);
+    while (back != nullptr && ds.size() != sel_len) {
+        ds.insert(ds.begin(), back);
+        back = walk(back, -m_dir);
+    }
+    if (ds.size() < min_len) {
+        return select_indices(seg, min_len, max_regrowth);
+    }
+    if (cur != nullptr) {
+        m_cp.add_active_endpoint(cur, cur->pos, 0);
+    }
+    return ds;
+}
+
+// cb_movetypes.cpp:463-484
+void Replica::ctcb_grow_chain(const std::vector<Dom*>& ds) {
+    if (ds.size() <= 1) {
+        return;
+    }
+    for (size_t i = 1; i < ds.size(); i++) {
+        Dom* d = ds[i];
+        m_dir = m_cp.get_dir(d);
+        cb_select_and_set(i, ds);
+        if (m_rejected) {
+            break;
+        }
+        m_cp.update_endpoints(d);
+        if (m_cp.is_growthpoint(d)) {
+            ctcb_grow_staple_and_update_endpoints(d);
+            if (m_rejected) {
+                break;
+            }
+        }
+    }
+}
+
+// cb_movetypes.cpp:541-562
+void Replica::ctcb_grow_staple_and_update_endpoints(Dom* gold) {
+    Dom* gnew = m_cp.domain_to_grow(gold);
+    int c_i = gnew->c;
+    if (c_i == 0) {
+        return;
+    }
+    if (m_regrow_old) {
+        cb_set_old_growth_point(*gnew, *gold);
+    }
+    else {
+        double de = set_growth_point(*gnew, *gold);
+        m_bias *= std::exp(-de);
+    }
+    if (!m_rejected) {
+        m_cp.update_endpoints(gnew);
+        std::vector<Dom*> staple = sys.chain_by_index(c_i);
+        grow_staple(gnew->d, staple);
+    }
+}
+
+// cb_movetypes.cpp:680-747
+bool Replica::ctcb_scaffold_move(int max_regrowth) {
+    m_regrow_old = false;
+    std::vector<Dom*> scaffold = sys.chains[0];
+    std::vector<Dom*> sds = select_indices(scaffold, 2, (unsigned)max_regrowth);
+    m_cp.calculate(sds, m_dir);
+    bool whole_ring = sys.cyclic && sds.size() == scaffold.size();
+    if (!whole_ring) {
+        m_cp.remove_active_endpoint(sds[0]);
+    }
+    std::set<int> staples = m_cp.staples_to_be_regrown();
+    std::vector<Dom*> to_unassign(sds.begin() + 1, sds.end());
+    cb_unassign_domains(to_unassign);
+    for (int c_i: staples) {
+        cb_unassign_domains(sys.chain_by_index(c_i));
+    }
+    if (m_cp.is_growthpoint(sds[0])) {
+        ctcb_grow_staple_and_update_endpoints(sds[0]);
+        if (m_rejected) {
+            return false;
+        }
+    }
+    ctcb_grow_chain(sds);
+    if (m_rejected) {
+        return false;
+    }
+    cb_add_external_bias();
+    cb_setup_regrow_old();
+    m_cp.reset_active_endpoints();
+    if (!whole_ring) {
+        m_cp.remove_active_endpoint(sds[0]);
+    }
+    cb_unassign_domains(to_unassign);
+    for (int c_i: staples) {
+        cb_unassign_domains(sys.chain_by_index(c_i));
+    }
+    if (m_cp.is_growthpoint(sds[0])) {
+        ctcb_grow_staple_and_update_endpoints(sds[0]);
+    }
+    ctcb_grow_chain(sds);
+    m_modifier = 1;
+    return cb_test_acceptance();
+}
+
+} // namespace orc

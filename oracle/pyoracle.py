@@ -1,0 +1,208 @@
+"""ctypes binding of the CPU oracle (oracle/_build/libldo_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+from latticednaorigami_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libldo_oracle.so")
+REF_NN = os.path.join(HERE, "_ref", "libref_nn.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if os.path.isdir("/root/reference"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        abi.declare_abi(L, "oracle_")
+        L.oracle_create.restype = C.c_void_p
+        L.oracle_create.argtypes = [C.POINTER(abi.SystemDesc), C.POINTER(abi.RunParams),
+                                    C.POINTER(abi.Config), C.c_int32,
+                                    C.POINTER(C.c_double), C.c_int32]
+        L.oracle_record_tape.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.oracle_tape_length.restype = C.c_int64
+        L.oracle_tape_length.argtypes = [C.c_void_p, C.c_int32]
+        L.oracle_get_tape.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64), C.c_int64]
+        L.oracle_energy_tables.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32,
+                                           C.c_int32, C.POINTER(C.c_double)]
+        L.oracle_recompute_energy.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
+        L.oracle_nn_hybridization_thermo.argtypes = [C.c_char_p, C.c_double, C.c_double,
+                                                     C.POINTER(C.c_double),
+                                                     C.POINTER(C.c_double)]
+        L.oracle_nn_reduced_energy.restype = C.c_double
+        L.oracle_nn_reduced_energy.argtypes = [C.c_char_p, C.c_double, C.c_double]
+        L.oracle_nn_longest_contig_complement.argtypes = [C.c_char_p, C.c_char_p,
+                                                          C.c_char_p, C.c_int32]
+        L.oracle_num_walks.restype = C.c_double
+        L.oracle_num_walks.argtypes = [C.c_int32] * 4
+        L.oracle_rotate_half.argtypes = [C.POINTER(C.c_int32)] * 3
+        L.oracle_rotate.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32,
+                                    C.POINTER(C.c_int32)]
+        L.oracle_philox.argtypes = [C.POINTER(C.c_uint32)] * 3
+        L.oracle_enumerate.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _check(L, h, rc):
+    if rc != 0:
+        raise abi.LdoError(rc, L.oracle_last_error(h).decode())
+
+
+class Oracle:
+    """The oracle's replicas of one system (rng_kind 0: Philox stream of the
+    sampler; 1: the reference's mt19937_64 RandomGens)."""
+
+    def __init__(self, system, temps=None, rng_kind=0):
+        self.L = lib()
+        self.system = system
+        temps = list(temps if temps is not None else (system.temps or [300.0]))
+        self.n = len(temps)
+        sd, rp, cfg, self._keep = system.to_abi()
+        t = (C.c_double * self.n)(*temps)
+        h = self.L.oracle_create(C.byref(sd), C.byref(rp), C.byref(cfg), self.n, t, rng_kind)
+        if not h:
+            raise abi.LdoError(abi.LDO_E_MISUSE, self.L.oracle_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sweep(self, steps):
+        _check(self.L, self.h, self.L.oracle_sweep(self.h, steps))
+
+    def run(self, n_swaps, interval):
+        _check(self.L, self.h, self.L.oracle_run(self.h, n_swaps, interval))
+
+    def set_ladder(self, temps, u=None, b=None, s=None):
+        n = len(temps)
+        one = [1.0] * n
+        arrs = [(C.c_double * n)(*v) for v in (temps, u or one, b or one, s or one)]
+        _check(self.L, self.h, self.L.oracle_set_ladder(self.h, n, *arrs))
+
+    def observables(self, r=0):
+        o = abi.Obs()
+        _check(self.L, self.h, self.L.oracle_get_observables(self.h, r, C.byref(o)))
+        return o.as_dict()
+
+    def counters(self):
+        c = abi.Counters()
+        _check(self.L, self.h, self.L.oracle_get_counters(self.h, C.byref(c)))
+        return c
+
+    def permutation(self):
+        p = (C.c_int32 * self.n)()
+        _check(self.L, self.h, self.L.oracle_get_permutation(self.h, p))
+        return list(p)
+
+    def config(self, r=0):
+        from latticednaorigami_amd.systems import config_from_abi, empty_config
+        cap_c = self.system.max_total_staples + 1
+        cap_d = self.system.max_domains()
+        cfg, keep = empty_config(cap_c, cap_d)
+        _check(self.L, self.h, self.L.oracle_get_config(self.h, r, C.byref(cfg), cap_c, cap_d))
+        return config_from_abi(cfg)
+
+    def record_tape(self, r, on=True):
+        _check(self.L, self.h, self.L.oracle_record_tape(self.h, r, int(on)))
+
+    def tape(self, r):
+        n = self.L.oracle_tape_length(self.h, r)
+        buf = (C.c_uint64 * max(1, n))()
+        _check(self.L, self.h, self.L.oracle_get_tape(self.h, r, buf, n))
+        return list(buf)[:n]
+
+    def tables(self, temp, ii, ij, stacking_mult=1.0):
+        out = (C.c_double * 5)()
+        _check(self.L, self.h, self.L.oracle_energy_tables(self.h, temp, stacking_mult, ii, ij,
+                                                           out))
+        return dict(zip(("hyb_e", "hyb_h", "hyb_s", "stack", "init_e"), list(out)))
+
+    def recompute_energy(self, r=0):
+        e = C.c_double()
+        _check(self.L, self.h, self.L.oracle_recompute_energy(self.h, r, C.byref(e)))
+        return e.value
+
+
+def nn_thermo(seq, temp, cation):
+    h, s = C.c_double(), C.c_double()
+    rc = lib().oracle_nn_hybridization_thermo(seq.encode(), temp, cation, C.byref(h), C.byref(s))
+    if rc:
+        raise ValueError(seq)
+    return h.value, s.value
+
+
+def nn_reduced_energy(seq, temp, cation):
+    return lib().oracle_nn_reduced_energy(seq.encode(), temp, cation)
+
+
+def longest_contig_complement(a, b):
+    buf = C.create_string_buffer(4096)
+    lib().oracle_nn_longest_contig_complement(a.encode(), b.encode(), buf, 4096)
+    v = buf.value.decode()
+    return v.split(" ") if v else []
+
+
+def num_walks(d, steps):
+    return lib().oracle_num_walks(d[0], d[1], d[2], steps)
+
+
+def rotate_half(v, axis):
+    a = (C.c_int32 * 3)(*v)
+    b = (C.c_int32 * 3)(*axis)
+    o = (C.c_int32 * 3)()
+    lib().oracle_rotate_half(a, b, o)
+    return tuple(o)
+
+
+def rotate(v, axis, turns):
+    a = (C.c_int32 * 3)(*v)
+    b = (C.c_int32 * 3)(*axis)
+    o = (C.c_int32 * 3)()
+    lib().oracle_rotate(a, b, turns, o)
+    return tuple(o)
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().oracle_philox(c, k, o)
+    return tuple(o)
+
+
+def ref_nn_lib():
+    """The reference's own nearest_neighbour.cpp (oracle/_ref), or None."""
+    if not os.path.exists(REF_NN):
+        return None
+    L = C.CDLL(REF_NN)
+    L.ref_nn_hybridization_thermo.argtypes = [C.c_char_p, C.c_double, C.c_double,
+                                              C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.ref_nn_init_thermo.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.ref_nn_longest_contig_complement.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]
+    L.ref_nn_stacking_energy.restype = C.c_double
+    L.ref_nn_stacking_energy.argtypes = [C.c_char_p, C.c_char_p]
+    return L
