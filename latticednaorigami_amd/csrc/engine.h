@@ -252,6 +252,22 @@ struct Eng {
         init_e = P->init_e[S->init_slot];
     }
 
+    // bounded appends to the move's domain lists
+    DEV void push_modified(int s) {
+        if (S->n_modified >= 2 * nslots) {
+            fail(ST_CAPACITY);
+            return;
+        }
+        modified[S->n_modified++] = (int16_t)s;
+    }
+    DEV void push_assigned(int s) {
+        if (S->n_assigned >= 2 * nslots) {
+            fail(ST_CAPACITY);
+            return;
+        }
+        assigned[S->n_assigned++] = (int16_t)s;
+    }
+
     DEV void fail(int code) {
         if (S->status == ST_OK) {
             S->status = code;
@@ -374,7 +390,7 @@ struct Eng {
     }
     DEV int hfind(uint64_t k) const {
         int i = home(k);
-        while (true) {
+        for (int n = 0; n <= hmask; n++) {
             uint64_t kk = hkey[i];
             if (kk == k) {
                 return i;
@@ -384,6 +400,7 @@ struct Eng {
             }
             i = (i + 1) & hmask;
         }
+        return -1;
     }
     // OrigamiSystem::position_occupancy (origami_system.cpp:193-202)
     DEV int occupancy(I3 p) {
@@ -392,7 +409,12 @@ struct Eng {
     }
     DEV int unbound_at(I3 p) {
         int i = hfind(pkey(p));
-        return (int)(int16_t)(hval[i] & 0xFFFFu);
+        int s = i < 0 ? -1 : (int)(int16_t)(hval[i] & 0xFFFFu);
+        if (s < 0 || s >= nslots) {
+            fail(ST_CONSTRAINT);
+            return 0;
+        }
+        return s;
     }
     DEV void hput(I3 p, int occ, int s) {
         if (p.x <= -LDO_COORD_OFFSET + 2 || p.x >= LDO_COORD_OFFSET - 2 ||
@@ -402,8 +424,13 @@ struct Eng {
         }
         uint64_t k = pkey(p);
         int i = home(k);
+        int n = 0;
         while (hkey[i] != k && hkey[i] != LDO_HASH_EMPTY) {
             i = (i + 1) & hmask;
+            if (++n > hmask) {
+                fail(ST_CAPACITY);
+                return;
+            }
         }
         hkey[i] = k;
         hval[i] = ((uint32_t)occ << 16) | (uint32_t)(uint16_t)(int16_t)s;
@@ -415,7 +442,7 @@ struct Eng {
             return;
         }
         int j = i;
-        while (true) {
+        for (int n = 0; n <= hmask; n++) {
             j = (j + 1) & hmask;
             uint64_t kj = hkey[j];
             if (kj == LDO_HASH_EMPTY) {

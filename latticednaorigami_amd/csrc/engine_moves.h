@@ -137,7 +137,7 @@ DEVNI double Eng::set_growth_point(int gnew, int gold) {
         S->rejected = 1;
     }
     else {
-        assigned[S->n_assigned++] = (int16_t)gnew;
+        push_assigned(gnew);
     }
     return de;
 }
@@ -171,7 +171,7 @@ DEVNI void Eng::grow_chain_seq(int first, int count, int dir) {
                 S->rejected = 1;
                 break;
             }
-            assigned[S->n_assigned++] = (int16_t)d;
+            push_assigned(d);
         }
         else {
             cb_select_and_set(d, prev_d);
@@ -187,6 +187,10 @@ DEVNI void Eng::grow_chain_seq(int first, int count, int dir) {
 DEVNI bool Eng::orientation_move() {
     bool accepted = false;
     int d = select_random_domain();
+    if (d < 0) {
+        fail(ST_MISUSE);
+        return false;
+    }
     int o_new = uniform_int(0, 5);
     int stt = dom[d].state;
     if (stt == OCC_BOUND || stt == OCC_MISBOUND) {
@@ -235,7 +239,7 @@ DEVNI bool Eng::insert_staple() {
     int gnew = slot(cs, gi);
     int gold = select_random_domain();
     int guard = 0;
-    while (cslot_of(gold) == cs) {
+    while (gold < 0 || cslot_of(gold) == cs) {
         gold = select_random_domain();
         if (++guard > (1 << 20) || S->status != ST_OK) {
             fail(ST_MISUSE);
@@ -280,7 +284,7 @@ DEVNI bool Eng::delete_staple() {
         prev[s].y = dom[s].y;
         prev[s].z = dom[s].z;
         prev[s].ore = dom[s].ore;
-        modified[S->n_modified++] = (int16_t)s;
+        push_modified(s);
         S->delta_e += unassign_domain(s);
     }
     S->delta_e -= init_e;
@@ -377,7 +381,7 @@ DEVNI void Eng::cb_select_and_set(int d, int prev_d) {
     else {
         set_checked_domain_config(d, i3(old[d].x, old[d].y, old[d].z), old[d].ore);
     }
-    assigned[S->n_assigned++] = (int16_t)d;
+    push_assigned(d);
 }
 
 // CBMCMovetype::unassign_domains (cb_movetypes.cpp:200-213) over the domains
@@ -389,7 +393,7 @@ DEVNI void Eng::cb_unassign_chain(int cs, int from, int to) {
         prev[s].y = dom[s].y;
         prev[s].z = dom[s].z;
         prev[s].ore = dom[s].ore;
-        modified[S->n_modified++] = (int16_t)s;
+        push_modified(s);
         unassign_domain(s);
     }
 }
@@ -424,7 +428,7 @@ DEVNI bool Eng::cb_test_acceptance() {
 DEVNI void Eng::cb_set_old_growth_point(int gnew, int gold) {
     double de = set_checked_domain_config(gnew, pos(gold), old[gnew].ore);
     S->bias *= exp(-de);
-    assigned[S->n_assigned++] = (int16_t)gnew;
+    push_assigned(gnew);
 }
 
 // CBStapleRegrowthMCMovetype::internal_attempt_move (cb_movetypes.cpp:292-348)
@@ -643,6 +647,10 @@ DEVNI void Eng::cp_scan_network(int d0) {
         }
         int k = f.idx++;
         int dd = (k < len - 1 - gd) ? slot(ci, gd + 1 + k) : slot(ci, gd - 1 - (k - (len - 1 - gd)));
+        if (n_netds >= nslots || n_gps >= nslots || n_iaes >= nslots) {
+            fail(ST_CAPACITY);
+            return;
+        }
         netds[n_netds++] = (int16_t)dd;
         netds_flag[dd] = 1;
         int b = dom[dd].bound;
@@ -976,7 +984,7 @@ DEVNI bool Eng::ctcb_scaffold_move(int max_regrowth) {
             prev[d].y = dom[d].y;
             prev[d].z = dom[d].z;
             prev[d].ore = dom[d].ore;
-            modified[S->n_modified++] = (int16_t)d;
+            push_modified(d);
             unassign_domain(d);
         }
         for (int k = 0; k < nst; k++) {

@@ -12,7 +12,13 @@ import pytest
 
 from latticednaorigami_amd import abi, systems
 
-pytestmark = pytest.mark.gpu
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skip(reason="the HIP engine's sweep kernel took an unexplained illegal-address "
+                            "fault in its first MI355X run (four-domain system, 330 K, "
+                            "lanes_per_replica=64); the engine stays off the GPU until a cause "
+                            "is shown"),
+]
 
 ENERGY_RTOL = 1e-12
 EXACT_KEYS = ["num_staples", "num_domains", "num_bound_domain_pairs",
