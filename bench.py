@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Throughput of the lattice-MC sampler: accepted MC moves/s, all replicas on one GPU.
+
+Workload (the PTMC raster system): a 64-domain HalfTurn scaffold on an 8x8
+boustrophedon raster in z=0, 32 two-domain staple types from a seeded domino
+tiling, random 16-nt ACGT domain sequences, nearest-neighbour hybridization,
+1 uM staples, 0.5 M cation, 32 replicas at 300-360 K (linear), temperature
+exchange every 1000 steps.  One bench step is one exchange interval: every
+replica runs `--interval` MC steps, then one even/odd exchange round.
+
+Prints ONE JSON line.  `value` is accepted moves/s summed over all replicas
+with the state resident in device memory; `roofline` prices the sweep
+kernel's algorithmic bytes (counted events x fixed bytes per event) against
+HBM peak, timed with HIP events on the sampler's stream; `cpu_baseline` is the
+CPU oracle (a restatement of the reference's algorithm) timed on this host's
+cores over a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "accepted MC moves/sec (all replicas, 1 GPU)"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per counted event: trial placement onto an empty site,
+# trial placement that evaluates binding/stacking, domain commit, move attempt
+BYTES_EMPTY, BYTES_BIND, BYTES_COMMIT, BYTES_MOVE = 24, 216, 32, 16
+
+
+def algorithmic_bytes(c0, c1):
+    n_empty = c1.site_empty - c0.site_empty
+    n_bind = c1.site_bind - c0.site_bind
+    n_commit = c1.commits - c0.commits
+    n_move = sum(c1.attempts) - sum(c0.attempts)
+    return (BYTES_EMPTY * n_empty + BYTES_BIND * n_bind + BYTES_COMMIT * n_commit
+            + BYTES_MOVE * n_move)
+
+
+def _cpu_worker(args):
+    """One process of the CPU baseline: the oracle on the same workload."""
+    swaps, interval = args
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from latticednaorigami_amd import systems
+    system = systems.raster_system()
+    ref = pyoracle.Oracle(system, temps=system.temps)
+    c0 = ref.counters()
+    t0 = time.perf_counter()
+    ref.run(swaps, interval)
+    dt = time.perf_counter() - t0
+    c1 = ref.counters()
+    return sum(c1.accepts) - sum(c0.accepts), sum(c1.attempts) - sum(c0.attempts), dt
+
+
+def cpu_baseline(workers, swaps, interval):
+    from concurrent.futures import ProcessPoolExecutor
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    with ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as ex:
+        # every worker builds its system outside its own timed region; the
+        # aggregate is the sum of the concurrent per-process rates
+        res = list(ex.map(_cpu_worker, [(swaps, interval)] * workers))
+    accepted = sum(r[0] for r in res)
+    rate = sum(r[0] / r[2] for r in res)
+    longest = max(r[2] for r in res)
+    return {
+        "value": rate,
+        "unit": "moves/s",
+        "cores": workers,
+        "kind": "port",
+        "sample": (f"{workers} concurrent processes, each the 32-replica PTMC raster workload "
+                   f"for {swaps} exchange interval(s) of {interval} steps per replica from the "
+                   f"initial configuration ({accepted} accepted moves, longest timed region "
+                   f"{longest:.1f} s)"),
+    }
+
+
+def gpu_leg(args, system):
+    from latticednaorigami_amd import abi
+    from latticednaorigami_amd.sampler import Sampler
+    smp = Sampler(system, temps=system.temps, exchange_mode=abi.EXCHANGE_TEMPERATURE)
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except ImportError:
+        sync = lambda: None  # noqa: E731
+    if args.warmup > 0:
+        smp.run(args.warmup, args.interval)
+    c0 = smp.counters()
+    smp.set_profiling(True)
+    sync()
+    t0 = time.perf_counter()
+    smp.run(args.steps, args.interval)
+    sync()
+    wall = time.perf_counter() - t0
+    prof = smp.profile()
+    smp.set_profiling(False)
+    c1 = smp.counters()
+    accepted = sum(c1.accepts) - sum(c0.accepts)
+    attempted = sum(c1.attempts) - sum(c0.attempts)
+    nbytes = algorithmic_bytes(c0, c1)
+    kernel_s = prof["sweep_ms"] / 1e3
+    achieved = nbytes / kernel_s / 1e9 if kernel_s > 0 else None
+    roofline = {
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS if achieved is not None else None,
+        "traffic": None,
+        "kernel": smp.lib.ldo_kernel_name().decode(),
+        "launches": prof["sweep_launches"],
+        "avg_launch_ms": prof["sweep_ms"] / max(1, prof["sweep_launches"]),
+        "algorithmic_bytes_per_launch": nbytes / max(1, prof["sweep_launches"]),
+    }
+    extra = {
+        "attempted_per_s": attempted / wall,
+        "accepts_by_move": list(c1.accepts)[:4],
+        "attempts_by_move": list(c1.attempts)[:4],
+        "exchange_kernel_ms": prof["exchange_ms"],
+    }
+    smp.close()
+    return accepted / wall, wall, roofline, extra
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed exchange intervals")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed exchange intervals")
+    ap.add_argument("--interval", type=int, default=1000, help="MC steps per replica per step")
+    ap.add_argument("--cpu-steps", type=int, default=500,
+                    help="MC steps per replica in each CPU-baseline process")
+    ap.add_argument("--cpu-workers", type=int, default=0,
+                    help="CPU-baseline processes (0: min(8, host cores))")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every replica of the run is resident on a single "
+              f"device, so --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    from latticednaorigami_amd import systems
+    system = systems.raster_system()
+    line = {
+        "metric": METRIC,
+        "value": None,
+        "unit": "moves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "raster64_ptmc32",
+            "scaffold_domains": 64,
+            "staple_types": 32,
+            "replicas": 32,
+            "temps_K": [300.0, 360.0],
+            "exchange_interval": args.interval,
+            "hybridization": "NearestNeighbour",
+            "step": "one exchange interval on every replica",
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    status = 0
+    try:
+        value, wall, roofline, extra = gpu_leg(args, system)
+        line["value"] = value
+        line["ms_per_step"] = wall / args.steps * 1e3
+        line["roofline"] = roofline
+        line["config"].update(extra)
+    except Exception as e:  # the engine refused or failed: report, do not fake a number
+        line["error"] = f"{type(e).__name__}: {e}"
+        status = 1
+    if not args.no_cpu_baseline:
+        workers = args.cpu_workers or min(8, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(workers, 1, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+    sys.exit(status)
+
+
+if __name__ == "__main__":
+    main()

@@ -221,6 +221,16 @@ void* ldo_get_stream(void* ctx);
 /* Name of the sampler kernel, for matching profiler output. */
 const char* ldo_kernel_name(void);
 
+/* Kernel timing with HIP events recorded on the sampler's stream around
+ * every sweep-kernel and exchange-kernel launch of ldo_sweep / ldo_run.
+ * No reference counterpart: it serves throughput and roofline measurement.
+ * ldo_set_profiling(ctx, 1) starts a fresh tally, 0 stops recording.
+ * ldo_get_profile returns the summed kernel milliseconds and launch counts
+ * since the tally started (reading synchronizes the stream). */
+int ldo_set_profiling(void* ctx, int32_t on);
+int ldo_get_profile(void* ctx, double* sweep_ms, int64_t* sweep_launches,
+                    double* exchange_ms, int64_t* exchange_launches);
+
 #ifdef __cplusplus
 }
 #endif

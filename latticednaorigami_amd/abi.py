@@ -187,6 +187,11 @@ def load_library(path=None):
     lib.ldo_get_stream.argtypes = [C.c_void_p]
     lib.ldo_kernel_name.restype = C.c_char_p
     lib.ldo_kernel_name.argtypes = []
+    lib.ldo_set_profiling.restype = C.c_int
+    lib.ldo_set_profiling.argtypes = [C.c_void_p, C.c_int32]
+    lib.ldo_get_profile.restype = C.c_int
+    lib.ldo_get_profile.argtypes = [C.c_void_p, _f64p, C.POINTER(C.c_int64), _f64p,
+                                    C.POINTER(C.c_int64)]
     if path is None:
         _LIB = lib
     return lib

@@ -258,6 +258,17 @@ struct Ctx {
     hipStream_t stream = nullptr;
     int64_t swap_counter = 0;
     std::string err;
+    // kernel timing (ldo_set_profiling): event pairs awaiting readout and a
+    // pool of spent events for reuse
+    struct EvPair {
+        hipEvent_t start, stop;
+        int kind; // 0 sweep, 1 exchange
+    };
+    bool profiling = false;
+    std::vector<EvPair> pending;
+    std::vector<hipEvent_t> ev_pool;
+    double prof_ms[2] = {0, 0};
+    int64_t prof_n[2] = {0, 0};
     // device buffers
     char* d_blocks = nullptr;
     int32_t* d_type_len = nullptr;
@@ -306,6 +317,52 @@ void upload(Ctx& c, void* dst, const void* src, size_t bytes, const char* what) 
 
 void zero(Ctx& c, void* dst, size_t bytes, const char* what) {
     hip_check(hipMemsetAsync(dst, 0, bytes, c.stream), what);
+}
+
+// Reads out every recorded event pair into the running totals.
+void drain_profile(Ctx& c) {
+    if (c.pending.empty()) {
+        return;
+    }
+    hip_check(hipStreamSynchronize(c.stream), "sync profile");
+    for (const Ctx::EvPair& p: c.pending) {
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, p.start, p.stop), "event time");
+        c.prof_ms[p.kind] += ms;
+        c.prof_n[p.kind] += 1;
+        c.ev_pool.push_back(p.start);
+        c.ev_pool.push_back(p.stop);
+    }
+    c.pending.clear();
+}
+
+hipEvent_t take_event(Ctx& c) {
+    if (!c.ev_pool.empty()) {
+        hipEvent_t e = c.ev_pool.back();
+        c.ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "event create");
+    return e;
+}
+
+// Launches through `launch`, bracketed by events on the sampler's stream
+// when profiling is on.
+template <typename F>
+void launch_timed(Ctx& c, int kind, F launch) {
+    if (!c.profiling) {
+        launch();
+        return;
+    }
+    hipEvent_t a = take_event(c), b = take_event(c);
+    hip_check(hipEventRecord(a, c.stream), "event record");
+    launch();
+    hip_check(hipEventRecord(b, c.stream), "event record");
+    c.pending.push_back({a, b, kind});
+    if (c.pending.size() >= 4096) {
+        drain_profile(c);
+    }
 }
 
 template <typename T>
@@ -562,6 +619,13 @@ bool engine_enabled() {
 void free_ctx(Ctx* c) {
     if (c->stream) {
         (void)hipStreamSynchronize(c->stream);
+    }
+    for (const Ctx::EvPair& p: c->pending) {
+        c->ev_pool.push_back(p.start);
+        c->ev_pool.push_back(p.stop);
+    }
+    for (hipEvent_t e: c->ev_pool) {
+        (void)hipEventDestroy(e);
     }
     void* bufs[] = {c->d_blocks, c->d_type_len, c->d_type_ident, c->d_tab, c->d_init,
                     c->d_tapes, c->d_tape_meta, c->d_xint, c->d_xdbl, c->d_xcnt, c->d_cfg};
@@ -901,8 +965,10 @@ int ldo_sweep(void* ctx, int64_t steps) {
         if (steps < 0) {
             throw AbiError {LDO_E_ARG, "negative step count"};
         }
-        hipLaunchKernelGGL(ldo_sweep_kernel, grid_of(*c), dim3(64), 0, c->stream, c->P, steps,
-                           (const int32_t*)nullptr);
+        launch_timed(*c, 0, [&]() {
+            hipLaunchKernelGGL(ldo_sweep_kernel, grid_of(*c), dim3(64), 0, c->stream, c->P,
+                               steps, (const int32_t*)nullptr);
+        });
         hip_check(hipGetLastError(), "launch sweep");
         check_statuses(*c);
     });
@@ -924,11 +990,15 @@ int ldo_run(void* ctx, int64_t n_swaps, int64_t interval) {
         X.counts = c->d_xcnt;
         for (int64_t s = 0; s < n_swaps; s++) {
             int64_t swap_i = ++c->swap_counter;
-            hipLaunchKernelGGL(ldo_sweep_kernel, grid_of(*c), dim3(64), 0, c->stream, c->P,
-                               interval, (const int32_t*)X.rep_slot);
+            launch_timed(*c, 0, [&]() {
+                hipLaunchKernelGGL(ldo_sweep_kernel, grid_of(*c), dim3(64), 0, c->stream, c->P,
+                                   interval, (const int32_t*)X.rep_slot);
+            });
             hip_check(hipGetLastError(), "launch sweep");
-            hipLaunchKernelGGL(ldo_exchange_kernel, dim3(1), dim3(64), 0, c->stream, c->P, X,
-                               swap_i);
+            launch_timed(*c, 1, [&]() {
+                hipLaunchKernelGGL(ldo_exchange_kernel, dim3(1), dim3(64), 0, c->stream, c->P, X,
+                                   swap_i);
+            });
             hip_check(hipGetLastError(), "launch exchange");
         }
         check_statuses(*c);
@@ -1006,5 +1076,29 @@ int ldo_get_permutation(void* ctx, int32_t* q_to_rep) {
 void* ldo_get_stream(void* ctx) { return ctx ? (void*)static_cast<Ctx*>(ctx)->stream : nullptr; }
 
 const char* ldo_kernel_name(void) { return "ldo_sweep_kernel"; }
+
+int ldo_set_profiling(void* ctx, int32_t on) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        drain_profile(*c);
+        c->profiling = on != 0;
+        if (c->profiling) {
+            c->prof_ms[0] = c->prof_ms[1] = 0;
+            c->prof_n[0] = c->prof_n[1] = 0;
+        }
+    });
+}
+
+int ldo_get_profile(void* ctx, double* sweep_ms, int64_t* sweep_launches, double* exchange_ms,
+                    int64_t* exchange_launches) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        drain_profile(*c);
+        *sweep_ms = c->prof_ms[0];
+        *sweep_launches = c->prof_n[0];
+        *exchange_ms = c->prof_ms[1];
+        *exchange_launches = c->prof_n[1];
+    });
+}
 
 } // extern "C"

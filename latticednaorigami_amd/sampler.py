@@ -87,3 +87,16 @@ class Sampler:
 
     def stream(self):
         return self.lib.ldo_get_stream(self.h)
+
+    def set_profiling(self, on=True):
+        """Record HIP events around every sweep and exchange kernel launch."""
+        self._check(self.lib.ldo_set_profiling(self.h, int(on)))
+
+    def profile(self):
+        """Summed kernel times (ms) and launch counts since profiling started."""
+        sm, en = C.c_double(), C.c_double()
+        sn, xn = C.c_int64(), C.c_int64()
+        self._check(self.lib.ldo_get_profile(self.h, C.byref(sm), C.byref(sn), C.byref(en),
+                                             C.byref(xn)))
+        return {"sweep_ms": sm.value, "sweep_launches": sn.value,
+                "exchange_ms": en.value, "exchange_launches": xn.value}
