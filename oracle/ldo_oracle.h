@@ -76,12 +76,16 @@ int oracle_rotate(const int32_t* v, const int32_t* axis, int32_t turns,
 /* Philox4x32-10: ctr[4], key[2] -> out[4] */
 void oracle_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
 
-/* Exact enumeration (enumerate.cpp:19-81, 639-664) for small systems at
- * temps[0]: fills probabilities of the joint state
- * (numstaples, numfulldomains, nummisdomains, numstackedpairs) into a dense
- * array out[ns][nf][nm][nst] with the given extents. */
-int oracle_enumerate(void* o, int32_t max_ns, int32_t max_nf, int32_t max_nm,
-                     int32_t max_nst, double* out, double* out_log_z);
+/* Exact enumeration of the grand-canonical ensemble at the first ladder
+ * temperature under the run's staple limits, with the weights of
+ * enumerate.cpp:257-275, 480-520, 639-664.  Writes rows of five doubles
+ * (numstaples, numfulldomains, nummisdomains, numstackedpairs, probability)
+ * into out_rows (capacity cap_rows rows), the row count into n_rows, ln Z and
+ * the number of configurations counted.  flags bit 0: enumerate every
+ * orientation explicitly instead of counting the orientations of unbound
+ * and misbound domains as a factor of 6. */
+int oracle_enumerate(void* o, int32_t flags, double* out_rows, int32_t cap_rows,
+                     int32_t* n_rows, double* out_log_z, double* out_n_configs);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,7 @@ struct Oracle {
     Rng exchange_rng;
     int64_t swap_counter {0};
     ldo_counters counters {};
+    Limits lim {0, 0};
     std::string err;
 };
 
@@ -80,6 +81,7 @@ void* oracle_create(const ldo_system_desc* sd, const ldo_run_params* run,
                              run->movetype_max_regrowth ? run->movetype_max_regrowth[m] : 0});
         }
         Limits lim {run->max_total_staples, run->max_type_staples};
+        o->lim = lim;
         std::vector<ChainSpec> cfg = specs_from(init, o->idents);
         o->rng_kind = rng_kind;
         o->seed = run->seed;
@@ -402,6 +404,28 @@ int oracle_rotate(const int32_t* v, const int32_t* axis, int32_t turns, int32_t*
 
 void oracle_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
     philox4x32_10(ctr, key, out);
+}
+
+int oracle_enumerate(void* o, int32_t flags, double* out_rows, int32_t cap_rows, int32_t* n_rows,
+                     double* out_log_z, double* out_n_configs) {
+    ORACLE_TRY(o, {
+        EnumResult r = enumerate_states(O->sys, O->idents, O->seqs, O->lim, O->ladder_t.at(0),
+                                        (flags & 1) != 0);
+        if ((int64_t)r.weights.size() > cap_rows) {
+            throw OrigamiError(LDO_E_ARG, "row capacity too small for the enumerated states");
+        }
+        int k = 0;
+        for (const auto& kv: r.weights) {
+            for (int q = 0; q < 4; q++) {
+                out_rows[5 * k + q] = kv.first[q];
+            }
+            out_rows[5 * k + 4] = (double)(kv.second / r.z);
+            k++;
+        }
+        *n_rows = k;
+        *out_log_z = (double)std::log(r.z);
+        *out_n_configs = (double)r.n_configs;
+    })
 }
 
 } // extern "C"

@@ -10,6 +10,7 @@
 #define ORACLE_CORE_HPP
 
 #include <cstdint>
+#include <array>
 #include <map>
 #include <random>
 #include <set>
@@ -403,6 +404,20 @@ class Replica {
     void ctcb_grow_chain(const std::vector<Dom*>& ds);
     void ctcb_grow_staple_and_update_endpoints(Dom* gold);
 };
+
+// ------------------------------------------------------------ enumeration
+// Exact enumeration of a small system's grand-canonical ensemble
+// (oracle_enum.cpp).  Weights are keyed by (numstaples, numfulldomains,
+// nummisdomains, numstackedpairs).
+struct EnumResult {
+    std::map<std::array<int, 4>, long double> weights;
+    long double z {0};
+    long double n_configs {0};
+};
+
+EnumResult enumerate_states(const ldo_system_desc& desc, const std::vector<std::vector<int>>& idents,
+                            const std::vector<std::vector<std::string>>& seqs, const Limits& lim,
+                            double temp, bool all_orientations);
 
 } // namespace orc
 

@@ -53,8 +53,9 @@ def lib():
         L.oracle_rotate.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32,
                                     C.POINTER(C.c_int32)]
         L.oracle_philox.argtypes = [C.POINTER(C.c_uint32)] * 3
-        L.oracle_enumerate.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
-                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.oracle_enumerate.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_int32,
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -125,6 +126,20 @@ class Oracle:
         cfg, keep = empty_config(cap_c, cap_d)
         _check(self.L, self.h, self.L.oracle_get_config(self.h, r, C.byref(cfg), cap_c, cap_d))
         return config_from_abi(cfg)
+
+    def enumerate(self, all_orientations=False, cap_rows=4096):
+        """Exact state probabilities {(numstaples, numfulldomains, nummisdomains,
+        numstackedpairs): p} at the first ladder temperature, with ln Z and the
+        number of configurations counted."""
+        rows = (C.c_double * (5 * cap_rows))()
+        n = C.c_int32()
+        lz, nc = C.c_double(), C.c_double()
+        _check(self.L, self.h, self.L.oracle_enumerate(self.h, int(all_orientations), rows,
+                                                       cap_rows, C.byref(n), C.byref(lz),
+                                                       C.byref(nc)))
+        probs = {tuple(int(rows[5 * k + q]) for q in range(4)): rows[5 * k + 4]
+                 for k in range(n.value)}
+        return probs, lz.value, nc.value
 
     def record_tape(self, r, on=True):
         _check(self.L, self.h, self.L.oracle_record_tape(self.h, r, int(on)))
