@@ -236,6 +236,9 @@ int oracle_get_counters(void* o, ldo_counters* out) {
                 c.attempts[m] += r->attempts[m];
                 c.accepts[m] += r->accepts[m];
             }
+            c.site_empty += r->sys.ev_site_empty;
+            c.site_bind += r->sys.ev_site_bind;
+            c.commits += r->sys.ev_commits;
         }
         *out = c;
     })

@@ -1137,6 +1137,7 @@ Delta System::internal_check(Dom& di, V3 pos, V3 ore) {
         di.pos = pos;
         di.ore = ore;
         update_occupancies(di, pos);
+        ev_site_bind++;
         dc = pot.bind_domain(di);
         violated = pot.violated;
         break;
@@ -1144,6 +1145,7 @@ Delta System::internal_check(Dom& di, V3 pos, V3 ore) {
         di.pos = pos;
         di.ore = ore;
         update_occupancies(di, pos);
+        ev_site_empty++;
         break;
     }
     return dc;
@@ -1240,6 +1242,10 @@ double System::check_domain_constraints(Dom& di, V3 pos, V3 ore) {
 }
 
 double System::unassign_domain(Dom& di) {
+    if (di.st == Occ::bound) {
+        ev_site_bind++; // the stacking walk of a committed unset
+    }
+    ev_commits++;
     Delta dc = internal_unassign(di);
     energy += dc.e;
     num_stacked += dc.stacked;
@@ -1257,12 +1263,14 @@ double System::set_checked_domain_config(Dom& di, V3 pos, V3 ore) {
     }
     else if (di.st == Occ::bound) {
         de += pot.hyb_e(di, *di.bd);
+        ev_site_bind++;
         Delta dc = pot.check_stacking(di, *di.bd);
         de += dc.e;
         num_stacked += dc.stacked;
     }
     energy += de;
     num_unassigned--;
+    ev_commits++;
     return de;
 }
 
@@ -1278,6 +1286,7 @@ double System::set_domain_config(Dom& di, V3 pos, V3 ore) {
         energy += dc.e;
         num_stacked += dc.stacked;
         num_unassigned--;
+        ev_commits++;
     }
     return dc.e;
 }

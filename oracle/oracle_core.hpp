@@ -231,6 +231,11 @@ class System {
     double stacking_energy {0};
     bool violated {false};
     int current_c_i {0};
+    // counted events of the throughput byte model: trial placements onto an
+    // empty site, binding/stacking evaluations, committed sets and unsets
+    int64_t ev_site_empty {0};
+    int64_t ev_site_bind {0};
+    int64_t ev_commits {0};
 
     Occ occupancy(V3 pos) const;
     Dom* unbound_at(V3 pos) const { return pos_to_unbound.at(pos); }
