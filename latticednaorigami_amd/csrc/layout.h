@@ -126,4 +126,57 @@ struct Layout {
     int64_t bytes; // block size (multiple of 256)
 };
 
+// Kernel-wide constants (one copy, passed by value to every launch).
+struct Params {
+    Layout L;
+    int32_t n_replicas;
+    int32_t domain_type;
+    int32_t misbinding_pot;
+    int32_t cyclic;
+    int32_t n_moves;
+    int32_t move_kind[LDO_MAX_MOVES];
+    double move_cum[LDO_MAX_MOVES];
+    int32_t move_maxreg[LDO_MAX_MOVES];
+    int32_t max_total_staples;
+    int32_t max_type_staples;
+    int32_t centering_freq;
+    int32_t centering_domain;
+    int32_t check_freq;
+    int32_t rng_mode;
+    uint64_t seed;
+    int32_t ident_lo;
+    int32_t n_ident;
+    int32_t n_slots; // ladder slots with tables
+    int32_t lanes_per_replica;
+    double fugacity; // reduced fugacity = staple_M (never updated)
+    const int32_t* type_len;   // [ntypes]
+    const int16_t* type_ident; // [ntypes * type_ident_stride] table index of each domain
+    int32_t type_ident_stride;
+    const double* hyb_e;       // [n_slots * n_ident^2]
+    const double* hyb_h;
+    const double* hyb_s;
+    const double* stack;
+    const double* init_e; // [n_slots]
+    const double* init_h;
+    const double* init_s;
+    char* blocks;
+    const uint64_t* tapes;    // concatenated tapes
+    const int64_t* tape_off;  // [n_replicas]
+    const int64_t* tape_len;  // [n_replicas]
+};
+
+// move kinds held in RepScalars::kind (the grow_chain dispatch)
+#define LDO_MOVE_KIND_ORIENT 0
+#define LDO_MOVE_KIND_MET 1
+#define LDO_MOVE_KIND_CB 2
+#define LDO_MOVE_KIND_CTCB 3
+
+// status codes written into RepScalars::status
+#define ST_OK 0
+#define ST_CONSTRAINT 1
+#define ST_MISUSE 2
+#define ST_CAPACITY 7
+#define ST_TAPE 8
+#define ST_COORD 9
+
 #endif

@@ -1,0 +1,68 @@
+// plan.h - host-side preparation of a sampler run.
+//
+// Everything the library derives from the C-ABI inputs before it touches a
+// device: validation, the replica-block layout, the launch parameters, the
+// energy tables of every ladder slot, per-slot identity tables, the packed
+// initial configuration, and the reading of replica blocks back into ABI
+// structs.  Plain C++ (no HIP), shared by sampler.hip and the host test
+// build of the engine.
+#ifndef LDO_PLAN_H
+#define LDO_PLAN_H
+
+#include <string>
+#include <vector>
+
+#include "../../include/ldo.h"
+#include "layout.h"
+#include "tables.h"
+
+namespace ldo {
+
+// an ABI call rejected with a status code and message
+struct AbiError {
+    int code;
+    std::string msg;
+};
+
+Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap);
+
+struct Plan {
+    SystemSpec spec;
+    Params P {};  // scalars and layout; the device pointers are the caller's
+    int R = 0;
+    int ntypes = 0;
+    std::vector<int32_t> type_len;   // [ntypes]
+    std::vector<int16_t> type_ident; // [ntypes * P.type_ident_stride] table index
+    std::vector<double> ladder_t, ladder_u, ladder_b, ladder_s;
+    std::vector<double> reduced_u;   // [R * (ntypes - 1)]
+};
+
+// Validates the inputs of ldo_create and fills `out`; throws AbiError.
+void make_plan(const ldo_system_desc* sys, const ldo_run_params* run, int32_t n_replicas,
+               const double* temps, Plan& out);
+
+// Dense energy tables of every ladder slot: tab = hyb_e | hyb_h | hyb_s |
+// stack, each [R][n_ident^2]; init = init_e | init_h | init_s, each [R].
+void slot_tables(const Plan& p, std::vector<double>& tab, std::vector<double>& init);
+
+// ladder_t | ladder_u | ladder_s | reduced_u, as the exchange reads them
+std::vector<double> ladder_values(const Plan& p);
+
+// Validates a configuration and packs it as chain_index | chain_identity |
+// positions | orientations (three ints per domain).
+void pack_config(const Plan& p, const ldo_config* cfg, std::vector<int32_t>& packed,
+                 int& n_domains);
+
+// Reads the configuration held in a host copy of a replica block.
+void read_config(const Plan& p, const char* block, ldo_config* cfg, int32_t cap_chains,
+                 int32_t cap_domains);
+
+void fill_observables(const Plan& p, const RepScalars& S, ldo_obs* out);
+void add_counters(const RepScalars& S, ldo_counters* out);
+
+// Throws the ABI error that a replica's status word stands for.
+void check_status(int32_t status, int replica);
+
+} // namespace ldo
+
+#endif

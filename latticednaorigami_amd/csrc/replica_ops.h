@@ -1,0 +1,200 @@
+// replica_ops.h - the per-replica work of the sampler's kernels.
+//
+// Each function is the body a kernel runs for one replica (or, for the
+// exchange, for the whole ladder).  They compile for the device, where the
+// kernels in sampler.hip call them one replica per lane or per wavefront,
+// and for the host, where a test build drives the same code replica by
+// replica against the CPU oracle.
+#ifndef LDO_REPLICA_OPS_H
+#define LDO_REPLICA_OPS_H
+
+#include "engine.h"
+#include "engine_fourbody.h"
+#include "engine_moves.h"
+#include "engine_system.h"
+#include "layout.h"
+
+// Build (mode 0: OrigamiSystem constructor) or replace (mode 1: set_config)
+// replica r's configuration.  c_idx/c_ident hold n_chains unique indices and
+// identities, c_pos/c_ore three ints per domain in chain order.
+DEV void init_replica(const Params& P, int r, const int32_t* c_idx, const int32_t* c_ident,
+                      const int32_t* c_pos, const int32_t* c_ore, int n_chains, int mode,
+                      const int32_t* rep_slot) {
+    Eng e;
+    char* blk = P.blocks + (size_t)r * P.L.bytes;
+    RepScalars* S = (RepScalars*)blk;
+    if (mode == 0) {
+        S->temp_slot = rep_slot ? rep_slot[r] : r;
+        S->init_slot = S->temp_slot;
+        S->visited = 0;
+        S->status = ST_OK;
+        e.attach(&P, blk, r);
+        for (int i = 0; i < P.L.hcap; i++) {
+            e.hkey[i] = LDO_HASH_EMPTY;
+        }
+        for (int c = 0; c < P.L.nchain; c++) {
+            e.ch[c].ident = -1;
+            e.ch[c].len = 0;
+        }
+        for (int t = 0; t < P.L.ntypes; t++) {
+            e.tcount[t] = 0;
+        }
+        S->n_free = P.L.smax;
+        for (int i = 0; i < P.L.smax; i++) {
+            e.freel[i] = (int16_t)(P.L.smax - i);
+        }
+        // initialize_scaffold (origami_system.cpp:680-693)
+        e.add_chain_with(0, c_idx[0]);
+        S->num_staples--;
+    }
+    else {
+        e.attach(&P, blk, r);
+        // set_config (origami_system.cpp:327-341)
+        for (int w = 0; w < S->n_chains; w++) {
+            int cs = e.order[w];
+            for (int d = 0; d < e.ch[cs].len; d++) {
+                e.unassign_domain(e.slot(cs, d));
+            }
+        }
+        for (int w = S->n_chains - 1; w != 0; w--) {
+            e.delete_chain(e.order[w]);
+        }
+    }
+    // initialize_staples (origami_system.cpp:659-678)
+    int maxidx = c_idx[0];
+    for (int c = 1; c < n_chains; c++) {
+        e.add_chain_with(c_ident[c], c_idx[c]);
+        maxidx = c_idx[c] > maxidx ? c_idx[c] : maxidx;
+    }
+    S->energy += (double)(S->n_chains - 1) * e.init_e;
+    S->current_c_i = maxidx;
+    // set_all_domains(config) (origami_system.cpp:588-616)
+    int k = 0;
+    for (int w = 0; w < S->n_chains; w++) {
+        int cs = e.order[w];
+        for (int d = 0; d < e.ch[cs].len; d++, k++) {
+            int s = e.slot(cs, d);
+            e.dom[s].x = c_pos[3 * k];
+            e.dom[s].y = c_pos[3 * k + 1];
+            e.dom[s].z = c_pos[3 * k + 2];
+            I3 o = i3(c_ore[3 * k], c_ore[3 * k + 1], c_ore[3 * k + 2]);
+            e.dom[s].ore = (int8_t)unit_index(o);
+        }
+    }
+    e.set_all_domains();
+}
+
+// `steps` steps of GCMCSimulation::simulate (simulation.cpp:568-653); with a
+// ladder (rep_slot) the replica first takes its slot's control variables
+// (update_control_qs) and ends with its dependent quantities.
+DEV void sweep_replica(const Params& P, int r, int64_t steps, const int32_t* rep_slot) {
+    Eng e;
+    e.attach(&P, P.blocks + (size_t)r * P.L.bytes, r);
+    if (e.S->status != ST_OK) {
+        return;
+    }
+    if (rep_slot != nullptr) {
+        e.begin_interval(rep_slot[r]);
+    }
+    e.simulate(steps);
+    if (rep_slot != nullptr) {
+        e.update_enthalpy_and_entropy();
+    }
+}
+
+// OrigamiSystem::update_enthalpy_and_entropy (origami_system.cpp:204-246)
+DEV void dependent_replica(const Params& P, int r) {
+    Eng e;
+    e.attach(&P, P.blocks + (size_t)r * P.L.bytes, r);
+    e.update_enthalpy_and_entropy();
+}
+
+DEV int32_t replica_status(const Params& P, int r) {
+    return ((const RepScalars*)(P.blocks + (size_t)r * P.L.bytes))->status;
+}
+
+// Exchange state on the device
+struct XState {
+    int32_t* q_to_rep;
+    int32_t* rep_slot;
+    const double* ladder_t;
+    const double* ladder_u;
+    const double* ladder_s;
+    const double* reduced_u; // [n_replicas * (ntypes-1)]
+    int64_t* counts;         // attempts, accepts
+};
+
+// The exchange stream: Philox4x32-10 keyed by (seed, 0xFFFFFFFF), counter
+// (draw, 0, swap index); or the recorded draws of replica 0's tape.
+DEV double exchange_uniform(const Params& P, int64_t swap_i, uint32_t& idx) {
+    if (P.rng_mode == 1) {
+        RepScalars* S0 = (RepScalars*)P.blocks;
+        int64_t pos = S0->tape_pos;
+        if (pos >= P.tape_len[0]) {
+            S0->status = ST_TAPE;
+            return 0.0;
+        }
+        S0->tape_pos = pos + 1;
+        return bits_to_double(P.tapes[P.tape_off[0] + pos]);
+    }
+    uint32_t c0 = idx++, c1 = 0u, c2 = (uint32_t)(uint64_t)swap_i,
+             c3 = (uint32_t)((uint64_t)swap_i >> 32);
+    uint32_t k0 = (uint32_t)P.seed, k1 = 0xFFFFFFFFu;
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0;
+        c1 = n1;
+        c2 = n2;
+        c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    uint64_t x = (uint64_t)c0 | ((uint64_t)c1 << 32);
+    return (double)(x >> 11) * 0x1.0p-53;
+}
+
+// OneDPTGCMCSimulation::attempt_exchange + calc_acceptance_p
+// (ptmc_simulation.cpp:255-313, 360-412).  The per-staple chemical
+// potential term runs over the staple types.
+DEV void exchange_round(const Params& P, const XState& X, int64_t swap_i) {
+    int R = P.n_replicas;
+    int nst = P.L.ntypes - 1;
+    uint32_t idx = 0;
+    for (int i = (int)(swap_i % 2); i < R - 1; i += 2) {
+        X.counts[0]++;
+        int r1 = X.q_to_rep[i], r2 = X.q_to_rep[i + 1];
+        RepScalars* S1 = (RepScalars*)(P.blocks + (size_t)r1 * P.L.bytes);
+        RepScalars* S2 = (RepScalars*)(P.blocks + (size_t)r2 * P.L.bytes);
+        const int16_t* tc1 = (const int16_t*)((char*)S1 + P.L.off_tcount);
+        const int16_t* tc2 = (const int16_t*)((char*)S2 + P.L.off_tcount);
+        double t1 = X.ladder_t[i], t2 = X.ladder_t[i + 1];
+        double sm1 = X.ladder_s[i], sm2 = X.ladder_s[i + 1];
+        double dbu_dn = 0;
+        for (int k = 0; k < nst; k++) {
+            double u1 = X.reduced_u[(size_t)r1 * nst + k] * t1 * X.ladder_u[i];
+            double u2 = X.reduced_u[(size_t)r2 * nst + k] * t2 * X.ladder_u[i + 1];
+            dbu_dn += (u2 / t2 - u1 / t1) * ((double)tc2[k + 1] - (double)tc1[k + 1]);
+        }
+        double DB = 1 / t2 - 1 / t1;
+        double DH = S2->hyb_h * t2 - S1->hyb_h * t1;
+        double Dst = S2->stack_e * t2 - S1->stack_e * t1;
+        double DBM = sm2 / t2 - sm1 / t1;
+        double DBias = 0.0 * t2 - 0.0 * t1;
+        double p = fmin(1.0, exp(DB * (DH + DBias) + DBM * Dst - dbu_dn));
+        bool accept = (p == 1) ? true : (p > exchange_uniform(P, swap_i, idx));
+        if (accept) {
+            X.counts[1]++;
+            X.q_to_rep[i] = r2;
+            X.q_to_rep[i + 1] = r1;
+            X.rep_slot[r2] = i;
+            X.rep_slot[r1] = i + 1;
+        }
+    }
+}
+
+#endif

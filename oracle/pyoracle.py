@@ -141,6 +141,12 @@ class Oracle:
                  for k in range(n.value)}
         return probs, lz.value, nc.value
 
+    def set_config(self, r, chains):
+        from latticednaorigami_amd.systems import config_to_abi
+        keep = []
+        cfg = config_to_abi(chains, keep)
+        _check(self.L, self.h, self.L.oracle_set_config(self.h, r, C.byref(cfg)))
+
     def record_tape(self, r, on=True):
         _check(self.L, self.h, self.L.oracle_record_tape(self.h, r, int(on)))
 

@@ -1,0 +1,277 @@
+// engine_host.cpp - host build of the HIP sampler's engine, for tests.
+//
+// TEST INFRASTRUCTURE ONLY.  The kernels of libldo_hip.so run the
+// __host__ __device__ functions of latticednaorigami_amd/csrc/replica_ops.h
+// one replica per lane or wavefront.  This library compiles the same
+// functions for the host (hipcc --cuda-host-only) and calls them replica by
+// replica over host memory, with the run preparation of plan.cpp, so the CPU
+// test suite can check the engine's own source against the oracle step for
+// step.  It exports eh_* functions with the signatures of the ldo_* ABI; it
+// is not a fallback of the product, which never loads it.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ldo.h"
+#include "../../latticednaorigami_amd/csrc/plan.h"
+#include "../../latticednaorigami_amd/csrc/replica_ops.h"
+
+namespace {
+
+std::string g_create_error;
+
+struct HostCtx {
+    ldo::Plan plan;
+    Params P {};
+    int R = 0;
+    int64_t swap_counter = 0;
+    std::string err;
+    char* blocks = nullptr;
+    std::vector<double> tab, init, xdbl;
+    std::vector<int32_t> xint; // q_to_rep | rep_slot
+    std::vector<int64_t> xcnt;
+    std::vector<uint64_t> tapes_all;
+    std::vector<int64_t> tape_meta;
+    std::vector<std::vector<uint64_t>> tapes;
+    ~HostCtx() { std::free(blocks); }
+};
+
+void set_tables(HostCtx& c) {
+    int n = c.R;
+    size_t n2 = (size_t)c.plan.spec.n_ident * c.plan.spec.n_ident;
+    ldo::slot_tables(c.plan, c.tab, c.init);
+    c.P.hyb_e = c.tab.data();
+    c.P.hyb_h = c.tab.data() + (size_t)n * n2;
+    c.P.hyb_s = c.tab.data() + 2 * (size_t)n * n2;
+    c.P.stack = c.tab.data() + 3 * (size_t)n * n2;
+    c.P.init_e = c.init.data();
+    c.P.init_h = c.init.data() + n;
+    c.P.init_s = c.init.data() + 2 * n;
+    c.xdbl = ldo::ladder_values(c.plan);
+}
+
+void set_tapes(HostCtx& c) {
+    c.tape_meta.assign(2 * c.R, 0);
+    c.tapes_all.clear();
+    for (int r = 0; r < c.R; r++) {
+        c.tape_meta[r] = (int64_t)c.tapes_all.size();
+        c.tape_meta[c.R + r] = (int64_t)c.tapes[r].size();
+        c.tapes_all.insert(c.tapes_all.end(), c.tapes[r].begin(), c.tapes[r].end());
+    }
+    if (c.tapes_all.empty()) {
+        c.tapes_all.push_back(0);
+    }
+    c.P.tapes = c.tapes_all.data();
+    c.P.tape_off = c.tape_meta.data();
+    c.P.tape_len = c.tape_meta.data() + c.R;
+}
+
+void check_statuses(const HostCtx& c) {
+    for (int r = 0; r < c.R; r++) {
+        ldo::check_status(replica_status(c.P, r), r);
+    }
+}
+
+void init_from(HostCtx& c, const ldo_config* cfg, int only_rep, int mode) {
+    std::vector<int32_t> h;
+    int nd = 0;
+    ldo::pack_config(c.plan, cfg, h, nd);
+    int n = cfg->n_chains;
+    for (int r = 0; r < c.R; r++) {
+        if (only_rep >= 0 && r != only_rep) {
+            continue;
+        }
+        init_replica(c.P, r, h.data(), h.data() + n, h.data() + 2 * n, h.data() + 2 * n + 3 * nd,
+                     n, mode, nullptr);
+    }
+    check_statuses(c);
+}
+
+template <typename F>
+int guarded(HostCtx* c, F f) {
+    std::string& err = c ? c->err : g_create_error;
+    try {
+        f();
+    } catch (ldo::AbiError& e) {
+        err = e.msg;
+        return e.code;
+    } catch (std::exception& e) {
+        err = e.what();
+        return LDO_E_CONSTRAINT;
+    }
+    return LDO_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int eh_create(const ldo_system_desc* sys, const ldo_run_params* run, const ldo_config* init,
+              int32_t n_replicas, const double* temps, void** out_ctx) {
+    HostCtx* c = new HostCtx;
+    int rc = guarded(nullptr, [&]() {
+        if (!init || !out_ctx) {
+            throw ldo::AbiError {LDO_E_ARG, "null argument"};
+        }
+        ldo::make_plan(sys, run, n_replicas, temps, c->plan);
+        c->P = c->plan.P;
+        c->R = n_replicas;
+        c->P.type_len = c->plan.type_len.data();
+        c->P.type_ident = c->plan.type_ident.data();
+        set_tables(*c);
+        c->xint.assign(2 * (size_t)n_replicas, 0);
+        for (int r = 0; r < n_replicas; r++) {
+            c->xint[r] = r;
+            c->xint[n_replicas + r] = r;
+        }
+        c->xcnt.assign(2, 0);
+        c->tapes.assign(n_replicas, {});
+        set_tapes(*c);
+        size_t total = (size_t)c->P.L.bytes * n_replicas;
+        c->blocks = (char*)std::aligned_alloc(256, total);
+        if (!c->blocks) {
+            throw std::runtime_error("out of host memory");
+        }
+        std::memset(c->blocks, 0, total);
+        c->P.blocks = c->blocks;
+        init_from(*c, init, -1, 0);
+    });
+    if (rc != LDO_OK) {
+        delete c;
+        return rc;
+    }
+    *out_ctx = c;
+    return LDO_OK;
+}
+
+void eh_destroy(void* ctx) { delete static_cast<HostCtx*>(ctx); }
+
+const char* eh_last_error(void* ctx) {
+    return ctx ? static_cast<HostCtx*>(ctx)->err.c_str() : g_create_error.c_str();
+}
+
+int eh_set_config(void* ctx, int32_t replica, const ldo_config* cfg) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        if (replica < 0 || replica >= c->R) {
+            throw ldo::AbiError {LDO_E_ARG, "replica out of range"};
+        }
+        init_from(*c, cfg, replica, 1);
+    });
+}
+
+int eh_get_config(void* ctx, int32_t replica, ldo_config* cfg, int32_t cap_chains,
+                  int32_t cap_domains) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        if (replica < 0 || replica >= c->R) {
+            throw ldo::AbiError {LDO_E_ARG, "replica out of range"};
+        }
+        ldo::read_config(c->plan, c->blocks + (size_t)replica * c->P.L.bytes, cfg, cap_chains,
+                         cap_domains);
+    });
+}
+
+int eh_set_ladder(void* ctx, int32_t n, const double* temps, const double* u, const double* b,
+                  const double* s) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        if (n != c->R) {
+            throw ldo::AbiError {LDO_E_ARG, "ladder size must equal the replica count"};
+        }
+        c->plan.ladder_t.assign(temps, temps + n);
+        c->plan.ladder_u.assign(u, u + n);
+        c->plan.ladder_b.assign(b, b + n);
+        c->plan.ladder_s.assign(s, s + n);
+        set_tables(*c);
+    });
+}
+
+int eh_set_tape(void* ctx, int32_t replica, const uint64_t* tape, int64_t n) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        if (replica < 0 || replica >= c->R || n < 0) {
+            throw ldo::AbiError {LDO_E_ARG, "replica out of range"};
+        }
+        c->tapes[replica].assign(tape, tape + n);
+        set_tapes(*c);
+        ((RepScalars*)(c->blocks + (size_t)replica * c->P.L.bytes))->tape_pos = 0;
+    });
+}
+
+int eh_sweep(void* ctx, int64_t steps) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        if (steps < 0) {
+            throw ldo::AbiError {LDO_E_ARG, "negative step count"};
+        }
+        for (int r = 0; r < c->R; r++) {
+            sweep_replica(c->P, r, steps, nullptr);
+        }
+        check_statuses(*c);
+    });
+}
+
+int eh_run(void* ctx, int64_t n_swaps, int64_t interval) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        if (n_swaps < 0 || interval < 0) {
+            throw ldo::AbiError {LDO_E_ARG, "negative swap count or interval"};
+        }
+        XState X;
+        X.q_to_rep = c->xint.data();
+        X.rep_slot = c->xint.data() + c->R;
+        X.ladder_t = c->xdbl.data();
+        X.ladder_u = c->xdbl.data() + c->R;
+        X.ladder_s = c->xdbl.data() + 2 * c->R;
+        X.reduced_u = c->xdbl.data() + 3 * c->R;
+        X.counts = c->xcnt.data();
+        for (int64_t s = 0; s < n_swaps; s++) {
+            int64_t swap_i = ++c->swap_counter;
+            for (int r = 0; r < c->R; r++) {
+                sweep_replica(c->P, r, interval, X.rep_slot);
+            }
+            exchange_round(c->P, X, swap_i);
+        }
+        check_statuses(*c);
+    });
+}
+
+int eh_get_observables(void* ctx, int32_t replica, ldo_obs* out) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        if (replica < 0 || replica >= c->R) {
+            throw ldo::AbiError {LDO_E_ARG, "replica out of range"};
+        }
+        for (int r = 0; r < c->R; r++) {
+            dependent_replica(c->P, r);
+        }
+        const RepScalars& S = *(const RepScalars*)(c->blocks + (size_t)replica * c->P.L.bytes);
+        ldo::fill_observables(c->plan, S, out);
+    });
+}
+
+int eh_get_counters(void* ctx, ldo_counters* out) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        std::memset(out, 0, sizeof *out);
+        for (int r = 0; r < c->R; r++) {
+            ldo::add_counters(*(const RepScalars*)(c->blocks + (size_t)r * c->P.L.bytes), out);
+        }
+        out->exchange_attempts = c->xcnt[0];
+        out->exchange_accepts = c->xcnt[1];
+    });
+}
+
+int eh_get_permutation(void* ctx, int32_t* q_to_rep) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        std::memcpy(q_to_rep, c->xint.data(), c->R * sizeof(int32_t));
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,73 @@
+"""The engine's own source, built for the host, against the CPU oracle.
+
+The HIP kernels run the __host__ __device__ functions of
+latticednaorigami_amd/csrc/replica_ops.h; tests/engine_host builds the same
+functions for the CPU.  These are the GPU parity cases (parity_cases.py)
+run on that build: they check the engine's logic step for step against the
+oracle wherever no GPU is at hand.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import pytest
+
+import parity_cases as pc
+from latticednaorigami_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HOST_DIR = os.path.join(HERE, "engine_host")
+HOST_LIB = os.path.join(HOST_DIR, "_build", "libldo_engine_host.so")
+NAMES = ["create", "destroy", "last_error", "set_config", "get_config", "set_ladder", "set_tape",
+         "sweep", "run", "get_observables", "get_counters", "get_permutation"]
+
+
+class HostLib:
+    """The host build's eh_* functions under the ldo_* names Sampler calls."""
+
+    def __init__(self, cdll):
+        abi.declare_abi(cdll, "eh_")
+        cdll.eh_create.restype = C.c_int
+        cdll.eh_create.argtypes = [C.POINTER(abi.SystemDesc), C.POINTER(abi.RunParams),
+                                   C.POINTER(abi.Config), C.c_int32, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_void_p)]
+        cdll.eh_set_tape.restype = C.c_int
+        cdll.eh_set_tape.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64), C.c_int64]
+        for n in NAMES:
+            setattr(self, "ldo_" + n, getattr(cdll, "eh_" + n))
+
+
+@pytest.fixture(scope="module")
+def sampler_cls():
+    if not os.path.exists(HOST_LIB):
+        subprocess.run(["make", "-C", HOST_DIR], check=True, capture_output=True)
+    lib = HostLib(C.CDLL(HOST_LIB))
+    from latticednaorigami_amd.sampler import Sampler
+
+    def make(system, **kw):
+        return Sampler(system, lib=lib, **kw)
+    return make
+
+
+def test_four_domain_constant_t_bit_exact(sampler_cls, oracle_mod):
+    pc.case_four_domain_constant_t_bit_exact(sampler_cls, oracle_mod)
+
+
+def test_lane_mappings_agree(sampler_cls, oracle_mod):
+    pc.case_lane_mappings_agree(sampler_cls, oracle_mod)
+
+
+def test_raster_nearest_neighbour_bit_exact(sampler_cls, oracle_mod):
+    pc.case_raster_nearest_neighbour_bit_exact(sampler_cls, oracle_mod)
+
+
+def test_ptmc_exchange_matches_oracle(sampler_cls, oracle_mod):
+    pc.case_ptmc_exchange_matches_oracle(sampler_cls, oracle_mod)
+
+
+def test_reference_rng_tape_replay(sampler_cls, oracle_mod):
+    pc.case_reference_rng_tape_replay(sampler_cls, oracle_mod)
+
+
+def test_config_roundtrip_and_errors(sampler_cls, oracle_mod):
+    pc.case_config_roundtrip_and_errors(sampler_cls, oracle_mod)

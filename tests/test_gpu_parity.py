@@ -1,16 +1,8 @@
-"""Parity of the HIP sampler with the CPU oracle on an MI355X.
-
-Both consume the same random stream (Philox keyed by seed and replica, one
-sub-stream per MC step; or a recorded reference-RNG tape), so every
-acceptance decision, lattice configuration, FP64 energy and move counter
-must agree exactly.  Energies are compared with a 1e-12 relative tolerance
-written out below; in practice they agree to the last bit.
-"""
-import sys
-
+"""Parity of the HIP sampler with the CPU oracle on an MI355X (cases in
+parity_cases.py)."""
 import pytest
 
-from latticednaorigami_amd import abi, systems
+import parity_cases as pc
 
 pytestmark = [
     pytest.mark.gpu,
@@ -20,31 +12,6 @@ pytestmark = [
                             "is shown"),
 ]
 
-ENERGY_RTOL = 1e-12
-EXACT_KEYS = ["num_staples", "num_domains", "num_bound_domain_pairs",
-              "num_fully_bound_domain_pairs", "num_misbound_domain_pairs",
-              "num_stacked_domain_pairs", "num_unassigned_domains", "step", "status"]
-
-
-def say(msg):
-    print(msg, flush=True)
-    sys.stdout.flush()
-
-
-def assert_same(gpu, ref, r):
-    a, b = gpu.observables(r), ref.observables(r)
-    for k in EXACT_KEYS:
-        assert a[k] == b[k], (r, k, a[k], b[k])
-    for k in ("energy", "enthalpy", "entropy", "stacking_energy"):
-        assert a[k] == pytest.approx(b[k], rel=ENERGY_RTOL, abs=1e-12), (r, k, a[k], b[k])
-    assert gpu.config(r) == ref.config(r), r
-
-
-def assert_counters(gpu, ref):
-    ca, cb = gpu.counters(), ref.counters()
-    assert list(ca.attempts) == list(cb.attempts)
-    assert list(ca.accepts) == list(cb.accepts)
-
 
 @pytest.fixture(scope="module")
 def sampler_cls():
@@ -53,87 +20,24 @@ def sampler_cls():
 
 
 def test_four_domain_constant_t_bit_exact(sampler_cls, oracle_mod):
-    say("case: four-domain system, 330 K, 5 x 10000 steps, lanes_per_replica=64")
-    s = systems.four_domain_system()
-    gpu = sampler_cls(s, temps=[330.0])
-    ref = oracle_mod.Oracle(s, temps=[330.0])
-    for _ in range(5):
-        gpu.sweep(10000)
-        ref.sweep(10000)
-        assert_same(gpu, ref, 0)
-    assert_counters(gpu, ref)
+    pc.case_four_domain_constant_t_bit_exact(sampler_cls, oracle_mod)
 
 
 def test_lane_mappings_agree(sampler_cls, oracle_mod):
-    say("case: four-domain system, 40 replicas at 310-350 K, lanes_per_replica=1 vs 64")
-    s = systems.four_domain_system()
-    temps = [310.0 + i for i in range(40)]
-    a = sampler_cls(s, temps=temps, lanes_per_replica=1)
-    b = sampler_cls(s, temps=temps, lanes_per_replica=64)
-    ref = oracle_mod.Oracle(s, temps=temps)
-    for x in (a, b):
-        x.sweep(3000)
-    ref.sweep(3000)
-    for r in range(0, 40, 7):
-        assert_same(a, ref, r)
-        assert_same(b, ref, r)
+    pc.case_lane_mappings_agree(sampler_cls, oracle_mod)
 
 
 def test_raster_nearest_neighbour_bit_exact(sampler_cls, oracle_mod):
-    say("case: 64-domain raster, NN energies, 300 K and 360 K, 4 x 5000 steps")
-    s = systems.raster_system(replicas=2)
-    gpu = sampler_cls(s, temps=s.temps)
-    ref = oracle_mod.Oracle(s, temps=s.temps)
-    for _ in range(4):
-        gpu.sweep(5000)
-        ref.sweep(5000)
-        for r in range(2):
-            assert_same(gpu, ref, r)
-    assert_counters(gpu, ref)
+    pc.case_raster_nearest_neighbour_bit_exact(sampler_cls, oracle_mod)
 
 
 def test_ptmc_exchange_matches_oracle(sampler_cls, oracle_mod):
-    say("case: 64-domain raster, 8-replica PTMC ladder 300-360 K, 12 swaps x 250 steps")
-    s = systems.raster_system(replicas=8)
-    gpu = sampler_cls(s, temps=s.temps, exchange_mode=abi.EXCHANGE_TEMPERATURE)
-    ref = oracle_mod.Oracle(s, temps=s.temps)
-    for _ in range(3):
-        gpu.run(4, 250)
-        ref.run(4, 250)
-        assert gpu.permutation() == ref.permutation()
-        for r in range(8):
-            assert_same(gpu, ref, r)
-    cg, cr = gpu.counters(), ref.counters()
-    assert cg.exchange_attempts == cr.exchange_attempts
-    assert cg.exchange_accepts == cr.exchange_accepts
+    pc.case_ptmc_exchange_matches_oracle(sampler_cls, oracle_mod)
 
 
 def test_reference_rng_tape_replay(sampler_cls, oracle_mod):
-    say("case: four-domain system, mt19937_64 tape recorded by the oracle, 30000 steps")
-    s = systems.four_domain_system()
-    ref = oracle_mod.Oracle(s, temps=[330.0], rng_kind=1)
-    ref.record_tape(0)
-    ref.sweep(30000)
-    gpu = sampler_cls(s, temps=[330.0], rng_mode=abi.RNG_TAPE)
-    gpu.set_tape(0, ref.tape(0))
-    gpu.sweep(30000)
-    assert_same(gpu, ref, 0)
+    pc.case_reference_rng_tape_replay(sampler_cls, oracle_mod)
 
 
 def test_config_roundtrip_and_errors(sampler_cls, oracle_mod):
-    say("case: set_config/get_config round trip, argument errors")
-    s = systems.four_domain_system()
-    gpu = sampler_cls(s, temps=[330.0, 340.0])
-    ref = oracle_mod.Oracle(s, temps=[330.0])
-    ref.sweep(20000)
-    chains = ref.config(0)
-    gpu.set_config(1, chains)
-    assert gpu.config(1) == chains
-    assert gpu.observables(1)["energy"] == pytest.approx(ref.recompute_energy(0), rel=1e-12)
-    with pytest.raises(abi.LdoError) as e:
-        gpu.set_config(5, chains)
-    assert e.value.code == abi.LDO_E_ARG
-    bad = [systems.Chain(0, 0, [(0, 0, 0), (1, 0, 0), (3, 0, 0), (4, 0, 0)], [(0, 0, 1)] * 4)]
-    with pytest.raises(abi.LdoError) as e:
-        gpu.set_config(0, bad)
-    assert e.value.code == abi.LDO_E_CONSTRAINT
+    pc.case_config_roundtrip_and_errors(sampler_cls, oracle_mod)
