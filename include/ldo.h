@@ -68,7 +68,29 @@ enum ldo_rng_mode { LDO_RNG_PHILOX = 0, LDO_RNG_TAPE = 1 };
 enum ldo_exchange_mode {
     LDO_EXCHANGE_NONE = 0,        /* independent constant-T chains */
     LDO_EXCHANGE_TEMPERATURE = 1, /* ut_parallel_tempering */
+    LDO_EXCHANGE_WINDOWS = 2,     /* ptmwus: umbrella windows at one temperature */
 };
+
+/* System-wide order parameters a move-level bias can act on
+ * (order_params.cpp: NumBoundDomainPairsOrderParam counts fully bound pairs,
+ * 288-312; NumStaplesOrderParam, 194-209). */
+enum ldo_order_param { LDO_OP_NUMFULLDOMAINS = 0, LDO_OP_NUMSTAPLES = 1 };
+
+/* One umbrella window (one ladder slot): a LinearStepWell restraint on the
+ * window's order parameter (bias_functions.cpp:139-151) and a Grid bias over
+ * the same parameter (242-284), both multiplied by bias_mult (SystemBiases,
+ * 475-496).  Grid values come separately (ldo_set_windows). */
+typedef struct ldo_window {
+    int32_t op;     /* ldo_order_param */
+    int32_t min_op; /* window [min_op, max_op] */
+    int32_t max_op;
+    int32_t pad0;
+    double well_bias;     /* restraint inside the window */
+    double min_bias;      /* restraint one step outside */
+    double slope;         /* restraint growth per further step */
+    double off_grid_bias; /* grid bias where no grid value is given */
+    double bias_mult;
+} ldo_window;
 
 /* System and energy model: the data setup_origami() reads
  * (origami_system.cpp:991-1030) and the OrigamiPotential options
@@ -144,6 +166,7 @@ typedef struct ldo_obs {
     int32_t num_stacked_domain_pairs;
     int32_t num_unassigned_domains;
     int32_t status; /* 0, or LDO_E_CONSTRAINT after a failed constraint check */
+    double bias;    /* the replica's move-level bias (kT) under its window */
 } ldo_obs;
 
 #define LDO_MAX_MOVETYPES 8
@@ -195,6 +218,15 @@ int ldo_set_ladder(void* ctx, int32_t n, const double* temps,
                    const double* staple_u_mults, const double* bias_mults,
                    const double* stacking_mults);
 
+/* Umbrella windows, one per ladder slot (n = the replica count): window
+ * parameters and grid biases grid[w * grid_len + p] for order-parameter
+ * values 0 <= p < grid_len.  Replaces the bias-function file and the
+ * umbrella-sampling grid of USGCMCSimulation (us_simulation.cpp:35-67,
+ * 503-516).  Each window's stored bias values restart from its replica's
+ * current configuration. */
+int ldo_set_windows(void* ctx, int32_t n, const ldo_window* windows, const double* grid,
+                    int32_t grid_len);
+
 /* Supply a recorded draw sequence for one replica (LDO_RNG_TAPE).  Entry k
  * is the k-th value the reference's RandomGens returned: a uniform_real as
  * its IEEE-754 bit pattern, a uniform_int as the integer. */
@@ -206,7 +238,9 @@ int ldo_sweep(void* ctx, int64_t steps);
 
 /* PTMC: n_swaps rounds of [apply ladder slot -> simulate(exchange_interval)
  * -> dependent q's -> device exchange] (PTGCMCSimulation::run,
- * ptmc_simulation.cpp:106-150). */
+ * ptmc_simulation.cpp:106-150).  With LDO_EXCHANGE_WINDOWS the exchange is
+ * PTMWUSGCMCSimulation::attempt_exchange (us_simulation.cpp:662-706,
+ * 770-864): adjacent windows swap configurations. */
 int ldo_run(void* ctx, int64_t n_swaps, int64_t exchange_interval);
 
 int ldo_get_observables(void* ctx, int32_t replica, ldo_obs* out);

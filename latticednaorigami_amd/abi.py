@@ -27,7 +27,8 @@ MOVE_MET_STAPLE_EXCHANGE = 1
 MOVE_CB_STAPLE_REGROWTH = 2
 MOVE_CTCB_SCAFFOLD_REGROWTH = 3
 RNG_PHILOX, RNG_TAPE = 0, 1
-EXCHANGE_NONE, EXCHANGE_TEMPERATURE = 0, 1
+EXCHANGE_NONE, EXCHANGE_TEMPERATURE, EXCHANGE_WINDOWS = 0, 1, 2
+OP_NUMFULLDOMAINS, OP_NUMSTAPLES = 0, 1
 MAX_MOVETYPES = 8
 
 MOVETYPE_NAMES = {
@@ -115,10 +116,25 @@ class Obs(C.Structure):
         ("num_stacked_domain_pairs", C.c_int32),
         ("num_unassigned_domains", C.c_int32),
         ("status", C.c_int32),
+        ("bias", C.c_double),
     ]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Window(C.Structure):
+    _fields_ = [
+        ("op", C.c_int32),
+        ("min_op", C.c_int32),
+        ("max_op", C.c_int32),
+        ("pad0", C.c_int32),
+        ("well_bias", C.c_double),
+        ("min_bias", C.c_double),
+        ("slope", C.c_double),
+        ("off_grid_bias", C.c_double),
+        ("bias_mult", C.c_double),
+    ]
 
 
 class Counters(C.Structure):
@@ -155,6 +171,7 @@ def declare_abi(lib, prefix):
         "get_permutation": (C.c_int, [v, _i32p]),
         "get_config": (C.c_int, [v, C.c_int32, C.POINTER(Config), C.c_int32, C.c_int32]),
         "set_config": (C.c_int, [v, C.c_int32, C.POINTER(Config)]),
+        "set_windows": (C.c_int, [v, C.c_int32, C.POINTER(Window), _f64p, C.c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, prefix + name)

@@ -561,7 +561,9 @@ struct Eng {
     DEVNI bool ctcb_scaffold_move(int max_regrowth);
     DEVNI bool attempt(int m);
     DEVNI void simulate(int64_t steps);
-    DEVNI void begin_interval(int slot);
+    DEVNI void begin_interval(int slot, bool recompute);
+    DEVNI int window_op(int staple_offset);
+    DEVNI double calc_move_bias(int staple_offset);
 
     // move-local scratch for the CTCB segment (kept in registers)
     int seg_first, seg_count, seg_dir;

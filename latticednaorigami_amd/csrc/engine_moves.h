@@ -254,6 +254,7 @@ DEVNI bool Eng::insert_staple() {
     if (S->rejected) {
         return false;
     }
+    S->delta_e += calc_move_bias(0); // add_external_bias (met_movetypes.cpp:69-75, 216)
     double boltz = exp(-S->delta_e);
     int n_new = tcount[ident];
     double pratio = (double)len / 6.0 / n_new * boltz;
@@ -288,6 +289,9 @@ DEVNI bool Eng::delete_staple() {
         S->delta_e += unassign_domain(s);
     }
     S->delta_e -= init_e;
+    // add_external_bias with the staple count reduced by one
+    // (met_movetypes.cpp:259-261)
+    S->delta_e += calc_move_bias(-1);
     double boltz = exp(-S->delta_e);
     int ni = tcount[ident];
     double pratio = ni * 6.0 / (double)len * boltz;
@@ -417,6 +421,7 @@ DEVNI bool Eng::cb_test_acceptance() {
     double ratio = S->new_bias / S->bias;
     if (test_acceptance(ratio)) {
         reset_origami();
+        calc_move_bias(0);
         return true;
     }
     S->n_modified = 0;
@@ -469,6 +474,7 @@ DEVNI bool Eng::cb_staple_move() {
         return false;
     }
     S->bias /= num_bound_staple_domains(cs);
+    S->bias *= exp(-calc_move_bias(0)); // add_external_bias (cb_movetypes.cpp:52-56)
     cb_setup_regrow_old();
     k = uniform_int(0, nb - 1);
     cb_unassign_chain(cs, 0, len);
@@ -994,6 +1000,9 @@ DEVNI bool Eng::ctcb_scaffold_move(int max_regrowth) {
         if (phase == 0 && S->rejected) {
             return false;
         }
+        if (phase == 0) {
+            S->bias *= exp(-calc_move_bias(0)); // add_external_bias (cb_movetypes.cpp:719)
+        }
     }
     S->modifier = 1;
     return cb_test_acceptance();
@@ -1057,6 +1066,7 @@ DEVNI void Eng::simulate(int64_t steps) {
         S->accepts[m] += acc ? 1 : 0;
         if (!acc) {
             reset_origami();
+            calc_move_bias(0);
         }
         if (P->centering_freq != 0 && S->step % P->centering_freq == 0) {
             center();
@@ -1072,7 +1082,7 @@ DEVNI void Eng::simulate(int64_t steps) {
 // slot met for the first time rebuilds its tables in the reference, which
 // also refreshes the initiation terms (origami_potential.cpp:1019-1042,
 // 1057-1063); on later visits the initiation terms keep their last values.
-DEVNI void Eng::begin_interval(int slot_) {
+DEVNI void Eng::begin_interval(int slot_, bool recompute) {
     S->temp_slot = slot_;
     uint64_t bit = 1ull << slot_;
     if (!(S->visited & bit)) {
@@ -1080,7 +1090,50 @@ DEVNI void Eng::begin_interval(int slot_) {
         S->init_slot = slot_;
     }
     select_tables();
-    update_energy();
+    if (recompute) {
+        update_energy();
+    }
+}
+
+// the window's order parameter: fully bound pairs (numfulldomains) or
+// staples (numstaples), the latter offset while a deletion is tested
+DEVNI int Eng::window_op(int staple_offset) {
+    const ldo_window& w = P->windows[S->temp_slot];
+    if (w.op == LDO_OP_NUMSTAPLES) {
+        return S->num_staples + staple_offset;
+    }
+    return S->num_fully;
+}
+
+// SystemOrderParams::update_move_params + SystemBiases::calc_move
+// (order_params.cpp:595-603; bias_functions.cpp:484-496) for the replica's
+// umbrella window: a LinearStepWell restraint (bias_functions.cpp:139-151)
+// then a Grid bias (242-284).  The stored values belong to the window.
+DEVNI double Eng::calc_move_bias(int staple_offset) {
+    if (P->windows == nullptr) {
+        return 0;
+    }
+    int q = S->temp_slot;
+    const ldo_window& w = P->windows[q];
+    int p = window_op(staple_offset);
+    double r;
+    if (p < w.min_op) {
+        r = w.slope * (w.min_op - p - 1) + w.min_bias;
+    }
+    else if (p > w.max_op) {
+        r = w.slope * (p - w.max_op - 1) + w.min_bias;
+    }
+    else {
+        r = w.well_bias;
+    }
+    double g = (p >= 0 && p < P->grid_len) ? P->win_grid[(size_t)q * P->grid_len + p]
+                                           : w.off_grid_bias;
+    double* st = P->win_stored + 2 * q;
+    double diff = r - st[0];
+    diff += g - st[1];
+    st[0] = r;
+    st[1] = g;
+    return diff * w.bias_mult;
 }
 
 #endif

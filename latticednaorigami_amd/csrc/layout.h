@@ -16,6 +16,8 @@
 
 #include <stdint.h>
 
+#include "../../include/ldo.h"
+
 #define LDO_NONE ((int16_t)-1)
 #define LDO_HASH_EMPTY 0xFFFFFFFFFFFFFFFFull
 #define LDO_COORD_BITS 21
@@ -163,6 +165,11 @@ struct Params {
     const uint64_t* tapes;    // concatenated tapes
     const int64_t* tape_off;  // [n_replicas]
     const int64_t* tape_len;  // [n_replicas]
+    int32_t exchange_mode;    // ldo_exchange_mode
+    int32_t grid_len;         // grid values per window
+    const ldo_window* windows; // [n_slots] umbrella windows, or null
+    const double* win_grid;    // [n_slots * grid_len]
+    double* win_stored;        // [n_slots * 2] stored restraint and grid bias
 };
 
 // move kinds held in RepScalars::kind (the grow_chain dispatch)

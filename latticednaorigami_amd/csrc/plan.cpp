@@ -194,6 +194,14 @@ void make_plan(const ldo_system_desc* sys, const ldo_run_params* run, int32_t n_
     }
     P.lanes_per_replica = lpr;
     P.n_slots = n_replicas;
+    P.exchange_mode = run->exchange_mode;
+    if (run->exchange_mode == LDO_EXCHANGE_WINDOWS) {
+        for (int r = 1; r < n_replicas; r++) {
+            if (temps[r] != temps[0]) {
+                throw AbiError {LDO_E_ARG, "umbrella windows share one temperature"};
+            }
+        }
+    }
     // chain lengths by identity, and the table index of every domain of
     // every chain type
     out.type_len.assign(sys->n_chain_types, 0);
@@ -370,6 +378,45 @@ void check_status(int32_t status, int replica) {
         break;
     }
     throw AbiError {code, "replica " + std::to_string(replica) + ": " + why};
+}
+
+void check_windows(const Plan& p, int32_t n, const ldo_window* windows, int32_t grid_len) {
+    if (n != p.R || !windows || grid_len < 0) {
+        throw AbiError {LDO_E_ARG, "one window per replica (ladder slot) is required"};
+    }
+    if (p.P.exchange_mode == LDO_EXCHANGE_TEMPERATURE) {
+        throw AbiError {LDO_E_NOTIMPL, "umbrella windows with a temperature ladder"};
+    }
+    for (int w = 0; w < n; w++) {
+        if (windows[w].op != LDO_OP_NUMFULLDOMAINS && windows[w].op != LDO_OP_NUMSTAPLES) {
+            throw AbiError {LDO_E_ARG, "unknown window order parameter"};
+        }
+    }
+}
+
+double window_restraint(const ldo_window& w, int p) {
+    if (p < w.min_op) {
+        return w.slope * (w.min_op - p - 1) + w.min_bias;
+    }
+    if (p > w.max_op) {
+        return w.slope * (p - w.max_op - 1) + w.min_bias;
+    }
+    return w.well_bias;
+}
+
+int window_op_value(const ldo_window& w, const RepScalars& S) {
+    return w.op == LDO_OP_NUMSTAPLES ? S.num_staples : S.num_fully;
+}
+
+std::vector<double> initial_window_state(const ldo_window* windows, int32_t n,
+                                         const std::vector<int32_t>& q_to_rep,
+                                         const std::vector<RepScalars>& scalars) {
+    std::vector<double> st(2 * (size_t)n, 0.0);
+    for (int w = 0; w < n; w++) {
+        const RepScalars& S = scalars[q_to_rep[w]];
+        st[2 * w] = window_restraint(windows[w], window_op_value(windows[w], S));
+    }
+    return st;
 }
 
 } // namespace ldo

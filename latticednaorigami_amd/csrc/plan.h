@@ -63,6 +63,22 @@ void add_counters(const RepScalars& S, ldo_counters* out);
 // Throws the ABI error that a replica's status word stands for.
 void check_status(int32_t status, int replica);
 
+// Validates umbrella windows for this run (one per ladder slot).
+void check_windows(const Plan& p, int32_t n, const ldo_window* windows, int32_t grid_len);
+
+// LinearStepWellBiasFunction::calc_bias (bias_functions.cpp:139-151)
+double window_restraint(const ldo_window& w, int op_value);
+
+// the value of a window's order parameter in a replica
+int window_op_value(const ldo_window& w, const RepScalars& S);
+
+// Stored bias values of every window for the replicas now in them: the
+// restraint at the replica's order parameter and a zero grid value, as a
+// freshly built SystemBiases holds them (bias_functions.cpp:107-122, 222-224).
+std::vector<double> initial_window_state(const ldo_window* windows, int32_t n,
+                                         const std::vector<int32_t>& q_to_rep,
+                                         const std::vector<RepScalars>& scalars);
+
 } // namespace ldo
 
 #endif

@@ -86,7 +86,9 @@ DEV void init_replica(const Params& P, int r, const int32_t* c_idx, const int32_
 
 // `steps` steps of GCMCSimulation::simulate (simulation.cpp:568-653); with a
 // ladder (rep_slot) the replica first takes its slot's control variables
-// (update_control_qs) and ends with its dependent quantities.
+// (update_control_qs: a temperature switch recomputes the energy; umbrella
+// windows share one temperature and only change the bias) and ends with its
+// dependent quantities.
 DEV void sweep_replica(const Params& P, int r, int64_t steps, const int32_t* rep_slot) {
     Eng e;
     e.attach(&P, P.blocks + (size_t)r * P.L.bytes, r);
@@ -94,7 +96,7 @@ DEV void sweep_replica(const Params& P, int r, int64_t steps, const int32_t* rep
         return;
     }
     if (rep_slot != nullptr) {
-        e.begin_interval(rep_slot[r]);
+        e.begin_interval(rep_slot[r], P.exchange_mode == LDO_EXCHANGE_TEMPERATURE);
     }
     e.simulate(steps);
     if (rep_slot != nullptr) {
@@ -195,6 +197,62 @@ DEV void exchange_round(const Params& P, const XState& X, int64_t swap_i) {
             X.rep_slot[r1] = i + 1;
         }
     }
+}
+
+// A replica's value of a window's order parameter (GridBiasFunction::get_point)
+DEV int replica_window_op(const Params& P, int r, const ldo_window& w) {
+    const RepScalars* S = (const RepScalars*)(P.blocks + (size_t)r * P.L.bytes);
+    return w.op == LDO_OP_NUMSTAPLES ? S->num_staples : S->num_fully;
+}
+
+// grid bias of window q at point p as the exchange reads it (a GridFloats
+// lookup: points without a value read 0)
+DEV double exchange_grid(const Params& P, int q, int p) {
+    return (p >= 0 && p < P.grid_len) ? P.win_grid[(size_t)q * P.grid_len + p] : 0.0;
+}
+
+// PTMWUSGCMCSimulation::attempt_exchange (us_simulation.cpp:770-864):
+// adjacent windows swap configurations when each window's order parameter
+// lies inside the other window.  Windows are ladder slots here, so a swap
+// exchanges the two replicas' slots; each window keeps its stored biases.
+DEV void exchange_windows_round(const Params& P, const XState& X, int64_t swap_i) {
+    int W = P.n_replicas;
+    uint32_t idx = 0;
+    for (int i = (int)(swap_i % 2); i < W - 1; i += 2) {
+        X.counts[0]++;
+        const ldo_window& w1 = P.windows[i];
+        const ldo_window& w2 = P.windows[i + 1];
+        int r1 = X.q_to_rep[i], r2 = X.q_to_rep[i + 1];
+        int p1 = replica_window_op(P, r1, w1);
+        int p2 = replica_window_op(P, r2, w2);
+        if (p2 < w1.min_op || p2 > w1.max_op || p1 < w2.min_op || p1 > w2.max_op) {
+            continue;
+        }
+        bool accept = true;
+        if (p1 != p2) {
+            double d1 = exchange_grid(P, i, p1) - exchange_grid(P, i + 1, p1);
+            double d2 = exchange_grid(P, i + 1, p2) - exchange_grid(P, i, p2);
+            double p = fmin(1.0, exp(d1 + d2));
+            accept = (p == 1) ? true : (p > exchange_uniform(P, swap_i, idx));
+        }
+        if (accept) {
+            X.counts[1]++;
+            X.q_to_rep[i] = r2;
+            X.q_to_rep[i + 1] = r1;
+            X.rep_slot[r2] = i;
+            X.rep_slot[r1] = i + 1;
+        }
+    }
+}
+
+// the stored move-level bias of the window in ladder slot q (0 without
+// windows).  The replica in slot q is rep_slot's inverse, not the block's
+// temp_slot, which follows a swap only when the next interval begins.
+DEV double slot_bias(const Params& P, int q) {
+    if (P.windows == nullptr) {
+        return 0;
+    }
+    return (P.win_stored[2 * q] + P.win_stored[2 * q + 1]) * P.windows[q].bias_mult;
 }
 
 #endif

@@ -73,7 +73,12 @@ extern "C" __global__ void ldo_dependent_kernel(Params P) {
 
 extern "C" __global__ void ldo_exchange_kernel(Params P, XState X, int64_t swap_i) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        exchange_round(P, X, swap_i);
+        if (P.exchange_mode == LDO_EXCHANGE_WINDOWS) {
+            exchange_windows_round(P, X, swap_i);
+        }
+        else {
+            exchange_round(P, X, swap_i);
+        }
     }
 }
 
@@ -123,6 +128,10 @@ struct Ctx {
     int64_t* d_xcnt = nullptr;      // exchange attempts, accepts
     int32_t* d_cfg = nullptr;
     size_t cfg_cap = 0;
+    ldo_window* d_windows = nullptr;
+    double* d_grid = nullptr;
+    double* d_win_stored = nullptr;
+    std::vector<ldo_window> windows; // host copy
 };
 
 struct HipError {
@@ -325,7 +334,8 @@ void free_ctx(Ctx* c) {
         (void)hipEventDestroy(e);
     }
     void* bufs[] = {c->d_blocks, c->d_type_len, c->d_type_ident, c->d_tab, c->d_init,
-                    c->d_tapes, c->d_tape_meta, c->d_xint, c->d_xdbl, c->d_xcnt, c->d_cfg};
+                    c->d_tapes, c->d_tape_meta, c->d_xint, c->d_xdbl, c->d_xcnt, c->d_cfg,
+                    c->d_windows, c->d_grid, c->d_win_stored};
     for (void* b: bufs) {
         if (b) {
             (void)hipFree(b);
@@ -467,6 +477,39 @@ int ldo_set_ladder(void* ctx, int32_t n, const double* temps, const double* u, c
     });
 }
 
+int ldo_set_windows(void* ctx, int32_t n, const ldo_window* windows, const double* grid,
+                    int32_t grid_len) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        ldo::check_windows(c->plan, n, windows, grid_len);
+        std::vector<int32_t> q2r(c->R);
+        download(*c, q2r.data(), c->d_xint, c->R * sizeof(int32_t), "copy permutation");
+        std::vector<RepScalars> scalars(c->R);
+        for (int r = 0; r < c->R; r++) {
+            download(*c, &scalars[r], c->d_blocks + (size_t)r * c->P.L.bytes, sizeof(RepScalars),
+                     "copy scalars");
+        }
+        std::vector<double> stored = ldo::initial_window_state(windows, n, q2r, scalars);
+        release(*c, c->d_windows, "hipFree windows");
+        release(*c, c->d_grid, "hipFree grid");
+        release(*c, c->d_win_stored, "hipFree window state");
+        size_t ng = std::max<size_t>(1, (size_t)n * grid_len);
+        hip_check(hipMalloc(&c->d_windows, n * sizeof(ldo_window)), "hipMalloc windows");
+        hip_check(hipMalloc(&c->d_grid, ng * sizeof(double)), "hipMalloc grid");
+        hip_check(hipMalloc(&c->d_win_stored, stored.size() * sizeof(double)),
+                  "hipMalloc window state");
+        upload(*c, c->d_windows, windows, n * sizeof(ldo_window), "copy windows");
+        upload(*c, c->d_grid, grid, (size_t)n * grid_len * sizeof(double), "copy grid");
+        upload(*c, c->d_win_stored, stored.data(), stored.size() * sizeof(double),
+               "copy window state");
+        c->windows.assign(windows, windows + n);
+        c->P.windows = c->d_windows;
+        c->P.win_grid = c->d_grid;
+        c->P.grid_len = grid_len;
+        c->P.win_stored = c->d_win_stored;
+    });
+}
+
 int ldo_set_tape(void* ctx, int32_t replica, const uint64_t* tape, int64_t n) {
     Ctx* c = static_cast<Ctx*>(ctx);
     return guarded(c, [&]() {
@@ -540,6 +583,16 @@ int ldo_get_observables(void* ctx, int32_t replica, ldo_obs* out) {
         RepScalars S;
         download(*c, &S, c->d_blocks + (size_t)replica * c->P.L.bytes, sizeof S, "copy scalars");
         ldo::fill_observables(c->plan, S, out);
+        out->bias = 0;
+        if (c->P.windows != nullptr) {
+            // the window the replica holds now: rep_slot, which a swap
+            // updates at once (temp_slot follows at the next interval)
+            int32_t q = 0;
+            download(*c, &q, c->d_xint + c->R + replica, sizeof q, "copy slot");
+            double st[2];
+            download(*c, st, c->d_win_stored + 2 * q, sizeof st, "copy window state");
+            out->bias = (st[0] + st[1]) * c->windows[q].bias_mult;
+        }
     });
 }
 

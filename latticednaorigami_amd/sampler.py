@@ -54,6 +54,14 @@ class Sampler:
         arrs = [(C.c_double * n)(*v) for v in (temps, u or one, b or one, s or one)]
         self._check(self.lib.ldo_set_ladder(self.h, n, *arrs))
 
+    def set_windows(self, windows, grid, grid_len):
+        """Umbrella windows, one per replica slot (abi.Window records) and
+        their grid biases grid[w * grid_len + p]."""
+        n = len(windows)
+        arr = (abi.Window * n)(*windows)
+        g = (C.c_double * max(1, len(grid)))(*grid)
+        self._check(self.lib.ldo_set_windows(self.h, n, arr, g, grid_len))
+
     def set_tape(self, r, values):
         buf = (C.c_uint64 * max(1, len(values)))(*values)
         self._check(self.lib.ldo_set_tape(self.h, r, buf, len(values)))

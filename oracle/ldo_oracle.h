@@ -34,6 +34,10 @@ const char* oracle_last_error(void* o);
 int oracle_set_ladder(void* o, int32_t n, const double* temps,
                       const double* staple_u_mults, const double* bias_mults,
                       const double* stacking_mults);
+/* Umbrella windows (see ldo_set_windows); with LDO_EXCHANGE_WINDOWS in the
+ * run parameters oracle_run performs the window exchange. */
+int oracle_set_windows(void* o, int32_t n, const ldo_window* windows, const double* grid,
+                       int32_t grid_len);
 int oracle_sweep(void* o, int64_t steps);
 int oracle_run(void* o, int64_t n_swaps, int64_t exchange_interval);
 int oracle_get_observables(void* o, int32_t replica, ldo_obs* out);

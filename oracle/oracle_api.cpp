@@ -27,8 +27,24 @@ struct Oracle {
     int64_t swap_counter {0};
     ldo_counters counters {};
     Limits lim {0, 0};
+    int exchange_mode {0};
+    // umbrella windows, one per ladder slot, and their stored bias values
+    std::vector<WindowBias> windows;
+    std::vector<WindowState> win_state;
     std::string err;
 };
+
+// Points every replica at the window of its current ladder slot.
+static void assign_windows(Oracle* O) {
+    if (O->windows.empty()) {
+        return;
+    }
+    for (size_t q = 0; q < O->q_to_rep.size(); q++) {
+        Replica& rep = *O->reps[O->q_to_rep[q]];
+        rep.win = &O->windows[q];
+        rep.win_state = &O->win_state[q];
+    }
+}
 
 static std::vector<ChainSpec> specs_from(const ldo_config* cfg,
                                          const std::vector<std::vector<int>>& idents) {
@@ -82,6 +98,7 @@ void* oracle_create(const ldo_system_desc* sd, const ldo_run_params* run,
         }
         Limits lim {run->max_total_staples, run->max_type_staples};
         o->lim = lim;
+        o->exchange_mode = run->exchange_mode;
         std::vector<ChainSpec> cfg = specs_from(init, o->idents);
         o->rng_kind = rng_kind;
         o->seed = run->seed;
@@ -150,8 +167,93 @@ int oracle_sweep(void* o, int64_t steps) {
 // PTGCMCSimulation::run / OneDPTGCMCSimulation::attempt_exchange for a
 // ut_parallel_tempering run (ptmc_simulation.cpp:106-161, 255-313, 360-412,
 // 658-663), with every rank's replica held in this process.
+// PTMWUSGCMCSimulation::run_swaps + attempt_exchange
+// (us_simulation.cpp:662-706, 770-864) with every window on this process:
+// adjacent windows swap configurations when each window's order parameter
+// lies inside the other window, with the grid biases of both windows.  A
+// window keeps its stored bias values across a swap, as each rank of the
+// reference keeps its own SystemBiases.
+static void run_windows(Oracle* O, int64_t n_swaps, int64_t interval) {
+    int W = (int)O->reps.size();
+    for (int64_t s = 0; s < n_swaps; s++) {
+        int64_t swap_i = ++O->swap_counter;
+        for (auto& rep: O->reps) {
+            rep->simulate(interval);
+        }
+        Rng& xr = (O->rng_kind == 1) ? O->reps[0]->rng : O->exchange_rng;
+        if (O->rng_kind != 1) {
+            xr.begin_step(swap_i);
+        }
+        for (int i = (int)(swap_i % 2); i < W - 1; i += 2) {
+            O->counters.exchange_attempts++;
+            const WindowBias& w1 = O->windows[i];
+            const WindowBias& w2 = O->windows[i + 1];
+            int r1 = O->q_to_rep[i], r2 = O->q_to_rep[i + 1];
+            int p1 = O->reps[r1]->window_op(0);
+            int p2 = O->reps[r2]->window_op(0);
+            if (p2 < w1.min_op || p2 > w1.max_op || p1 < w2.min_op || p1 > w2.max_op) {
+                continue;
+            }
+            bool accepted = true;
+            if (p1 != p2) {
+                // GridFloats lookups with operator[]: points without a grid
+                // value read 0
+                auto g = [](const WindowBias& w, int p) {
+                    return (p >= 0 && p < (int)w.grid.size()) ? w.grid[p] : 0.0;
+                };
+                double d1 = g(w1, p1) - g(w2, p1);
+                double d2 = g(w2, p2) - g(w1, p2);
+                double p = std::min(1.0, std::exp(d1 + d2));
+                accepted = (p == 1) ? true : (p > xr.uniform_real());
+            }
+            if (accepted) {
+                O->counters.exchange_accepts++;
+                O->q_to_rep[i] = r2;
+                O->q_to_rep[i + 1] = r1;
+            }
+        }
+        assign_windows(O);
+    }
+}
+
+int oracle_set_windows(void* o, int32_t n, const ldo_window* windows, const double* grid,
+                       int32_t grid_len) {
+    ORACLE_TRY(o, {
+        if (n != (int)O->reps.size() || grid_len < 0) {
+            throw OrigamiError(LDO_E_ARG, "one window per replica is required");
+        }
+        O->windows.assign(n, WindowBias {});
+        O->win_state.assign(n, WindowState {});
+        for (int w = 0; w < n; w++) {
+            WindowBias& b = O->windows[w];
+            b.op = windows[w].op;
+            b.min_op = windows[w].min_op;
+            b.max_op = windows[w].max_op;
+            b.well_bias = windows[w].well_bias;
+            b.min_bias = windows[w].min_bias;
+            b.slope = windows[w].slope;
+            b.off_grid = windows[w].off_grid_bias;
+            b.mult = windows[w].bias_mult;
+            b.grid.assign(grid + (size_t)w * grid_len, grid + (size_t)(w + 1) * grid_len);
+        }
+        assign_windows(O);
+        // LinearStepWellBiasFunction stores its value when constructed
+        // (bias_functions.cpp:107-122); a GridBiasFunction starts from 0
+        // (222-224)
+        for (int w = 0; w < n; w++) {
+            Replica& rep = *O->reps[O->q_to_rep[w]];
+            O->win_state[w].restraint = O->windows[w].restraint(rep.window_op(0));
+            O->win_state[w].grid = 0;
+        }
+    })
+}
+
 int oracle_run(void* o, int64_t n_swaps, int64_t interval) {
     ORACLE_TRY(o, {
+        if (O->exchange_mode == LDO_EXCHANGE_WINDOWS) {
+            run_windows(O, n_swaps, interval);
+            return LDO_OK;
+        }
         int R = (int)O->reps.size();
         std::vector<int> slot_of(R);
         for (int64_t s = 0; s < n_swaps; s++) {
@@ -164,7 +266,7 @@ int oracle_run(void* o, int64_t n_swaps, int64_t interval) {
                 Replica& rep = *O->reps[r];
                 int q = slot_of[r];
                 double temp = O->ladder_t[q];
-                rep.sys.update_temp(temp, 1.0);
+                rep.sys.update_temp(temp, O->ladder_s[q]);
                 rep.sys.update_staple_us(temp, O->ladder_u[q]);
                 rep.simulate(interval);
                 rep.sys.update_enthalpy_and_entropy();
@@ -225,6 +327,7 @@ int oracle_get_observables(void* o, int32_t r, ldo_obs* out) {
         out->num_stacked_domain_pairs = rep.sys.num_stacked;
         out->num_unassigned_domains = rep.sys.num_unassigned;
         out->status = rep.status;
+        out->bias = rep.win ? (rep.win_state->restraint + rep.win_state->grid) * rep.win->mult : 0;
     })
 }
 

@@ -285,6 +285,26 @@ struct Limits {
     int max_type_staples;
 };
 
+// Move-level biases of one umbrella window: a LinearStepWell restraint
+// (bias_functions.cpp:139-151) as the first bias level and a Grid bias
+// (242-284) as the second, on one system-wide order parameter, multiplied by
+// the window's bias multiplier (SystemBiases::calc_move, 484-496).
+struct WindowBias {
+    int op {0}; // LDO_OP_*
+    int min_op {0}, max_op {0};
+    double well_bias {0}, min_bias {0}, slope {0};
+    double off_grid {0}, mult {1};
+    std::vector<double> grid;
+    double restraint(int p) const; // LinearStepWellBiasFunction::calc_bias
+    double grid_bias(int p) const; // GridBiasFunction::calc_bias
+};
+
+// the stored values of a window's bias functions (BiasFunction::m_bias)
+struct WindowState {
+    double restraint {0};
+    double grid {0};
+};
+
 class Replica; // forward
 
 // Constraintpoints + StapleNetwork (top_constraint_points.cpp:14-601)
@@ -348,8 +368,14 @@ class Replica {
     int64_t step {0};
     int status {0};
     std::vector<int64_t> attempts, accepts;
+    // the umbrella window this replica samples in, and that window's stored
+    // bias values (both null without biases)
+    const WindowBias* win {nullptr};
+    WindowState* win_state {nullptr};
 
     void simulate(int64_t steps); // simulation.cpp:568-653
+    int window_op(int staple_offset) const;
+    double calc_move_bias(int staple_offset);
 
   private:
     std::vector<MovesetEntry> m_moves;
@@ -401,7 +427,7 @@ class Replica {
     void cb_setup_regrow_old();
     bool cb_test_acceptance();
     double cb_set_old_growth_point(Dom& gnew, Dom& gold);
-    void cb_add_external_bias() {}
+    void cb_add_external_bias();
 
     bool ctcb_scaffold_move(int max_regrowth);
     std::vector<Dom*> select_indices(const std::vector<Dom*>& seg, unsigned min_len,

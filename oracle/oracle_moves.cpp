@@ -298,6 +298,7 @@ void Replica::simulate(int64_t steps) {
         accepts[i] += accepted;
         if (!accepted) {
             reset_origami();
+            calc_move_bias(0);
         }
         if (m_centering_freq != 0 && step % m_centering_freq == 0) {
             sys.center(m_centering_domain);
@@ -601,6 +602,7 @@ bool Replica::insert_staple() {
         return false;
     }
     std::vector<Dom*>& staple = sys.chain_by_index(c_i);
+    m_delta_e += calc_move_bias(0); // add_external_bias (met_movetypes.cpp:69-75, 216)
     double boltz = std::exp(-m_delta_e);
     int n_new = (int)sys.ident_to_index[ident].size();
     size_t len = sys.identities[ident].size();
@@ -626,6 +628,9 @@ bool Replica::delete_staple() {
     int n_bd = num_bound_staple_domains(staple);
     met_unassign_domains(staple);
     m_delta_e -= sys.pot.init_e();
+    // add_external_bias with the staple count reduced by one
+    // (met_movetypes.cpp:259-261)
+    m_delta_e += calc_move_bias(-1);
     double boltz = std::exp(-m_delta_e);
     int ni = (int)sys.ident_to_index[ident].size();
     size_t len = sys.identities[ident].size();
@@ -796,6 +801,7 @@ bool Replica::cb_test_acceptance() {
     long double ratio = m_new_bias / m_bias;
     if (test_acceptance(ratio)) {
         reset_origami();
+        calc_move_bias(0);
         return true;
     }
     m_modified.clear();
@@ -957,6 +963,54 @@ bool Replica::ctcb_scaffold_move(int max_regrowth) {
     ctcb_grow_chain(sds);
     m_modifier = 1;
     return cb_test_acceptance();
+}
+
+// ------------------------------------------------------ move-level biases
+double WindowBias::restraint(int p) const {
+    if (p < min_op) {
+        return slope * (min_op - p - 1) + min_bias;
+    }
+    if (p > max_op) {
+        return slope * (p - max_op - 1) + min_bias;
+    }
+    return well_bias;
+}
+
+double WindowBias::grid_bias(int p) const {
+    if (p >= 0 && p < (int)grid.size()) {
+        return grid[p];
+    }
+    return off_grid;
+}
+
+int Replica::window_op(int staple_offset) const {
+    if (win->op == LDO_OP_NUMSTAPLES) {
+        return sys.num_staples + staple_offset;
+    }
+    return sys.num_fully_bound_pairs;
+}
+
+// SystemOrderParams::update_move_params + SystemBiases::calc_move
+// (order_params.cpp:595-603; bias_functions.cpp:484-496): refresh the
+// window's bias functions at the current order parameter and return the
+// change since their stored values, times the bias multiplier.
+double Replica::calc_move_bias(int staple_offset) {
+    if (win == nullptr) {
+        return 0;
+    }
+    int p = window_op(staple_offset);
+    double r = win->restraint(p);
+    double g = win->grid_bias(p);
+    double diff = r - win_state->restraint;
+    diff += g - win_state->grid;
+    win_state->restraint = r;
+    win_state->grid = g;
+    return diff * win->mult;
+}
+
+// CBMCMovetype::add_external_bias (cb_movetypes.cpp:52-56)
+void Replica::cb_add_external_bias() {
+    m_bias *= std::exp(-calc_move_bias(0));
 }
 
 } // namespace orc

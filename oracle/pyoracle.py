@@ -69,12 +69,12 @@ class Oracle:
     """The oracle's replicas of one system (rng_kind 0: Philox stream of the
     sampler; 1: the reference's mt19937_64 RandomGens)."""
 
-    def __init__(self, system, temps=None, rng_kind=0):
+    def __init__(self, system, temps=None, rng_kind=0, exchange_mode=abi.EXCHANGE_NONE):
         self.L = lib()
         self.system = system
         temps = list(temps if temps is not None else (system.temps or [300.0]))
         self.n = len(temps)
-        sd, rp, cfg, self._keep = system.to_abi()
+        sd, rp, cfg, self._keep = system.to_abi(exchange_mode=exchange_mode)
         t = (C.c_double * self.n)(*temps)
         h = self.L.oracle_create(C.byref(sd), C.byref(rp), C.byref(cfg), self.n, t, rng_kind)
         if not h:
@@ -97,6 +97,12 @@ class Oracle:
 
     def run(self, n_swaps, interval):
         _check(self.L, self.h, self.L.oracle_run(self.h, n_swaps, interval))
+
+    def set_windows(self, windows, grid, grid_len):
+        n = len(windows)
+        arr = (abi.Window * n)(*windows)
+        g = (C.c_double * max(1, len(grid)))(*grid)
+        _check(self.L, self.h, self.L.oracle_set_windows(self.h, n, arr, g, grid_len))
 
     def set_ladder(self, temps, u=None, b=None, s=None):
         n = len(temps)

@@ -37,6 +37,8 @@ struct HostCtx {
     std::vector<uint64_t> tapes_all;
     std::vector<int64_t> tape_meta;
     std::vector<std::vector<uint64_t>> tapes;
+    std::vector<ldo_window> windows;
+    std::vector<double> grid, win_stored;
     ~HostCtx() { std::free(blocks); }
 };
 
@@ -191,6 +193,29 @@ int eh_set_ladder(void* ctx, int32_t n, const double* temps, const double* u, co
     });
 }
 
+int eh_set_windows(void* ctx, int32_t n, const ldo_window* windows, const double* grid,
+                   int32_t grid_len) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        ldo::check_windows(c->plan, n, windows, grid_len);
+        std::vector<int32_t> q2r(c->xint.begin(), c->xint.begin() + c->R);
+        std::vector<RepScalars> scalars(c->R);
+        for (int r = 0; r < c->R; r++) {
+            scalars[r] = *(const RepScalars*)(c->blocks + (size_t)r * c->P.L.bytes);
+        }
+        c->win_stored = ldo::initial_window_state(windows, n, q2r, scalars);
+        c->windows.assign(windows, windows + n);
+        c->grid.assign(grid, grid + (size_t)n * grid_len);
+        if (c->grid.empty()) {
+            c->grid.push_back(0);
+        }
+        c->P.windows = c->windows.data();
+        c->P.win_grid = c->grid.data();
+        c->P.grid_len = grid_len;
+        c->P.win_stored = c->win_stored.data();
+    });
+}
+
 int eh_set_tape(void* ctx, int32_t replica, const uint64_t* tape, int64_t n) {
     HostCtx* c = static_cast<HostCtx*>(ctx);
     return guarded(c, [&]() {
@@ -235,7 +260,12 @@ int eh_run(void* ctx, int64_t n_swaps, int64_t interval) {
             for (int r = 0; r < c->R; r++) {
                 sweep_replica(c->P, r, interval, X.rep_slot);
             }
-            exchange_round(c->P, X, swap_i);
+            if (c->P.exchange_mode == LDO_EXCHANGE_WINDOWS) {
+                exchange_windows_round(c->P, X, swap_i);
+            }
+            else {
+                exchange_round(c->P, X, swap_i);
+            }
         }
         check_statuses(*c);
     });
@@ -252,6 +282,7 @@ int eh_get_observables(void* ctx, int32_t replica, ldo_obs* out) {
         }
         const RepScalars& S = *(const RepScalars*)(c->blocks + (size_t)replica * c->P.L.bytes);
         ldo::fill_observables(c->plan, S, out);
+        out->bias = slot_bias(c->P, c->xint[c->R + replica]);
     });
 }
 

@@ -97,6 +97,58 @@ def case_ptmc_exchange_matches_oracle(sampler_cls, oracle_mod):
     assert cg.exchange_accepts == cr.exchange_accepts
 
 
+def _raster_windows(n):
+    """One numstaples window per replica: nested LinearStepWell restraints
+    [0, 2 + w] (the raster holds 0-4 staples over these runs, so every pair
+    of windows overlaps and swaps are tried) and a seeded grid bias of 41
+    points per window, so swap acceptance is not trivially one."""
+    import random
+    rnd = random.Random(4242)
+    wins, grid = [], []
+    for w in range(n):
+        wins.append(abi.Window(abi.OP_NUMSTAPLES, 0, 2 + w, 0, 0.0, 2.0, 1.5, 0.0, 1.0))
+        grid += [rnd.uniform(-1.0, 1.0) for _ in range(41)]
+    return wins, grid, 41
+
+
+def case_windows_exchange_matches_oracle(sampler_cls, oracle_mod):
+    say("case: 64-domain raster, 8 numstaples windows at 330 K, 2 x 2000 biased steps, "
+        "then 12 window swaps x 250 steps")
+    s = systems.raster_system(replicas=8)
+    temps = [330.0] * 8
+    wins, grid, glen = _raster_windows(8)
+    us = sampler_cls(s, temps=temps)
+    ref = oracle_mod.Oracle(s, temps=temps)
+    us.set_windows(wins, grid, glen)
+    ref.set_windows(wins, grid, glen)
+    for _ in range(2):
+        us.sweep(2000)
+        ref.sweep(2000)
+        for r in range(8):
+            assert_same(us, ref, r)
+            assert us.observables(r)["bias"] == ref.observables(r)["bias"], r
+    assert_counters(us, ref)
+    gpu = sampler_cls(s, temps=temps, exchange_mode=abi.EXCHANGE_WINDOWS)
+    ref = oracle_mod.Oracle(s, temps=temps, exchange_mode=abi.EXCHANGE_WINDOWS)
+    gpu.set_windows(wins, grid, glen)
+    ref.set_windows(wins, grid, glen)
+    for _ in range(3):
+        gpu.run(4, 250)
+        ref.run(4, 250)
+        assert gpu.permutation() == ref.permutation()
+        for r in range(8):
+            assert_same(gpu, ref, r)
+            assert gpu.observables(r)["bias"] == ref.observables(r)["bias"], r
+    cg, cr = gpu.counters(), ref.counters()
+    assert cg.exchange_attempts == cr.exchange_attempts
+    assert cg.exchange_accepts == cr.exchange_accepts
+    assert 0 < cg.exchange_accepts < cg.exchange_attempts
+    assert_counters(gpu, ref)
+    with pytest.raises(abi.LdoError) as e:
+        gpu.set_windows(wins[:4], grid[:4 * glen], glen)
+    assert e.value.code == abi.LDO_E_ARG
+
+
 def case_reference_rng_tape_replay(sampler_cls, oracle_mod):
     say("case: four-domain system, mt19937_64 tape recorded by the oracle, 30000 steps")
     s = systems.four_domain_system()

@@ -35,6 +35,10 @@ def test_ptmc_exchange_matches_oracle(sampler_cls, oracle_mod):
     pc.case_ptmc_exchange_matches_oracle(sampler_cls, oracle_mod)
 
 
+def test_windows_exchange_matches_oracle(sampler_cls, oracle_mod):
+    pc.case_windows_exchange_matches_oracle(sampler_cls, oracle_mod)
+
+
 def test_reference_rng_tape_replay(sampler_cls, oracle_mod):
     pc.case_reference_rng_tape_replay(sampler_cls, oracle_mod)
 

@@ -105,3 +105,114 @@ def test_monte_carlo_matches_enumeration(oracle_mod, temp, seed):
     for k, v in exact.items():
         marg[k[0]] = marg.get(k[0], 0) + v
     check(lambda k: k[0], marg)
+
+
+def _window(op, lo, hi, well=0.0, min_bias=0.0, slope=0.0, off_grid=0.0, mult=1.0):
+    from latticednaorigami_amd import abi
+    return abi.Window(op, lo, hi, 0, well, min_bias, slope, off_grid, mult)
+
+
+def _biased_run(oracle_mod, op, win, grid, misbinding=None, moves=None, n_samples=2500):
+    """Seeded biased Monte Carlo of the two-staple four-domain system at 345 K
+    against the exact distribution reweighted by exp(-B(op)) state by state.
+    Returns (target, per-replica counts, n_samples)."""
+    from latticednaorigami_amd import abi
+    n_rep, temp = 8, 345.0
+    s = _system(2, seed=13)
+    if misbinding is not None:
+        s.misbinding_pot = misbinding
+    if moves is not None:
+        kept = [m for m in s.moves if m.kind in moves]
+        tot = sum(m.freq for m in kept)
+        s.moves = [systems.Move(m.kind, m.freq / tot, m.max_regrowth) for m in kept]
+    o = oracle_mod.Oracle(s, temps=[temp] * n_rep)
+    exact, _, _ = o.enumerate()
+    lo, hi, well, min_bias, slope = win
+    w = _window(op, lo, hi, well, min_bias, slope)
+    glen = len(grid)
+    o.set_windows([w] * n_rep, grid * n_rep, glen)
+
+    def bias(key):
+        p = key[0] if op == abi.OP_NUMSTAPLES else key[1]
+        r = well if lo <= p <= hi else (slope * ((lo - p - 1) if p < lo else (p - hi - 1)) + min_bias)
+        g = grid[p] if 0 <= p < glen else 0.0
+        return r + g
+
+    weights = {k: v * math.exp(-bias(k)) for k, v in exact.items()}
+    z = sum(weights.values())
+    target = {k: v / z for k, v in weights.items()}
+    assert max(abs(target[k] - exact[k]) for k in exact) > 0.1  # the bias matters
+    o.sweep(20000)
+    counts = [dict() for _ in range(n_rep)]
+    for _ in range(n_samples):
+        o.sweep(100)
+        for r in range(n_rep):
+            ob = o.observables(r)
+            k = (ob["num_staples"], ob["num_fully_bound_domain_pairs"],
+                 ob["num_misbound_domain_pairs"], ob["num_stacked_domain_pairs"])
+            counts[r][k] = counts[r].get(k, 0) + 1
+            assert ob["bias"] == pytest.approx(bias(k), abs=1e-12)
+    return target, counts, n_samples
+
+
+def _mean_se(counts, key, n_samples):
+    fr = [c.get(key, 0) / n_samples for c in counts]
+    m = sum(fr) / len(fr)
+    se = math.sqrt(sum((f - m) ** 2 for f in fr) / (len(fr) - 1) / len(fr))
+    return m, se
+
+
+_MET = 1
+_ALL_MOVES = None
+
+
+@pytest.mark.parametrize("op,win,grid,misbinding,moves", [
+    # numstaples: grid biases 0 / +1.5 / -2.0 kT on 0 / 1 / 2 staples
+    (1, (0, 2, 0.0, 0.0, 0.0), [0.0, 1.5, -2.0], None, _ALL_MOVES),
+    # numfulldomains: a LinearStepWell that favours 2-3 fully bound pairs
+    (0, (2, 3, -1.0, 1.0, 1.5), [], None, _ALL_MOVES),
+    # numfulldomains: a deep well on exactly one fully bound pair, so that
+    # half-bound staples dominate and CB/CTCB regrowth moves the order
+    # parameter (their exp(-dB) factor decides acceptance: a sign error in
+    # it gives p(1,2,0,1) = 0.99 against 0.16).  Misbinding is disallowed so
+    # that every bound staple domain is fully bound (see the next test).
+    (0, (1, 1, -6.0, 3.0, 3.0), [], "disallowed", _ALL_MOVES),
+    # the same well with Opposing misbinding (staples with one bound and one
+    # misbound domain are then common) without CB staple regrowth
+    (0, (1, 1, -6.0, 3.0, 3.0), [], None, "no_cb"),
+])
+def test_biased_monte_carlo_matches_reweighted_enumeration(oracle_mod, op, win, grid,
+                                                           misbinding, moves):
+    """Move-level biases (add_external_bias in every move type, re-sync after
+    rejection) sample exp(-B(op)) times the unbiased ensemble: the exact
+    distribution reweighted state by state."""
+    from latticednaorigami_amd import abi
+    mis = abi.MISBIND_DISALLOWED if misbinding == "disallowed" else None
+    kinds = None
+    if moves == "no_cb":
+        kinds = {abi.MOVE_ORIENTATION_ROTATION, abi.MOVE_MET_STAPLE_EXCHANGE,
+                 abi.MOVE_CTCB_SCAFFOLD_REGROWTH}
+    target, counts, n = _biased_run(oracle_mod, op, win, grid, mis, kinds)
+    for key, p in target.items():
+        if p < 0.01:
+            continue
+        m, se = _mean_se(counts, key, n)
+        assert abs(m - p) <= 4 * se + 0.005, (key, m, p, se)
+
+
+def test_cb_staple_regrowth_redraws_the_old_growth_point(oracle_mod):
+    """A behaviour of the reference kept on purpose: CB staple regrowth draws
+    the growth point of the old-conformation regrowth afresh
+    (cb_movetypes.cpp:335 -> movetypes.cpp:385-393) instead of reusing the
+    forward one.  For a staple with one fully bound and one misbound domain
+    the two Rosenbluth weights differ by ~1e5, so detailed balance fails for
+    the move (1,1,1,0) -> (1,2,0,1).  Under a deep well on one fully bound
+    pair this over-samples the fully bound staple several fold, which the
+    oracle (and the engine, step for step equal to it) must reproduce."""
+    from latticednaorigami_amd import abi
+    target, counts, n = _biased_run(oracle_mod, 0, (1, 1, -6.0, 3.0, 3.0), [], None,
+                                    {abi.MOVE_MET_STAPLE_EXCHANGE, abi.MOVE_CB_STAPLE_REGROWTH,
+                                     abi.MOVE_CTCB_SCAFFOLD_REGROWTH}, n_samples=800)
+    m, se = _mean_se(counts, (1, 2, 0, 1), n)
+    assert target[(1, 2, 0, 1)] < 0.15
+    assert m > 3 * target[(1, 2, 0, 1)], (m, se)
