@@ -495,6 +495,9 @@ struct Eng {
     DEVNI void fb_forward_triplets(int a, int b);
     DEVNI void fb_central_triplets(int di, int dj);
     DEVNI void fb_backward_single_junction(int a, int b);
+    DEVNI void fb_backward_outer(int j3, int j4, int k2, int k1);
+    DEVNI void fb_forward_outer(int j1, int j2, int k1, int k2);
+    DEVNI void fb_central_outer(int j1, int j2, int k1, int k2, int k2b);
     DEVNI void fb_forward_single_junction(int a, int b);
     DEVNI void fb_central_single_junction(int a, int b);
     DEVNI void fb_regular_pair(int a, int b, int i);

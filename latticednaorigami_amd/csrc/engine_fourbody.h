@@ -260,148 +260,154 @@ DEVNI void Eng::fb_central_triplets(int di, int dj) {
     }
 }
 
+// The single-junction checks below enumerate up to three outer candidate
+// pairs and, for each, up to three inner ones.  The lists are kept in scalar
+// variables (LDO_NONE for an entry the reference does not add; eb() is false
+// on it), each outer candidate in a function of its own, and no list lives in
+// a local array: gfx950 code for a small stack array filled on divergent
+// branches was found to store the last entry at the wrong index and leave the
+// next one unwritten, which the loop then used as a domain slot.  The order
+// of the fb_junction calls, and of every read before them, is the reference's.
+
+// inner candidates of one outer pair (k2, k1) of check_backward_single_junction
+DEVNI void Eng::fb_backward_outer(int j3, int j4, int k2, int k1) {
+    if (!eb(k1) || pair_stacked(k1, k2)) {
+        return;
+    }
+    int k1_next = walk(k1, d_of(k1) - d_of(k2));
+    int k1b = bd(k1);
+    int k1b_for = fwd(k1b);
+    int k1b_bac = bwd(k1b);
+    int e1 = k1b_for, e2 = k1b_bac;
+    if (eb(k1_next)) {
+        int k1nb = bd(k1_next);
+        if (k1b_for == k1nb) {
+            e1 = LDO_NONE;
+        }
+        if (k1b_bac == k1nb) {
+            e2 = LDO_NONE;
+        }
+    }
+    if (eb(k1_next)) {
+        fb_junction(k1_next, k1, j3, j4, k1, k2);
+    }
+    if (eb(e1)) {
+        fb_junction(e1, k1b, j3, j4, k1, k2);
+    }
+    if (eb(e2)) {
+        fb_junction(e2, k1b, j3, j4, k1, k2);
+    }
+}
+
 // check_backward_single_junction (:654-748)
 DEVNI void Eng::fb_backward_single_junction(int a, int b) {
     int j3 = a, j4 = b;
-    int k2s[3], k1s[3];
-    int n1 = 0;
     int j3_bac = bwd(j3);
-    k2s[n1] = j3;
-    k1s[n1++] = j3_bac;
     int j3b = bd(j3);
     int j3b_for = fwd(j3b);
     int j3b_bac = bwd(j3b);
+    int c1, c2 = LDO_NONE; // outer candidates (j3b, c) after (j3, j3_bac)
     if (j3b_for == bd(j4)) {
-        k2s[n1] = j3b;
-        k1s[n1++] = j3b_bac;
+        c1 = j3b_bac;
     }
     else if (eb(j3_bac) && eb(j3b_bac) && bd(j3_bac) == j3b_bac) {
-        k2s[n1] = j3b;
-        k1s[n1++] = j3b_for;
+        c1 = j3b_for;
     }
     else if (eb(j3_bac) && eb(j3b_for) && bd(j3_bac) == j3b_for) {
-        k2s[n1] = j3b;
-        k1s[n1++] = j3b_bac;
+        c1 = j3b_bac;
     }
     else {
-        k2s[n1] = j3b;
-        k1s[n1++] = j3b_for;
-        k2s[n1] = j3b;
-        k1s[n1++] = j3b_bac;
+        c1 = j3b_for;
+        c2 = j3b_bac;
     }
-    for (int q = 0; q < n1; q++) {
-        int k2 = k2s[q], k1 = k1s[q];
-        if (!eb(k1)) {
-            continue;
+    fb_backward_outer(j3, j4, j3, j3_bac);
+    fb_backward_outer(j3, j4, j3b, c1);
+    fb_backward_outer(j3, j4, j3b, c2);
+}
+
+// inner candidates of one outer pair (k1, k2) of check_forward_single_junction
+DEVNI void Eng::fb_forward_outer(int j1, int j2, int k1, int k2) {
+    if (!eb(k2) || pair_stacked(k1, k2)) {
+        return;
+    }
+    int k2_next = walk(k2, d_of(k2) - d_of(k1));
+    int k2b = bd(k2);
+    int k2b_for = fwd(k2b);
+    int k2b_bac = bwd(k2b);
+    int e1 = k2b_for, e2 = k2b_bac;
+    if (eb(k2_next)) {
+        int k2nb = bd(k2_next);
+        if (k2b_for == k2nb) {
+            e1 = LDO_NONE;
         }
-        if (pair_stacked(k1, k2)) {
-            continue;
+        if (k2b_bac == k2nb) {
+            e2 = LDO_NONE;
         }
-        int j2s[3], j1s[3];
-        int n2 = 0;
-        int k1_next = walk(k1, d_of(k1) - d_of(k2));
-        j2s[n2] = k1;
-        j1s[n2++] = k1_next;
-        int k1b = bd(k1);
-        int k1b_for = fwd(k1b);
-        int k1b_bac = bwd(k1b);
-        if (eb(k1_next)) {
-            int k1nb = bd(k1_next);
-            if (k1b_for != k1nb) {
-                j2s[n2] = k1b;
-                j1s[n2++] = k1b_for;
-            }
-            if (k1b_bac != k1nb) {
-                j2s[n2] = k1b;
-                j1s[n2++] = k1b_bac;
-            }
-        }
-        else {
-            j2s[n2] = k1b;
-            j1s[n2++] = k1b_for;
-            j2s[n2] = k1b;
-            j1s[n2++] = k1b_bac;
-        }
-        for (int r = 0; r < n2; r++) {
-            int j2 = j2s[r], j1 = j1s[r];
-            if (!eb(j1)) {
-                continue;
-            }
-            fb_junction(j1, j2, j3, j4, k1, k2);
-        }
+    }
+    if (eb(k2_next)) {
+        fb_junction(j1, j2, k2, k2_next, k1, k2);
+    }
+    if (eb(e1)) {
+        fb_junction(j1, j2, k2b, e1, k1, k2);
+    }
+    if (eb(e2)) {
+        fb_junction(j1, j2, k2b, e2, k1, k2);
     }
 }
 
 // check_forward_single_junction (:750-844)
 DEVNI void Eng::fb_forward_single_junction(int a, int b) {
     int j1 = a, j2 = b;
-    int k1s[3], k2s[3];
-    int n1 = 0;
     int j2_for = fwd(j2);
-    k1s[n1] = j2;
-    k2s[n1++] = j2_for;
     int j1b = bd(j1);
     int j2b = bd(j2);
     int j2b_for = fwd(j2b);
     int j2b_bac = bwd(j2b);
+    int c1, c2 = LDO_NONE; // outer candidates (j2b, c) after (j2, j2_for)
     if (walk(j1b, 1) == j2b) {
-        k1s[n1] = j2b;
-        k2s[n1++] = j2b_for;
+        c1 = j2b_for;
     }
     else if (eb(j2_for) && eb(j2b_bac) && bd(j2_for) == j2b_bac) {
-        k1s[n1] = j2b;
-        k2s[n1++] = j2b_for;
+        c1 = j2b_for;
     }
     else if (eb(j2_for) && eb(j2b_for) && bd(j2_for) == j2b_for) {
-        k1s[n1] = j2b;
-        k2s[n1++] = j2b_bac;
+        c1 = j2b_bac;
     }
     else {
-        k1s[n1] = j2b;
-        k2s[n1++] = j2b_for;
-        k1s[n1] = j2b;
-        k2s[n1++] = j2b_bac;
+        c1 = j2b_for;
+        c2 = j2b_bac;
     }
-    for (int q = 0; q < n1; q++) {
-        int k1 = k1s[q], k2 = k2s[q];
-        if (!eb(k2)) {
-            continue;
+    fb_forward_outer(j1, j2, j2, j2_for);
+    fb_forward_outer(j1, j2, j2b, c1);
+    fb_forward_outer(j1, j2, j2b, c2);
+}
+
+// inner candidates of one outer pair (j2, j1) of check_central_single_junction
+DEVNI void Eng::fb_central_outer(int j1, int j2, int k1, int k2, int k2b) {
+    if (!eb(j1)) {
+        return;
+    }
+    int k2_for = fwd(k2);
+    int k2b_for = fwd(k2b);
+    int k2b_bac = bwd(k2b);
+    int e1 = k2b_for, e2 = k2b_bac;
+    if (eb(k2_for)) {
+        int k2fb = bd(k2_for);
+        if (k2b_for == k2fb) {
+            e1 = LDO_NONE;
         }
-        if (pair_stacked(k1, k2)) {
-            continue;
+        if (k2b_bac == k2fb) {
+            e2 = LDO_NONE;
         }
-        int j3s[3], j4s[3];
-        int n2 = 0;
-        int k2_next = walk(k2, d_of(k2) - d_of(k1));
-        j3s[n2] = k2;
-        j4s[n2++] = k2_next;
-        int k2b = bd(k2);
-        int k2b_for = fwd(k2b);
-        int k2b_bac = bwd(k2b);
-        if (eb(k2_next)) {
-            int k2nb = bd(k2_next);
-            if (k2b_for != k2nb) {
-                j3s[n2] = k2b;
-                j4s[n2++] = k2b_for;
-            }
-            if (k2b_bac != k2nb) {
-                j3s[n2] = k2b;
-                j4s[n2++] = k2b_bac;
-            }
-        }
-        else {
-            j3s[n2] = k2b;
-            j4s[n2++] = k2b_for;
-            j3s[n2] = k2b;
-            j4s[n2++] = k2b_bac;
-        }
-        for (int r = 0; r < n2; r++) {
-            int j3 = j3s[r], j4 = j4s[r];
-            if (!eb(j4)) {
-                continue;
-            }
-            fb_junction(j1, j2, j3, j4, k1, k2);
-        }
+    }
+    if (eb(k2_for)) {
+        fb_junction(j1, j2, k2, k2_for, k1, k2);
+    }
+    if (eb(e1)) {
+        fb_junction(j1, j2, k2b, e1, k1, k2);
+    }
+    if (eb(e2)) {
+        fb_junction(j1, j2, k2b, e2, k1, k2);
     }
 }
 
@@ -415,67 +421,22 @@ DEVNI void Eng::fb_central_single_junction(int a, int b) {
             return;
         }
     }
-    int j2s[3], j1s[3];
-    int n1 = 0;
     int k1_bac = bwd(k1);
-    j2s[n1] = k1;
-    j1s[n1++] = k1_bac;
     int k1b_for = fwd(k1b);
     int k1b_bac = bwd(k1b);
+    int e1 = k1b_for, e2 = k1b_bac; // outer candidates (k1b, e) after (k1, k1_bac)
     if (eb(k1_bac)) {
         int k1bacb = bd(k1_bac);
-        if (k1b_for != k1bacb) {
-            j2s[n1] = k1b;
-            j1s[n1++] = k1b_for;
+        if (k1b_for == k1bacb) {
+            e1 = LDO_NONE;
         }
-        if (k1b_bac != k1bacb) {
-            j2s[n1] = k1b;
-            j1s[n1++] = k1b_bac;
+        if (k1b_bac == k1bacb) {
+            e2 = LDO_NONE;
         }
     }
-    else {
-        j2s[n1] = k1b;
-        j1s[n1++] = k1b_for;
-        j2s[n1] = k1b;
-        j1s[n1++] = k1b_bac;
-    }
-    for (int q = 0; q < n1; q++) {
-        int j2 = j2s[q], j1 = j1s[q];
-        if (!eb(j1)) {
-            continue;
-        }
-        int j3s[3], j4s[3];
-        int n2 = 0;
-        int k2_for = fwd(k2);
-        j3s[n2] = k2;
-        j4s[n2++] = k2_for;
-        int k2b_for = fwd(k2b);
-        int k2b_bac = bwd(k2b);
-        if (eb(k2_for)) {
-            int k2fb = bd(k2_for);
-            if (k2b_for != k2fb) {
-                j3s[n2] = k2b;
-                j4s[n2++] = k2b_for;
-            }
-            if (k2b_bac != k2fb) {
-                j3s[n2] = k2b;
-                j4s[n2++] = k2b_bac;
-            }
-        }
-        else {
-            j3s[n2] = k2b;
-            j4s[n2++] = k2b_for;
-            j3s[n2] = k2b;
-            j4s[n2++] = k2b_bac;
-        }
-        for (int r = 0; r < n2; r++) {
-            int j3 = j3s[r], j4 = j4s[r];
-            if (!eb(j4)) {
-                continue;
-            }
-            fb_junction(j1, j2, j3, j4, k1, k2);
-        }
-    }
+    fb_central_outer(k1_bac, k1, k1, k2, k2b);
+    fb_central_outer(e1, k1b, k1, k2, k2b);
+    fb_central_outer(e2, k1b, k1, k2, k2b);
 }
 
 // check_regular_pair_constraints (:288-320)

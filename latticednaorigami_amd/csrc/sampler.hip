@@ -312,16 +312,6 @@ int guarded(Ctx* c, F f) {
     return LDO_OK;
 }
 
-// The engine kernels (sweep, exchange, dependent quantities) took an
-// illegal-address fault in their first run on an MI355X.  The cause has not
-// been identified: the same code is clean under AddressSanitizer and
-// UndefinedBehaviorSanitizer in a host build over hundreds of seeds.  Until a
-// cause is shown the engine stays switched off by default.
-bool engine_enabled() {
-    const char* v = std::getenv("LDO_ENABLE_UNVERIFIED_ENGINE");
-    return v != nullptr && std::strcmp(v, "1") == 0;
-}
-
 void free_ctx(Ctx* c) {
     if (c->stream) {
         (void)hipStreamSynchronize(c->stream);
@@ -355,12 +345,6 @@ int ldo_create(const ldo_system_desc* sys, const ldo_run_params* run, const ldo_
                int32_t n_replicas, const double* temps, void** out_ctx) {
     Ctx* c = new Ctx;
     int rc = guarded(nullptr, [&]() {
-        if (!engine_enabled()) {
-            throw AbiError {LDO_E_NOTIMPL,
-                            "the HIP engine is switched off: its sweep kernel took an "
-                            "unexplained illegal-address fault on gfx950; set "
-                            "LDO_ENABLE_UNVERIFIED_ENGINE=1 to run it anyway"};
-        }
         if (!init || !out_ctx) {
             throw AbiError {LDO_E_ARG, "null argument or no replicas"};
         }

@@ -4,13 +4,7 @@ import pytest
 
 import parity_cases as pc
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="the HIP engine's sweep kernel took an unexplained illegal-address "
-                            "fault in its first MI355X run (four-domain system, 330 K, "
-                            "lanes_per_replica=64); the engine stays off the GPU until a cause "
-                            "is shown"),
-]
+pytestmark = [pytest.mark.gpu]
 
 
 @pytest.fixture(scope="module")
