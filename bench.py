@@ -113,7 +113,7 @@ def gpu_leg(args, system):
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS if achieved is not None else None,
         "traffic": None,
-        "kernel": smp.lib.ldo_kernel_name().decode(),
+        "kernel": smp.lib.ldo_kernel_name(smp.h).decode(),
         "launches": prof["sweep_launches"],
         "avg_launch_ms": prof["sweep_ms"] / max(1, prof["sweep_launches"]),
         "algorithmic_bytes_per_launch": nbytes / max(1, prof["sweep_launches"]),

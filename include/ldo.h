@@ -252,8 +252,10 @@ int ldo_get_permutation(void* ctx, int32_t* q_to_rep);
 /* The HIP stream the sampler's kernels are launched on (hipStream_t). */
 void* ldo_get_stream(void* ctx);
 
-/* Name of the sampler kernel, for matching profiler output. */
-const char* ldo_kernel_name(void);
+/* Name of the sweep kernel this context launches (ldo_sweep_kernel, or
+ * ldo_sweep_lds_kernel when every replica block is held in LDS), for matching
+ * profiler output. */
+const char* ldo_kernel_name(void* ctx);
 
 /* Kernel timing with HIP events recorded on the sampler's stream around
  * every sweep-kernel and exchange-kernel launch of ldo_sweep / ldo_run.

@@ -203,7 +203,7 @@ def load_library(path=None):
     lib.ldo_get_stream.restype = C.c_void_p
     lib.ldo_get_stream.argtypes = [C.c_void_p]
     lib.ldo_kernel_name.restype = C.c_char_p
-    lib.ldo_kernel_name.argtypes = []
+    lib.ldo_kernel_name.argtypes = [C.c_void_p]
     lib.ldo_set_profiling.restype = C.c_int
     lib.ldo_set_profiling.argtypes = [C.c_void_p, C.c_int32]
     lib.ldo_get_profile.restype = C.c_int

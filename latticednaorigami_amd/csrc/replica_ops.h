@@ -89,9 +89,11 @@ DEV void init_replica(const Params& P, int r, const int32_t* c_idx, const int32_
 // (update_control_qs: a temperature switch recomputes the energy; umbrella
 // windows share one temperature and only change the bias) and ends with its
 // dependent quantities.
-DEV void sweep_replica(const Params& P, int r, int64_t steps, const int32_t* rep_slot) {
-    Eng e;
-    e.attach(&P, P.blocks + (size_t)r * P.L.bytes, r);
+// One sweep of replica r on a caller-provided engine object and replica block
+// (the block in device memory, or a copy of it held closer to the lane).
+DEV void sweep_replica_on(Eng& e, const Params& P, int r, char* blk, int64_t steps,
+                          const int32_t* rep_slot) {
+    e.attach(&P, blk, r);
     if (e.S->status != ST_OK) {
         return;
     }
@@ -102,6 +104,11 @@ DEV void sweep_replica(const Params& P, int r, int64_t steps, const int32_t* rep
     if (rep_slot != nullptr) {
         e.update_enthalpy_and_entropy();
     }
+}
+
+DEV void sweep_replica(const Params& P, int r, int64_t steps, const int32_t* rep_slot) {
+    Eng e;
+    sweep_replica_on(e, P, r, P.blocks + (size_t)r * P.L.bytes, steps, rep_slot);
 }
 
 // OrigamiSystem::update_enthalpy_and_entropy (origami_system.cpp:204-246)

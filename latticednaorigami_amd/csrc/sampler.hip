@@ -7,12 +7,16 @@
 //                        steps of GCMCSimulation::simulate; in PTMC mode it
 //                        first switches to its ladder slot (update_control_qs)
 //                        and ends with its dependent quantities
+//   ldo_sweep_lds_kernel the same sweep with the replica held in LDS: one
+//                        workgroup per replica copies its block, the launch
+//                        parameters and the engine object into LDS, runs the
+//                        chain there and writes the block back
 //   ldo_exchange_kernel  one PTMC exchange round over even/odd ladder pairs
 //                        (OneDPTGCMCSimulation::attempt_exchange)
 // A replica is driven by one lane.  With lanes_per_replica = 64 every
 // replica has a wavefront of its own (lane 0 works), which keeps replicas on
-// separate SIMDs when there are few of them; with 1, 64 replicas share a
-// wavefront.
+// separate CUs when there are few of them, and its block is held in that
+// CU's LDS when it fits; with 1, 64 replicas share a wavefront.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -60,6 +64,38 @@ extern "C" __global__ void ldo_sweep_kernel(Params P, int64_t steps, const int32
     int r = replica_of_thread(P, active);
     if (active) {
         sweep_replica(P, r, steps, rep_slot);
+    }
+}
+
+// One workgroup of 64 lanes per replica.  Every access of the Markov chain
+// to its replica state (the block, the engine's section pointers and the
+// launch parameters) goes to LDS instead of device memory or scratch.  The
+// dynamic LDS holds the block (P.L.bytes, a multiple of 256); the host picks
+// this kernel only when that fits.
+extern "C" __global__ void __launch_bounds__(64)
+    ldo_sweep_lds_kernel(Params P, int64_t steps, const int32_t* rep_slot) {
+    extern __shared__ uint4 lds_block[];
+    __shared__ Params lds_params;
+    __shared__ Eng lds_engine;
+    int r = blockIdx.x;
+    if (r >= P.n_replicas) {
+        return;
+    }
+    size_t n16 = (size_t)P.L.bytes / 16;
+    uint4* g = (uint4*)(P.blocks + (size_t)r * P.L.bytes);
+    for (size_t i = threadIdx.x; i < n16; i += blockDim.x) {
+        lds_block[i] = g[i];
+    }
+    if (threadIdx.x == 0) {
+        lds_params = P;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sweep_replica_on(lds_engine, lds_params, r, (char*)lds_block, steps, rep_slot);
+    }
+    __syncthreads();
+    for (size_t i = threadIdx.x; i < n16; i += blockDim.x) {
+        g[i] = lds_block[i];
     }
 }
 
@@ -132,6 +168,7 @@ struct Ctx {
     double* d_grid = nullptr;
     double* d_win_stored = nullptr;
     std::vector<ldo_window> windows; // host copy
+    bool lds_sweep = false;           // sweep with the replica blocks in LDS
 };
 
 struct HipError {
@@ -252,6 +289,48 @@ dim3 grid_of(const Ctx& c) {
         return dim3((c.R + 63) / 64);
     }
     return dim3(c.R);
+}
+
+// Whether each replica's block can live in its workgroup's LDS during a
+// sweep: one wavefront per replica, and the block plus the kernel's static
+// LDS within what the device grants a workgroup.
+void choose_sweep_kernel(Ctx& c) {
+    c.lds_sweep = false;
+    const char* force = std::getenv("LDO_SWEEP_KERNEL");
+    if (force != nullptr && std::strcmp(force, "global") == 0) {
+        return;
+    }
+    if (c.P.lanes_per_replica != 64) {
+        return;
+    }
+    int dev = 0, optin = 0;
+    hip_check(hipGetDevice(&dev), "get device");
+    hip_check(hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev),
+              "shared memory opt-in limit");
+    hipFuncAttributes fa;
+    hip_check(hipFuncGetAttributes(&fa, (const void*)ldo_sweep_lds_kernel), "kernel attributes");
+    size_t need = (size_t)c.P.L.bytes + fa.sharedSizeBytes;
+    if (optin <= 0 || need > (size_t)optin) {
+        return;
+    }
+    hip_check(hipFuncSetAttribute((const void*)ldo_sweep_lds_kernel,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.P.L.bytes),
+              "dynamic LDS size");
+    c.lds_sweep = true;
+}
+
+void launch_sweep(Ctx& c, int64_t steps, const int32_t* rep_slot) {
+    launch_timed(c, 0, [&]() {
+        if (c.lds_sweep) {
+            hipLaunchKernelGGL(ldo_sweep_lds_kernel, dim3(c.R), dim3(64), (size_t)c.P.L.bytes,
+                               c.stream, c.P, steps, rep_slot);
+        }
+        else {
+            hipLaunchKernelGGL(ldo_sweep_kernel, grid_of(c), dim3(64), 0, c.stream, c.P, steps,
+                               rep_slot);
+        }
+    });
+    hip_check(hipGetLastError(), "launch sweep");
 }
 
 void upload_config(Ctx& c, const ldo_config* cfg, int& n_domains) {
@@ -391,6 +470,7 @@ int ldo_create(const ldo_system_desc* sys, const ldo_run_params* run, const ldo_
                            (const int32_t*)nullptr);
         hip_check(hipGetLastError(), "launch init");
         check_statuses(*c);
+        choose_sweep_kernel(*c);
     });
     if (rc != LDO_OK) {
         free_ctx(c);
@@ -516,11 +596,7 @@ int ldo_sweep(void* ctx, int64_t steps) {
         if (steps < 0) {
             throw AbiError {LDO_E_ARG, "negative step count"};
         }
-        launch_timed(*c, 0, [&]() {
-            hipLaunchKernelGGL(ldo_sweep_kernel, grid_of(*c), dim3(64), 0, c->stream, c->P,
-                               steps, (const int32_t*)nullptr);
-        });
-        hip_check(hipGetLastError(), "launch sweep");
+        launch_sweep(*c, steps, nullptr);
         check_statuses(*c);
     });
 }
@@ -541,11 +617,7 @@ int ldo_run(void* ctx, int64_t n_swaps, int64_t interval) {
         X.counts = c->d_xcnt;
         for (int64_t s = 0; s < n_swaps; s++) {
             int64_t swap_i = ++c->swap_counter;
-            launch_timed(*c, 0, [&]() {
-                hipLaunchKernelGGL(ldo_sweep_kernel, grid_of(*c), dim3(64), 0, c->stream, c->P,
-                                   interval, (const int32_t*)X.rep_slot);
-            });
-            hip_check(hipGetLastError(), "launch sweep");
+            launch_sweep(*c, interval, X.rep_slot);
             launch_timed(*c, 1, [&]() {
                 hipLaunchKernelGGL(ldo_exchange_kernel, dim3(1), dim3(64), 0, c->stream, c->P, X,
                                    swap_i);
@@ -605,7 +677,10 @@ int ldo_get_permutation(void* ctx, int32_t* q_to_rep) {
 
 void* ldo_get_stream(void* ctx) { return ctx ? (void*)static_cast<Ctx*>(ctx)->stream : nullptr; }
 
-const char* ldo_kernel_name(void) { return "ldo_sweep_kernel"; }
+const char* ldo_kernel_name(void* ctx) {
+    const Ctx* c = static_cast<const Ctx*>(ctx);
+    return (c != nullptr && c->lds_sweep) ? "ldo_sweep_lds_kernel" : "ldo_sweep_kernel";
+}
 
 int ldo_set_profiling(void* ctx, int32_t on) {
     Ctx* c = static_cast<Ctx*>(ctx);

@@ -81,6 +81,29 @@ def case_raster_nearest_neighbour_bit_exact(sampler_cls, oracle_mod):
     assert_counters(gpu, ref)
 
 
+def case_sweep_kernels_agree(sampler_cls, oracle_mod):
+    say("case: 64-domain raster, 2 replicas, sweep with blocks in LDS and in device memory, "
+        "3 x 2000 steps")
+    import os
+    s = systems.raster_system(replicas=2)
+    os.environ["LDO_SWEEP_KERNEL"] = "global"
+    try:
+        glob = sampler_cls(s, temps=s.temps)
+    finally:
+        del os.environ["LDO_SWEEP_KERNEL"]
+    lds = sampler_cls(s, temps=s.temps)
+    ref = oracle_mod.Oracle(s, temps=s.temps)
+    for _ in range(3):
+        glob.sweep(2000)
+        lds.sweep(2000)
+        ref.sweep(2000)
+        for r in range(2):
+            assert_same(glob, ref, r)
+            assert_same(lds, ref, r)
+    assert_counters(glob, ref)
+    assert_counters(lds, ref)
+
+
 def case_ptmc_exchange_matches_oracle(sampler_cls, oracle_mod):
     say("case: 64-domain raster, 8-replica PTMC ladder 300-360 K, 12 swaps x 250 steps")
     s = systems.raster_system(replicas=8)

@@ -61,6 +61,10 @@ def test_raster_nearest_neighbour_bit_exact(sampler_cls, oracle_mod):
     pc.case_raster_nearest_neighbour_bit_exact(sampler_cls, oracle_mod)
 
 
+def test_sweep_kernels_agree(sampler_cls, oracle_mod):
+    pc.case_sweep_kernels_agree(sampler_cls, oracle_mod)
+
+
 def test_ptmc_exchange_matches_oracle(sampler_cls, oracle_mod):
     pc.case_ptmc_exchange_matches_oracle(sampler_cls, oracle_mod)
 
