@@ -154,6 +154,20 @@ struct Eng {
     bool net_external;
 
     // ------------------------------------------------------------ wiring
+    // Fills a block section of `bytes` bytes with a 32-bit pattern in 16-byte
+    // stores.  Sections start 16-byte aligned and own the padding up to the
+    // next 16-byte boundary (the replica-block layout rounds every section).
+    DEV void fill_section(void* base, int64_t bytes, uint32_t pattern) {
+        struct alignas(16) Quad {
+            uint32_t a, b, c, d;
+        };
+        Quad q {pattern, pattern, pattern, pattern};
+        Quad* dst = (Quad*)base;
+        for (int64_t i = 0, n = (bytes + 15) / 16; i < n; i++) {
+            dst[i] = q;
+        }
+    }
+
     DEV void attach(const Params* p, char* blk, int r) {
         P = p;
         rep = r;

@@ -310,6 +310,7 @@ DEVNI void Eng::cb_select_and_set(int d, int prev_d) {
     I3 pp = pos(prev_d);
     I3 cpos[6];
     int core[6];
+    int cub[6]; // unbound domain at the trial site, or LDO_NONE for an empty site
     double w[6];
     int n = 0;
     for (int v = 0; v < 6; v++) {
@@ -325,6 +326,7 @@ DEVNI void Eng::cb_select_and_set(int d, int prev_d) {
             if (!S->violated) {
                 cpos[n] = pn;
                 core[n] = on;
+                cub[n] = ub;
                 w[n] = exp(-de);
                 n++;
             }
@@ -335,16 +337,30 @@ DEVNI void Eng::cb_select_and_set(int d, int prev_d) {
         else {
             cpos[n] = pn;
             core[n] = -1;
-            double de = check_domain_constraints(d, pn, 0);
-            w[n] = 6 * exp(-de);
+            cub[n] = LDO_NONE;
+            if (dom[d].state == OCC_UNASSIGNED) {
+                // check_domain_constraints at an empty site, with the
+                // occupancy already known: the domain is left unassigned at
+                // pn and weighs 6 exp(-0) = 6
+                dom[d].x = pn.x;
+                dom[d].y = pn.y;
+                dom[d].z = pn.z;
+                dom[d].ore = 0;
+                S->site_empty++;
+                w[n] = 6.0;
+            }
+            else {
+                double de = check_domain_constraints(d, pn, 0);
+                w[n] = 6 * exp(-de);
+            }
             n++;
         }
     }
     double sum = 0;
     if (S->kind == LDO_MOVE_KIND_CTCB) {
         for (int i = 0; i < n; i++) {
-            if (occupancy(cpos[i]) == OCC_UNBOUND) {
-                int occd = unbound_at(cpos[i]);
+            if (cub[i] >= 0) {
+                int occd = cub[i];
                 bool same_chain = cslot_of(occd) == cslot_of(d);
                 bool endpoint = cp_endpoint_reached(d, cpos[i]);
                 if (!(same_chain || endpoint)) {
@@ -489,21 +505,18 @@ DEVNI bool Eng::cb_staple_move() {
 // ------------------------------------------------- constraint points
 // Constraintpoints::reset_internal (top_constraint_points.cpp:191-205)
 DEVNI void Eng::cp_reset() {
-    for (int s = 0; s < nslots; s++) {
-        seg[s] = -1;
-        gp[s] = LDO_NONE;
-        stem[s] = LDO_NONE;
-        inact[s] = LDO_NONE;
-        netseg[s] = -1;
-        netds_flag[s] = 0;
-    }
-    for (int c = 0; c < nchain; c++) {
-        checked[c] = 0;
-        netcs[c] = 0;
-        regrow[c] = 0;
-        dir0[c] = 0;
-        dir1[c] = 0;
-    }
+    // -1 in every int8 and int16 entry is the byte pattern 0xff
+    fill_section(seg, nslots, 0xffffffffu);
+    fill_section(gp, 2 * (int64_t)nslots, 0xffffffffu);
+    fill_section(stem, 2 * (int64_t)nslots, 0xffffffffu);
+    fill_section(inact, 2 * (int64_t)nslots, 0xffffffffu);
+    fill_section(netseg, nslots, 0xffffffffu);
+    fill_section(netds_flag, nslots, 0u);
+    fill_section(checked, nchain, 0u);
+    fill_section(netcs, nchain, 0u);
+    fill_section(regrow, nchain, 0u);
+    fill_section(dir0, nchain, 0u);
+    fill_section(dir1, nchain, 0u);
     S->n_active = 0;
     S->n_active0 = 0;
     n_netds = 0;

@@ -122,6 +122,19 @@ DEVNI Delta Eng::internal_unassign(int s, bool need_delta) {
 
 // OrigamiSystem::check_domain_constraints (:343-355)
 DEVNI double Eng::check_domain_constraints(int s, I3 p, int o) {
+    // An unassigned domain checked at an empty site: placing it and taking it
+    // off again leaves the site hash as it was (the key is appended at the
+    // end of its probe run, so its removal shifts nothing back) and the
+    // domain unassigned at p with orientation o.  Only that and the event
+    // count remain, so they are written directly.
+    if (dom[s].state == OCC_UNASSIGNED && occupancy(p) == OCC_UNASSIGNED) {
+        dom[s].x = p.x;
+        dom[s].y = p.y;
+        dom[s].z = p.z;
+        dom[s].ore = (int8_t)o;
+        S->site_empty++;
+        return 0;
+    }
     Delta dc = internal_check(s, p, o);
     internal_unassign(s, false);
     return dc.e;
