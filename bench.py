@@ -80,6 +80,95 @@ def cpu_baseline(workers, swaps, interval):
     }
 
 
+def _cpu_batch_worker(args):
+    """One process of the batching CPU baseline: the oracle on every
+    `stride`-th chain of the batch workload, `steps` MC steps each."""
+    first, stride, n_chains, steps = args
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from latticednaorigami_amd import systems
+    ss, ts = systems.batch_systems()
+    refs = [pyoracle.Oracle(ss[k], temps=[ts[k]]) for k in range(first, len(ss), stride)][:n_chains]
+    accepted = 0
+    t0 = time.perf_counter()
+    for ref in refs:
+        ref.sweep(steps)
+    dt = time.perf_counter() - t0
+    for ref in refs:
+        accepted += sum(ref.counters().accepts)
+    return accepted, dt
+
+
+def cpu_baseline_batch(workers, chains_per_worker, steps):
+    from concurrent.futures import ProcessPoolExecutor
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    stride = 4096 // (workers * chains_per_worker)
+    with ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as ex:
+        res = list(ex.map(_cpu_batch_worker,
+                          [(w * stride, workers * stride, chains_per_worker, steps)
+                           for w in range(workers)]))
+    accepted = sum(r[0] for r in res)
+    rate = sum(r[0] / r[1] for r in res)
+    longest = max(r[1] for r in res)
+    return {
+        "value": rate,
+        "unit": "moves/s",
+        "cores": workers,
+        "kind": "port",
+        "sample": (f"{workers} concurrent processes, each {chains_per_worker} chains of the "
+                   f"4096-chain batch (every {stride}-th chain, all lengths represented) for "
+                   f"{steps} MC steps from the initial configuration ({accepted} accepted "
+                   f"moves, longest timed region {longest:.1f} s)"),
+    }
+
+
+def gpu_leg_batch(args):
+    """The batching workload: 4096 independent constant-T chains."""
+    from latticednaorigami_amd import abi, systems
+    from latticednaorigami_amd.sampler import BatchSampler
+    ss, ts = systems.batch_systems()
+    b = BatchSampler(ss, ts)
+    chosen, tune_ms = b.tune(args.tune_steps)
+    if args.batch_kernel is not None:
+        b.set_kernel(args.batch_kernel)
+        chosen = args.batch_kernel
+    for _ in range(args.warmup):
+        b.sweep(args.interval)
+    c0 = b.counters()
+    kernel_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kernel_ms += b.sweep(args.interval)
+    wall = time.perf_counter() - t0
+    c1 = b.counters()
+    accepted = sum(c1.accepts) - sum(c0.accepts)
+    attempted = sum(c1.attempts) - sum(c0.attempts)
+    nbytes = algorithmic_bytes(c0, c1)
+    achieved = nbytes / (kernel_ms / 1e3) / 1e9 if kernel_ms > 0 else None
+    roofline = {
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS if achieved is not None else None,
+        "traffic": None,
+        "kernel": abi.BATCH_KERNEL_NAMES[chosen],
+        "launches": args.steps,
+        "avg_launch_ms": kernel_ms / max(1, args.steps),
+        "algorithmic_bytes_per_launch": nbytes / max(1, args.steps),
+    }
+    extra = {
+        "attempted_per_s": attempted / wall,
+        "accepts_by_move": list(c1.accepts)[:4],
+        "attempts_by_move": list(c1.attempts)[:4],
+        "tune_ms": {abi.BATCH_KERNEL_NAMES[k]: v for k, v in tune_ms.items()},
+    }
+    b.close()
+    return accepted / wall, wall, roofline, extra
+
+
 def gpu_leg(args, system):
     from latticednaorigami_amd import abi
     from latticednaorigami_amd.sampler import Sampler
@@ -139,6 +228,18 @@ def main():
     ap.add_argument("--cpu-workers", type=int, default=0,
                     help="CPU-baseline processes (0: min(8, host cores))")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="raster64_ptmc32",
+                    choices=["raster64_ptmc32", "batch4096"],
+                    help="raster64_ptmc32: the 32-replica PTMC raster run; batch4096: 4096 "
+                         "independent constant-T chains of 8-128 domains")
+    ap.add_argument("--tune-steps", type=int, default=100,
+                    help="batch4096: MC steps per chain in each autotuner trial")
+    ap.add_argument("--batch-kernel", type=int, default=None, choices=[0, 1, 2],
+                    help="batch4096: force a sweep kernel instead of the autotuner's choice")
+    ap.add_argument("--cpu-chains", type=int, default=64,
+                    help="batch4096: chains per CPU-baseline process")
+    ap.add_argument("--cpu-batch-steps", type=int, default=20000,
+                    help="batch4096: MC steps per chain in the CPU baseline")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -146,6 +247,8 @@ def main():
               f"device, so --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "batch4096":
+        return main_batch(args)
     from latticednaorigami_amd import systems
     system = systems.raster_system()
     line = {
@@ -187,6 +290,48 @@ def main():
     if not args.no_cpu_baseline:
         workers = args.cpu_workers or min(8, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(workers, 1, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+    sys.exit(status)
+
+
+def main_batch(args):
+    line = {
+        "metric": "accepted MC moves/sec (all chains, 1 GPU)",
+        "value": None,
+        "unit": "moves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "batch4096",
+            "chains": 4096,
+            "scaffold_domains": [8, 128],
+            "temps_K": [310.0, 350.0],
+            "hybridization": "Uniform",
+            "step": f"{args.interval} MC steps on every chain (one sweep launch)",
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    status = 0
+    try:
+        value, wall, roofline, extra = gpu_leg_batch(args)
+        line["value"] = value
+        line["ms_per_step"] = wall / args.steps * 1e3
+        line["roofline"] = roofline
+        line["config"].update(extra)
+    except Exception as e:
+        line["error"] = f"{type(e).__name__}: {e}"
+        status = 1
+    if not args.no_cpu_baseline:
+        workers = args.cpu_workers or min(8, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline_batch(workers, args.cpu_chains, args.cpu_batch_steps)
     print(json.dumps(line), flush=True)
     sys.exit(status)
 

@@ -267,6 +267,46 @@ int ldo_set_profiling(void* ctx, int32_t on);
 int ldo_get_profile(void* ctx, double* sweep_ms, int64_t* sweep_launches,
                     double* exchange_ms, int64_t* exchange_launches);
 
+/* ------------------------------------------------------------- batching
+ * Many independent constant-temperature chains, each a system of its own
+ * (scaffold, staple types, energies, moveset, seed), advanced together by one
+ * kernel launch per sweep.  The reference runs each such chain as a separate
+ * ConstantTGCMCSimulation process (constant_temp_simulation.hpp:31); no
+ * single reference interface batches them.  Chain k runs exactly the chain a
+ * one-replica context of system k at temps[k] would run (same seed, same
+ * random stream), so ldo_create / ldo_sweep on system k alone reproduces it.
+ * Philox random streams only.  Errors as for ldo_create. */
+int ldo_batch_create(int32_t n_chains, const ldo_system_desc* systems,
+                     const ldo_run_params* runs, const ldo_config* init,
+                     const double* temps, void** out_batch);
+void ldo_batch_destroy(void* batch);
+const char* ldo_batch_last_error(void* batch);
+
+/* Sweep kernels a batch can use: one chain per workgroup with its block in
+ * LDS, one chain per wavefront with its block in device memory, or one chain
+ * per lane (64 chains per wavefront). */
+enum ldo_batch_kernel {
+    LDO_BATCH_LDS = 0,
+    LDO_BATCH_WAVE = 1,
+    LDO_BATCH_LANE = 2
+};
+
+/* Times every available sweep kernel on `steps_per_trial` steps of every
+ * chain (real steps: the chains advance), keeps the fastest and returns it
+ * with the milliseconds of each trial (ms[k] < 0: kernel k unavailable). */
+int ldo_batch_tune(void* batch, int64_t steps_per_trial, int32_t* chosen, double* ms);
+int ldo_batch_set_kernel(void* batch, int32_t kernel);
+
+/* `steps` MC steps on every chain; the kernel time in *kernel_ms. */
+int ldo_batch_sweep(void* batch, int64_t steps, double* kernel_ms);
+
+/* Observables of chains [first, first + n), counters summed over all chains,
+ * and one chain's configuration. */
+int ldo_batch_get_observables(void* batch, int32_t first, int32_t n, ldo_obs* out);
+int ldo_batch_get_counters(void* batch, ldo_counters* out);
+int ldo_batch_get_config(void* batch, int32_t chain, ldo_config* cfg, int32_t cap_chains,
+                         int32_t cap_domains);
+
 #ifdef __cplusplus
 }
 #endif

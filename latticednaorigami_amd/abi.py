@@ -179,6 +179,33 @@ def declare_abi(lib, prefix):
         fn.argtypes = args
 
 
+BATCH_LDS, BATCH_WAVE, BATCH_LANE = 0, 1, 2
+BATCH_KERNEL_NAMES = {BATCH_LDS: "ldo_batch_sweep_lds_kernel",
+                      BATCH_WAVE: "ldo_batch_sweep_kernel (one chain per wavefront)",
+                      BATCH_LANE: "ldo_batch_sweep_kernel (one chain per lane)"}
+
+
+def declare_batch_abi(lib):
+    """Attach argtypes/restype for the ldo_batch_* entry points."""
+    v = C.c_void_p
+    sig = {
+        "create": (C.c_int, [C.c_int32, C.POINTER(SystemDesc), C.POINTER(RunParams),
+                             C.POINTER(Config), _f64p, C.POINTER(C.c_void_p)]),
+        "destroy": (None, [v]),
+        "last_error": (C.c_char_p, [v]),
+        "set_kernel": (C.c_int, [v, C.c_int32]),
+        "tune": (C.c_int, [v, C.c_int64, _i32p, _f64p]),
+        "sweep": (C.c_int, [v, C.c_int64, _f64p]),
+        "get_observables": (C.c_int, [v, C.c_int32, C.c_int32, C.POINTER(Obs)]),
+        "get_counters": (C.c_int, [v, C.POINTER(Counters)]),
+        "get_config": (C.c_int, [v, C.c_int32, C.POINTER(Config), C.c_int32, C.c_int32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, "ldo_batch_" + name)
+        fn.restype = res
+        fn.argtypes = args
+
+
 _LIB = None
 
 
@@ -209,6 +236,7 @@ def load_library(path=None):
     lib.ldo_get_profile.restype = C.c_int
     lib.ldo_get_profile.argtypes = [C.c_void_p, _f64p, C.POINTER(C.c_int64), _f64p,
                                     C.POINTER(C.c_int64)]
+    declare_batch_abi(lib)
     if path is None:
         _LIB = lib
     return lib

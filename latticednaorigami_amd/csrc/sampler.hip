@@ -125,6 +125,89 @@ extern "C" __global__ void ldo_status_kernel(Params P, int32_t* out) {
     }
 }
 
+
+// ---------------------------------------------------------------- batches
+// Chain k of a batch is replica 0 of its own system: plist[k] carries that
+// system's layout, tables and block pointer (P.blocks = the chain's block).
+struct BatchCfg {
+    int64_t off;       // first int of the chain's packed configuration
+    int32_t n_chains;
+    int32_t n_domains;
+};
+
+extern "C" __global__ void ldo_batch_init_kernel(const Params* plist, int32_t n,
+                                                 const int32_t* cfg, const BatchCfg* meta) {
+    int k = blockIdx.x;
+    if (k >= n || threadIdx.x != 0) {
+        return;
+    }
+    const BatchCfg& m = meta[k];
+    const int32_t* h = cfg + m.off;
+    int nc = m.n_chains, nd = m.n_domains;
+    init_replica(plist[k], 0, h, h + nc, h + 2 * nc, h + 2 * nc + 3 * nd, nc, 0, nullptr);
+}
+
+// One workgroup per chain with the chain's block, its parameters and the
+// engine object in LDS (dynamic LDS = the largest block of the batch).
+extern "C" __global__ void __launch_bounds__(64)
+    ldo_batch_sweep_lds_kernel(const Params* plist, int32_t n, int64_t steps) {
+    extern __shared__ uint4 lds_block[];
+    __shared__ Params lds_params;
+    __shared__ Eng lds_engine;
+    int k = blockIdx.x;
+    if (k >= n) {
+        return;
+    }
+    if (threadIdx.x == 0) {
+        lds_params = plist[k];
+    }
+    __syncthreads();
+    size_t n16 = (size_t)lds_params.L.bytes / 16;
+    uint4* g = (uint4*)lds_params.blocks;
+    for (size_t i = threadIdx.x; i < n16; i += blockDim.x) {
+        lds_block[i] = g[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sweep_replica_on(lds_engine, lds_params, 0, (char*)lds_block, steps, nullptr);
+    }
+    __syncthreads();
+    for (size_t i = threadIdx.x; i < n16; i += blockDim.x) {
+        g[i] = lds_block[i];
+    }
+}
+
+// Blocks in device memory: one chain per wavefront (lane 0 works) or, with
+// per_lane, one chain per lane.
+extern "C" __global__ void ldo_batch_sweep_kernel(const Params* plist, int32_t n, int64_t steps,
+                                                  int32_t per_lane) {
+    int k;
+    bool active;
+    if (per_lane) {
+        k = blockIdx.x * blockDim.x + threadIdx.x;
+        active = k < n;
+    }
+    else {
+        k = blockIdx.x;
+        active = threadIdx.x == 0 && k < n;
+    }
+    if (active) {
+        sweep_replica(plist[k], 0, steps, nullptr);
+    }
+}
+
+extern "C" __global__ void ldo_batch_scalars_kernel(const Params* plist, int32_t n,
+                                                    int32_t dependent, RepScalars* out) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) {
+        return;
+    }
+    if (dependent) {
+        dependent_replica(plist[k], 0);
+    }
+    out[k] = *(const RepScalars*)plist[k].blocks;
+}
+
 // ------------------------------------------------------------------- host
 namespace {
 
@@ -703,6 +786,371 @@ int ldo_get_profile(void* ctx, double* sweep_ms, int64_t* sweep_launches, double
         *sweep_launches = c->prof_n[0];
         *exchange_ms = c->prof_ms[1];
         *exchange_launches = c->prof_n[1];
+    });
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------ batch host
+namespace {
+
+struct BatchCtx {
+    int n = 0;
+    std::vector<ldo::Plan> plans;
+    std::vector<Params> params;
+    hipStream_t stream = nullptr;
+    std::string err;
+    char* d_blocks = nullptr;
+    double* d_tab = nullptr;
+    double* d_init = nullptr;
+    int32_t* d_tlen = nullptr;
+    int16_t* d_tident = nullptr;
+    uint64_t* d_tapes = nullptr;
+    int64_t* d_tmeta = nullptr;
+    Params* d_params = nullptr;
+    int32_t* d_cfg = nullptr;
+    BatchCfg* d_meta = nullptr;
+    RepScalars* d_scalars = nullptr;
+    size_t max_block = 0;
+    bool lds_ok = false;
+    int kernel = LDO_BATCH_WAVE;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+std::string g_batch_create_error;
+
+template <typename F>
+int bguarded(BatchCtx* b, F f) {
+    std::string& err = b ? b->err : g_batch_create_error;
+    try {
+        f();
+    } catch (AbiError& e) {
+        err = e.msg;
+        return e.code;
+    } catch (HipError& e) {
+        err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+        return LDO_E_HIP;
+    } catch (std::exception& e) {
+        err = e.what();
+        return LDO_E_CONSTRAINT;
+    }
+    return LDO_OK;
+}
+
+template <typename T>
+void bupload(BatchCtx& b, T*& dst, const std::vector<T>& src, const char* what) {
+    size_t bytes = std::max<size_t>(1, src.size()) * sizeof(T);
+    hip_check(hipMalloc(&dst, bytes), what);
+    if (!src.empty()) {
+        hip_check(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice,
+                                 b.stream),
+                  what);
+        hip_check(hipStreamSynchronize(b.stream), what);
+    }
+}
+
+void bdownload(BatchCtx& b, void* dst, const void* src, size_t bytes, const char* what) {
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, b.stream), what);
+    hip_check(hipStreamSynchronize(b.stream), what);
+}
+
+template <typename T>
+void bfree(T*& p) {
+    if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+    }
+}
+
+void free_batch(BatchCtx* b) {
+    if (b->stream) {
+        (void)hipStreamSynchronize(b->stream);
+    }
+    bfree(b->d_blocks);
+    bfree(b->d_tab);
+    bfree(b->d_init);
+    bfree(b->d_tlen);
+    bfree(b->d_tident);
+    bfree(b->d_tapes);
+    bfree(b->d_tmeta);
+    bfree(b->d_params);
+    bfree(b->d_cfg);
+    bfree(b->d_meta);
+    bfree(b->d_scalars);
+    if (b->ev0) {
+        (void)hipEventDestroy(b->ev0);
+    }
+    if (b->ev1) {
+        (void)hipEventDestroy(b->ev1);
+    }
+    if (b->stream) {
+        (void)hipStreamDestroy(b->stream);
+    }
+    delete b;
+}
+
+void batch_scalars(BatchCtx& b, bool dependent, std::vector<RepScalars>& out) {
+    hipLaunchKernelGGL(ldo_batch_scalars_kernel, dim3((b.n + 63) / 64), dim3(64), 0, b.stream,
+                       b.d_params, b.n, (int32_t)dependent, b.d_scalars);
+    hip_check(hipGetLastError(), "launch scalars");
+    out.resize(b.n);
+    bdownload(b, out.data(), b.d_scalars, (size_t)b.n * sizeof(RepScalars), "copy scalars");
+}
+
+void batch_check(BatchCtx& b) {
+    std::vector<RepScalars> s;
+    batch_scalars(b, false, s);
+    for (int k = 0; k < b.n; k++) {
+        ldo::check_status(s[k].status, k);
+    }
+}
+
+// Launches `steps` steps on every chain with the batch's sweep kernel and
+// returns the kernel time.
+double batch_launch(BatchCtx& b, int kernel, int64_t steps) {
+    hip_check(hipEventRecord(b.ev0, b.stream), "event record");
+    if (kernel == LDO_BATCH_LDS) {
+        hipLaunchKernelGGL(ldo_batch_sweep_lds_kernel, dim3(b.n), dim3(64), b.max_block, b.stream,
+                           b.d_params, b.n, steps);
+    }
+    else if (kernel == LDO_BATCH_WAVE) {
+        hipLaunchKernelGGL(ldo_batch_sweep_kernel, dim3(b.n), dim3(64), 0, b.stream, b.d_params,
+                           b.n, steps, 0);
+    }
+    else {
+        hipLaunchKernelGGL(ldo_batch_sweep_kernel, dim3((b.n + 63) / 64), dim3(64), 0, b.stream,
+                           b.d_params, b.n, steps, 1);
+    }
+    hip_check(hipGetLastError(), "launch batch sweep");
+    hip_check(hipEventRecord(b.ev1, b.stream), "event record");
+    hip_check(hipEventSynchronize(b.ev1), "sync batch sweep");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, b.ev0, b.ev1), "event time");
+    batch_check(b);
+    return ms;
+}
+
+} // namespace
+
+extern "C" {
+
+int ldo_batch_create(int32_t n_chains, const ldo_system_desc* systems, const ldo_run_params* runs,
+                     const ldo_config* init, const double* temps, void** out_batch) {
+    BatchCtx* b = new BatchCtx;
+    int rc = bguarded(nullptr, [&]() {
+        if (n_chains < 1 || !systems || !runs || !init || !temps || !out_batch) {
+            throw AbiError {LDO_E_ARG, "null argument or no chains"};
+        }
+        b->n = n_chains;
+        b->plans.resize(n_chains);
+        std::vector<double> tab, init_e;
+        std::vector<int32_t> tlen, cfg;
+        std::vector<int16_t> tident;
+        std::vector<size_t> tab_off(n_chains), tlen_off(n_chains), tident_off(n_chains);
+        std::vector<size_t> block_off(n_chains);
+        std::vector<BatchCfg> meta(n_chains);
+        size_t blocks = 0;
+        for (int k = 0; k < n_chains; k++) {
+            if (runs[k].rng_mode != LDO_RNG_PHILOX || runs[k].exchange_mode != LDO_EXCHANGE_NONE) {
+                throw AbiError {LDO_E_ARG, "batch chains use Philox streams and no exchange"};
+            }
+            ldo::Plan& pl = b->plans[k];
+            try {
+                ldo::make_plan(&systems[k], &runs[k], 1, &temps[k], pl);
+            } catch (AbiError& e) {
+                throw AbiError {e.code, "chain " + std::to_string(k) + ": " + e.msg};
+            }
+            std::vector<double> t1, i1;
+            ldo::slot_tables(pl, t1, i1);
+            tab_off[k] = tab.size();
+            tab.insert(tab.end(), t1.begin(), t1.end());
+            init_e.insert(init_e.end(), i1.begin(), i1.end());
+            tlen_off[k] = tlen.size();
+            tlen.insert(tlen.end(), pl.type_len.begin(), pl.type_len.end());
+            tident_off[k] = tident.size();
+            tident.insert(tident.end(), pl.type_ident.begin(), pl.type_ident.end());
+            block_off[k] = blocks;
+            blocks += (size_t)pl.P.L.bytes;
+            b->max_block = std::max(b->max_block, (size_t)pl.P.L.bytes);
+            std::vector<int32_t> h;
+            int nd = 0;
+            try {
+                ldo::pack_config(pl, &init[k], h, nd);
+            } catch (AbiError& e) {
+                throw AbiError {e.code, "chain " + std::to_string(k) + ": " + e.msg};
+            }
+            meta[k] = BatchCfg {(int64_t)cfg.size(), init[k].n_chains, nd};
+            cfg.insert(cfg.end(), h.begin(), h.end());
+        }
+        hip_check(hipStreamCreate(&b->stream), "stream");
+        hip_check(hipEventCreate(&b->ev0), "event create");
+        hip_check(hipEventCreate(&b->ev1), "event create");
+        bupload(*b, b->d_tab, tab, "copy tables");
+        bupload(*b, b->d_init, init_e, "copy init");
+        bupload(*b, b->d_tlen, tlen, "copy type_len");
+        bupload(*b, b->d_tident, tident, "copy type_ident");
+        bupload(*b, b->d_tapes, std::vector<uint64_t>(1, 0), "copy tapes");
+        bupload(*b, b->d_tmeta, std::vector<int64_t>(2, 0), "copy tape meta");
+        bupload(*b, b->d_cfg, cfg, "copy configurations");
+        bupload(*b, b->d_meta, meta, "copy configuration index");
+        hip_check(hipMalloc(&b->d_blocks, blocks), "hipMalloc blocks");
+        hip_check(hipMemsetAsync(b->d_blocks, 0, blocks, b->stream), "clear blocks");
+        hip_check(hipMalloc(&b->d_scalars, (size_t)n_chains * sizeof(RepScalars)),
+                  "hipMalloc scalars");
+        b->params.resize(n_chains);
+        for (int k = 0; k < n_chains; k++) {
+            const ldo::Plan& pl = b->plans[k];
+            Params P = pl.P;
+            size_t n2 = (size_t)pl.spec.n_ident * pl.spec.n_ident;
+            double* t0 = b->d_tab + tab_off[k];
+            P.hyb_e = t0;
+            P.hyb_h = t0 + n2;
+            P.hyb_s = t0 + 2 * n2;
+            P.stack = t0 + 3 * n2;
+            P.init_e = b->d_init + 3 * (size_t)k;
+            P.init_h = b->d_init + 3 * (size_t)k + 1;
+            P.init_s = b->d_init + 3 * (size_t)k + 2;
+            P.type_len = b->d_tlen + tlen_off[k];
+            P.type_ident = b->d_tident + tident_off[k];
+            P.tapes = b->d_tapes;
+            P.tape_off = b->d_tmeta;
+            P.tape_len = b->d_tmeta + 1;
+            P.blocks = b->d_blocks + block_off[k];
+            P.windows = nullptr;
+            P.win_grid = nullptr;
+            P.win_stored = nullptr;
+            b->params[k] = P;
+        }
+        bupload(*b, b->d_params, b->params, "copy parameters");
+        hipLaunchKernelGGL(ldo_batch_init_kernel, dim3(n_chains), dim3(64), 0, b->stream,
+                           b->d_params, n_chains, b->d_cfg, b->d_meta);
+        hip_check(hipGetLastError(), "launch batch init");
+        batch_check(*b);
+        // LDS residency when the largest block fits a workgroup's LDS
+        int dev = 0, optin = 0;
+        hip_check(hipGetDevice(&dev), "get device");
+        hip_check(hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev),
+                  "shared memory opt-in limit");
+        hipFuncAttributes fa;
+        hip_check(hipFuncGetAttributes(&fa, (const void*)ldo_batch_sweep_lds_kernel),
+                  "kernel attributes");
+        if (optin > 0 && b->max_block + fa.sharedSizeBytes <= (size_t)optin) {
+            hip_check(hipFuncSetAttribute((const void*)ldo_batch_sweep_lds_kernel,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)b->max_block),
+                      "dynamic LDS size");
+            b->lds_ok = true;
+            b->kernel = LDO_BATCH_LDS;
+        }
+    });
+    if (rc != LDO_OK) {
+        free_batch(b);
+        return rc;
+    }
+    *out_batch = b;
+    return LDO_OK;
+}
+
+void ldo_batch_destroy(void* batch) {
+    if (batch) {
+        free_batch(static_cast<BatchCtx*>(batch));
+    }
+}
+
+const char* ldo_batch_last_error(void* batch) {
+    return batch ? static_cast<BatchCtx*>(batch)->err.c_str() : g_batch_create_error.c_str();
+}
+
+int ldo_batch_set_kernel(void* batch, int32_t kernel) {
+    BatchCtx* b = static_cast<BatchCtx*>(batch);
+    return bguarded(b, [&]() {
+        if (kernel < LDO_BATCH_LDS || kernel > LDO_BATCH_LANE) {
+            throw AbiError {LDO_E_ARG, "unknown batch kernel"};
+        }
+        if (kernel == LDO_BATCH_LDS && !b->lds_ok) {
+            throw AbiError {LDO_E_ARG, "the largest chain block does not fit in LDS"};
+        }
+        b->kernel = kernel;
+    });
+}
+
+int ldo_batch_tune(void* batch, int64_t steps_per_trial, int32_t* chosen, double* ms) {
+    BatchCtx* b = static_cast<BatchCtx*>(batch);
+    return bguarded(b, [&]() {
+        if (steps_per_trial < 1) {
+            throw AbiError {LDO_E_ARG, "steps_per_trial must be positive"};
+        }
+        double best = -1;
+        for (int k = LDO_BATCH_LDS; k <= LDO_BATCH_LANE; k++) {
+            double t = -1;
+            if (k != LDO_BATCH_LDS || b->lds_ok) {
+                t = batch_launch(*b, k, steps_per_trial);
+                if (best < 0 || t < best) {
+                    best = t;
+                    b->kernel = k;
+                }
+            }
+            if (ms) {
+                ms[k] = t;
+            }
+        }
+        if (chosen) {
+            *chosen = b->kernel;
+        }
+    });
+}
+
+int ldo_batch_sweep(void* batch, int64_t steps, double* kernel_ms) {
+    BatchCtx* b = static_cast<BatchCtx*>(batch);
+    return bguarded(b, [&]() {
+        if (steps < 0) {
+            throw AbiError {LDO_E_ARG, "negative step count"};
+        }
+        double t = batch_launch(*b, b->kernel, steps);
+        if (kernel_ms) {
+            *kernel_ms = t;
+        }
+    });
+}
+
+int ldo_batch_get_observables(void* batch, int32_t first, int32_t n, ldo_obs* out) {
+    BatchCtx* b = static_cast<BatchCtx*>(batch);
+    return bguarded(b, [&]() {
+        if (first < 0 || n < 0 || first + n > b->n || (n > 0 && !out)) {
+            throw AbiError {LDO_E_ARG, "chain range out of bounds"};
+        }
+        std::vector<RepScalars> s;
+        batch_scalars(*b, true, s);
+        for (int i = 0; i < n; i++) {
+            ldo::fill_observables(b->plans[first + i], s[first + i], &out[i]);
+            out[i].bias = 0;
+        }
+    });
+}
+
+int ldo_batch_get_counters(void* batch, ldo_counters* out) {
+    BatchCtx* b = static_cast<BatchCtx*>(batch);
+    return bguarded(b, [&]() {
+        std::memset(out, 0, sizeof *out);
+        std::vector<RepScalars> s;
+        batch_scalars(*b, false, s);
+        for (int k = 0; k < b->n; k++) {
+            ldo::add_counters(s[k], out);
+        }
+    });
+}
+
+int ldo_batch_get_config(void* batch, int32_t chain, ldo_config* cfg, int32_t cap_chains,
+                         int32_t cap_domains) {
+    BatchCtx* b = static_cast<BatchCtx*>(batch);
+    return bguarded(b, [&]() {
+        if (chain < 0 || chain >= b->n) {
+            throw AbiError {LDO_E_ARG, "chain out of range"};
+        }
+        const Params& P = b->params[chain];
+        std::vector<char> blk(P.L.bytes);
+        bdownload(*b, blk.data(), P.blocks, P.L.bytes, "copy block");
+        ldo::read_config(b->plans[chain], blk.data(), cfg, cap_chains, cap_domains);
     });
 }
 

@@ -108,3 +108,78 @@ class Sampler:
                                              C.byref(xn)))
         return {"sweep_ms": sm.value, "sweep_launches": sn.value,
                 "exchange_ms": en.value, "exchange_launches": xn.value}
+
+
+class BatchSampler:
+    """Many independent constant-T chains, each a system of its own, advanced
+    by one kernel launch per sweep (the batch C ABI, ldo_batch_*).  Chain k
+    follows exactly the chain Sampler(systems[k], temps=[temps[k]]) follows."""
+
+    def __init__(self, systems, temps, lib=None):
+        if len(systems) != len(temps) or not systems:
+            raise ValueError("one temperature per system, at least one system")
+        self.lib = lib if lib is not None else abi.load_library()
+        self.systems = list(systems)
+        n = self.n = len(systems)
+        packed = [s.to_abi() for s in systems]
+        self._keep = [p[3] for p in packed]
+        sds = (abi.SystemDesc * n)(*[p[0] for p in packed])
+        rps = (abi.RunParams * n)(*[p[1] for p in packed])
+        cfgs = (abi.Config * n)(*[p[2] for p in packed])
+        self._keep += [sds, rps, cfgs]
+        t = (C.c_double * n)(*temps)
+        h = C.c_void_p()
+        rc = self.lib.ldo_batch_create(n, sds, rps, cfgs, t, C.byref(h))
+        if rc != 0:
+            raise abi.LdoError(rc, self.lib.ldo_batch_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise abi.LdoError(rc, self.lib.ldo_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ldo_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tune(self, steps_per_trial):
+        """Time each available sweep kernel on real steps; keep the fastest.
+        Returns (chosen kernel, {kernel: ms or None})."""
+        chosen = C.c_int32()
+        ms = (C.c_double * 3)()
+        self._check(self.lib.ldo_batch_tune(self.h, steps_per_trial, C.byref(chosen), ms))
+        return chosen.value, {k: (ms[k] if ms[k] >= 0 else None) for k in range(3)}
+
+    def set_kernel(self, kernel):
+        self._check(self.lib.ldo_batch_set_kernel(self.h, kernel))
+
+    def sweep(self, steps):
+        """`steps` MC steps on every chain; returns the kernel milliseconds."""
+        ms = C.c_double()
+        self._check(self.lib.ldo_batch_sweep(self.h, steps, C.byref(ms)))
+        return ms.value
+
+    def observables(self, first=0, n=None):
+        n = self.n - first if n is None else n
+        out = (abi.Obs * max(1, n))()
+        self._check(self.lib.ldo_batch_get_observables(self.h, first, n, out))
+        return [out[i].as_dict() for i in range(n)]
+
+    def counters(self):
+        c = abi.Counters()
+        self._check(self.lib.ldo_batch_get_counters(self.h, C.byref(c)))
+        return c
+
+    def config(self, k):
+        s = self.systems[k]
+        cfg, keep = empty_config(s.max_total_staples + 1, s.max_domains())
+        self._check(self.lib.ldo_batch_get_config(self.h, k, C.byref(cfg), s.max_total_staples + 1,
+                                                  s.max_domains()))
+        return config_from_abi(cfg)

@@ -10,7 +10,7 @@ the workload definition.
 import ctypes as C
 import json
 import math
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 from fractions import Fraction
 
 import numpy as np
@@ -344,3 +344,78 @@ def raster_system(n_domains=64, width=8, replicas=32, t_lo=300.0, t_hi=360.0,
         temps=temps,
     )
     return sysd
+
+
+def batch_chain_system(n_domains, width=8, seed=20261202, tiling_seed=1):
+    """One chain of the batching workload: an `n_domains` HalfTurn scaffold on
+    a width-8 boustrophedon raster in z=0 and its two-domain staple types.
+    The staples come from the domino rule of the raster workload on the
+    enclosing full rows (dominoes with both cells on the scaffold), then
+    consecutive scaffold domains left unpaired are paired along the path, so
+    every type binds two lattice-adjacent scaffold domains.  Uniform energies
+    as the four-domain workload, 1 uM staples."""
+    rows = -(-n_domains // width)
+    index_of = {}
+    for i in range(rows * width):
+        col, row, _ = raster_position(i, width)
+        index_of[(row, col)] = i
+    pairs = domino_tiling(rows + (rows % 2), width, 10 * rows * width, tiling_seed)
+    used = set()
+    staples = []
+    for a, b in pairs:
+        i, j = index_of.get(a, n_domains), index_of.get(b, n_domains)
+        if i < n_domains and j < n_domains:
+            i, j = sorted((i, j))
+            staples.append([i + 1, j + 1])
+            used.update((i, j))
+    i = 0
+    while i + 1 < n_domains:
+        if i not in used and i + 1 not in used:
+            staples.append([i + 1, i + 2])
+            used.update((i, i + 1))
+            i += 2
+        else:
+            i += 1
+    staples.sort()
+    n_types = len(staples)
+    return System(
+        identities=[[-(i + 1) for i in range(n_domains)]] + staples,
+        cyclic=False,
+        domain_type=abi.HALFTURN,
+        hybridization_pot=abi.HYB_UNIFORM,
+        misbinding_pot=abi.MISBIND_OPPOSING,
+        binding_h=-120.0 * KCAL_PER_MOL_TO_K,
+        binding_s=-0.33 * KCAL_PER_MOL_TO_K,
+        stacking_ene=-1000.0,
+        cation_M=1.0,
+        staple_M=1e-6,
+        chains=[Chain(0, 0, [raster_position(i, width) for i in range(n_domains)],
+                      [(0, 0, 1)] * n_domains)],
+        moves=standard_moveset(min(12, n_domains)),
+        max_total_staples=2 * n_types,
+        max_type_staples=2,
+        max_staple_size=2,
+        centering_freq=0,
+        constraint_check_freq=0,
+        seed=seed,
+        name=f"{n_domains}-domain HalfTurn raster chain, {n_types} staple types, Uniform",
+        temps=[330.0],
+    )
+
+
+def batch_systems(n_chains=4096, len_lo=8, len_hi=128, t_lo=310.0, t_hi=350.0, length_seed=3,
+                  temp_seed=4, seed=20261202):
+    """The batching workload: `n_chains` independent constant-T chains with
+    scaffold lengths L ~ U{8..128} and temperatures T ~ U(310, 350) K; chain
+    k has seed `seed + k`.  Returns (systems, temperatures)."""
+    lengths = np.random.default_rng(length_seed).integers(len_lo, len_hi + 1, size=n_chains)
+    temps = np.random.default_rng(temp_seed).uniform(t_lo, t_hi, size=n_chains)
+    cache = {}
+    systems = []
+    for k, L in enumerate(lengths):
+        L = int(L)
+        if L not in cache:
+            cache[L] = batch_chain_system(L)
+        s = replace(cache[L], seed=seed + k, temps=[float(temps[k])])
+        systems.append(s)
+    return systems, [float(t) for t in temps]

@@ -172,6 +172,34 @@ def case_windows_exchange_matches_oracle(sampler_cls, oracle_mod):
     assert e.value.code == abi.LDO_E_ARG
 
 
+def case_batch_matches_oracle(batch_cls, oracle_mod):
+    say("case: batch of 12 chains (8-40 domains, 310-350 K), 3 x 500 steps, each segment on "
+        "a different sweep kernel (LDS, wavefront, lane)")
+    ss, ts = systems.batch_systems(12, len_lo=8, len_hi=40)
+    b = batch_cls(ss, ts)
+    refs = [oracle_mod.Oracle(s, temps=[t]) for s, t in zip(ss, ts)]
+    for kernel in (abi.BATCH_LDS, abi.BATCH_WAVE, abi.BATCH_LANE):
+        b.set_kernel(kernel)
+        b.sweep(500)
+        obs = b.observables()
+        for k, ref in enumerate(refs):
+            ref.sweep(500)
+            a, r = obs[k], ref.observables(0)
+            for key in EXACT_KEYS:
+                assert a[key] == r[key], (kernel, k, key, a[key], r[key])
+            for key in ("energy", "enthalpy", "entropy", "stacking_energy"):
+                assert a[key] == pytest.approx(r[key], rel=ENERGY_RTOL, abs=1e-12), (kernel, k, key)
+            assert b.config(k) == ref.config(0), (kernel, k)
+    tot = b.counters()
+    sums = [ref.counters() for ref in refs]
+    assert list(tot.attempts) == [sum(c.attempts[i] for c in sums) for i in range(len(tot.attempts))]
+    assert list(tot.accepts) == [sum(c.accepts[i] for c in sums) for i in range(len(tot.accepts))]
+    for key in ("site_empty", "site_bind", "commits"):
+        assert getattr(tot, key) == sum(getattr(c, key) for c in sums), key
+    chosen, ms = b.tune(20)
+    assert chosen in ms and ms[chosen] is not None
+
+
 def case_reference_rng_tape_replay(sampler_cls, oracle_mod):
     say("case: four-domain system, mt19937_64 tape recorded by the oracle, 30000 steps")
     s = systems.four_domain_system()

@@ -37,6 +37,11 @@ def test_windows_exchange_matches_oracle(sampler_cls, oracle_mod):
     pc.case_windows_exchange_matches_oracle(sampler_cls, oracle_mod)
 
 
+def test_batch_matches_oracle(oracle_mod):
+    from latticednaorigami_amd.sampler import BatchSampler
+    pc.case_batch_matches_oracle(BatchSampler, oracle_mod)
+
+
 def test_reference_rng_tape_replay(sampler_cls, oracle_mod):
     pc.case_reference_rng_tape_replay(sampler_cls, oracle_mod)
 
