@@ -136,8 +136,27 @@ typedef struct ldo_run_params {
     int32_t rng_mode;      /* ldo_rng_mode */
     int32_t exchange_mode; /* ldo_exchange_mode */
     int32_t lanes_per_replica; /* 0: pick automatically; 1 or 64 */
-    int32_t pad0;
+    int32_t flags;             /* ldo_run_flags, 0 = the reference's behaviour */
 } ldo_run_params;
+
+/* Opt-in deviations from the reference (ldo_run_params.flags).
+ *
+ * LDO_FLAG_CTCB_SCAFFOLD_ENDPOINTS: when a staple network bound to the
+ * regrown scaffold segment is also bound outside it, the reference turns
+ * each in-segment (staple domain, scaffold domain) pair into an active
+ * endpoint keyed by the staple domain's chain
+ * (Constraintpoints::add_active_endpoints_on_scaffold,
+ * top_constraint_points.cpp:553-557), an endpoint no scaffold growth ever
+ * consults.  A staple of three or more domains bound across the segment
+ * boundary then leaves its in-segment scaffold domains unconstrained: CTCB
+ * regrowth pulls them off the staple while both Rosenbluth weights drop the
+ * binding trial, so the move does not satisfy detailed balance.  With this
+ * flag the endpoint is placed on the scaffold domain at its own position, as
+ * the growthpoint pairs of the same function are.  Two-domain staples that
+ * bind only the scaffold never reach this code path. */
+enum ldo_run_flags {
+    LDO_FLAG_CTCB_SCAFFOLD_ENDPOINTS = 1
+};
 
 /* One configuration: the Chain records of origami_system.hpp:46-62. */
 typedef struct ldo_config {

@@ -27,6 +27,7 @@ MOVE_MET_STAPLE_EXCHANGE = 1
 MOVE_CB_STAPLE_REGROWTH = 2
 MOVE_CTCB_SCAFFOLD_REGROWTH = 3
 RNG_PHILOX, RNG_TAPE = 0, 1
+FLAG_CTCB_SCAFFOLD_ENDPOINTS = 1
 EXCHANGE_NONE, EXCHANGE_TEMPERATURE, EXCHANGE_WINDOWS = 0, 1, 2
 OP_NUMFULLDOMAINS, OP_NUMSTAPLES = 0, 1
 MAX_MOVETYPES = 8
@@ -84,7 +85,7 @@ class RunParams(C.Structure):
         ("rng_mode", C.c_int32),
         ("exchange_mode", C.c_int32),
         ("lanes_per_replica", C.c_int32),
-        ("pad0", C.c_int32),
+        ("flags", C.c_int32),
     ]
 
 

@@ -530,8 +530,8 @@ DEVNI void Eng::cp_add_active(int s, I3 p, int sg) {
     }
     Endpoint& e = active[S->n_active++];
     e.cslot = (int16_t)cslot_of(s);
-    e.seg = (int8_t)sg;
-    e.d = (int8_t)d_of(s);
+    e.seg = sg;
+    e.d = (int16_t)d_of(s);
     e.x = p.x;
     e.y = p.y;
     e.z = p.z;
@@ -755,10 +755,13 @@ DEVNI void Eng::cp_find(int first, int count, int dir) {
                     cp_add_active(g1, pos(g1), 0);
                 }
             }
+            // the reference keys these endpoints by the staple domain (a1),
+            // where no scaffold growth looks for them
+            bool on_scaffold = (P->flags & LDO_FLAG_CTCB_SCAFFOLD_ENDPOINTS) != 0;
             for (int a = 0; a < n_iaes; a++) {
                 int a1 = iaes[2 * a], a2 = iaes[2 * a + 1];
                 if (cslot_of(a2) == 0) {
-                    cp_add_active(a1, pos(a2), 0);
+                    cp_add_active(on_scaffold ? a2 : a1, pos(a2), 0);
                 }
             }
         }

@@ -48,8 +48,8 @@ struct ChainRec {
 // Active endpoint of the constraint-point machinery, 16 bytes.
 struct Endpoint {
     int16_t cslot; // chain slot of the endpoint domain's chain
-    int8_t seg;
-    int8_t d;      // domain index within the chain
+    int16_t d;     // domain index within the chain (scaffolds up to 30000 domains)
+    int32_t seg;
     int32_t x, y, z;
 };
 
@@ -150,6 +150,7 @@ struct Params {
     int32_t n_ident;
     int32_t n_slots; // ladder slots with tables
     int32_t lanes_per_replica;
+    int32_t flags;   // ldo_run_flags
     double fugacity; // reduced fugacity = staple_M (never updated)
     const int32_t* type_len;   // [ntypes]
     const int16_t* type_ident; // [ntypes * type_ident_stride] table index of each domain

@@ -23,14 +23,18 @@ Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap) {
     L.nchain = 1 + smax;
     L.ntypes = ntypes;
     L.tcap = tcap;
+    // at most nslots occupied sites: load factor <= 1/2
     int hcap = 64, hbits = 6;
-    while (hcap < 4 * L.nslots) {
+    while (hcap < 2 * L.nslots) {
         hcap *= 2;
         hbits++;
     }
     L.hcap = hcap;
     L.hbits = hbits;
-    L.ecap = 4 * L.nslots + 16;
+    // one CTCB pass adds at most one active endpoint per domain of the
+    // scanned networks (find_growthpoints_endpoints) and one per inactive
+    // endpoint reached (update_endpoints), plus the segment ends
+    L.ecap = 2 * L.nslots + 16;
     int64_t off = round_up(sizeof(RepScalars), 16);
     auto take = [&](int64_t& field, int64_t bytes) {
         field = off;
@@ -193,15 +197,12 @@ void make_plan(const ldo_system_desc* sys, const ldo_run_params* run, int32_t n_
         throw AbiError {LDO_E_ARG, "lanes_per_replica must be 0, 1 or 64"};
     }
     P.lanes_per_replica = lpr;
+    if (run->flags & ~LDO_FLAG_CTCB_SCAFFOLD_ENDPOINTS) {
+        throw AbiError {LDO_E_ARG, "unknown ldo_run_params.flags bits"};
+    }
+    P.flags = run->flags;
     P.n_slots = n_replicas;
     P.exchange_mode = run->exchange_mode;
-    if (run->exchange_mode == LDO_EXCHANGE_WINDOWS) {
-        for (int r = 1; r < n_replicas; r++) {
-            if (temps[r] != temps[0]) {
-                throw AbiError {LDO_E_ARG, "umbrella windows share one temperature"};
-            }
-        }
-    }
     // chain lengths by identity, and the table index of every domain of
     // every chain type
     out.type_len.assign(sys->n_chain_types, 0);

@@ -86,8 +86,9 @@ DEV void init_replica(const Params& P, int r, const int32_t* c_idx, const int32_
 
 // `steps` steps of GCMCSimulation::simulate (simulation.cpp:568-653); with a
 // ladder (rep_slot) the replica first takes its slot's control variables
-// (update_control_qs: a temperature switch recomputes the energy; umbrella
-// windows share one temperature and only change the bias) and ends with its
+// (update_control_qs: a temperature switch recomputes the energy; windows
+// swap only within a ladder of one temperature and change only the bias) and
+// ends with its
 // dependent quantities.
 // One sweep of replica r on a caller-provided engine object and replica block
 // (the block in device memory, or a copy of it held closer to the lane).
@@ -222,10 +223,9 @@ DEV double exchange_grid(const Params& P, int q, int p) {
 // adjacent windows swap configurations when each window's order parameter
 // lies inside the other window.  Windows are ladder slots here, so a swap
 // exchanges the two replicas' slots; each window keeps its stored biases.
-DEV void exchange_windows_round(const Params& P, const XState& X, int64_t swap_i) {
-    int W = P.n_replicas;
-    uint32_t idx = 0;
-    for (int i = (int)(swap_i % 2); i < W - 1; i += 2) {
+DEV void exchange_window_group(const Params& P, const XState& X, int64_t swap_i, int g0, int g1,
+                               uint32_t& idx) {
+    for (int i = g0 + (int)(swap_i % 2); i < g1 - 1; i += 2) {
         X.counts[0]++;
         const ldo_window& w1 = P.windows[i];
         const ldo_window& w2 = P.windows[i + 1];
@@ -249,6 +249,19 @@ DEV void exchange_windows_round(const Params& P, const XState& X, int64_t swap_i
             X.rep_slot[r2] = i;
             X.rep_slot[r1] = i + 1;
         }
+    }
+}
+
+// One exchange round over every window ladder.
+DEV void exchange_windows_round(const Params& P, const XState& X, int64_t swap_i) {
+    int W = P.n_replicas;
+    uint32_t idx = 0;
+    // each run of consecutive slots at one temperature is a window ladder of
+    // its own (one PTMWUS run of the reference), paired from its first slot
+    for (int g0 = 0, g1; g0 < W; g0 = g1) {
+        for (g1 = g0 + 1; g1 < W && X.ladder_t[g1] == X.ladder_t[g0]; g1++) {
+        }
+        exchange_window_group(P, X, swap_i, g0, g1, idx);
     }
 }
 

@@ -67,6 +67,7 @@ class System:
     seed: int = 0
     name: str = ""
     temps: list = None
+    flags: int = 0  # ldo_run_flags (abi.FLAG_*); 0 reproduces the reference
 
     # ------------------------------------------------------------- sizes
     @property
@@ -130,6 +131,7 @@ class System:
         rp.rng_mode = rng_mode
         rp.exchange_mode = exchange_mode
         rp.lanes_per_replica = lanes_per_replica
+        rp.flags = self.flags
 
         cfg = config_to_abi(self.chains, keep)
         return sd, rp, cfg, keep
@@ -257,6 +259,36 @@ def four_domain_system(seed=20261202, max_total_staples=4):
     )
 
 
+def three_domain_system(domain_type=abi.HALFTURN, misbinding_pot=abi.MISBIND_OPPOSING,
+                        max_total_staples=1, seed=20261202, flags=0):
+    """An enumerable system with one three-domain staple type: a 3-domain
+    scaffold [-1,-2,-3] and staple [1,2,3], energies and moveset as the
+    four-domain workload (CTCB max_regrowth 3)."""
+    return System(
+        identities=[[-1, -2, -3], [1, 2, 3]],
+        cyclic=False,
+        domain_type=domain_type,
+        hybridization_pot=abi.HYB_UNIFORM,
+        misbinding_pot=misbinding_pot,
+        binding_h=-120.0 * KCAL_PER_MOL_TO_K,
+        binding_s=-0.33 * KCAL_PER_MOL_TO_K,
+        stacking_ene=-1000.0,
+        cation_M=1.0,
+        staple_M=1e-6,
+        chains=[Chain(0, 0, [(i, 0, 0) for i in range(3)], [(0, 0, 1)] * 3)],
+        moves=standard_moveset(3),
+        max_total_staples=max_total_staples,
+        max_type_staples=max_total_staples,
+        max_staple_size=3,
+        centering_freq=10000,
+        constraint_check_freq=10000,
+        seed=seed,
+        name="three-domain scaffold, one three-domain staple type, Uniform",
+        temps=[349.0],
+        flags=flags,
+    )
+
+
 def raster_position(i, width):
     row, col = divmod(i, width)
     if row % 2 == 1:
@@ -344,6 +376,138 @@ def raster_system(n_domains=64, width=8, replicas=32, t_lo=300.0, t_hi=360.0,
         temps=temps,
     )
     return sysd
+
+
+def layered_position(i, width, layers_cells):
+    """Position of scaffold domain i on a stack of width x width boustrophedon
+    layers: even layers run the raster forwards, odd layers backwards, so the
+    path steps +z between layers."""
+    layer, k = divmod(i, layers_cells)
+    if layer % 2 == 1:
+        k = layers_cells - 1 - k
+    col, row, _ = raster_position(k, width)
+    return (col, row, layer)
+
+
+def _is_path3(cells):
+    """Three lattice cells forming a path a-b-c: returns (a, b, c) or None."""
+    def adj(u, v):
+        return sum(abs(x - y) for x, y in zip(u, v)) == 1
+    for mid in cells:
+        ends = [c for c in cells if c != mid]
+        if adj(mid, ends[0]) and adj(mid, ends[1]):
+            return ends[0], mid, ends[1]
+    return None
+
+
+def tromino_tiling(path, flips, seed):
+    """Seeded random tiling of the cells of `path` (a lattice path) into
+    paths of three cells, plus one pair when len(path) % 3 == 2: start from
+    consecutive triples along the path and apply `flips` random retilings of
+    two adjacent triples (a uniform choice among the ways to split their six
+    cells into two paths of three).  Returns a list of cell tuples."""
+    import itertools
+    rng = np.random.default_rng(seed)
+    n = len(path)
+    pieces = [tuple(path[i:i + 3]) for i in range(0, n - n % 3, 3)]
+    rest = tuple(path[n - n % 3:])
+    owner = {}
+    for k, pc in enumerate(pieces):
+        for c in pc:
+            owner[c] = k
+    steps = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    for _ in range(flips):
+        a = int(rng.integers(len(pieces)))
+        c = pieces[a][int(rng.integers(3))]
+        d = steps[int(rng.integers(6))]
+        nb = (c[0] + d[0], c[1] + d[1], c[2] + d[2])
+        b = owner.get(nb)
+        if b is None or b == a:
+            continue
+        cells = sorted(pieces[a] + pieces[b])
+        splits = []
+        for trio in itertools.combinations(cells, 3):
+            if cells[0] not in trio:
+                continue
+            other = tuple(x for x in cells if x not in trio)
+            p1, p2 = _is_path3(list(trio)), _is_path3(list(other))
+            if p1 and p2:
+                splits.append((p1, p2))
+        p1, p2 = splits[int(rng.integers(len(splits)))]
+        pieces[a], pieces[b] = p1, p2
+        for x in p1:
+            owner[x] = a
+        for x in p2:
+            owner[x] = b
+    return pieces + ([rest] if rest else [])
+
+
+def mwus3d_system(width=5, layers=8, windows=64, temps=(310.0, 315.0, 320.0, 325.0),
+                  seed=20261202, tiling_seed=1, sequence_seed=2,
+                  flags=abi.FLAG_CTCB_SCAFFOLD_ENDPOINTS):
+    """The large 3D umbrella workload: a 200-domain ThreeQuarterTurn scaffold
+    on a 5x5 raster x 8 layers, 66 three-domain staple types and one
+    two-domain type from a seeded tromino tiling of the scaffold's cells
+    (every staple binds lattice-adjacent scaffold domains, its middle domain
+    adjacent to both ends), random 8-nt sequences, NearestNeighbour energies,
+    0.5 M cation, 1 uM staples, at most 2 staples per type.  Replicas:
+    `windows` numfulldomains windows at each of the temperatures, ordered
+    temperature-major, so each temperature's windows form one ladder.
+    CTCB constrains the in-segment scaffold domains of externally bound
+    staple networks (abi.FLAG_CTCB_SCAFFOLD_ENDPOINTS) by default: under the
+    reference's rule a three-domain staple bound across a segment boundary
+    is left dangling, its site then blocks the old-configuration regrowth,
+    and the rejected move leaves scaffold domains unassigned (the run stops
+    with a constraint error within a few thousand steps).
+    Returns (system, window list, grid values, grid length)."""
+    cells = width * width
+    n = cells * layers
+    path = [layered_position(i, width, cells) for i in range(n)]
+    index_of = {p: i for i, p in enumerate(path)}
+    pieces = tromino_tiling(path, 20 * n, tiling_seed)
+    staples = []
+    for pc in pieces:
+        ids = [index_of[c] + 1 for c in pc]
+        if ids[-1] < ids[0]:
+            ids.reverse()
+        staples.append(ids)
+    staples.sort()
+    seqs = random_sequences(n, 8, sequence_seed)
+    identities = [[-(i + 1) for i in range(n)]] + staples
+    sequences = [seqs] + [[_revcomp(seqs[v - 1]) for v in st] for st in staples]
+    all_temps = [t for t in temps for _ in range(windows)]
+    sysd = System(
+        identities=identities,
+        sequences=sequences,
+        cyclic=False,
+        domain_type=abi.THREEQUARTERTURN,
+        hybridization_pot=abi.HYB_NEARESTNEIGHBOUR,
+        misbinding_pot=abi.MISBIND_OPPOSING,
+        stacking_ene=-1000.0,
+        cation_M=0.5,
+        staple_M=1e-6,
+        chains=[Chain(0, 0, path, [(0, 0, 1)] * n)],
+        moves=standard_moveset(12),
+        max_total_staples=2 * len(staples),
+        max_type_staples=2,
+        max_staple_size=3,
+        centering_freq=0,
+        constraint_check_freq=0,
+        seed=seed,
+        name=(f"{n}-domain ThreeQuarterTurn {width}x{width}x{layers} scaffold, "
+              f"{len(staples)} staple types, NN"),
+        temps=all_temps,
+        flags=flags,
+    )
+    # LinearStepWell windows tiling numfulldomains 0..n with one point of
+    # overlap on each side, 1 kT per point outside the well, no grid bias
+    wins = []
+    for w in range(windows):
+        lo = max(0, (w * n) // windows - 1)
+        hi = min(n, ((w + 1) * n) // windows + 1)
+        wins.append(abi.Window(abi.OP_NUMFULLDOMAINS, lo, hi, 0, 0.0, 0.0, 1.0, 0.0, 1.0))
+    wins = wins * len(temps)
+    return sysd, wins, [], 0
 
 
 def batch_chain_system(n_domains, width=8, seed=20261202, tiling_seed=1):

@@ -96,7 +96,10 @@ void* oracle_create(const ldo_system_desc* sd, const ldo_run_params* run,
             moves.push_back({run->movetype_kind[m], run->movetype_freq[m],
                              run->movetype_max_regrowth ? run->movetype_max_regrowth[m] : 0});
         }
-        Limits lim {run->max_total_staples, run->max_type_staples};
+        if (run->flags & ~LDO_FLAG_CTCB_SCAFFOLD_ENDPOINTS) {
+            throw OrigamiError(LDO_E_ARG, "unknown ldo_run_params.flags bits");
+        }
+        Limits lim {run->max_total_staples, run->max_type_staples, run->flags};
         o->lim = lim;
         o->exchange_mode = run->exchange_mode;
         std::vector<ChainSpec> cfg = specs_from(init, o->idents);
@@ -184,7 +187,17 @@ static void run_windows(Oracle* O, int64_t n_swaps, int64_t interval) {
         if (O->rng_kind != 1) {
             xr.begin_step(swap_i);
         }
-        for (int i = (int)(swap_i % 2); i < W - 1; i += 2) {
+        // each run of consecutive slots at one temperature is its own window
+        // ladder (one reference PTMWUS run), paired from its first slot
+        std::vector<int> pairs;
+        for (int g0 = 0, g1; g0 < W; g0 = g1) {
+            for (g1 = g0 + 1; g1 < W && O->ladder_t[g1] == O->ladder_t[g0]; g1++) {
+            }
+            for (int i = g0 + (int)(swap_i % 2); i < g1 - 1; i += 2) {
+                pairs.push_back(i);
+            }
+        }
+        for (int i: pairs) {
             O->counters.exchange_attempts++;
             const WindowBias& w1 = O->windows[i];
             const WindowBias& w2 = O->windows[i + 1];

@@ -283,6 +283,7 @@ struct MovesetEntry {
 struct Limits {
     int max_total_staples;
     int max_type_staples;
+    int flags {0}; // ldo_run_flags
 };
 
 // Move-level biases of one umbrella window: a LinearStepWell restraint
@@ -311,6 +312,9 @@ class Replica; // forward
 class Constraintpoints {
   public:
     explicit Constraintpoints(System& s): m_sys {s} {}
+    // LDO_FLAG_CTCB_SCAFFOLD_ENDPOINTS: endpoints of externally bound
+    // networks go on the in-segment scaffold domains (off: the reference)
+    bool scaffold_endpoints {false};
     void reset();
     void calculate(const std::vector<Dom*>& scaffold_domains, int dir);
     std::set<int> staples_to_be_regrown() const { return m_regrowth_staples; }

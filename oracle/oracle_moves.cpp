@@ -222,7 +222,10 @@ void Constraintpoints::find_growthpoints_endpoints(const std::vector<Dom*>& sds,
             }
             for (auto& ia: m_net_iaes) {
                 if (ia.second->c == 0) {
-                    add_active_endpoint(ia.first, ia.second->pos, seg);
+                    // the reference keys this endpoint by the staple domain
+                    // (ia.first), where no scaffold growth looks for it
+                    add_active_endpoint(scaffold_endpoints ? ia.second : ia.first,
+                                        ia.second->pos, seg);
                 }
             }
         }
@@ -265,6 +268,7 @@ Replica::Replica(const ldo_system_desc& sd, const std::vector<std::vector<int>>&
         m_centering_domain {centering_domain},
         m_check_freq {check_freq},
         m_cp {sys} {
+    m_cp.scaffold_endpoints = (lim.flags & LDO_FLAG_CTCB_SCAFFOLD_ENDPOINTS) != 0;
     rng.init(rng_kind, seed, stream);
     double cum = 0;
     for (auto& m: moves) {

@@ -172,6 +172,61 @@ def case_windows_exchange_matches_oracle(sampler_cls, oracle_mod):
     assert e.value.code == abi.LDO_E_ARG
 
 
+def case_three_quarter_turn_and_three_domain_staples(sampler_cls, oracle_mod):
+    say("case: ThreeQuarterTurn four-domain system (8 replicas, 340-354 K) and the "
+        "three-domain staple system with both CTCB endpoint rules (8 replicas, 345-352 K), "
+        "3 x 5000 steps")
+    tq = systems.four_domain_system(seed=17)
+    tq.domain_type = abi.THREEQUARTERTURN
+    cases = [(tq, [340.0 + 2 * i for i in range(8)])]
+    for flags in (0, abi.FLAG_CTCB_SCAFFOLD_ENDPOINTS):
+        for dt in (abi.HALFTURN, abi.THREEQUARTERTURN):
+            s = systems.three_domain_system(domain_type=dt, max_total_staples=2, flags=flags)
+            cases.append((s, [345.0 + i for i in range(8)]))
+    for s, temps in cases:
+        gpu = sampler_cls(s, temps=temps)
+        ref = oracle_mod.Oracle(s, temps=temps)
+        for _ in range(3):
+            gpu.sweep(5000)
+            ref.sweep(5000)
+            for r in range(len(temps)):
+                assert_same(gpu, ref, r)
+        assert_counters(gpu, ref)
+
+
+def case_mwus3d_window_ladders_by_temperature(sampler_cls, oracle_mod):
+    say("case: 200-domain ThreeQuarterTurn 5x5x8 scaffold, 3-domain staples, NN; "
+        "numfulldomains windows: 3 at 318 K, 2 at 322 K, 3 at 326 K (three ladders), "
+        "6 window swaps x 500 steps")
+    s, wins, grid, glen = systems.mwus3d_system(windows=64, temps=(318.0,))
+    pick = [0, 1, 2, 0, 1, 0, 1, 2]
+    temps = [318.0] * 3 + [322.0] * 2 + [326.0] * 3
+    wins = [wins[i] for i in pick]
+    for i, w in enumerate(wins):  # wide wells so that the bare scaffold swaps
+        wins[i] = abi.Window(w.op, 0, w.max_op + 4 * i, 0, 0.0, 0.0, 1.0, 0.0, 1.0)
+    s.temps = temps
+    gpu = sampler_cls(s, temps=temps, exchange_mode=abi.EXCHANGE_WINDOWS)
+    ref = oracle_mod.Oracle(s, temps=temps, exchange_mode=abi.EXCHANGE_WINDOWS)
+    gpu.set_windows(wins, grid, glen)
+    ref.set_windows(wins, grid, glen)
+    groups = [range(0, 3), range(3, 5), range(5, 8)]
+    for _ in range(3):
+        gpu.run(2, 500)
+        ref.run(2, 500)
+        perm = gpu.permutation()
+        assert perm == ref.permutation()
+        for g in groups:
+            assert sorted(perm[q] for q in g) == list(g)  # swaps stay in their ladder
+        for r in range(8):
+            assert_same(gpu, ref, r)
+            assert gpu.observables(r)["bias"] == ref.observables(r)["bias"], r
+    cg, cr = gpu.counters(), ref.counters()
+    # even rounds pair (0,1), (3,4), (5,6); odd rounds (1,2), (6,7)
+    assert cg.exchange_attempts == cr.exchange_attempts == 3 * 3 + 3 * 2
+    assert cg.exchange_accepts == cr.exchange_accepts > 0
+    assert_counters(gpu, ref)
+
+
 def case_batch_matches_oracle(batch_cls, oracle_mod):
     say("case: batch of 12 chains (8-40 domains, 310-350 K), 3 x 500 steps, each segment on "
         "a different sweep kernel (LDS, wavefront, lane)")

@@ -37,6 +37,14 @@ def test_windows_exchange_matches_oracle(sampler_cls, oracle_mod):
     pc.case_windows_exchange_matches_oracle(sampler_cls, oracle_mod)
 
 
+def test_mwus3d_window_ladders_by_temperature(sampler_cls, oracle_mod):
+    pc.case_mwus3d_window_ladders_by_temperature(sampler_cls, oracle_mod)
+
+
+def test_three_quarter_turn_and_three_domain_staples(sampler_cls, oracle_mod):
+    pc.case_three_quarter_turn_and_three_domain_staples(sampler_cls, oracle_mod)
+
+
 def test_batch_matches_oracle(oracle_mod):
     from latticednaorigami_amd.sampler import BatchSampler
     pc.case_batch_matches_oracle(BatchSampler, oracle_mod)

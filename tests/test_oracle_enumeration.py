@@ -216,3 +216,75 @@ def test_cb_staple_regrowth_redraws_the_old_growth_point(oracle_mod):
     m, se = _mean_se(counts, (1, 2, 0, 1), n)
     assert target[(1, 2, 0, 1)] < 0.15
     assert m > 3 * target[(1, 2, 0, 1)], (m, se)
+
+
+def _mc_counts(o, n_rep, n_samples, stride=100, burn=20000):
+    o.sweep(burn)
+    counts = [dict() for _ in range(n_rep)]
+    for _ in range(n_samples):
+        o.sweep(stride)
+        for r in range(n_rep):
+            ob = o.observables(r)
+            k = (ob["num_staples"], ob["num_fully_bound_domain_pairs"],
+                 ob["num_misbound_domain_pairs"], ob["num_stacked_domain_pairs"])
+            counts[r][k] = counts[r].get(k, 0) + 1
+    return counts
+
+
+def test_three_quarter_turn_monte_carlo_matches_enumeration(oracle_mod):
+    """ThreeQuarterTurn twist, kink and junction rules (domain.cpp:69-118):
+    the four-domain system with 3/4-turn domains at 345 K, where unbound,
+    misbound, one- and two-staple states are all populated."""
+    from latticednaorigami_amd import abi
+    s = _system(2, seed=17)
+    s.domain_type = abi.THREEQUARTERTURN
+    o = oracle_mod.Oracle(s, temps=[345.0] * 8)
+    exact, _, _ = o.enumerate()
+    assert sum(1 for p in exact.values() if p > 0.02) >= 5
+    counts = _mc_counts(o, 8, 2500)
+    for key, p in exact.items():
+        if p < 0.01:
+            continue
+        m, se = _mean_se(counts, key, 2500)
+        assert abs(m - p) <= 4 * se + 0.005, (key, m, p, se)
+
+
+@pytest.mark.parametrize("domain_type", [0, 1])
+def test_three_domain_staples_match_enumeration_with_scaffold_endpoints(oracle_mod, domain_type):
+    """Staples of three domains (Met insertion/deletion with sites-before /
+    sites-minus-L counting, CB regrowth from either end or the middle, CTCB
+    through a staple network bound across the segment boundary) sample the
+    enumerated ensemble at 349 K, where the bare scaffold, a misbound scaffold
+    and the fully bound staple are all probable, once CTCB constrains the
+    in-segment scaffold domains of an externally bound network
+    (LDO_FLAG_CTCB_SCAFFOLD_ENDPOINTS)."""
+    from latticednaorigami_amd import abi
+    s = systems.three_domain_system(domain_type=domain_type,
+                                    flags=abi.FLAG_CTCB_SCAFFOLD_ENDPOINTS)
+    o = oracle_mod.Oracle(s, temps=[349.0] * 8)
+    exact, _, _ = o.enumerate()
+    assert exact[(1, 3, 0, 2)] == pytest.approx(0.567, abs=0.01)
+    counts = _mc_counts(o, 8, 2500)
+    for key, p in exact.items():
+        if p < 0.01:
+            continue
+        m, se = _mean_se(counts, key, 2500)
+        assert abs(m - p) <= 4 * se + 0.005, (key, m, p, se)
+
+
+def test_reference_ctcb_rule_unbinds_three_domain_staples(oracle_mod):
+    """The reference keys the endpoint of an in-segment (staple domain,
+    scaffold domain) pair of an externally bound network by the staple's
+    chain (top_constraint_points.cpp:553-557), so CTCB regrowth of a segment
+    that ends inside a fully bound three-domain staple moves the scaffold
+    domain off the staple with both Rosenbluth weights blind to the lost
+    pair: the fully bound state (exact probability 0.567) is almost never
+    seen, and the scaffold spends its time unbound."""
+    s = systems.three_domain_system()
+    o = oracle_mod.Oracle(s, temps=[349.0] * 8)
+    exact, _, _ = o.enumerate()
+    counts = _mc_counts(o, 8, 1000)
+    m_full, _ = _mean_se(counts, (1, 3, 0, 2), 1000)
+    m_bare, _ = _mean_se(counts, (0, 0, 0, 0), 1000)
+    assert exact[(1, 3, 0, 2)] > 0.5 and m_full < 0.05
+    assert exact[(0, 0, 0, 0)] < 0.4 and m_bare > 0.7
