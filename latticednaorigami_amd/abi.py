@@ -207,6 +207,36 @@ def declare_batch_abi(lib):
         fn.argtypes = args
 
 
+class MlpLayout(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("w1", "emb", "w1e", "b1", "w2", "b2", "w3", "b3", "w4",
+                                         "b4", "total")]
+
+
+MLP_SITES, MLP_CHANNELS, MLP_FEATURES, MLP_EMBED, MLP_HIDDEN, MLP_OUT = 343, 13, 4459, 16, 256, 36
+
+
+def declare_mlp_abi(lib):
+    """Attach argtypes/restype for the ldo_mlp_* entry points (ldo_mlp.h)."""
+    v = C.c_void_p
+    u8p = C.POINTER(C.c_uint8)
+    f32p = C.POINTER(C.c_float)
+    sig = {
+        "layout_of": (C.c_int, [C.c_int32, C.POINTER(MlpLayout)]),
+        "create": (C.c_int, [C.c_int32, C.c_int32, f32p, C.c_uint64, C.POINTER(C.c_void_p)]),
+        "destroy": (None, [v]),
+        "last_error": (C.c_char_p, [v]),
+        "get_params": (C.c_int, [v, f32p]),
+        "set_params": (C.c_int, [v, f32p]),
+        "loss_grad": (C.c_int, [v, C.c_int32, u8p, _i32p, u8p, _i32p, _f64p, f32p]),
+        "adam_step": (C.c_int, [v, C.c_float, C.c_float, C.c_float, C.c_float]),
+        "log_probs": (C.c_int, [v, C.c_int32, u8p, _i32p, u8p, f32p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, "ldo_mlp_" + name)
+        fn.restype = res
+        fn.argtypes = args
+
+
 _LIB = None
 
 
@@ -238,6 +268,7 @@ def load_library(path=None):
     lib.ldo_get_profile.argtypes = [C.c_void_p, _f64p, C.POINTER(C.c_int64), _f64p,
                                     C.POINTER(C.c_int64)]
     declare_batch_abi(lib)
+    declare_mlp_abi(lib)
     if path is None:
         _LIB = lib
     return lib

@@ -6,7 +6,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libldo_hip.so")
-SOURCES = ["sampler.hip", "plan.cpp", "tables.cpp"]
+SOURCES = ["sampler.hip", "mlp.hip", "plan.cpp", "tables.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
