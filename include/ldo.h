@@ -255,6 +255,32 @@ int ldo_set_tape(void* ctx, int32_t replica, const uint64_t* tape, int64_t n);
  * (GCMCSimulation::simulate, simulation.cpp:568-653). */
 int ldo_sweep(void* ctx, int64_t steps);
 
+/* Learned domain regrowth: no counterpart in the reference (the learned
+ * proposal of this sampler).  One attempt picks a staple uniformly and one
+ * of its two end domains d (probability 1/2 each); the network of
+ * ldo_mlp.h, given the 7x7x7 box around d's chain neighbour with d left out,
+ * proposes d's new site (one of the neighbour's 6) and orientation; the move
+ * is accepted with min(1, exp(-dE - dB) q(old) / q(new)) if every staple
+ * stays connected to the scaffold.  With probability `fallback` the attempt
+ * is a plain CB staple regrowth instead.  Counters: attempts/accepts[7] are
+ * learned moves, [6] the CB fallbacks (so the move set may use slots 0-5).
+ * The parameters are copied to the device; n_ident must be at least the
+ * number of scaffold domains (identity index = |identity| - 1). */
+int ldo_learned_set_network(void* ctx, int32_t n_ident, const float* params, double fallback);
+
+/* `attempts` learned-move attempts on every replica, each with the tables
+ * and umbrella window of its last sweep. */
+int ldo_learned_sweep(void* ctx, int64_t attempts);
+
+/* Training samples from the current configurations (no state change): for
+ * every staple of every replica and both of its ends, the network's input
+ * box around the end's chain neighbour (end left out, channels [343]), the
+ * end's identity index, the legal outputs [36] and the output the end
+ * occupies (target).  Writes at most `cap` samples, replica by replica, and
+ * their number to n_out. */
+int ldo_learned_samples(void* ctx, int32_t cap, uint8_t* channels, int32_t* ident,
+                        uint8_t* legal, int32_t* target, int32_t* n_out);
+
 /* PTMC: n_swaps rounds of [apply ladder slot -> simulate(exchange_interval)
  * -> dependent q's -> device exchange] (PTGCMCSimulation::run,
  * ptmc_simulation.cpp:106-150).  With LDO_EXCHANGE_WINDOWS the exchange is

@@ -21,7 +21,8 @@
  * from the anchor; channel 0 = empty, 1 + o = scaffold with orientation o,
  * 7 + o = staple with orientation o (o indexes +x,-x,+y,-y,+z,-z).  Output
  * k = 6 * n + o: neighbour n (the anchor plus unit vector n) and orientation
- * o.  A site holding a scaffold domain reads as scaffold.
+ * o.  A site holding a scaffold domain reads as scaffold; two domains of one
+ * kind read as the smaller orientation index.
  *
  * Parameters are one flat FP32 vector in the order of ldo_mlp_layout;
  * matrices are row-major with the input index first (W[in][out]).

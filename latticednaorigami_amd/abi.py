@@ -237,6 +237,21 @@ def declare_mlp_abi(lib):
         fn.argtypes = args
 
 
+def declare_learned_abi(lib, prefix):
+    """argtypes/restype of the learned-move entry points (ldo_learned_*)."""
+    u8p = C.POINTER(C.c_uint8)
+    sig = {
+        "learned_set_network": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_float),
+                                          C.c_double]),
+        "learned_sweep": (C.c_int, [C.c_void_p, C.c_int64]),
+        "learned_samples": (C.c_int, [C.c_void_p, C.c_int32, u8p, _i32p, u8p, _i32p, _i32p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, prefix + name)
+        fn.restype = res
+        fn.argtypes = args
+
+
 _LIB = None
 
 
@@ -269,6 +284,7 @@ def load_library(path=None):
                                     C.POINTER(C.c_int64)]
     declare_batch_abi(lib)
     declare_mlp_abi(lib)
+    declare_learned_abi(lib, "ldo_")
     if path is None:
         _LIB = lib
     return lib

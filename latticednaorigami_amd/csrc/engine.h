@@ -245,11 +245,13 @@ struct Eng {
     }
 
     // ------------------------------------------------------------ rng
-    // Philox4x32-10 (Salmon et al., SC'11); counter (draw, 0, step lo, hi),
-    // key (seed lo, replica)
+    // Philox4x32-10 (Salmon et al., SC'11); counter (draw, stream, step lo,
+    // hi) with stream 0 for MC steps and 1 for learned-move attempts (whose
+    // own count replaces the step), key (seed lo, replica)
     DEV uint64_t philox64() {
-        uint32_t c0 = (uint32_t)S->rng_idx, c1 = 0u;
-        uint32_t c2 = (uint32_t)(uint64_t)S->step, c3 = (uint32_t)((uint64_t)S->step >> 32);
+        uint64_t ctr = S->rng_sub ? (uint64_t)S->learned_step : (uint64_t)S->step;
+        uint32_t c0 = (uint32_t)S->rng_idx, c1 = (uint32_t)S->rng_sub;
+        uint32_t c2 = (uint32_t)ctr, c3 = (uint32_t)(ctr >> 32);
         uint32_t k0 = (uint32_t)P->seed, k1 = (uint32_t)rep;
 #pragma unroll
         for (int r = 0; r < 10; r++) {
@@ -577,6 +579,57 @@ struct Eng {
     DEVNI void ctcb_grow_staple_from(int gold);
     DEVNI bool ctcb_scaffold_move(int max_regrowth);
     DEVNI bool attempt(int m);
+    DEVNI bool attempt_kind(int kind, int max_regrowth);
+    // learned domain regrowth (lane 0 parts)
+    DEVNI bool chain_connected(int cs);
+    DEVNI int learned_begin(LearnedMove& m);
+    DEVNI void learned_finish(LearnedMove& m);
+    // channel of the learned proposal's input at site p, leaving domain
+    // `excl` out: 0 empty; 1 + o for a scaffold domain, 7 + o for a staple
+    // domain only; with two domains of one kind the smaller orientation index
+    // (read-only: any lane may call it)
+    DEV int site_channel(I3 p, int excl) const {
+        int i = hfind(pkey(p));
+        if (i < 0) {
+            return 0;
+        }
+        int s = (int)(int16_t)(hval[i] & 0xFFFFu);
+        int occ = (int)((hval[i] >> 16) & 3u);
+        int c[2] = {s, (occ != OCC_UNBOUND && s >= 0) ? (int)dom[s].bound : (int)LDO_NONE};
+        int scaf = 6, stap = 6;
+        for (int k = 0; k < 2; k++) {
+            int x = c[k];
+            if (x < 0 || x == excl) {
+                continue;
+            }
+            int o = dom[x].ore;
+            if (cslot_of(x) == 0) {
+                scaf = o < scaf ? o : scaf;
+            }
+            else {
+                stap = o < stap ? o : stap;
+            }
+        }
+        if (scaf < 6) {
+            return 1 + scaf;
+        }
+        return stap < 6 ? 7 + stap : 0;
+    }
+    // domains at site p other than `excl` (0, 1 or 2)
+    DEV int site_count(I3 p, int excl) const {
+        int i = hfind(pkey(p));
+        if (i < 0) {
+            return 0;
+        }
+        int s = (int)(int16_t)(hval[i] & 0xFFFFu);
+        int occ = (int)((hval[i] >> 16) & 3u);
+        int n = (s >= 0 && s != excl) ? 1 : 0;
+        if (occ != OCC_UNBOUND && s >= 0) {
+            int b = dom[s].bound;
+            n += (b >= 0 && b != excl) ? 1 : 0;
+        }
+        return n;
+    }
     DEVNI void simulate(int64_t steps);
     DEVNI void begin_interval(int slot, bool recompute);
     DEVNI int window_op(int staple_offset);

@@ -1027,6 +1027,10 @@ DEVNI bool Eng::ctcb_scaffold_move(int max_regrowth) {
 // MCMovetype::attempt_move and the reset_internal chain (movetypes.cpp:41-51,
 // 242-250; met_movetypes.cpp:45-48, 139-143; cb_movetypes.cpp:44-50, 457-461)
 DEVNI bool Eng::attempt(int m) {
+    return attempt_kind(P->move_kind[m], P->move_maxreg[m]);
+}
+
+DEVNI bool Eng::attempt_kind(int kind, int max_regrowth) {
     S->n_modified = 0;
     S->n_assigned = 0;
     S->n_added = 0;
@@ -1038,7 +1042,6 @@ DEVNI bool Eng::attempt(int m) {
     S->new_modifier = 1;
     S->regrow_old = 0;
     S->insertion_sites = S->num_domains;
-    int kind = P->move_kind[m];
     switch (kind) {
     case 0:
         S->kind = LDO_MOVE_KIND_ORIENT;
@@ -1054,7 +1057,7 @@ DEVNI bool Eng::attempt(int m) {
         return cb_staple_move();
     case 3:
         S->kind = LDO_MOVE_KIND_CTCB;
-        return ctcb_scaffold_move(P->move_maxreg[m]);
+        return ctcb_scaffold_move(max_regrowth);
     }
     fail(ST_MISUSE);
     return false;
@@ -1150,6 +1153,148 @@ DEVNI double Eng::calc_move_bias(int staple_offset) {
     st[0] = r;
     st[1] = g;
     return diff * w.bias_mult;
+}
+
+// ------------------------------------------------------ learned proposal
+// Learned domain regrowth (no reference counterpart): a staple end domain d
+// moves next to its chain neighbour a (the anchor) to one of the 6 sites x 6
+// orientations, drawn from the network's masked softmax over a context that
+// leaves d out, so forward and reverse moves share one distribution.
+// Acceptance is Metropolis-Hastings: min(1, exp(-dE - dB) q(old) / q(new)).
+// With probability learned_fallback the attempt is a plain CB staple
+// regrowth instead.  Returns 1 when the lanes must build the context and
+// evaluate the network, 0 when the attempt is complete.
+// whether staple chain cs reaches the scaffold through bound pairs
+DEVNI bool Eng::chain_connected(int cs) {
+    for (int k = 0; k < ch[cs].len; k++) {
+        int b = dom[slot(cs, k)].bound;
+        if (b < 0 || cslot_of(b) == cs) {
+            continue;
+        }
+        if (cslot_of(b) == 0 || scan_for_scaffold(b, cs)) {
+            return true;
+        }
+    }
+    return false;
+}
+
+DEVNI int Eng::learned_begin(LearnedMove& m) {
+    S->rng_sub = 1;
+    S->learned_step += 1;
+    S->rng_idx = 0;
+    m.mode = 0;
+    int ls = P->learned_slot;
+    if (uniform_real() < P->learned_fallback) {
+        S->attempts[ls - 1]++;
+        bool acc = attempt_kind(2, 0);
+        S->accepts[ls - 1] += acc ? 1 : 0;
+        if (!acc) {
+            reset_origami();
+            calc_move_bias(0);
+        }
+        S->rng_sub = 0;
+        return 0;
+    }
+    S->attempts[ls]++;
+    if (S->num_staples == 0) {
+        S->rng_sub = 0;
+        return 0;
+    }
+    int w = uniform_int(1, S->num_staples);
+    int cs = order[w];
+    int len = ch[cs].len;
+    bool first = uniform_real() < 0.5;
+    int d = first ? slot(cs, 0) : slot(cs, len - 1);
+    int a = first ? slot(cs, 1) : slot(cs, len - 2);
+    int idv = ident_of(d);
+    int id = (idv < 0 ? -idv : idv) - 1;
+    if (len < 2 || id < 0 || id >= P->net_n_ident) {
+        fail(ST_MISUSE);
+        S->rng_sub = 0;
+        return 0;
+    }
+    I3 ap = pos(a);
+    int n_old = unit_index(sub3(pos(d), ap));
+    m.d = d;
+    m.a = a;
+    m.id = id;
+    m.k_old = 6 * n_old + dom[d].ore;
+    m.ax = ap.x;
+    m.ay = ap.y;
+    m.az = ap.z;
+    m.mode = 1;
+    return 1;
+}
+
+DEVNI void Eng::learned_finish(LearnedMove& m) {
+    int ls = P->learned_slot;
+    double w[36];
+    float mx = -INFINITY;
+    for (int k = 0; k < 36; k++) {
+        if (m.legal[k / 6]) {
+            mx = fmaxf(mx, m.logits[k]);
+        }
+    }
+    double z = 0;
+    for (int k = 0; k < 36; k++) {
+        w[k] = m.legal[k / 6] ? exp((double)m.logits[k] - (double)mx) : 0.0;
+        z += w[k];
+    }
+    double u = uniform_real() * z;
+    double cum = 0;
+    int kn = -1;
+    for (int k = 0; k < 36; k++) {
+        if (w[k] == 0) {
+            continue;
+        }
+        kn = k;
+        cum += w[k];
+        if (u < cum) {
+            break;
+        }
+    }
+    bool accepted = false;
+    if (kn == m.k_old) {
+        accepted = true;
+    }
+    else {
+        S->n_modified = 0;
+        S->n_assigned = 0;
+        S->n_added = 0;
+        S->rejected = 0;
+        S->violated = 0;
+        S->modifier = 1;
+        int d = m.d;
+        int cs = cslot_of(d);
+        int partner = dom[d].bound;
+        prev[d].x = dom[d].x;
+        prev[d].y = dom[d].y;
+        prev[d].z = dom[d].z;
+        prev[d].ore = dom[d].ore;
+        push_modified(d);
+        double de = unassign_domain(d);
+        I3 pn = add3(i3(m.ax, m.ay, m.az), unit3(kn / 6));
+        de += set_domain_config(d, pn, kn % 6);
+        push_assigned(d);
+        if (!S->violated) {
+            // every staple must stay connected to the scaffold (the ensemble
+            // of the reference's moves): d's own, and a staple d left
+            bool ok = chain_connected(cs);
+            if (ok && partner >= 0 && cslot_of(partner) != 0 && cslot_of(partner) != cs) {
+                ok = chain_connected(cslot_of(partner));
+            }
+            if (ok) {
+                de += calc_move_bias(0);
+                accepted = test_acceptance(exp(-de) * (w[m.k_old] / w[kn]));
+            }
+        }
+        if (!accepted) {
+            reset_origami();
+            calc_move_bias(0);
+        }
+    }
+    S->accepts[ls] += accepted ? 1 : 0;
+    S->rng_sub = 0;
 }
 
 #endif

@@ -26,7 +26,9 @@ DEVNI void Eng::update_occupancies(int s, I3 p) {
             }
             nst = OCC_MISBOUND;
         }
-        hput(p, nst, LDO_NONE);
+        // a bound site keeps the domain placed last (the learned proposal
+        // reads both domains of a site through it)
+        hput(p, nst, s);
         dom[s].state = (int8_t)nst;
         dom[dj].state = (int8_t)nst;
         dom[dj].bound = (int16_t)s;
@@ -294,7 +296,7 @@ DEVNI void Eng::center() {
                 hput(np, OCC_UNBOUND, s);
             }
             else if (stt != OCC_UNASSIGNED) {
-                hput(np, stt, LDO_NONE);
+                hput(np, stt, s);
             }
         }
     }

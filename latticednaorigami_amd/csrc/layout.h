@@ -79,6 +79,7 @@ struct RepScalars {
     int64_t site_empty;
     int64_t site_bind;
     int64_t commits;
+    int64_t learned_step; // learned-move attempts so far (their Philox counter)
     int32_t num_domains;
     int32_t num_staples;
     int32_t num_bound;
@@ -104,7 +105,8 @@ struct RepScalars {
     int32_t n_active0;
     int32_t rng_idx;
     int32_t added[4];
-    int32_t pad[2];
+    int32_t rng_sub; // 0: MC step stream; 1: learned-move stream
+    int32_t pad[1];
 };
 
 // Byte offsets of the sections of a replica block, and sizes.
@@ -171,6 +173,27 @@ struct Params {
     const ldo_window* windows; // [n_slots] umbrella windows, or null
     const double* win_grid;    // [n_slots * grid_len]
     double* win_stored;        // [n_slots * 2] stored restraint and grid bias
+    // learned proposal (ldo_mlp.h layout), or null
+    const float* net;
+    int64_t net_off[10]; // w1, emb, w1e, b1, w2, b2, w3, b3, w4, b4
+    int32_t net_n_ident;
+    int32_t learned_slot; // counters: [slot] learned moves, [slot - 1] CB fallbacks
+    double learned_fallback;
+};
+
+// One learned-move attempt, shared by the lanes of a replica's wavefront:
+// lane 0 fills d..k_old, the lanes fill ch/legal and the network's logits.
+struct LearnedMove {
+    int32_t mode; // 1: the network is needed
+    int32_t d;    // moving staple end domain
+    int32_t a;    // its chain neighbour (the anchor)
+    int32_t id;   // identity index of d (embedding row)
+    int32_t k_old;
+    int32_t ax, ay, az;
+    uint8_t legal[6];
+    uint8_t ch[343];
+    float h[2][256];
+    float logits[36];
 };
 
 // move kinds held in RepScalars::kind (the grow_chain dispatch)

@@ -18,6 +18,7 @@
 
 #include "../../include/ldo.h"
 #include "../../include/ldo_mlp.h"
+#include "plan.h"
 
 namespace {
 
@@ -257,31 +258,6 @@ struct MlpCtx {
 
 std::string g_mlp_create_error;
 
-void layout_of(int n_ident, ldo_mlp_layout& L) {
-    int64_t o = 0;
-    L.w1 = o;
-    o += (int64_t)LDO_MLP_FEATURES * H;
-    L.emb = o;
-    o += (int64_t)n_ident * NE;
-    L.w1e = o;
-    o += (int64_t)NE * H;
-    L.b1 = o;
-    o += H;
-    L.w2 = o;
-    o += (int64_t)H * H;
-    L.b2 = o;
-    o += H;
-    L.w3 = o;
-    o += (int64_t)H * H;
-    L.b3 = o;
-    o += H;
-    L.w4 = o;
-    o += (int64_t)H * NO;
-    L.b4 = o;
-    o += NO;
-    L.total = o;
-}
-
 template <class F>
 int mguarded(MlpCtx* c, F f) {
     std::string& err = c ? c->err : g_mlp_create_error;
@@ -429,7 +405,7 @@ int ldo_mlp_layout_of(int32_t n_ident, ldo_mlp_layout* out) {
     if (n_ident < 1 || !out) {
         return LDO_E_ARG;
     }
-    layout_of(n_ident, *out);
+    ldo::mlp_layout(n_ident, *out);
     return LDO_OK;
 }
 
@@ -442,7 +418,7 @@ int ldo_mlp_create(int32_t n_ident, int32_t max_batch, const float* params, uint
         }
         c->n_ident = n_ident;
         c->cap = max_batch;
-        layout_of(n_ident, c->L);
+        ldo::mlp_layout(n_ident, c->L);
         hcheck(hipStreamCreate(&c->stream), "stream");
         int64_t n = c->L.total;
         dalloc(c->d_p, n, "params");

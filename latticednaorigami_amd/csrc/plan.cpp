@@ -13,6 +13,26 @@ int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
 } // namespace
 
+void mlp_layout(int n_ident, ldo_mlp_layout& L) {
+    const int64_t H = LDO_MLP_HIDDEN;
+    int64_t o = 0;
+    auto take = [&](int64_t& field, int64_t n) {
+        field = o;
+        o += n;
+    };
+    take(L.w1, (int64_t)LDO_MLP_FEATURES * H);
+    take(L.emb, (int64_t)n_ident * LDO_MLP_EMBED);
+    take(L.w1e, (int64_t)LDO_MLP_EMBED * H);
+    take(L.b1, H);
+    take(L.w2, H * H);
+    take(L.b2, H);
+    take(L.w3, H * H);
+    take(L.b3, H);
+    take(L.w4, H * LDO_MLP_OUT);
+    take(L.b4, LDO_MLP_OUT);
+    L.total = o;
+}
+
 Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap) {
     Layout L;
     std::memset(&L, 0, sizeof L);

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/ldo.h"
+#include "../../include/ldo_mlp.h"
 #include "layout.h"
 #include "tables.h"
 
@@ -25,6 +26,9 @@ struct AbiError {
 };
 
 Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap);
+
+// element offsets of the learned proposal's flat parameter vector (ldo_mlp.h)
+void mlp_layout(int n_ident, ldo_mlp_layout& L);
 
 struct Plan {
     SystemSpec spec;

@@ -275,4 +275,80 @@ DEV double slot_bias(const Params& P, int q) {
     return (P.win_stored[2 * q] + P.win_stored[2 * q + 1]) * P.windows[q].bias_mult;
 }
 
+// ------------------------------------------------------ learned proposal
+// The lane parts of a learned-move attempt (Eng::learned_begin/finish): lane
+// `lane` of `nl` handles sites lane, lane + nl, ... of the input box and
+// hidden units lane, lane + nl, ...; every value is computed by one lane in a
+// fixed order, so any lane count gives the same bits.
+
+// the one-hot channels of the 7x7x7 box around the anchor (site index
+// (dz+3)*49 + (dy+3)*7 + (dx+3)) and the six neighbour sites that can take d
+// (at most one other domain there)
+DEV void learned_context_lane(const Eng& e, LearnedMove& m, int lane, int nl) {
+    for (int s = lane; s < 343; s += nl) {
+        I3 p = i3(m.ax + s % 7 - 3, m.ay + (s / 7) % 7 - 3, m.az + s / 49 - 3);
+        m.ch[s] = (uint8_t)e.site_channel(p, m.d);
+    }
+    for (int n = lane; n < 6; n += nl) {
+        I3 p = add3(i3(m.ax, m.ay, m.az), unit3(n));
+        m.legal[n] = e.site_count(p, m.d) <= 1 ? 1 : 0;
+    }
+}
+
+// h[0] = relu(b1 + sum over sites of W1[site*13 + channel] + emb[id] W1e)
+DEV void net_layer1_lane(const Params& P, LearnedMove& m, int lane, int nl) {
+    const float* W = P.net;
+    const float* e = W + P.net_off[1] + (int64_t)m.id * 16;
+    for (int j = lane; j < 256; j += nl) {
+        float s = W[P.net_off[3] + j];
+        for (int site = 0; site < 343; site++) {
+            s += W[P.net_off[0] + (int64_t)(site * 13 + m.ch[site]) * 256 + j];
+        }
+        for (int k = 0; k < 16; k++) {
+            s = fmaf(e[k], W[P.net_off[2] + k * 256 + j], s);
+        }
+        m.h[0][j] = fmaxf(s, 0.f);
+    }
+}
+
+// out[j] = b[j] + sum_i in[i] W[i][j] (fused multiply-adds in i order),
+// ReLU when `relu`
+DEV void net_dense_lane(const float* W, const float* b, const float* in, float* out, int n_in,
+                        int n_out, bool relu, int lane, int nl) {
+    for (int j = lane; j < n_out; j += nl) {
+        float s = b[j];
+        for (int i = 0; i < n_in; i++) {
+            s = fmaf(in[i], W[(int64_t)i * n_out + j], s);
+        }
+        out[j] = relu ? fmaxf(s, 0.f) : s;
+    }
+}
+
+// network phase `ph` (0: context, 1: layer 1, 2-3: hidden, 4: logits); the
+// kernel separates phases with barriers, the host build runs them in order
+DEV void learned_phase_lane(const Eng& e, const Params& P, LearnedMove& m, int ph, int lane,
+                            int nl) {
+    const float* W = P.net;
+    switch (ph) {
+    case 0:
+        learned_context_lane(e, m, lane, nl);
+        break;
+    case 1:
+        net_layer1_lane(P, m, lane, nl);
+        break;
+    case 2:
+        net_dense_lane(W + P.net_off[4], W + P.net_off[5], m.h[0], m.h[1], 256, 256, true, lane, nl);
+        break;
+    case 3:
+        net_dense_lane(W + P.net_off[6], W + P.net_off[7], m.h[1], m.h[0], 256, 256, true, lane, nl);
+        break;
+    case 4:
+        net_dense_lane(W + P.net_off[8], W + P.net_off[9], m.h[0], m.logits, 256, 36, false, lane,
+                       nl);
+        break;
+    }
+}
+
+#define LDO_LEARNED_PHASES 5
+
 #endif

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "../../include/ldo.h"
+#include "../../include/ldo_mlp.h"
 #include "layout.h"
 #include "plan.h"
 #include "replica_ops.h"
@@ -122,6 +123,111 @@ extern "C" __global__ void ldo_status_kernel(Params P, int32_t* out) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < P.n_replicas) {
         out[r] = replica_status(P, r);
+    }
+}
+
+// `attempts` learned domain-regrowth attempts per replica, one workgroup of
+// 64 lanes per replica: lane 0 runs the Markov chain, all lanes build the
+// input box and evaluate the network between barriers.  Each replica acts
+// with the control variables of its last sweep.
+extern "C" __global__ void __launch_bounds__(64)
+    ldo_learned_kernel(Params P, int64_t attempts) {
+    __shared__ Params sp;
+    __shared__ Eng e;
+    __shared__ LearnedMove m;
+    __shared__ int go;
+    int r = blockIdx.x, lane = threadIdx.x;
+    if (r >= P.n_replicas) {
+        return;
+    }
+    if (lane == 0) {
+        sp = P;
+        e.attach(&sp, sp.blocks + (size_t)r * sp.L.bytes, r);
+        e.select_tables();
+    }
+    __syncthreads();
+    for (int64_t it = 0; it < attempts; it++) {
+        if (lane == 0) {
+            go = e.S->status == ST_OK ? e.learned_begin(m) : -1;
+        }
+        __syncthreads();
+        int g = go;
+        if (g < 0) {
+            break;
+        }
+        if (g == 1) {
+            for (int ph = 0; ph < LDO_LEARNED_PHASES; ph++) {
+                learned_phase_lane(e, sp, m, ph, lane, 64);
+                __syncthreads();
+            }
+            if (lane == 0) {
+                e.learned_finish(m);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Training samples of the learned proposal from the current configurations:
+// for every staple and both of its ends, the input box around the end's
+// chain neighbour (the end left out), the legal outputs and the end's
+// actual site and orientation.  Replica r writes from sample r * cap on and
+// its count to counts[r].
+extern "C" __global__ void __launch_bounds__(64)
+    ldo_learned_samples_kernel(Params P, int32_t cap, uint8_t* ch_out, int32_t* id_out,
+                               uint8_t* legal_out, int32_t* tgt_out, int32_t* counts) {
+    __shared__ Params sp;
+    __shared__ Eng e;
+    __shared__ LearnedMove m;
+    __shared__ int n_st;
+    int r = blockIdx.x, lane = threadIdx.x;
+    if (r >= P.n_replicas) {
+        return;
+    }
+    if (lane == 0) {
+        sp = P;
+        e.attach(&sp, sp.blocks + (size_t)r * sp.L.bytes, r);
+        n_st = e.S->status == ST_OK ? e.S->num_staples : 0;
+    }
+    __syncthreads();
+    int n = 0;
+    for (int w = 1; w <= n_st; w++) {
+        for (int end = 0; end < 2 && n < cap; end++) {
+            if (lane == 0) {
+                int cs = e.order[w];
+                int len = e.ch[cs].len;
+                int d = end == 0 ? e.slot(cs, 0) : e.slot(cs, len - 1);
+                int a = end == 0 ? e.slot(cs, 1) : e.slot(cs, len - 2);
+                int idv = e.ident_of(d);
+                m.d = d;
+                m.a = a;
+                m.id = (idv < 0 ? -idv : idv) - 1;
+                I3 ap = e.pos(a);
+                m.ax = ap.x;
+                m.ay = ap.y;
+                m.az = ap.z;
+                m.k_old = 6 * unit_index(sub3(e.pos(d), ap)) + e.dom[d].ore;
+            }
+            __syncthreads();
+            learned_context_lane(e, m, lane, 64);
+            __syncthreads();
+            size_t o = (size_t)r * cap + n;
+            for (int s = lane; s < 343; s += 64) {
+                ch_out[o * 343 + s] = m.ch[s];
+            }
+            if (lane < 36) {
+                legal_out[o * 36 + lane] = m.legal[lane / 6];
+            }
+            if (lane == 0) {
+                id_out[o] = m.id;
+                tgt_out[o] = m.k_old;
+            }
+            n++;
+            __syncthreads();
+        }
+    }
+    if (lane == 0) {
+        counts[r] = n;
     }
 }
 
@@ -252,6 +358,7 @@ struct Ctx {
     double* d_win_stored = nullptr;
     std::vector<ldo_window> windows; // host copy
     bool lds_sweep = false;           // sweep with the replica blocks in LDS
+    float* d_net = nullptr;           // learned proposal parameters
 };
 
 struct HipError {
@@ -487,7 +594,7 @@ void free_ctx(Ctx* c) {
     }
     void* bufs[] = {c->d_blocks, c->d_type_len, c->d_type_ident, c->d_tab, c->d_init,
                     c->d_tapes, c->d_tape_meta, c->d_xint, c->d_xdbl, c->d_xcnt, c->d_cfg,
-                    c->d_windows, c->d_grid, c->d_win_stored};
+                    c->d_windows, c->d_grid, c->d_win_stored, c->d_net};
     for (void* b: bufs) {
         if (b) {
             (void)hipFree(b);
@@ -670,6 +777,109 @@ int ldo_set_tape(void* ctx, int32_t replica, const uint64_t* tape, int64_t n) {
         int64_t cursor = 0;
         upload(*c, c->d_blocks + (size_t)replica * c->P.L.bytes + offsetof(RepScalars, tape_pos),
                &cursor, sizeof cursor, "reset tape cursor");
+    });
+}
+
+int ldo_learned_set_network(void* ctx, int32_t n_ident, const float* params, double fallback) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        if (!params || n_ident < 1) {
+            throw AbiError {LDO_E_ARG, "parameters and n_ident >= 1 required"};
+        }
+        ldo_mlp_layout L;
+        ldo::mlp_layout(n_ident, L);
+        if (!(fallback >= 0.0 && fallback <= 1.0)) {
+            throw AbiError {LDO_E_ARG, "fallback probability outside [0, 1]"};
+        }
+        if (c->P.n_moves > LDO_MAX_MOVES - 2) {
+            throw AbiError {LDO_E_ARG, "the learned move needs two free counter slots"};
+        }
+        if (c->P.rng_mode != 0) {
+            throw AbiError {LDO_E_ARG, "the learned move draws from the Philox streams"};
+        }
+        if (n_ident < c->P.L.ns) {
+            throw AbiError {LDO_E_ARG, "n_ident must cover every scaffold domain identity"};
+        }
+        release(*c, c->d_net, "hipFree network");
+        hip_check(hipMalloc(&c->d_net, L.total * sizeof(float)), "hipMalloc network");
+        upload(*c, c->d_net, params, L.total * sizeof(float), "copy network");
+        c->P.net = c->d_net;
+        int64_t offs[10] = {L.w1, L.emb, L.w1e, L.b1, L.w2, L.b2, L.w3, L.b3, L.w4, L.b4};
+        for (int k = 0; k < 10; k++) {
+            c->P.net_off[k] = offs[k];
+        }
+        c->P.net_n_ident = n_ident;
+        c->P.learned_slot = LDO_MAX_MOVES - 1;
+        c->P.learned_fallback = fallback;
+    });
+}
+
+int ldo_learned_sweep(void* ctx, int64_t attempts) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        if (attempts < 0) {
+            throw AbiError {LDO_E_ARG, "negative attempt count"};
+        }
+        if (!c->P.net) {
+            throw AbiError {LDO_E_MISUSE, "no network (ldo_learned_set_network)"};
+        }
+        hipLaunchKernelGGL(ldo_learned_kernel, dim3(c->R), dim3(64), 0, c->stream, c->P,
+                           attempts);
+        hip_check(hipGetLastError(), "launch learned");
+        check_statuses(*c);
+    });
+}
+
+int ldo_learned_samples(void* ctx, int32_t cap, uint8_t* channels, int32_t* ident,
+                        uint8_t* legal, int32_t* target, int32_t* n_out) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        if (cap < 0 || !channels || !ident || !legal || !target || !n_out) {
+            throw AbiError {LDO_E_ARG, "buffers and cap >= 0 required"};
+        }
+        int per = 2 * c->P.max_total_staples;
+        size_t n = (size_t)c->R * per;
+        uint8_t *d_ch = nullptr, *d_lg = nullptr;
+        int32_t *d_id = nullptr, *d_tg = nullptr, *d_cnt = nullptr;
+        auto cleanup = [&]() {
+            for (void* b: {(void*)d_ch, (void*)d_lg, (void*)d_id, (void*)d_tg, (void*)d_cnt}) {
+                if (b) {
+                    (void)hipFree(b);
+                }
+            }
+        };
+        try {
+            hip_check(hipMalloc(&d_ch, std::max<size_t>(1, n * 343)), "hipMalloc samples");
+            hip_check(hipMalloc(&d_lg, std::max<size_t>(1, n * 36)), "hipMalloc samples");
+            hip_check(hipMalloc(&d_id, std::max<size_t>(1, n) * 4), "hipMalloc samples");
+            hip_check(hipMalloc(&d_tg, std::max<size_t>(1, n) * 4), "hipMalloc samples");
+            hip_check(hipMalloc(&d_cnt, (size_t)c->R * 4), "hipMalloc samples");
+            hipLaunchKernelGGL(ldo_learned_samples_kernel, dim3(c->R), dim3(64), 0, c->stream,
+                               c->P, per, d_ch, d_id, d_lg, d_tg, d_cnt);
+            hip_check(hipGetLastError(), "launch samples");
+            std::vector<int32_t> cnt(c->R), id(n), tg(n);
+            std::vector<uint8_t> ch(n * 343), lg(n * 36);
+            download(*c, cnt.data(), d_cnt, cnt.size() * 4, "copy counts");
+            download(*c, ch.data(), d_ch, ch.size(), "copy samples");
+            download(*c, lg.data(), d_lg, lg.size(), "copy samples");
+            download(*c, id.data(), d_id, id.size() * 4, "copy samples");
+            download(*c, tg.data(), d_tg, tg.size() * 4, "copy samples");
+            int32_t k = 0;
+            for (int r = 0; r < c->R; r++) {
+                for (int i = 0; i < cnt[r] && k < cap; i++, k++) {
+                    size_t o = (size_t)r * per + i;
+                    std::memcpy(channels + (size_t)k * 343, &ch[o * 343], 343);
+                    std::memcpy(legal + (size_t)k * 36, &lg[o * 36], 36);
+                    ident[k] = id[o];
+                    target[k] = tg[o];
+                }
+            }
+            *n_out = k;
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
     });
 }
 

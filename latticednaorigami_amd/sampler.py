@@ -62,6 +62,34 @@ class Sampler:
         g = (C.c_double * max(1, len(grid)))(*grid)
         self._check(self.lib.ldo_set_windows(self.h, n, arr, g, grid_len))
 
+    def learned_set_network(self, n_ident, params, fallback=0.1):
+        """Enable learned domain regrowth with these network parameters
+        (ldo_mlp.h layout); `fallback` is the plain CB probability."""
+        import numpy as np
+        self._net = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self.lib.ldo_learned_set_network(
+            self.h, n_ident, self._net.ctypes.data_as(C.POINTER(C.c_float)), fallback))
+
+    def learned_sweep(self, attempts):
+        self._check(self.lib.ldo_learned_sweep(self.h, attempts))
+
+    def learned_samples(self, cap):
+        """Training samples of the current configurations:
+        (channels [n, 343], ident [n], legal [n, 36], target [n])."""
+        import numpy as np
+        ch = np.zeros((max(1, cap), 343), dtype=np.uint8)
+        lg = np.zeros((max(1, cap), 36), dtype=np.uint8)
+        idn = np.zeros(max(1, cap), dtype=np.int32)
+        tg = np.zeros(max(1, cap), dtype=np.int32)
+        n = C.c_int32()
+        u8 = C.POINTER(C.c_uint8)
+        i32 = C.POINTER(C.c_int32)
+        self._check(self.lib.ldo_learned_samples(
+            self.h, cap, ch.ctypes.data_as(u8), idn.ctypes.data_as(i32), lg.ctypes.data_as(u8),
+            tg.ctypes.data_as(i32), C.byref(n)))
+        k = n.value
+        return ch[:k], idn[:k], lg[:k], tg[:k]
+
     def set_tape(self, r, values):
         buf = (C.c_uint64 * max(1, len(values)))(*values)
         self._check(self.lib.ldo_set_tape(self.h, r, buf, len(values)))

@@ -39,6 +39,11 @@ int oracle_set_ladder(void* o, int32_t n, const double* temps,
 int oracle_set_windows(void* o, int32_t n, const ldo_window* windows, const double* grid,
                        int32_t grid_len);
 int oracle_sweep(void* o, int64_t steps);
+/* learned domain regrowth (the sampler's ldo_learned_* restated) */
+int oracle_learned_set_network(void* o, int32_t n_ident, const float* params, double fallback);
+int oracle_learned_sweep(void* o, int64_t attempts);
+int oracle_learned_samples(void* o, int32_t cap, uint8_t* channels, int32_t* ident,
+                           uint8_t* legal, int32_t* target, int32_t* n_out);
 int oracle_run(void* o, int64_t n_swaps, int64_t exchange_interval);
 int oracle_get_observables(void* o, int32_t replica, ldo_obs* out);
 int oracle_get_counters(void* o, ldo_counters* out);

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "../include/ldo_mlp.h"
 #include "ldo_oracle.h"
 #include "nn_thermo.hpp"
 #include "oracle_core.hpp"
@@ -23,6 +24,7 @@ struct Oracle {
     std::vector<int> q_to_rep;
     int rng_kind {0};
     uint64_t seed {0};
+    std::vector<float> net; // learned proposal parameters
     Rng exchange_rng;
     int64_t swap_counter {0};
     ldo_counters counters {};
@@ -156,6 +158,97 @@ int oracle_set_ladder(void* o, int32_t n, const double* t, const double* u, cons
         O->ladder_u.assign(u, u + n);
         O->ladder_b.assign(b, b + n);
         O->ladder_s.assign(s, s + n);
+    })
+}
+
+// element offsets of the flat parameter vector, in the order ldo_mlp.h
+// lists the tensors
+static ldo_mlp_layout mlp_layout_of(int n_ident) {
+    const int64_t H = LDO_MLP_HIDDEN;
+    const int64_t sizes[10] = {(int64_t)LDO_MLP_FEATURES * H, (int64_t)n_ident * LDO_MLP_EMBED,
+                               (int64_t)LDO_MLP_EMBED * H, H, H * H, H, H * H, H,
+                               H * LDO_MLP_OUT, LDO_MLP_OUT};
+    int64_t* f[10];
+    ldo_mlp_layout L;
+    f[0] = &L.w1;
+    f[1] = &L.emb;
+    f[2] = &L.w1e;
+    f[3] = &L.b1;
+    f[4] = &L.w2;
+    f[5] = &L.b2;
+    f[6] = &L.w3;
+    f[7] = &L.b3;
+    f[8] = &L.w4;
+    f[9] = &L.b4;
+    int64_t o = 0;
+    for (int k = 0; k < 10; k++) {
+        *f[k] = o;
+        o += sizes[k];
+    }
+    L.total = o;
+    return L;
+}
+
+int oracle_learned_set_network(void* o, int32_t n_ident, const float* params, double fallback) {
+    ORACLE_TRY(o, {
+        if (!params || n_ident < 1 || !(fallback >= 0.0 && fallback <= 1.0) ||
+            O->rng_kind == 1) {
+            throw OrigamiError(LDO_E_ARG, "bad learned-move arguments");
+        }
+        ldo_mlp_layout L = mlp_layout_of(n_ident);
+        O->net.assign(params, params + L.total);
+        int64_t offs[10] = {L.w1, L.emb, L.w1e, L.b1, L.w2, L.b2, L.w3, L.b3, L.w4, L.b4};
+        for (auto& r: O->reps) {
+            if ((int)r->sys.chains[0].size() > n_ident) {
+                throw OrigamiError(LDO_E_ARG, "n_ident must cover every scaffold domain");
+            }
+            r->net = O->net.data();
+            for (int k = 0; k < 10; k++) {
+                r->net_off[k] = offs[k];
+            }
+            r->net_n_ident = n_ident;
+            r->learned_fallback = fallback;
+            r->attempts.resize(LDO_MAX_MOVETYPES, 0);
+            r->accepts.resize(LDO_MAX_MOVETYPES, 0);
+        }
+    })
+}
+
+int oracle_learned_sweep(void* o, int64_t attempts) {
+    ORACLE_TRY(o, {
+        for (auto& r: O->reps) {
+            if (!r->net) {
+                throw OrigamiError(LDO_E_MISUSE, "no network");
+            }
+            for (int64_t i = 0; i < attempts; i++) {
+                r->learned_attempt();
+            }
+        }
+    })
+}
+
+int oracle_learned_samples(void* o, int32_t cap, uint8_t* channels, int32_t* ident,
+                           uint8_t* legal, int32_t* target, int32_t* n_out) {
+    ORACLE_TRY(o, {
+        int32_t k = 0;
+        for (auto& r: O->reps) {
+            for (int w = 1; w <= r->sys.num_staples; w++) {
+                const std::vector<Dom*>& ch = r->sys.chains[w];
+                int len = (int)ch.size();
+                for (int end = 0; end < 2 && k < cap; end++) {
+                    Dom* d = end == 0 ? ch[0] : ch[len - 1];
+                    Dom* a = end == 0 ? ch[1] : ch[len - 2];
+                    uint8_t lg[6];
+                    target[k] = r->learned_context(d, a, channels + (size_t)k * 343, lg);
+                    for (int q = 0; q < 36; q++) {
+                        legal[(size_t)k * 36 + q] = lg[q / 6];
+                    }
+                    ident[k] = std::abs(d->d_ident) - 1;
+                    k++;
+                }
+            }
+        }
+        *n_out = k;
     })
 }
 

@@ -117,7 +117,7 @@ void Rng::init(int kind, uint64_t seed, uint32_t stream) {
 }
 
 uint64_t Rng::philox64() {
-    uint32_t ctr[4] = {m_idx, 0u, (uint32_t)(uint64_t)m_step,
+    uint32_t ctr[4] = {m_idx, m_sub, (uint32_t)(uint64_t)m_step,
                        (uint32_t)((uint64_t)m_step >> 32)};
     uint32_t key[2] = {(uint32_t)m_seed, m_stream};
     uint32_t out[4];

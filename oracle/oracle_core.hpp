@@ -82,6 +82,13 @@ class Rng {
     void init(int kind, uint64_t seed, uint32_t stream);
     void begin_step(int64_t step) {
         m_step = step;
+        m_sub = 0;
+        m_idx = 0;
+    }
+    // sub-stream `sub` (1: learned-move attempts) at counter `ctr`
+    void begin_sub(uint32_t sub, int64_t ctr) {
+        m_step = ctr;
+        m_sub = sub;
         m_idx = 0;
     }
     double uniform_real();
@@ -95,6 +102,7 @@ class Rng {
     uint64_t m_seed {0};
     uint32_t m_stream {0};
     int64_t m_step {0};
+    uint32_t m_sub {0};
     uint32_t m_idx {0};
     bool m_record {false};
     std::mt19937_64 m_mt;
@@ -380,6 +388,19 @@ class Replica {
     void simulate(int64_t steps); // simulation.cpp:568-653
     int window_op(int staple_offset) const;
     double calc_move_bias(int staple_offset);
+
+    // learned domain regrowth (no reference counterpart; the sampler's
+    // ldo_learned_*): network parameters in the ldo_mlp.h layout
+    const float* net {nullptr};
+    int64_t net_off[10] {};
+    int net_n_ident {0};
+    double learned_fallback {0};
+    int64_t learned_step {0};
+    void learned_attempt();
+    // the network input around the chain neighbour of staple end d (d left
+    // out); returns the output d occupies
+    int learned_context(Dom* d, Dom* a, uint8_t ch[343], uint8_t legal[6]) const;
+    void learned_logits(const uint8_t ch[343], int id, float logits[36]) const;
 
   private:
     std::vector<MovesetEntry> m_moves;

@@ -4,6 +4,9 @@
 // TEST INFRASTRUCTURE ONLY (see ldo_oracle.h).
 #include <algorithm>
 #include <cmath>
+#include <functional>
+#include <set>
+#include <unordered_map>
 
 #include "oracle_core.hpp"
 
@@ -1015,6 +1018,204 @@ double Replica::calc_move_bias(int staple_offset) {
 // CBMCMovetype::add_external_bias (cb_movetypes.cpp:52-56)
 void Replica::cb_add_external_bias() {
     m_bias *= std::exp(-calc_move_bias(0));
+}
+
+
+// ------------------------------------------------------ learned proposal
+// Learned domain regrowth, restated from the sampler's definition (ldo.h,
+// ldo_mlp.h); the network arithmetic follows the sampler's order (layer 1 as
+// sequential float sums over the sites, then fused multiply-adds over the
+// embedding, dense layers as fused multiply-adds in input order) so that
+// every decision matches.
+static int unit_of(V3 v) {
+    for (int k = 0; k < 6; k++) {
+        if (kUnit[k] == v) {
+            return k;
+        }
+    }
+    return -1;
+}
+
+int Replica::learned_context(Dom* d, Dom* a, uint8_t ch[343], uint8_t legal[6]) const {
+    std::unordered_map<V3, std::vector<Dom*>, V3Hash> at;
+    for (auto& c: sys.chains) {
+        for (Dom* x: c) {
+            if (x->st != Occ::unassigned && x != d) {
+                at[x->pos].push_back(x);
+            }
+        }
+    }
+    V3 A = a->pos;
+    for (int s = 0; s < 343; s++) {
+        V3 p {A.x + s % 7 - 3, A.y + (s / 7) % 7 - 3, A.z + s / 49 - 3};
+        auto it = at.find(p);
+        int scaf = 6, stap = 6;
+        if (it != at.end()) {
+            for (Dom* x: it->second) {
+                int o = unit_of(x->ore);
+                if (x->c == 0) {
+                    scaf = std::min(scaf, o);
+                }
+                else {
+                    stap = std::min(stap, o);
+                }
+            }
+        }
+        ch[s] = (uint8_t)(scaf < 6 ? 1 + scaf : (stap < 6 ? 7 + stap : 0));
+    }
+    for (int n = 0; n < 6; n++) {
+        V3 p = A + kUnit[n];
+        auto it = at.find(p);
+        legal[n] = (it == at.end() || it->second.size() <= 1) ? 1 : 0;
+    }
+    return 6 * unit_of(d->pos - A) + unit_of(d->ore);
+}
+
+void Replica::learned_logits(const uint8_t ch[343], int id, float logits[36]) const {
+    const float* W = net;
+    float h0[256], h1[256];
+    const float* e = W + net_off[1] + (int64_t)id * 16;
+    for (int j = 0; j < 256; j++) {
+        float s = W[net_off[3] + j];
+        for (int site = 0; site < 343; site++) {
+            s += W[net_off[0] + (int64_t)(site * 13 + ch[site]) * 256 + j];
+        }
+        for (int k = 0; k < 16; k++) {
+            s = std::fmaf(e[k], W[net_off[2] + k * 256 + j], s);
+        }
+        h0[j] = std::fmax(s, 0.f);
+    }
+    auto dense = [&](int64_t wo, int64_t bo, const float* in, float* out, int n_out, bool relu) {
+        for (int j = 0; j < n_out; j++) {
+            float s = W[bo + j];
+            for (int i = 0; i < 256; i++) {
+                s = std::fmaf(in[i], W[wo + (int64_t)i * n_out + j], s);
+            }
+            out[j] = relu ? std::fmax(s, 0.f) : s;
+        }
+    };
+    dense(net_off[4], net_off[5], h0, h1, 256, true);
+    dense(net_off[6], net_off[7], h1, h0, 256, true);
+    dense(net_off[8], net_off[9], h0, logits, 36, false);
+}
+
+// whether staple chain c reaches the scaffold through bound pairs
+static bool chain_connected(const std::vector<Dom*>& chain,
+                            const std::function<bool(Dom*, std::set<int>&)>& scan) {
+    for (Dom* x: chain) {
+        Dom* b = x->bd;
+        if (b == nullptr || b->c == x->c) {
+            continue;
+        }
+        if (b->c == 0) {
+            return true;
+        }
+        std::set<int> part {x->c};
+        if (scan(b, part)) {
+            return true;
+        }
+    }
+    return false;
+}
+
+void Replica::learned_attempt() {
+    learned_step++;
+    rng.begin_sub(1, learned_step);
+    const int fb = LDO_MAX_MOVETYPES - 2, ls = LDO_MAX_MOVETYPES - 1;
+    if (rng.uniform_real() < learned_fallback) {
+        attempts[fb]++;
+        m_kind = LDO_MOVE_CB_STAPLE_REGROWTH;
+        bool acc = attempt(LDO_MOVE_CB_STAPLE_REGROWTH, 0);
+        accepts[fb] += acc;
+        if (!acc) {
+            reset_origami();
+            calc_move_bias(0);
+        }
+        return;
+    }
+    attempts[ls]++;
+    if (sys.num_staples == 0) {
+        return;
+    }
+    int w = rng.uniform_int(1, sys.num_staples);
+    const std::vector<Dom*>& chain = sys.chains[w];
+    int len = (int)chain.size();
+    bool first = rng.uniform_real() < 0.5;
+    Dom* d = first ? chain[0] : chain[len - 1];
+    Dom* a = first ? chain[1] : chain[len - 2];
+    int id = std::abs(d->d_ident) - 1;
+    if (len < 2 || id < 0 || id >= net_n_ident) {
+        throw OrigamiError(LDO_E_MISUSE, "learned move: identity outside the network");
+    }
+    uint8_t ch[343], legal[6];
+    int k_old = learned_context(d, a, ch, legal);
+    float logits[36];
+    learned_logits(ch, id, logits);
+    double w8[36];
+    float mx = -INFINITY;
+    for (int k = 0; k < 36; k++) {
+        if (legal[k / 6]) {
+            mx = std::fmax(mx, logits[k]);
+        }
+    }
+    double z = 0;
+    for (int k = 0; k < 36; k++) {
+        w8[k] = legal[k / 6] ? std::exp((double)logits[k] - (double)mx) : 0.0;
+        z += w8[k];
+    }
+    double u = rng.uniform_real() * z;
+    double cum = 0;
+    int kn = -1;
+    for (int k = 0; k < 36; k++) {
+        if (w8[k] == 0) {
+            continue;
+        }
+        kn = k;
+        cum += w8[k];
+        if (u < cum) {
+            break;
+        }
+    }
+    bool accepted = false;
+    if (kn == k_old) {
+        accepted = true;
+    }
+    else {
+        m_modified.clear();
+        m_assigned.clear();
+        m_added.clear();
+        m_prev_pos.clear();
+        m_prev_ore.clear();
+        m_modifier = 1;
+        sys.violated = false;
+        Dom* partner = d->bd;
+        auto key = std::make_pair(d->c, d->d);
+        m_prev_pos[key] = d->pos;
+        m_prev_ore[key] = d->ore;
+        m_modified.push_back(key);
+        double de = sys.unassign_domain(*d);
+        V3 pn = a->pos + kUnit[kn / 6];
+        de += sys.set_domain_config(*d, pn, kUnit[kn % 6]);
+        m_assigned.push_back(key);
+        if (!sys.violated) {
+            auto scan = [this](Dom* x, std::set<int>& part) {
+                return scan_for_scaffold_domain(x, part);
+            };
+            bool ok = chain_connected(chain, scan);
+            if (ok && partner != nullptr && partner->c != 0 && partner->c != d->c) {
+                ok = chain_connected(sys.chain_by_index(partner->c), scan);
+            }
+            if (ok) {
+                de += calc_move_bias(0);
+                accepted = test_acceptance((long double)(std::exp(-de) * (w8[k_old] / w8[kn])));
+            }
+        }
+        if (!accepted) {
+            reset_origami();
+            calc_move_bias(0);
+        }
+    }
+    accepts[ls] += accepted;
 }
 
 } // namespace orc

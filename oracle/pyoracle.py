@@ -56,6 +56,12 @@ def lib():
         L.oracle_enumerate.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_int32,
                                        C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double)]
+        u8p = C.POINTER(C.c_uint8)
+        i32p = C.POINTER(C.c_int32)
+        L.oracle_learned_set_network.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float),
+                                                 C.c_double]
+        L.oracle_learned_sweep.argtypes = [C.c_void_p, C.c_int64]
+        L.oracle_learned_samples.argtypes = [C.c_void_p, C.c_int32, u8p, i32p, u8p, i32p, i32p]
         _lib = L
     return _lib
 
@@ -94,6 +100,31 @@ class Oracle:
 
     def sweep(self, steps):
         _check(self.L, self.h, self.L.oracle_sweep(self.h, steps))
+
+    def learned_set_network(self, n_ident, params, fallback=0.1):
+        import numpy as np
+        self._net = np.ascontiguousarray(params, dtype=np.float32)
+        _check(self.L, self.h, self.L.oracle_learned_set_network(
+            self.h, n_ident, self._net.ctypes.data_as(C.POINTER(C.c_float)), fallback))
+
+    def learned_sweep(self, attempts):
+        _check(self.L, self.h, self.L.oracle_learned_sweep(self.h, attempts))
+
+    def learned_samples(self, cap):
+        """(channels [n, 343], ident [n], legal [n, 36], target [n])"""
+        import numpy as np
+        ch = np.zeros((max(1, cap), 343), dtype=np.uint8)
+        lg = np.zeros((max(1, cap), 36), dtype=np.uint8)
+        idn = np.zeros(max(1, cap), dtype=np.int32)
+        tg = np.zeros(max(1, cap), dtype=np.int32)
+        n = C.c_int32()
+        u8 = C.POINTER(C.c_uint8)
+        i32 = C.POINTER(C.c_int32)
+        _check(self.L, self.h, self.L.oracle_learned_samples(
+            self.h, cap, ch.ctypes.data_as(u8), idn.ctypes.data_as(i32), lg.ctypes.data_as(u8),
+            tg.ctypes.data_as(i32), C.byref(n)))
+        k = n.value
+        return ch[:k], idn[:k], lg[:k], tg[:k]
 
     def run(self, n_swaps, interval):
         _check(self.L, self.h, self.L.oracle_run(self.h, n_swaps, interval))

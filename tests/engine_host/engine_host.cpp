@@ -39,8 +39,31 @@ struct HostCtx {
     std::vector<std::vector<uint64_t>> tapes;
     std::vector<ldo_window> windows;
     std::vector<double> grid, win_stored;
+    std::vector<float> net;
     ~HostCtx() { std::free(blocks); }
 };
+
+// the learned kernel's per-replica loop with the 64 lanes run in turn
+// between the phases the kernel separates with barriers
+void learned_replica(const Params& P, int r, int64_t attempts) {
+    Eng e;
+    LearnedMove m;
+    e.attach(&P, P.blocks + (size_t)r * P.L.bytes, r);
+    e.select_tables();
+    for (int64_t it = 0; it < attempts; it++) {
+        if (e.S->status != ST_OK) {
+            break;
+        }
+        if (e.learned_begin(m) == 1) {
+            for (int ph = 0; ph < LDO_LEARNED_PHASES; ph++) {
+                for (int lane = 0; lane < 64; lane++) {
+                    learned_phase_lane(e, P, m, ph, lane, 64);
+                }
+            }
+            e.learned_finish(m);
+        }
+    }
+}
 
 void set_tables(HostCtx& c) {
     int n = c.R;
@@ -238,6 +261,81 @@ int eh_sweep(void* ctx, int64_t steps) {
             sweep_replica(c->P, r, steps, nullptr);
         }
         check_statuses(*c);
+    });
+}
+
+int eh_learned_set_network(void* ctx, int32_t n_ident, const float* params, double fallback) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        if (!params || n_ident < c->P.L.ns || c->P.n_moves > LDO_MAX_MOVES - 2 ||
+            !(fallback >= 0.0 && fallback <= 1.0) || c->P.rng_mode != 0) {
+            throw ldo::AbiError {LDO_E_ARG, "bad learned-move arguments"};
+        }
+        ldo_mlp_layout L;
+        ldo::mlp_layout(n_ident, L);
+        c->net.assign(params, params + L.total);
+        c->P.net = c->net.data();
+        int64_t offs[10] = {L.w1, L.emb, L.w1e, L.b1, L.w2, L.b2, L.w3, L.b3, L.w4, L.b4};
+        for (int k = 0; k < 10; k++) {
+            c->P.net_off[k] = offs[k];
+        }
+        c->P.net_n_ident = n_ident;
+        c->P.learned_slot = LDO_MAX_MOVES - 1;
+        c->P.learned_fallback = fallback;
+    });
+}
+
+int eh_learned_sweep(void* ctx, int64_t attempts) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        if (attempts < 0 || !c->P.net) {
+            throw ldo::AbiError {LDO_E_ARG, "bad learned sweep"};
+        }
+        for (int r = 0; r < c->R; r++) {
+            learned_replica(c->P, r, attempts);
+        }
+        check_statuses(*c);
+    });
+}
+
+int eh_learned_samples(void* ctx, int32_t cap, uint8_t* channels, int32_t* ident,
+                       uint8_t* legal, int32_t* target, int32_t* n_out) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        int32_t k = 0;
+        for (int r = 0; r < c->R; r++) {
+            Eng e;
+            LearnedMove m;
+            e.attach(&c->P, c->P.blocks + (size_t)r * c->P.L.bytes, r);
+            for (int w = 1; w <= e.S->num_staples; w++) {
+                for (int end = 0; end < 2 && k < cap; end++) {
+                    int cs = e.order[w];
+                    int len = e.ch[cs].len;
+                    int d = end == 0 ? e.slot(cs, 0) : e.slot(cs, len - 1);
+                    int a = end == 0 ? e.slot(cs, 1) : e.slot(cs, len - 2);
+                    int idv = e.ident_of(d);
+                    m.d = d;
+                    m.a = a;
+                    m.id = (idv < 0 ? -idv : idv) - 1;
+                    I3 ap = e.pos(a);
+                    m.ax = ap.x;
+                    m.ay = ap.y;
+                    m.az = ap.z;
+                    m.k_old = 6 * unit_index(sub3(e.pos(d), ap)) + e.dom[d].ore;
+                    for (int lane = 0; lane < 64; lane++) {
+                        learned_context_lane(e, m, lane, 64);
+                    }
+                    std::memcpy(channels + (size_t)k * 343, m.ch, 343);
+                    for (int q = 0; q < 36; q++) {
+                        legal[(size_t)k * 36 + q] = m.legal[q / 6];
+                    }
+                    ident[k] = m.id;
+                    target[k] = m.k_old;
+                    k++;
+                }
+            }
+        }
+        *n_out = k;
     });
 }
 

@@ -227,6 +227,50 @@ def case_mwus3d_window_ladders_by_temperature(sampler_cls, oracle_mod):
     assert_counters(gpu, ref)
 
 
+def random_network(n_ident, seed):
+    """Seeded network parameters in the ldo_mlp.h layout, scaled so that the
+    proposal is far from uniform."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    H, F, E, O = 256, 4459, 16, 36
+    parts = [rng.uniform(-1, 1, F * H) * np.sqrt(6 / 343), rng.uniform(-1, 1, n_ident * E),
+             rng.uniform(-1, 1, E * H) * np.sqrt(6 / E), rng.uniform(-0.1, 0.1, H),
+             rng.uniform(-1, 1, H * H) * np.sqrt(6 / H), rng.uniform(-0.1, 0.1, H),
+             rng.uniform(-1, 1, H * H) * np.sqrt(6 / H), rng.uniform(-0.1, 0.1, H),
+             rng.uniform(-1, 1, H * O) * np.sqrt(6 / H) * 3, rng.uniform(-0.5, 0.5, O)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def case_learned_move_matches_oracle(sampler_cls, oracle_mod):
+    say("case: learned domain regrowth (random network, CB fallback 0.1) on the 64-domain "
+        "raster at 330 K and 345 K: 3 x (150 learned attempts + 500 steps), samples")
+    import numpy as np
+    s = systems.raster_system(replicas=2)
+    temps = [330.0, 345.0]
+    gpu = sampler_cls(s, temps=temps)
+    ref = oracle_mod.Oracle(s, temps=temps)
+    gpu.sweep(3000)
+    ref.sweep(3000)
+    net = random_network(64, 11)
+    gpu.learned_set_network(64, net, 0.1)
+    ref.learned_set_network(64, net, 0.1)
+    for _ in range(3):
+        gpu.learned_sweep(150)
+        ref.learned_sweep(150)
+        for r in range(2):
+            assert_same(gpu, ref, r)
+        gpu.sweep(500)
+        ref.sweep(500)
+    assert_counters(gpu, ref)
+    c = gpu.counters()
+    assert c.attempts[7] == 2 * 450 - c.attempts[6] and c.attempts[6] > 0
+    assert 0 < c.accepts[7] < c.attempts[7]
+    a, b = gpu.learned_samples(4096), ref.learned_samples(4096)
+    assert len(a[0]) > 0
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
 def case_batch_matches_oracle(batch_cls, oracle_mod):
     say("case: batch of 12 chains (8-40 domains, 310-350 K), 3 x 500 steps, each segment on "
         "a different sweep kernel (LDS, wavefront, lane)")

@@ -33,7 +33,8 @@ class HostLib:
                                    C.POINTER(C.c_void_p)]
         cdll.eh_set_tape.restype = C.c_int
         cdll.eh_set_tape.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64), C.c_int64]
-        for n in NAMES:
+        abi.declare_learned_abi(cdll, "eh_")
+        for n in NAMES + ["learned_set_network", "learned_sweep", "learned_samples"]:
             setattr(self, "ldo_" + n, getattr(cdll, "eh_" + n))
 
 
@@ -87,6 +88,10 @@ def test_mwus3d_window_ladders_by_temperature(sampler_cls, oracle_mod):
 
 def test_three_quarter_turn_and_three_domain_staples(sampler_cls, oracle_mod):
     pc.case_three_quarter_turn_and_three_domain_staples(sampler_cls, oracle_mod)
+
+
+def test_learned_move_matches_oracle(sampler_cls, oracle_mod):
+    pc.case_learned_move_matches_oracle(sampler_cls, oracle_mod)
 
 
 def test_reference_ctcb_rule_fails_alike_on_3d_system(sampler_cls, oracle_mod):

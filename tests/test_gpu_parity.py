@@ -45,6 +45,10 @@ def test_three_quarter_turn_and_three_domain_staples(sampler_cls, oracle_mod):
     pc.case_three_quarter_turn_and_three_domain_staples(sampler_cls, oracle_mod)
 
 
+def test_learned_move_matches_oracle(sampler_cls, oracle_mod):
+    pc.case_learned_move_matches_oracle(sampler_cls, oracle_mod)
+
+
 def test_batch_matches_oracle(oracle_mod):
     from latticednaorigami_amd.sampler import BatchSampler
     pc.case_batch_matches_oracle(BatchSampler, oracle_mod)

@@ -288,3 +288,84 @@ def test_reference_ctcb_rule_unbinds_three_domain_staples(oracle_mod):
     m_bare, _ = _mean_se(counts, (0, 0, 0, 0), 1000)
     assert exact[(1, 3, 0, 2)] > 0.5 and m_full < 0.05
     assert exact[(0, 0, 0, 0)] < 0.4 and m_bare > 0.7
+
+
+def _learned_system():
+    """A staple [1, 5] whose domain 5 has no complement (misbinding
+    disallowed): bound through domain 1, its end domain 5 always dangles, so
+    its orientation carries no energy and is uniform in equilibrium."""
+    from latticednaorigami_amd import abi
+    s = systems.four_domain_system(seed=13, max_total_staples=2)
+    s.max_type_staples = 2
+    s.identities = [[-1, -2, -3, -4], [1, 5], [3, 4]]
+    s.misbinding_pot = abi.MISBIND_DISALLOWED
+    return s
+
+
+def _dangling_orientations(oracle_mod, network, n_samples=500):
+    """Per-replica frequencies of the six orientations of unbound staple
+    domains, sampled between rounds of 20 learned attempts and 5 plain steps
+    at 300 K."""
+    import numpy as np
+    import parity_cases as pc
+    o = oracle_mod.Oracle(_learned_system(), temps=[300.0] * 8)
+    o.learned_set_network(5, pc.random_network(5, network), 0.0)
+    o.sweep(5000)
+    units = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    hist = np.zeros((8, 6))
+    for _ in range(n_samples):
+        o.learned_sweep(20)
+        o.sweep(5)
+        for r in range(8):
+            cfg = o.config(r)
+            occ = {}
+            for ch in cfg:
+                for p in ch.positions:
+                    occ[p] = occ.get(p, 0) + 1
+            for ch in cfg[1:]:
+                for p, q in zip(ch.positions, ch.orientations):
+                    if occ[p] == 1:
+                        hist[r, units.index(q)] += 1
+    return hist / hist.sum(axis=1, keepdims=True), o.counters()
+
+
+def test_learned_move_keeps_unbound_orientations_uniform(oracle_mod):
+    """The learned proposal (a seeded random network whose output layer is
+    scaled so that it prefers some orientations several-fold) moves the
+    dangling domain far more often than the plain moves do.  Its
+    Metropolis-Hastings factor q(old)/q(new) must cancel that preference: the
+    sampled orientation of unbound staple domains stays uniform (1/6 each,
+    within 4 standard errors over 8 replicas).  Without the factor the
+    frequencies are 0.12-0.25 (35 standard errors off)."""
+    import numpy as np
+    fr, c = _dangling_orientations(oracle_mod, 5)
+    assert c.accepts[7] > 2000
+    m = fr.mean(axis=0)
+    se = fr.std(axis=0, ddof=1) / np.sqrt(len(fr))
+    assert np.all(np.abs(m - 1 / 6) <= 4 * se + 0.005), (m, se)
+
+
+def test_learned_move_matches_enumeration(oracle_mod):
+    """Learned attempts (with the CB fallback) interleaved with the plain
+    moves sample the enumerated distribution of the four-domain system."""
+    import parity_cases as pc
+    s = _system(2, seed=13)
+    o = oracle_mod.Oracle(s, temps=[345.0] * 8)
+    exact, _, _ = o.enumerate()
+    o.learned_set_network(4, pc.random_network(4, 5), 0.1)
+    o.sweep(20000)
+    counts = [dict() for _ in range(8)]
+    n = 600
+    for _ in range(n):
+        o.learned_sweep(10)
+        o.sweep(100)
+        for r in range(8):
+            ob = o.observables(r)
+            k = (ob["num_staples"], ob["num_fully_bound_domain_pairs"],
+                 ob["num_misbound_domain_pairs"], ob["num_stacked_domain_pairs"])
+            counts[r][k] = counts[r].get(k, 0) + 1
+    for key, p in exact.items():
+        if p < 0.01:
+            continue
+        m, se = _mean_se(counts, key, n)
+        assert abs(m - p) <= 4 * se + 0.005, (key, m, p, se)
