@@ -260,6 +260,69 @@ def gpu_leg_mwus3d(args):
     return accepted / wall, wall, roofline, extra
 
 
+def gpu_leg_learned(args):
+    """The learned proposal on the PTMC raster system: collect samples from
+    the plain sampler, train the network on the device, then compare
+    effective samples per second of the plain and the learned form."""
+    import numpy as np
+    from latticednaorigami_amd import learned, systems
+    from latticednaorigami_amd.sampler import Sampler
+    s = systems.raster_system()
+    temps = s.temps
+    t0 = time.perf_counter()
+    data, n_cfg = learned.collect_samples(s, temps, args.learned_rounds, 500,
+                                          seed_steps=args.learned_seed_steps)
+    t_collect = time.perf_counter() - t0
+    n = len(data[1])
+    print(f"learned: {n} samples from {n_cfg} configurations in {t_collect:.1f} s",
+          file=sys.stderr, flush=True)
+    rng = np.random.default_rng(0)
+    perm = rng.permutation(n)
+    hold = perm[: max(1, n // 10)]
+    fit = perm[max(1, n // 10):]
+    split = lambda idx: tuple(a[idx] for a in data)  # noqa: E731
+    t0 = time.perf_counter()
+    params, hist = learned.train(split(fit), 64, args.learned_train_steps, batch=256, lr=1e-3,
+                                 log_every=500)
+    t_train = time.perf_counter() - t0
+    held = learned.mean_loss(params, 64, split(hold))
+    uniform = float(np.mean(np.log(split(hold)[2].sum(axis=1))))
+    keys = ["num_fully_bound_domain_pairs", "num_staples"]
+    res = {}
+    for form in ("plain", "learned"):
+        smp = Sampler(s, temps=temps)
+        smp.sweep(args.learned_seed_steps)
+        if form == "learned":
+            smp.learned_set_network(64, params, 0.1)
+            la = args.learned_attempts
+        else:
+            la = 0
+        series, wall = learned.run_series(smp, args.learned_eval_rounds, args.interval, la, keys)
+        c = smp.counters()
+        print(f"learned: {form} form evaluated in {wall:.1f} s", file=sys.stderr, flush=True)
+        res[form] = {
+            "seconds": wall,
+            "ess_per_s": {k: learned.ess(series[k]) / wall for k in keys},
+            "series": series,
+            "learned_accepts": c.accepts[7],
+            "learned_attempts": c.attempts[7],
+            "accepted_moves_per_s": sum(c.accepts) / wall,
+        }
+        smp.close()
+    gap = learned.free_energy_gap(res["plain"]["series"][keys[0]],
+                                  res["learned"]["series"][keys[0]])
+    ratio = {k: res["learned"]["ess_per_s"][k] / res["plain"]["ess_per_s"][k]
+             if res["plain"]["ess_per_s"][k] > 0 else None for k in keys}
+    for f in res.values():
+        f.pop("series")
+    return res, ratio, gap, {"samples": n, "configurations": n_cfg,
+                             "collect_s": t_collect, "train_s": t_train,
+                             "train_steps": args.learned_train_steps,
+                             "first_loss": float(np.mean(hist[:20])),
+                             "last_loss": float(np.mean(hist[-20:])),
+                             "heldout_loss": held, "uniform_loss": uniform}
+
+
 def gpu_leg(args, system):
     from latticednaorigami_amd import abi
     from latticednaorigami_amd.sampler import Sampler
@@ -320,7 +383,7 @@ def main():
                     help="CPU-baseline processes (0: min(8, host cores))")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="raster64_ptmc32",
-                    choices=["raster64_ptmc32", "batch4096", "mwus3d"],
+                    choices=["raster64_ptmc32", "batch4096", "mwus3d", "learned"],
                     help="raster64_ptmc32: the 32-replica PTMC raster run; batch4096: 4096 "
                          "independent constant-T chains of 8-128 domains; mwus3d: the "
                          "200-domain 3D scaffold, 64 umbrella windows x 4 T "
@@ -335,6 +398,15 @@ def main():
                     help="batch4096: MC steps per chain in the CPU baseline")
     ap.add_argument("--cpu-swaps", type=int, default=400,
                     help="mwus3d: exchange intervals in each CPU-baseline process")
+    ap.add_argument("--learned-rounds", type=int, default=200,
+                    help="learned: sample-collection rounds (500 steps each)")
+    ap.add_argument("--learned-seed-steps", type=int, default=20000,
+                    help="learned: plain steps before collection and before each evaluation")
+    ap.add_argument("--learned-train-steps", type=int, default=3000)
+    ap.add_argument("--learned-eval-rounds", type=int, default=200,
+                    help="learned: evaluation rounds (--interval plain steps each)")
+    ap.add_argument("--learned-attempts", type=int, default=100,
+                    help="learned: learned attempts per replica per evaluation round")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -346,6 +418,8 @@ def main():
         return main_batch(args)
     if args.workload == "mwus3d":
         return main_mwus3d(args)
+    if args.workload == "learned":
+        return main_learned(args)
     from latticednaorigami_amd import systems
     system = systems.raster_system()
     line = {
@@ -476,6 +550,45 @@ def main_mwus3d(args):
     if not args.no_cpu_baseline:
         workers = args.cpu_workers or min(8, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline_mwus3d(workers, 8, args.cpu_swaps, args.interval)
+    print(json.dumps(line), flush=True)
+    sys.exit(status)
+
+
+def main_learned(args):
+    line = {
+        "metric": "ESS/sec of numfulldomains (all replicas, 1 GPU), learned vs plain form",
+        "value": None,
+        "unit": "effective samples/s",
+        "n_gpus": 1,
+        "steps": args.learned_eval_rounds,
+        "warmup": args.learned_seed_steps,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 chain, bf16/fp32 network",
+        "data": "synthetic",
+        "config": {
+            "workload": "learned_raster64",
+            "system": "64-domain raster, NN, 32 replicas at 300-360 K, no exchange",
+            "network": "7x7x7x13 one-hot + 16-dim identity embedding, 3 x 256 ReLU, 36 outputs",
+            "fallback_cb": 0.1,
+            "round": f"{args.learned_attempts} learned attempts + {args.interval} plain steps "
+                     f"per replica (plain form: the plain steps only)",
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    status = 0
+    try:
+        res, ratio, gap, train = gpu_leg_learned(args)
+        line["value"] = res["learned"]["ess_per_s"]["num_fully_bound_domain_pairs"]
+        line["ms_per_step"] = res["learned"]["seconds"] / args.learned_eval_rounds * 1e3
+        line["config"].update({"forms": res, "ess_ratio_learned_over_plain": ratio,
+                               "free_energy_gap_kT": gap, "training": train})
+    except Exception as e:
+        line["error"] = f"{type(e).__name__}: {e}"
+        status = 1
     print(json.dumps(line), flush=True)
     sys.exit(status)
 
