@@ -20,6 +20,9 @@ HIPCC_FLAGS = [
 ]
 
 
+DRIVER = os.path.join(LIB_DIR, "ldo_run")
+
+
 def build(verbose=False):
     os.makedirs(LIB_DIR, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -29,6 +32,14 @@ def build(verbose=False):
         print(" ".join(cmd))
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # the reference-compatible driver: host C++ over the C ABI
+    cmd = ["g++", "-O2", "-std=c++17", "-o", DRIVER + ".tmp",
+           os.path.join(CSRC, "driver", "ldo_run.cpp"), "-L" + LIB_DIR, "-lldo_hip",
+           "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    os.replace(DRIVER + ".tmp", DRIVER)
     return LIB
 
 

@@ -1,0 +1,760 @@
+// ldo_run.cpp - the reference's simulation loop on the sampler's C ABI.
+//
+// Reads the reference's inputs (the .inp parameter file of parser.cpp, the
+// system JSON of OrigamiInputFile, the moveset JSON of
+// setup_movetypes, files.cpp:140-218, 296-306) and writes the reference's
+// output files (files.cpp:493-791: .vsf .trj .vcf .states .ores .counts
+// .staples .staplestates .times .ene .ops, and .swp for parallel tempering,
+// ptmc_simulation.cpp:91-104, 315-322) while every replica runs on the GPU
+// through ldo.h.  Supported: simulation_type constant_temp and
+// ut_parallel_tempering with the movetypes OrientationRotation,
+// MetStapleExchange, CBStapleRegrowth and CTCBScaffoldRegrowth.
+//
+//   ldo_run <parameter file> [key=value ...]
+//
+// Later key=value arguments override the file, as the reference's command
+// line overrides its config file.  Outputs are written after the MC steps
+// whose index is a multiple of the output's frequency, as GCMCSimulation::
+// simulate writes them (simulation.cpp:620-650); with parallel tempering the
+// frequencies must be multiples of exchange_interval.  ldo_flags sets
+// ldo_run_params.flags (no reference key).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/ldo.h"
+
+namespace {
+
+struct Fail : std::runtime_error {
+    int code;
+    Fail(int c, const std::string& m): std::runtime_error(m), code(c) {}
+};
+
+// ------------------------------------------------------------------ JSON
+struct Json {
+    enum Kind { Null, Bool, Num, Str, Arr, Obj } kind = Null;
+    bool b = false;
+    double num = 0;
+    std::string str;
+    std::vector<Json> arr;
+    std::map<std::string, Json> obj;
+
+    const Json& at(const std::string& k) const {
+        auto it = obj.find(k);
+        if (kind != Obj || it == obj.end()) {
+            throw Fail(LDO_E_IO, "JSON key missing: " + k);
+        }
+        return it->second;
+    }
+    bool has(const std::string& k) const { return kind == Obj && obj.count(k) > 0; }
+};
+
+struct JsonReader {
+    const std::string& s;
+    size_t i = 0;
+    explicit JsonReader(const std::string& t): s(t) {}
+
+    void ws() {
+        while (i < s.size() && (s[i] == ' ' || s[i] == '\n' || s[i] == '\r' || s[i] == '\t')) {
+            i++;
+        }
+    }
+    [[noreturn]] void bad() { throw Fail(LDO_E_IO, "malformed JSON at byte " + std::to_string(i)); }
+    char peek() {
+        ws();
+        if (i >= s.size()) {
+            bad();
+        }
+        return s[i];
+    }
+    void expect(char c) {
+        if (peek() != c) {
+            bad();
+        }
+        i++;
+    }
+    Json value() {
+        char c = peek();
+        Json v;
+        if (c == '{') {
+            i++;
+            v.kind = Json::Obj;
+            if (peek() == '}') {
+                i++;
+                return v;
+            }
+            while (true) {
+                Json k = value();
+                if (k.kind != Json::Str) {
+                    bad();
+                }
+                expect(':');
+                v.obj[k.str] = value();
+                if (peek() == ',') {
+                    i++;
+                    continue;
+                }
+                expect('}');
+                return v;
+            }
+        }
+        if (c == '[') {
+            i++;
+            v.kind = Json::Arr;
+            if (peek() == ']') {
+                i++;
+                return v;
+            }
+            while (true) {
+                v.arr.push_back(value());
+                if (peek() == ',') {
+                    i++;
+                    continue;
+                }
+                expect(']');
+                return v;
+            }
+        }
+        if (c == '"') {
+            i++;
+            v.kind = Json::Str;
+            while (i < s.size() && s[i] != '"') {
+                if (s[i] == '\\' && i + 1 < s.size()) {
+                    i++;
+                }
+                v.str += s[i++];
+            }
+            if (i >= s.size()) {
+                bad();
+            }
+            i++;
+            return v;
+        }
+        if (s.compare(i, 4, "true") == 0 || s.compare(i, 5, "false") == 0) {
+            v.kind = Json::Bool;
+            v.b = s[i] == 't';
+            i += v.b ? 4 : 5;
+            return v;
+        }
+        if (s.compare(i, 4, "null") == 0) {
+            i += 4;
+            return v;
+        }
+        size_t end = i;
+        while (end < s.size() && std::string("+-0123456789.eE").find(s[end]) != std::string::npos) {
+            end++;
+        }
+        if (end == i) {
+            bad();
+        }
+        v.kind = Json::Num;
+        v.num = std::stod(s.substr(i, end - i));
+        i = end;
+        return v;
+    }
+};
+
+std::string slurp(const std::string& path) {
+    std::ifstream f(path);
+    if (!f.good()) {
+        throw Fail(LDO_E_IO, path + ": cannot be read");
+    }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    return ss.str();
+}
+
+Json read_json(const std::string& path) {
+    std::string text = slurp(path);
+    JsonReader r(text);
+    return r.value();
+}
+
+// --------------------------------------------------------------- params
+struct Params {
+    std::map<std::string, std::string> kv;
+
+    std::string str(const std::string& k, const std::string& def) const {
+        auto it = kv.find(k);
+        return it == kv.end() ? def : it->second;
+    }
+    double num(const std::string& k, double def) const {
+        auto it = kv.find(k);
+        return it == kv.end() ? def : std::stod(it->second);
+    }
+    long long integer(const std::string& k, long long def) const {
+        auto it = kv.find(k);
+        return it == kv.end() ? def : std::stoll(it->second);
+    }
+    bool flag(const std::string& k, bool def) const {
+        auto it = kv.find(k);
+        if (it == kv.end()) {
+            return def;
+        }
+        return it->second == "true" || it->second == "1";
+    }
+    std::vector<double> list(const std::string& k) const {
+        std::vector<double> out;
+        std::stringstream ss(str(k, ""));
+        double v;
+        while (ss >> v) {
+            out.push_back(v);
+        }
+        return out;
+    }
+    std::vector<std::string> words(const std::string& k) const {
+        std::vector<std::string> out;
+        std::stringstream ss(str(k, ""));
+        std::string w;
+        while (ss >> w) {
+            out.push_back(w);
+        }
+        return out;
+    }
+};
+
+std::string trim(const std::string& s) {
+    size_t a = s.find_first_not_of(" \t\r"), b = s.find_last_not_of(" \t\r");
+    return a == std::string::npos ? "" : s.substr(a, b - a + 1);
+}
+
+void parse_assignment(const std::string& line, Params& p) {
+    std::string l = line.substr(0, line.find('#'));
+    size_t eq = l.find('=');
+    if (eq == std::string::npos) {
+        if (!trim(l).empty()) {
+            throw Fail(LDO_E_IO, "parameter line without '=': " + line);
+        }
+        return;
+    }
+    p.kv[trim(l.substr(0, eq))] = trim(l.substr(eq + 1));
+}
+
+// "a/b" fractions or plain numbers (utility.cpp:266-282)
+double fraction(const Json& v) {
+    if (v.kind == Json::Num) {
+        return v.num;
+    }
+    size_t slash = v.str.find('/');
+    if (slash == std::string::npos) {
+        return std::stod(v.str);
+    }
+    return std::stod(v.str.substr(0, slash)) / std::stod(v.str.substr(slash + 1));
+}
+
+int enum_of(const std::string& v, const std::vector<std::string>& names, const char* what) {
+    for (size_t k = 0; k < names.size(); k++) {
+        if (names[k] == v) {
+            return (int)k;
+        }
+    }
+    throw Fail(LDO_E_NOTIMPL, std::string(what) + " " + v + " is not supported");
+}
+
+// ----------------------------------------------------------------- run
+struct Chain {
+    int index, identity;
+    std::vector<int> pos, ore; // 3 per domain
+};
+
+struct Run {
+    Params p;
+    std::vector<std::vector<int>> identities;
+    int max_staple_size = 2, max_total_staples = 999, n_scaffold = 0;
+    void* ctx = nullptr;
+    int R = 1;
+    std::vector<double> temps;
+
+    void check(int rc) {
+        if (rc != LDO_OK) {
+            throw Fail(rc, ldo_last_error(ctx));
+        }
+    }
+
+    std::vector<Chain> config(int r) {
+        int cap_c = max_total_staples + 1;
+        int cap_d = n_scaffold + max_total_staples * max_staple_size;
+        std::vector<int32_t> idx(cap_c), idn(cap_c), len(cap_c), pos(3 * cap_d), ore(3 * cap_d);
+        ldo_config c {};
+        c.chain_index = idx.data();
+        c.chain_identity = idn.data();
+        c.chain_length = len.data();
+        c.positions = pos.data();
+        c.orientations = ore.data();
+        check(ldo_get_config(ctx, r, &c, cap_c, cap_d));
+        std::vector<Chain> out;
+        int o = 0;
+        for (int k = 0; k < c.n_chains; k++) {
+            Chain ch {idx[k], idn[k], {}, {}};
+            ch.pos.assign(pos.begin() + 3 * o, pos.begin() + 3 * (o + len[k]));
+            ch.ore.assign(ore.begin() + 3 * o, ore.begin() + 3 * (o + len[k]));
+            o += len[k];
+            out.push_back(ch);
+        }
+        return out;
+    }
+};
+
+// occupancy state of every domain (OrigamiStateOutputFile codes: 1 unbound,
+// 2 bound, 3 misbound) from the positions: a site holds one domain, or a
+// pair that is bound when the identities are complementary
+std::vector<std::vector<int>> domain_states(const Run& run, const std::vector<Chain>& chains) {
+    std::map<std::vector<int>, std::vector<std::pair<int, int>>> at;
+    for (size_t c = 0; c < chains.size(); c++) {
+        for (size_t d = 0; d < chains[c].pos.size() / 3; d++) {
+            std::vector<int> p(chains[c].pos.begin() + 3 * d, chains[c].pos.begin() + 3 * d + 3);
+            at[p].push_back({(int)c, (int)d});
+        }
+    }
+    std::vector<std::vector<int>> st(chains.size());
+    for (size_t c = 0; c < chains.size(); c++) {
+        st[c].assign(chains[c].pos.size() / 3, 1);
+    }
+    for (auto& kv: at) {
+        if (kv.second.size() == 2) {
+            auto a = kv.second[0], b = kv.second[1];
+            int ia = run.identities[chains[a.first].identity][a.second];
+            int ib = run.identities[chains[b.first].identity][b.second];
+            int code = (ia == -ib) ? 2 : 3;
+            st[a.first][a.second] = code;
+            st[b.first][b.second] = code;
+        }
+    }
+    return st;
+}
+
+struct Outputs {
+    std::string base;
+    long long f_cfg = 0, f_vtf = 0, f_counts = 0, f_times = 0, f_ene = 0, f_ops = 0;
+    std::ofstream trj, vcf, states, ores, counts, staples, staplestates, times, ene, ops;
+    std::vector<std::string> op_tags;
+    int max_domains = 0;
+
+    static void open(std::ofstream& f, const std::string& path) {
+        f.open(path);
+        if (!f.good()) {
+            throw Fail(LDO_E_IO, path + ": output path does not exist");
+        }
+    }
+};
+
+std::unique_ptr<Outputs> setup_outputs(const Run& run, const std::string& base) {
+    auto o = std::make_unique<Outputs>();
+    const Params& p = run.p;
+    o->base = base;
+    o->max_domains = run.max_staple_size * run.max_total_staples + run.n_scaffold;
+    {
+        std::ofstream vsf;
+        Outputs::open(vsf, base + ".vsf");
+        vsf << "atom 0:" << run.n_scaffold << " radius 0.25 type scaffold\n";
+        vsf << "atom " << run.n_scaffold << ":" << o->max_domains - 1;
+        vsf << " radius 0.25 type staple";
+    }
+    o->f_cfg = p.integer("configs_output_freq", 0);
+    o->f_vtf = p.integer("vtf_output_freq", 0);
+    o->f_counts = p.integer("counts_output_freq", 0);
+    o->f_times = p.integer("times_output_freq", 0);
+    o->f_ene = p.integer("energies_output_freq", 0);
+    o->f_ops = p.integer("order_params_output_freq", 0);
+    if (o->f_cfg) {
+        Outputs::open(o->trj, base + ".trj");
+    }
+    if (o->f_vtf) {
+        Outputs::open(o->vcf, base + ".vcf");
+        Outputs::open(o->states, base + ".states");
+        Outputs::open(o->ores, base + ".ores");
+    }
+    if (o->f_counts) {
+        Outputs::open(o->counts, base + ".counts");
+        Outputs::open(o->staples, base + ".staples");
+        Outputs::open(o->staplestates, base + ".staplestates");
+    }
+    if (o->f_times) {
+        Outputs::open(o->times, base + ".times");
+        o->times << "step time\n";
+    }
+    if (o->f_ene) {
+        Outputs::open(o->ene, base + ".ene");
+        o->ene << "step tenergy henthalpy hentropy stacking bias\n";
+        o->ene.precision(10);
+    }
+    if (o->f_ops) {
+        Outputs::open(o->ops, base + ".ops");
+        o->op_tags = p.words("ops_to_output");
+        for (auto& t: o->op_tags) {
+            if (t != "numfulldomains" && t != "nummisdomains" && t != "numstackedpairs" &&
+                t != "numstaples") {
+                throw Fail(LDO_E_NOTIMPL, "order parameter " + t + " is not supported");
+            }
+            o->ops << t << ", ";
+        }
+        o->ops << "\n";
+        o->ops.precision(10);
+        o->ops.flush();
+    }
+    return o;
+}
+
+bool due(long long freq, long long step) { return freq != 0 && step % freq == 0; }
+
+void write_outputs(Run& run, Outputs& o, int r, long long step, double seconds) {
+    bool need_cfg = due(o.f_cfg, step) || due(o.f_vtf, step) || due(o.f_counts, step);
+    std::vector<Chain> chains;
+    std::vector<std::vector<int>> st;
+    if (need_cfg) {
+        chains = run.config(r);
+        st = domain_states(run, chains);
+    }
+    if (due(o.f_cfg, step)) {
+        o.trj << step << "\n";
+        for (auto& ch: chains) {
+            o.trj << ch.index << " " << ch.identity << "\n";
+            for (int v: ch.pos) {
+                o.trj << v << " ";
+            }
+            o.trj << "\n";
+            for (int v: ch.ore) {
+                o.trj << v << " ";
+            }
+            o.trj << "\n";
+        }
+        o.trj << "\n";
+        o.trj.flush();
+    }
+    if (due(o.f_vtf, step)) {
+        o.vcf << "timestep\n";
+        int written = 0;
+        for (size_t c = 0; c < chains.size(); c++) {
+            int n = 0;
+            for (size_t d = 0; d < chains[c].pos.size() / 3; d++, n++) {
+                for (int i = 0; i < 3; i++) {
+                    o.vcf << chains[c].pos[3 * d + i] << " ";
+                    o.ores << chains[c].ore[3 * d + i] << " ";
+                }
+                o.vcf << "\n";
+                o.states << st[c][d] << " ";
+            }
+            for (; n < run.max_staple_size; n++) {
+                o.vcf << "0 0 0 \n";
+                o.ores << "0 0 0 ";
+                o.states << "-1 ";
+            }
+            written += n;
+        }
+        for (int k = written; k < o.max_domains; k++) {
+            o.vcf << "0 0 0 \n";
+            o.ores << "0 0 0 ";
+            o.states << "-1 ";
+        }
+        o.vcf << "\n";
+        o.ores << "\n";
+        o.states << "\n";
+        o.vcf.flush();
+        o.ores.flush();
+        o.states.flush();
+    }
+    ldo_obs ob {};
+    if (due(o.f_counts, step) || due(o.f_ene, step) || due(o.f_ops, step)) {
+        run.check(ldo_get_observables(run.ctx, r, &ob));
+    }
+    if (due(o.f_counts, step)) {
+        int n_types = (int)run.identities.size();
+        std::vector<int> per(n_types, 0);
+        std::vector<int> full(n_types, 0);
+        for (size_t c = 1; c < chains.size(); c++) {
+            per[chains[c].identity]++;
+            bool all = true;
+            for (int s: st[c]) {
+                all = all && s == 2;
+            }
+            full[chains[c].identity] |= all ? 1 : 0;
+        }
+        int unique = 0;
+        for (int t = 1; t < n_types; t++) {
+            unique += per[t] > 0;
+        }
+        o.counts << step << " " << ob.num_staples << " " << unique << " "
+                 << ob.num_bound_domain_pairs << " " << ob.num_fully_bound_domain_pairs << " "
+                 << ob.num_misbound_domain_pairs << " \n";
+        o.staples << step << " ";
+        o.staplestates << step << " ";
+        for (int t = 1; t < n_types; t++) {
+            o.staples << per[t] << " ";
+            o.staplestates << full[t] << " ";
+        }
+        o.staples << "\n";
+        o.staplestates << "\n";
+        o.counts.flush();
+        o.staples.flush();
+        o.staplestates.flush();
+    }
+    if (due(o.f_times, step)) {
+        o.times << step << " " << seconds << "\n";
+        o.times.flush();
+    }
+    if (due(o.f_ene, step)) {
+        o.ene << step << " " << ob.energy << " " << ob.enthalpy << " " << ob.entropy << " "
+              << ob.stacking_energy << " " << ob.bias << " \n";
+        o.ene.flush();
+    }
+    if (due(o.f_ops, step)) {
+        o.ops << step;
+        for (auto& t: o.op_tags) {
+            int v = t == "numfulldomains"    ? ob.num_fully_bound_domain_pairs
+                    : t == "nummisdomains"   ? ob.num_misbound_domain_pairs
+                    : t == "numstackedpairs" ? ob.num_stacked_domain_pairs
+                                             : ob.num_staples;
+            o.ops << " " << v;
+        }
+        o.ops << "\n";
+        o.ops.flush();
+    }
+}
+
+long long gcd_of(long long a, long long b) { return b == 0 ? a : gcd_of(b, a % b); }
+
+int main_run(int argc, char** argv) {
+    if (argc < 2 || std::string(argv[1]) == "-h" || std::string(argv[1]) == "--help") {
+        std::cout << "usage: ldo_run <parameter file> [key=value ...]\n";
+        return argc < 2 ? 2 : 0;
+    }
+    Run run;
+    Params& p = run.p;
+    {
+        std::stringstream ss(slurp(argv[1]));
+        std::string line;
+        while (std::getline(ss, line)) {
+            parse_assignment(line, p);
+        }
+        for (int a = 2; a < argc; a++) {
+            parse_assignment(argv[a], p);
+        }
+    }
+    // system JSON (OrigamiInputFile, files.cpp:140-218)
+    const Json sys = read_json(p.str("origami_input_filename", "")).at("origami");
+    for (auto& ch: sys.at("identities").arr) {
+        std::vector<int> ids;
+        for (auto& v: ch.arr) {
+            ids.push_back((int)v.num);
+        }
+        run.identities.push_back(ids);
+    }
+    run.n_scaffold = (int)run.identities.at(0).size();
+    std::vector<std::string> seq_store;
+    std::vector<const char*> seqs;
+    if (sys.has("sequences")) {
+        for (auto& ch: sys.at("sequences").arr) {
+            for (auto& v: ch.arr) {
+                seq_store.push_back(v.str);
+            }
+        }
+        for (auto& s: seq_store) {
+            seqs.push_back(s.c_str());
+        }
+    }
+    std::vector<int32_t> lens, flat;
+    for (auto& ids: run.identities) {
+        lens.push_back((int32_t)ids.size());
+        flat.insert(flat.end(), ids.begin(), ids.end());
+    }
+    ldo_system_desc sd {};
+    sd.n_chain_types = (int32_t)run.identities.size();
+    sd.chain_lengths = lens.data();
+    sd.identities = flat.data();
+    sd.sequences = seqs.empty() ? nullptr : seqs.data();
+    sd.cyclic = sys.has("cyclic") && sys.at("cyclic").b;
+    sd.domain_type = enum_of(p.str("domain_type", "Halfturn"), {"HalfTurn", "ThreeQuarterTurn"},
+                             "domain_type");
+    if (p.str("binding_pot", "FourBody") != "FourBody") {
+        throw Fail(LDO_E_NOTIMPL, "binding_pot " + p.str("binding_pot", "") + " is not supported");
+    }
+    sd.hybridization_pot = enum_of(p.str("hybridization_pot", "NearestNeighbour"),
+                                   {"NearestNeighbour", "Uniform", "Specified"},
+                                   "hybridization_pot");
+    sd.misbinding_pot = enum_of(p.str("misbinding_pot", "Opposing"), {"Opposing", "Disallowed"},
+                                "misbinding_pot");
+    sd.stacking_pot = enum_of(p.str("stacking_pot", "Constant"),
+                              {"Constant", "SequenceSpecific"}, "stacking_pot");
+    sd.apply_mean_field_cor = p.flag("apply_mean_field_cor", false);
+    sd.binding_h = p.num("binding_h", 0);
+    sd.binding_s = p.num("binding_s", 0);
+    sd.misbinding_h = p.num("misbinding_h", 0);
+    sd.misbinding_s = p.num("misbinding_s", 0);
+    sd.stacking_ene = p.num("stacking_ene", 1);
+    sd.cation_M = p.num("cation_M", 1);
+    sd.staple_M = p.num("staple_M", 1);
+    // moveset JSON (setup_movetypes, files.cpp:296-306)
+    const Json mv = read_json(p.str("movetype_file", "")).at("origami").at("movetypes");
+    std::vector<int32_t> kinds, maxreg;
+    std::vector<double> freqs;
+    for (auto& m: mv.arr) {
+        kinds.push_back(enum_of(m.at("type").str,
+                                {"OrientationRotation", "MetStapleExchange", "CBStapleRegrowth",
+                                 "CTCBScaffoldRegrowth"},
+                                "movetype"));
+        freqs.push_back(fraction(m.at("freq")));
+        maxreg.push_back(m.has("max_regrowth") ? (int32_t)m.at("max_regrowth").num : 0);
+    }
+    run.max_total_staples = (int)p.integer("max_total_staples", 999);
+    run.max_staple_size = (int)p.integer("max_staple_size", 2);
+    ldo_run_params rp {};
+    rp.n_movetypes = (int32_t)kinds.size();
+    rp.movetype_kind = kinds.data();
+    rp.movetype_freq = freqs.data();
+    rp.movetype_max_regrowth = maxreg.data();
+    rp.max_total_staples = run.max_total_staples;
+    rp.max_type_staples = (int32_t)p.integer("max_type_staples", 999);
+    rp.max_staple_size = run.max_staple_size;
+    rp.centering_freq = (int32_t)p.integer("centering_freq", 0);
+    rp.centering_domain = (int32_t)p.integer("centering_domain", 0);
+    rp.constraint_check_freq = (int32_t)p.integer("constraint_check_freq", 0);
+    long long seed = p.integer("random_seed", -1);
+    if (seed < 0) {
+        seed = (long long)std::chrono::steady_clock::now().time_since_epoch().count() & 0x7FFFFFFF;
+    }
+    rp.seed = (uint64_t)seed;
+    rp.flags = (int32_t)p.integer("ldo_flags", 0);
+    // initial configuration: the first of the file's configurations
+    std::vector<int32_t> c_idx, c_idn, c_len, c_pos, c_ore;
+    const Json& cfg0 = sys.at("configurations").arr.at(0);
+    for (auto& ch: cfg0.at("chains").arr) {
+        c_idx.push_back((int32_t)ch.at("index").num);
+        c_idn.push_back((int32_t)ch.at("identity").num);
+        c_len.push_back((int32_t)ch.at("positions").arr.size());
+        for (auto& v: ch.at("positions").arr) {
+            for (auto& x: v.arr) {
+                c_pos.push_back((int32_t)x.num);
+            }
+        }
+        for (auto& v: ch.at("orientations").arr) {
+            for (auto& x: v.arr) {
+                c_ore.push_back((int32_t)x.num);
+            }
+        }
+    }
+    ldo_config init {};
+    init.n_chains = (int32_t)c_idx.size();
+    init.chain_index = c_idx.data();
+    init.chain_identity = c_idn.data();
+    init.chain_length = c_len.data();
+    init.positions = c_pos.data();
+    init.orientations = c_ore.data();
+
+    std::string sim = p.str("simulation_type", "constant_temp");
+    bool pt = sim == "ut_parallel_tempering";
+    if (!pt && sim != "constant_temp") {
+        throw Fail(LDO_E_NOTIMPL, "simulation_type " + sim + " is not supported");
+    }
+    if (pt) {
+        run.temps = p.list("temps");
+        run.R = (int)p.integer("num_reps", (long long)run.temps.size());
+        if ((int)run.temps.size() != run.R) {
+            throw Fail(LDO_E_ARG, "temps must list num_reps temperatures");
+        }
+        for (const char* k: {"chem_pot_mults", "bias_mults", "stacking_mults"}) {
+            for (double v: p.list(k)) {
+                if (v != 1.0) {
+                    throw Fail(LDO_E_NOTIMPL, std::string(k) + " other than 1");
+                }
+            }
+        }
+        rp.exchange_mode = LDO_EXCHANGE_TEMPERATURE;
+    }
+    else {
+        run.temps = {p.num("temp", 300)};
+    }
+    rp.rng_mode = LDO_RNG_PHILOX;
+    int rc = ldo_create(&sd, &rp, &init, run.R, run.temps.data(), &run.ctx);
+    if (rc != LDO_OK) {
+        throw Fail(rc, ldo_last_error(nullptr));
+    }
+
+    std::string base = p.str("output_filebase", "");
+    std::vector<std::unique_ptr<Outputs>> outs;
+    for (int r = 0; r < run.R; r++) {
+        outs.push_back(setup_outputs(run, pt ? base + "-" + std::to_string(r) : base));
+    }
+    const Outputs& o0 = *outs[0];
+    long long chunk = 0;
+    for (long long f: {o0.f_cfg, o0.f_vtf, o0.f_counts, o0.f_times, o0.f_ene, o0.f_ops}) {
+        if (f > 0) {
+            chunk = gcd_of(f, chunk);
+        }
+    }
+    auto start = std::chrono::steady_clock::now();
+    auto elapsed = [&]() {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    };
+    long long step = 0;
+    if (!pt) {
+        long long total = p.integer("ct_steps", 0);
+        if (chunk == 0) {
+            chunk = total;
+        }
+        while (step < total) {
+            long long n = std::min(chunk - step % chunk, total - step);
+            run.check(ldo_sweep(run.ctx, n));
+            step += n;
+            write_outputs(run, *outs[0], 0, step, elapsed());
+        }
+    }
+    else {
+        long long swaps = p.integer("swaps", 0);
+        long long interval = p.integer("exchange_interval", 1);
+        // each ldo_run interval starts by switching every replica to its
+        // ladder slot, so an output may not fall inside an interval
+        if (interval < 1 || (chunk > 0 && chunk % interval != 0)) {
+            throw Fail(LDO_E_NOTIMPL, "output frequencies must be multiples of exchange_interval");
+        }
+        std::ofstream swp;
+        Outputs::open(swp, base + ".swp");
+        for (int r = 0; r < run.R; r++) {
+            swp << run.temps[r] << "/" << p.num("staple_u_mult", 1) << "/ ";
+        }
+        swp << "\n";
+        long long cfg_freq = o0.f_cfg;
+        std::vector<int32_t> q_to_rep(run.R);
+        for (long long s = 1; s <= swaps; s++) {
+            run.check(ldo_run(run.ctx, 1, interval));
+            step += interval;
+            for (int r = 0; r < run.R; r++) {
+                write_outputs(run, *outs[r], r, step, elapsed());
+            }
+            if (cfg_freq != 0 && step % cfg_freq == 0) {
+                run.check(ldo_get_permutation(run.ctx, q_to_rep.data()));
+                for (int q = 0; q < run.R; q++) {
+                    swp << q_to_rep[q] << " ";
+                }
+                swp << "\n";
+                swp.flush();
+            }
+        }
+    }
+    ldo_destroy(run.ctx);
+    return 0;
+}
+
+} // namespace
+
+int main(int argc, char** argv) {
+    try {
+        return main_run(argc, argv);
+    } catch (Fail& e) {
+        std::cerr << "ldo_run: " << e.what() << " (status " << e.code << ")\n";
+        return 1;
+    } catch (std::exception& e) {
+        std::cerr << "ldo_run: " << e.what() << "\n";
+        return 1;
+    }
+}
