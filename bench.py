@@ -31,6 +31,12 @@ HBM_PEAK_GBS = 8000.0
 BYTES_EMPTY, BYTES_BIND, BYTES_COMMIT, BYTES_MOVE = 24, 216, 32, 16
 
 
+# HBM bytes per 1000-step sweep launch of the raster workload, measured with
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a copy of known size
+# (profiles/ldo_sweep_lds_kernel_raster64_ptmc32_pmc.md)
+RASTER_TRAFFIC_PER_1000 = 7.78e6
+
+
 def algorithmic_bytes(c0, c1):
     n_empty = c1.site_empty - c0.site_empty
     n_bind = c1.site_bind - c0.site_bind
@@ -355,7 +361,7 @@ def gpu_leg(args, system):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS if achieved is not None else None,
-        "traffic": None,
+        "traffic": RASTER_TRAFFIC_PER_1000 if args.interval == 1000 else None,
         "kernel": smp.lib.ldo_kernel_name(smp.h).decode(),
         "launches": prof["sweep_launches"],
         "avg_launch_ms": prof["sweep_ms"] / max(1, prof["sweep_launches"]),
@@ -383,7 +389,7 @@ def main():
                     help="CPU-baseline processes (0: min(8, host cores))")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="raster64_ptmc32",
-                    choices=["raster64_ptmc32", "batch4096", "mwus3d", "learned"],
+                    choices=["raster64_ptmc32", "batch4096", "mwus3d", "learned", "hbm_copy"],
                     help="raster64_ptmc32: the 32-replica PTMC raster run; batch4096: 4096 "
                          "independent constant-T chains of 8-128 domains; mwus3d: the "
                          "200-domain 3D scaffold, 64 umbrella windows x 4 T "
@@ -420,6 +426,8 @@ def main():
         return main_mwus3d(args)
     if args.workload == "learned":
         return main_learned(args)
+    if args.workload == "hbm_copy":
+        return main_hbm_copy(args)
     from latticednaorigami_amd import systems
     system = systems.raster_system()
     line = {
@@ -591,6 +599,26 @@ def main_learned(args):
         status = 1
     print(json.dumps(line), flush=True)
     sys.exit(status)
+
+
+def main_hbm_copy(args):
+    """Calibration for the HBM traffic counters: `--steps` device copies of a
+    256 MiB buffer (256 MiB read and 256 MiB written per copy), so that
+    rocprofv3's FETCH_SIZE / WRITE_SIZE per dispatch of this copy give the
+    factor from counter units to bytes."""
+    import torch
+    n = 64 << 20
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"workload": "hbm_copy", "bytes_read_per_copy": 4 * n,
+                      "bytes_written_per_copy": 4 * n, "copies": args.steps,
+                      "GB_per_s": 8 * n * args.steps / dt / 1e9}), flush=True)
 
 
 if __name__ == "__main__":
