@@ -295,14 +295,17 @@ def gpu_leg_learned(args):
     uniform = float(np.mean(np.log(split(hold)[2].sum(axis=1))))
     keys = ["num_fully_bound_domain_pairs", "num_staples"]
     res = {}
+    eval_temps = [args.learned_eval_temp] * len(temps)
     for form in ("plain", "learned"):
-        smp = Sampler(s, temps=temps)
+        smp = Sampler(s, temps=eval_temps)
         smp.sweep(args.learned_seed_steps)
         if form == "learned":
             smp.learned_set_network(64, params, 0.1)
             la = args.learned_attempts
         else:
             la = 0
+        # each form relaxes under its own moves before the measured window
+        learned.run_series(smp, args.learned_eval_rounds // 4, args.interval, la, keys)
         series, wall = learned.run_series(smp, args.learned_eval_rounds, args.interval, la, keys)
         c = smp.counters()
         print(f"learned: {form} form evaluated in {wall:.1f} s", file=sys.stderr, flush=True)
@@ -406,8 +409,10 @@ def main():
                     help="mwus3d: exchange intervals in each CPU-baseline process")
     ap.add_argument("--learned-rounds", type=int, default=200,
                     help="learned: sample-collection rounds (500 steps each)")
-    ap.add_argument("--learned-seed-steps", type=int, default=20000,
+    ap.add_argument("--learned-seed-steps", type=int, default=100000,
                     help="learned: plain steps before collection and before each evaluation")
+    ap.add_argument("--learned-eval-temp", type=float, default=350.0,
+                    help="learned: temperature of every replica in the evaluation")
     ap.add_argument("--learned-train-steps", type=int, default=3000)
     ap.add_argument("--learned-eval-rounds", type=int, default=200,
                     help="learned: evaluation rounds (--interval plain steps each)")
@@ -578,7 +583,8 @@ def main_learned(args):
         "data": "synthetic",
         "config": {
             "workload": "learned_raster64",
-            "system": "64-domain raster, NN, 32 replicas at 300-360 K, no exchange",
+            "system": (f"64-domain raster, NN; samples from 32 replicas at 300-360 K, "
+                       f"evaluation with 32 replicas at {args.learned_eval_temp} K, no exchange"),
             "network": "7x7x7x13 one-hot + 16-dim identity embedding, 3 x 256 ReLU, 36 outputs",
             "fallback_cb": 0.1,
             "round": f"{args.learned_attempts} learned attempts + {args.interval} plain steps "

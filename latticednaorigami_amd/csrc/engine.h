@@ -584,6 +584,13 @@ struct Eng {
     DEVNI bool chain_connected(int cs);
     DEVNI int learned_begin(LearnedMove& m);
     DEVNI void learned_finish(LearnedMove& m);
+    DEVNI int learned_continue(LearnedMove& m);
+    DEVNI int learned_exchange_begin(LearnedMove& m);
+    DEVNI int learned_exchange_continue(LearnedMove& m);
+    DEVNI int learned_path(LearnedMove& m, int i);
+    DEVNI int complement_of(int s);
+    DEVNI int learned_end(LearnedMove& m, bool changed, bool accepted);
+    DEVNI void learned_softmax(const LearnedMove& m, double* w, double& z);
     // channel of the learned proposal's input at site p, leaving domain
     // `excl` out: 0 empty; 1 + o for a scaffold domain, 7 + o for a staple
     // domain only; with two domains of one kind the smaller orientation index

@@ -1195,7 +1195,12 @@ DEVNI int Eng::learned_begin(LearnedMove& m) {
         S->rng_sub = 0;
         return 0;
     }
+    if (uniform_real() < 0.5) {
+        S->attempts[ls - 2]++;
+        return learned_exchange_begin(m);
+    }
     S->attempts[ls]++;
+    m.kind = 1;
     if (S->num_staples == 0) {
         S->rng_sub = 0;
         return 0;
@@ -1226,20 +1231,34 @@ DEVNI int Eng::learned_begin(LearnedMove& m) {
     return 1;
 }
 
-DEVNI void Eng::learned_finish(LearnedMove& m) {
-    int ls = P->learned_slot;
-    double w[36];
+// unnormalised masked softmax of the network's logits and its sum
+DEVNI void Eng::learned_softmax(const LearnedMove& m, double* w, double& z) {
     float mx = -INFINITY;
     for (int k = 0; k < 36; k++) {
         if (m.legal[k / 6]) {
             mx = fmaxf(mx, m.logits[k]);
         }
     }
-    double z = 0;
+    z = 0;
     for (int k = 0; k < 36; k++) {
         w[k] = m.legal[k / 6] ? exp((double)m.logits[k] - (double)mx) : 0.0;
         z += w[k];
     }
+}
+
+DEVNI int Eng::learned_continue(LearnedMove& m) {
+    if (m.kind == 1) {
+        learned_finish(m);
+        return 0;
+    }
+    return learned_exchange_continue(m);
+}
+
+DEVNI void Eng::learned_finish(LearnedMove& m) {
+    int ls = P->learned_slot;
+    double w[36];
+    double z;
+    learned_softmax(m, w, z);
     double u = uniform_real() * z;
     double cum = 0;
     int kn = -1;
@@ -1254,6 +1273,12 @@ DEVNI void Eng::learned_finish(LearnedMove& m) {
         }
     }
     bool accepted = false;
+    if (kn < 0) {
+        // unreachable (d's own site is always legal); kept as a bound on kn
+        S->accepts[ls] += 0;
+        S->rng_sub = 0;
+        return;
+    }
     if (kn == m.k_old) {
         accepted = true;
     }
@@ -1295,6 +1320,209 @@ DEVNI void Eng::learned_finish(LearnedMove& m) {
     }
     S->accepts[ls] += accepted ? 1 : 0;
     S->rng_sub = 0;
+}
+
+// Learned staple exchange (no reference counterpart), for staples of two
+// domains: an insertion draws a staple type uniformly and one of its two
+// domains (the growth domain), binds it to its complementary scaffold domain
+// (which must be unbound), and places the other domain from the network's
+// distribution around the growth domain.  A deletion draws one of the N
+// staples uniformly.  With P_ins the summed probability of the growth paths
+// that produce the staple (a path per domain bound to its complement:
+// (1 / types) (1 / 2) q), the stationary weight of the reference's ensemble
+// (staple_M / 6^3 per staple, exp(-E) with the initiation term) gives the
+// acceptance ratio exp(-dE - dB) (staple_M / 216) (1 / N) / P_ins for an
+// insertion and its inverse for a deletion (N counted with the staple).
+// the scaffold domain a staple domain grows onto: scaffold domain v - 1 for
+// identity v when it carries identity -v, else none
+DEVNI int Eng::complement_of(int s) {
+    int v = ident_of(s);
+    if (v < 1 || v > ns) {
+        return -1;
+    }
+    int x = slot(0, v - 1);
+    return ident_of(x) == -v ? x : -1;
+}
+
+DEVNI int Eng::learned_end(LearnedMove& m, bool changed, bool accepted) {
+    if (changed && !accepted) {
+        reset_origami();
+        calc_move_bias(0);
+    }
+    S->accepts[P->learned_slot - 2] += accepted ? 1 : 0;
+    S->rng_sub = 0;
+    m.mode = 0;
+    return 0;
+}
+
+// sets up the scoring of growth path i of staple m.cs (domain i grows, the
+// other is placed by the network) for a deletion
+DEVNI int Eng::learned_path(LearnedMove& m, int i) {
+    int gi = slot(m.cs, i), dj = slot(m.cs, 1 - i);
+    int idv = ident_of(dj);
+    m.a = gi;
+    m.d = dj;
+    m.id = (idv < 0 ? -idv : idv) - 1;
+    I3 ap = pos(gi);
+    m.ax = ap.x;
+    m.ay = ap.y;
+    m.az = ap.z;
+    m.k_old = 6 * unit_index(sub3(pos(dj), ap)) + dom[dj].ore;
+    m.stage = i;
+    m.mode = 1;
+    return 1;
+}
+
+DEVNI int Eng::learned_exchange_begin(LearnedMove& m) {
+    S->n_modified = 0;
+    S->n_assigned = 0;
+    S->n_added = 0;
+    S->rejected = 0;
+    S->violated = 0;
+    S->modifier = 1;
+    S->delta_e = 0;
+    m.qsum = 0;
+    int types = P->L.ntypes - 1;
+    if (uniform_real() < 0.5) {
+        m.kind = 2;
+        if (S->num_staples == P->max_total_staples) {
+            return learned_end(m, false, false);
+        }
+        int ident = uniform_int(1, types);
+        if (tcount[ident] == P->max_type_staples || P->type_len[ident] != 2) {
+            return learned_end(m, false, false);
+        }
+        int i = uniform_int(0, 1);
+        int cs = add_chain(ident);
+        if (cs < 0) {
+            return learned_end(m, false, false);
+        }
+        S->delta_e += init_e;
+        S->added[S->n_added++] = cs;
+        int gi = slot(cs, i), dj = slot(cs, 1 - i);
+        int x = complement_of(gi);
+        if (x < 0 || dom[x].state != OCC_UNBOUND) {
+            return learned_end(m, true, false);
+        }
+        S->delta_e += set_domain_config(gi, pos(x), dom[x].ore ^ 1);
+        push_assigned(gi);
+        if (S->violated) {
+            return learned_end(m, true, false);
+        }
+        int idv = ident_of(dj);
+        m.cs = cs;
+        m.a = gi;
+        m.d = dj;
+        m.id = (idv < 0 ? -idv : idv) - 1;
+        I3 ap = pos(gi);
+        m.ax = ap.x;
+        m.ay = ap.y;
+        m.az = ap.z;
+        m.k_old = -1;
+        m.stage = 0;
+        m.mode = 1;
+        return 1;
+    }
+    m.kind = 3;
+    if (S->num_staples == 0) {
+        return learned_end(m, false, false);
+    }
+    int w = uniform_int(1, S->num_staples);
+    int cs = order[w];
+    if (ch[cs].len != 2 || staple_is_connector(cs)) {
+        return learned_end(m, false, false);
+    }
+    m.cs = cs;
+    for (int k = 0; k < 2; k++) {
+        int s = slot(cs, k);
+        m.valid[k] = (dom[s].state == OCC_BOUND && dom[s].bound == complement_of(s)) ? 1 : 0;
+    }
+    if (!m.valid[0] && !m.valid[1]) {
+        return learned_end(m, false, false);
+    }
+    return learned_path(m, m.valid[0] ? 0 : 1);
+}
+
+DEVNI int Eng::learned_exchange_continue(LearnedMove& m) {
+    double w[36];
+    double z;
+    learned_softmax(m, w, z);
+    double types = (double)(P->L.ntypes - 1);
+    if (m.kind == 2 && m.k_old < 0) {
+        // place the second domain from the network's distribution
+        double u = uniform_real() * z;
+        double cum = 0;
+        int kn = -1;
+        for (int k = 0; k < 36; k++) {
+            if (w[k] == 0) {
+                continue;
+            }
+            kn = k;
+            cum += w[k];
+            if (u < cum) {
+                break;
+            }
+        }
+        if (kn < 0) {
+            // every site next to the growth domain is taken: no placement
+            // can be proposed, so the insertion is rejected
+            return learned_end(m, true, false);
+        }
+        m.qsum = w[kn] / z;
+        int dj = m.d;
+        S->delta_e += set_domain_config(dj, add3(pos(m.a), unit3(kn / 6)), kn % 6);
+        push_assigned(dj);
+        if (S->violated) {
+            return learned_end(m, true, false);
+        }
+        if (dom[dj].state == OCC_BOUND && dom[dj].bound == complement_of(dj)) {
+            // the staple also grows from the other domain: score that path
+            int gi = m.a;
+            int idv = ident_of(gi);
+            m.a = dj;
+            m.d = gi;
+            m.id = (idv < 0 ? -idv : idv) - 1;
+            I3 ap = pos(dj);
+            m.ax = ap.x;
+            m.ay = ap.y;
+            m.az = ap.z;
+            m.k_old = 6 * unit_index(sub3(pos(gi), ap)) + dom[gi].ore;
+            m.stage = 1;
+            return 1;
+        }
+    }
+    else {
+        m.qsum += w[m.k_old] / z;
+        if (m.kind == 3 && m.stage == 0 && m.valid[1]) {
+            return learned_path(m, 1);
+        }
+    }
+    double p_ins = 0.5 * m.qsum / types;
+    if (m.kind == 2) {
+        S->delta_e += calc_move_bias(0);
+        double ratio = exp(-S->delta_e) * (P->fugacity / 216.0) / (double)S->num_staples / p_ins;
+        return learned_end(m, true, test_acceptance(ratio));
+    }
+    // deletion
+    int cs = m.cs;
+    double n_before = (double)S->num_staples;
+    for (int k = 0; k < 2; k++) {
+        int s = slot(cs, k);
+        prev[s].x = dom[s].x;
+        prev[s].y = dom[s].y;
+        prev[s].z = dom[s].z;
+        prev[s].ore = dom[s].ore;
+        push_modified(s);
+        S->delta_e += unassign_domain(s);
+    }
+    S->delta_e -= init_e;
+    S->delta_e += calc_move_bias(-1);
+    double ratio = exp(-S->delta_e) * p_ins / ((P->fugacity / 216.0) / n_before);
+    bool acc = test_acceptance(ratio);
+    if (acc) {
+        delete_chain(cs);
+    }
+    return learned_end(m, true, acc);
 }
 
 #endif

@@ -185,11 +185,16 @@ struct Params {
 // lane 0 fills d..k_old, the lanes fill ch/legal and the network's logits.
 struct LearnedMove {
     int32_t mode; // 1: the network is needed
-    int32_t d;    // moving staple end domain
+    int32_t d;    // moving staple end domain (the domain the network places)
     int32_t a;    // its chain neighbour (the anchor)
     int32_t id;   // identity index of d (embedding row)
-    int32_t k_old;
+    int32_t k_old; // output d occupies (scored), or -1 (sampled)
     int32_t ax, ay, az;
+    int32_t kind;  // 1 domain regrowth, 2 staple insertion, 3 staple deletion
+    int32_t stage; // growth path being scored (exchange)
+    int32_t cs;    // staple chain slot (exchange)
+    int32_t valid[2];
+    double qsum;   // summed proposal probability of the staple's growth paths
     uint8_t legal[6];
     uint8_t ch[343];
     float h[2][256];

@@ -155,14 +155,18 @@ extern "C" __global__ void __launch_bounds__(64)
         if (g < 0) {
             break;
         }
-        if (g == 1) {
+        // an attempt asks for up to three network evaluations (a domain
+        // regrowth one, an exchange one per growth path and placement)
+        while (g == 1) {
             for (int ph = 0; ph < LDO_LEARNED_PHASES; ph++) {
                 learned_phase_lane(e, sp, m, ph, lane, 64);
                 __syncthreads();
             }
             if (lane == 0) {
-                e.learned_finish(m);
+                go = e.learned_continue(m);
             }
+            __syncthreads();
+            g = go;
         }
         __syncthreads();
     }
@@ -791,8 +795,8 @@ int ldo_learned_set_network(void* ctx, int32_t n_ident, const float* params, dou
         if (!(fallback >= 0.0 && fallback <= 1.0)) {
             throw AbiError {LDO_E_ARG, "fallback probability outside [0, 1]"};
         }
-        if (c->P.n_moves > LDO_MAX_MOVES - 2) {
-            throw AbiError {LDO_E_ARG, "the learned move needs two free counter slots"};
+        if (c->P.n_moves > LDO_MAX_MOVES - 3) {
+            throw AbiError {LDO_E_ARG, "the learned moves need three free counter slots"};
         }
         if (c->P.rng_mode != 0) {
             throw AbiError {LDO_E_ARG, "the learned move draws from the Philox streams"};

@@ -397,6 +397,9 @@ class Replica {
     double learned_fallback {0};
     int64_t learned_step {0};
     void learned_attempt();
+    void learned_exchange();
+    Dom* complement_of(Dom* s) const;
+    void learned_weights(Dom* d, Dom* a, int id, double w[36], double& z, int& k_of_d) const;
     // the network input around the chain neighbour of staple end d (d left
     // out); returns the output d occupies
     int learned_context(Dom* d, Dom* a, uint8_t ch[343], uint8_t legal[6]) const;

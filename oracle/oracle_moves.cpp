@@ -1133,6 +1133,11 @@ void Replica::learned_attempt() {
         }
         return;
     }
+    if (rng.uniform_real() < 0.5) {
+        attempts[ls - 2]++;
+        learned_exchange();
+        return;
+    }
     attempts[ls]++;
     if (sys.num_staples == 0) {
         return;
@@ -1177,6 +1182,9 @@ void Replica::learned_attempt() {
         }
     }
     bool accepted = false;
+    if (kn < 0) {
+        return;
+    }
     if (kn == k_old) {
         accepted = true;
     }
@@ -1216,6 +1224,158 @@ void Replica::learned_attempt() {
         }
     }
     accepts[ls] += accepted;
+}
+
+
+// the scaffold domain a staple domain grows onto in the learned exchange:
+// scaffold domain v - 1 for identity v when it carries identity -v
+Dom* Replica::complement_of(Dom* s) const {
+    int v = s->d_ident;
+    if (v < 1 || v > (int)sys.chains[0].size()) {
+        return nullptr;
+    }
+    Dom* x = sys.chains[0][v - 1];
+    return x->d_ident == -v ? x : nullptr;
+}
+
+// unnormalised masked softmax of the network around anchor a with d left
+// out, its sum, and the output d occupies
+void Replica::learned_weights(Dom* d, Dom* a, int id, double w[36], double& z,
+                              int& k_of_d) const {
+    uint8_t ch[343], legal[6];
+    k_of_d = learned_context(d, a, ch, legal);
+    float logits[36];
+    learned_logits(ch, id, logits);
+    float mx = -INFINITY;
+    for (int k = 0; k < 36; k++) {
+        if (legal[k / 6]) {
+            mx = std::fmax(mx, logits[k]);
+        }
+    }
+    z = 0;
+    for (int k = 0; k < 36; k++) {
+        w[k] = legal[k / 6] ? std::exp((double)logits[k] - (double)mx) : 0.0;
+        z += w[k];
+    }
+}
+
+// learned staple exchange, restated from the sampler's definition (the
+// acceptance ratios are derived there from the Met move's stationary weight)
+void Replica::learned_exchange() {
+    const int xs = LDO_MAX_MOVETYPES - 3;
+    m_modified.clear();
+    m_assigned.clear();
+    m_added.clear();
+    m_prev_pos.clear();
+    m_prev_ore.clear();
+    m_modifier = 1;
+    m_delta_e = 0;
+    m_rejected = false;
+    sys.violated = false;
+    double types = (double)(sys.identities.size() - 1);
+    auto end = [&](bool changed, bool acc) {
+        if (changed && !acc) {
+            reset_origami();
+            calc_move_bias(0);
+        }
+        accepts[xs] += acc;
+    };
+    auto id_of = [](Dom* s) { return std::abs(s->d_ident) - 1; };
+    if (rng.uniform_real() < 0.5) {
+        if (sys.num_staples == m_lim.max_total_staples) {
+            return end(false, false);
+        }
+        int ident = rng.uniform_int(1, (int)types);
+        if ((int)sys.ident_to_index[ident].size() == m_lim.max_type_staples ||
+            sys.identities[ident].size() != 2) {
+            return end(false, false);
+        }
+        int i = rng.uniform_int(0, 1);
+        int c_i = sys.add_chain(ident);
+        m_delta_e += sys.pot.init_e();
+        m_added.push_back(c_i);
+        std::vector<Dom*> chain = sys.chains.back();
+        Dom* g = chain[i];
+        Dom* dj = chain[1 - i];
+        Dom* x = complement_of(g);
+        if (x == nullptr || x->st != Occ::unbound) {
+            return end(true, false);
+        }
+        m_delta_e += sys.set_domain_config(*g, x->pos, -x->ore);
+        m_assigned.push_back({g->c, g->d});
+        if (sys.violated) {
+            return end(true, false);
+        }
+        double w[36], z;
+        int unused;
+        learned_weights(dj, g, id_of(dj), w, z, unused);
+        double u = rng.uniform_real() * z;
+        double cum = 0;
+        int kn = -1;
+        for (int k = 0; k < 36; k++) {
+            if (w[k] == 0) {
+                continue;
+            }
+            kn = k;
+            cum += w[k];
+            if (u < cum) {
+                break;
+            }
+        }
+        if (kn < 0) {
+            return end(true, false);
+        }
+        double q = w[kn] / z;
+        m_delta_e += sys.set_domain_config(*dj, g->pos + kUnit[kn / 6], kUnit[kn % 6]);
+        m_assigned.push_back({dj->c, dj->d});
+        if (sys.violated) {
+            return end(true, false);
+        }
+        if (dj->st == Occ::bound && dj->bd == complement_of(dj)) {
+            int k_g;
+            learned_weights(g, dj, id_of(g), w, z, k_g);
+            q += w[k_g] / z;
+        }
+        m_delta_e += calc_move_bias(0);
+        double ratio = std::exp(-m_delta_e) * (sys.reduced_fugacity / 216.0) /
+                       (double)sys.num_staples / (0.5 * q / types);
+        return end(true, test_acceptance((long double)ratio));
+    }
+    if (sys.num_staples == 0) {
+        return end(false, false);
+    }
+    int w8 = rng.uniform_int(1, sys.num_staples);
+    std::vector<Dom*> staple = sys.chains[w8];
+    if (staple.size() != 2 || staple_is_connector(staple)) {
+        return end(false, false);
+    }
+    bool valid[2];
+    for (int k = 0; k < 2; k++) {
+        valid[k] = staple[k]->st == Occ::bound && staple[k]->bd == complement_of(staple[k]);
+    }
+    if (!valid[0] && !valid[1]) {
+        return end(false, false);
+    }
+    double q = 0;
+    for (int k = 0; k < 2; k++) {
+        if (valid[k]) {
+            double w[36], z;
+            int k_d;
+            learned_weights(staple[1 - k], staple[k], id_of(staple[1 - k]), w, z, k_d);
+            q += w[k_d] / z;
+        }
+    }
+    double p_ins = 0.5 * q / types;
+    double n_before = (double)sys.num_staples;
+    met_unassign_domains(staple);
+    m_delta_e -= sys.pot.init_e();
+    m_delta_e += calc_move_bias(-1);
+    double ratio = std::exp(-m_delta_e) * p_ins / ((sys.reduced_fugacity / 216.0) / n_before);
+    bool acc = test_acceptance((long double)ratio);
+    if (acc) {
+        sys.delete_chain(staple[0]->c);
+    }
+    end(true, acc);
 }
 
 } // namespace orc

@@ -54,13 +54,14 @@ void learned_replica(const Params& P, int r, int64_t attempts) {
         if (e.S->status != ST_OK) {
             break;
         }
-        if (e.learned_begin(m) == 1) {
+        int g = e.learned_begin(m);
+        while (g == 1) {
             for (int ph = 0; ph < LDO_LEARNED_PHASES; ph++) {
                 for (int lane = 0; lane < 64; lane++) {
                     learned_phase_lane(e, P, m, ph, lane, 64);
                 }
             }
-            e.learned_finish(m);
+            g = e.learned_continue(m);
         }
     }
 }
@@ -267,7 +268,7 @@ int eh_sweep(void* ctx, int64_t steps) {
 int eh_learned_set_network(void* ctx, int32_t n_ident, const float* params, double fallback) {
     HostCtx* c = static_cast<HostCtx*>(ctx);
     return guarded(c, [&]() {
-        if (!params || n_ident < c->P.L.ns || c->P.n_moves > LDO_MAX_MOVES - 2 ||
+        if (!params || n_ident < c->P.L.ns || c->P.n_moves > LDO_MAX_MOVES - 3 ||
             !(fallback >= 0.0 && fallback <= 1.0) || c->P.rng_mode != 0) {
             throw ldo::AbiError {LDO_E_ARG, "bad learned-move arguments"};
         }

@@ -227,9 +227,9 @@ def case_mwus3d_window_ladders_by_temperature(sampler_cls, oracle_mod):
     assert_counters(gpu, ref)
 
 
-def random_network(n_ident, seed):
-    """Seeded network parameters in the ldo_mlp.h layout, scaled so that the
-    proposal is far from uniform."""
+def random_network(n_ident, seed, out_scale=3.0):
+    """Seeded network parameters in the ldo_mlp.h layout; the output layer is
+    scaled by `out_scale` (3: a proposal far from uniform)."""
     import numpy as np
     rng = np.random.default_rng(seed)
     H, F, E, O = 256, 4459, 16, 36
@@ -237,13 +237,15 @@ def random_network(n_ident, seed):
              rng.uniform(-1, 1, E * H) * np.sqrt(6 / E), rng.uniform(-0.1, 0.1, H),
              rng.uniform(-1, 1, H * H) * np.sqrt(6 / H), rng.uniform(-0.1, 0.1, H),
              rng.uniform(-1, 1, H * H) * np.sqrt(6 / H), rng.uniform(-0.1, 0.1, H),
-             rng.uniform(-1, 1, H * O) * np.sqrt(6 / H) * 3, rng.uniform(-0.5, 0.5, O)]
+             rng.uniform(-1, 1, H * O) * np.sqrt(6 / H) * out_scale,
+             rng.uniform(-0.5, 0.5, O) * out_scale / 3]
     return np.concatenate(parts).astype(np.float32)
 
 
 def case_learned_move_matches_oracle(sampler_cls, oracle_mod):
-    say("case: learned domain regrowth (random network, CB fallback 0.1) on the 64-domain "
-        "raster at 330 K and 345 K: 3 x (150 learned attempts + 500 steps), samples")
+    say("case: learned moves (domain regrowth and staple exchange, random network, CB "
+        "fallback 0.1) on the 64-domain raster at 330 K and 345 K: 3 x (150 learned attempts "
+        "+ 500 steps), samples")
     import numpy as np
     s = systems.raster_system(replicas=2)
     temps = [330.0, 345.0]
@@ -263,8 +265,10 @@ def case_learned_move_matches_oracle(sampler_cls, oracle_mod):
         ref.sweep(500)
     assert_counters(gpu, ref)
     c = gpu.counters()
-    assert c.attempts[7] == 2 * 450 - c.attempts[6] and c.attempts[6] > 0
-    assert 0 < c.accepts[7] < c.attempts[7]
+    # slots 5, 6, 7: learned staple exchange, CB fallback, learned domain regrowth
+    assert c.attempts[5] + c.attempts[6] + c.attempts[7] == 2 * 450
+    assert min(c.attempts[5], c.attempts[6], c.attempts[7]) > 0
+    assert 0 < c.accepts[7] < c.attempts[7] and 0 < c.accepts[5] < c.attempts[5]
     a, b = gpu.learned_samples(4096), ref.learned_samples(4096)
     assert len(a[0]) > 0
     for x, y in zip(a, b):
