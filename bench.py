@@ -313,8 +313,10 @@ def gpu_leg_learned(args):
             "seconds": wall,
             "ess_per_s": {k: learned.ess(series[k]) / wall for k in keys},
             "series": series,
-            "learned_accepts": c.accepts[7],
-            "learned_attempts": c.attempts[7],
+            "learned_regrowth": [c.accepts[7], c.attempts[7]],
+            "learned_exchange": [c.accepts[5], c.attempts[5]],
+            "cb_fallback": [c.accepts[6], c.attempts[6]],
+            "met_exchange": [c.accepts[1], c.attempts[1]],
             "accepted_moves_per_s": sum(c.accepts) / wall,
         }
         smp.close()

@@ -255,15 +255,23 @@ int ldo_set_tape(void* ctx, int32_t replica, const uint64_t* tape, int64_t n);
  * (GCMCSimulation::simulate, simulation.cpp:568-653). */
 int ldo_sweep(void* ctx, int64_t steps);
 
-/* Learned domain regrowth: no counterpart in the reference (the learned
- * proposal of this sampler).  One attempt picks a staple uniformly and one
- * of its two end domains d (probability 1/2 each); the network of
- * ldo_mlp.h, given the 7x7x7 box around d's chain neighbour with d left out,
- * proposes d's new site (one of the neighbour's 6) and orientation; the move
- * is accepted with min(1, exp(-dE - dB) q(old) / q(new)) if every staple
- * stays connected to the scaffold.  With probability `fallback` the attempt
- * is a plain CB staple regrowth instead.  Counters: attempts/accepts[7] are
- * learned moves, [6] the CB fallbacks (so the move set may use slots 0-5).
+/* Learned moves: no counterpart in the reference (the learned proposal of
+ * this sampler).  With probability `fallback` an attempt is a plain CB
+ * staple regrowth.  Otherwise, with probability 1/2 each:
+ *  - domain regrowth: a staple drawn uniformly and one of its two end
+ *    domains d; the network of ldo_mlp.h, given the 7x7x7 box around d's
+ *    chain neighbour with d left out, proposes d's new site (one of the
+ *    neighbour's 6) and orientation; accepted with
+ *    min(1, exp(-dE - dB) q(old) / q(new)) if every staple stays connected
+ *    to the scaffold;
+ *  - staple exchange (two-domain staples; scaffold domain k must carry
+ *    identity -(k+1)): an insertion (1/2) binds a uniformly drawn domain of
+ *    a uniformly drawn type to its complementary scaffold domain and places
+ *    the other domain from the network; a deletion (1/2) removes a uniformly
+ *    drawn staple; acceptance targets the reference's grand-canonical
+ *    weight, summing the proposal over the staple's growth paths.
+ * Counters: attempts/accepts[7] domain regrowth, [6] CB fallbacks, [5]
+ * staple exchange (so the move set may use slots 0-4).
  * The parameters are copied to the device; n_ident must be at least the
  * number of scaffold domains (identity index = |identity| - 1). */
 int ldo_learned_set_network(void* ctx, int32_t n_ident, const float* params, double fallback);
