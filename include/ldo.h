@@ -302,6 +302,11 @@ int ldo_get_counters(void* ctx, ldo_counters* out);
 /* Ladder slot -> replica permutation (m_q_to_repi, written to .swp). */
 int ldo_get_permutation(void* ctx, int32_t* q_to_rep);
 
+/* Set the permutation, as a restart from a swap file does (the file's last
+ * line, ptmc_simulation.cpp:54-84); every replica takes its new slot's
+ * control variables when the next ldo_run interval begins. */
+int ldo_set_permutation(void* ctx, const int32_t* q_to_rep);
+
 /* The HIP stream the sampler's kernels are launched on (hipStream_t). */
 void* ldo_get_stream(void* ctx);
 

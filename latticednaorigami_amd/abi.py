@@ -275,6 +275,8 @@ def load_library(path=None):
     lib.ldo_set_tape.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64), C.c_int64]
     lib.ldo_get_stream.restype = C.c_void_p
     lib.ldo_get_stream.argtypes = [C.c_void_p]
+    lib.ldo_set_permutation.restype = C.c_int
+    lib.ldo_set_permutation.argtypes = [C.c_void_p, _i32p]
     lib.ldo_kernel_name.restype = C.c_char_p
     lib.ldo_kernel_name.argtypes = [C.c_void_p]
     lib.ldo_set_profiling.restype = C.c_int

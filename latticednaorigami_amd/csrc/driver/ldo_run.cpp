@@ -524,6 +524,74 @@ void write_outputs(Run& run, Outputs& o, int r, long long step, double seconds) 
 
 long long gcd_of(long long a, long long b) { return b == 0 ? a : gcd_of(b, a % b); }
 
+// OrigamiTrajInputFile::read_config (files.cpp:129-218): skip `frame`
+// frames (each ends with an empty line), read the step line, then chains
+// (index identity / positions / orientations) until an empty line
+std::vector<Chain> read_traj_frame(const std::string& path, int frame) {
+    std::ifstream f(path);
+    if (!f.good()) {
+        throw Fail(LDO_E_IO, "Trajectory input file " + path + " does not exist");
+    }
+    std::string line;
+    for (int i = 0; i < frame; i++) {
+        while (true) {
+            if (!std::getline(f, line)) {
+                throw Fail(LDO_E_IO, "Step " + std::to_string(frame) +
+                                         " not found in trajectory input file" + path);
+            }
+            if (line.empty()) {
+                break;
+            }
+        }
+    }
+    std::getline(f, line); // the step number
+    std::vector<Chain> out;
+    auto ints = [](const std::string& s) {
+        std::vector<int> v;
+        std::stringstream ss(s);
+        int x;
+        while (ss >> x) {
+            v.push_back(x);
+        }
+        v.resize(v.size() / 3 * 3);
+        return v;
+    };
+    while (std::getline(f, line) && !line.empty()) {
+        Chain ch {};
+        std::stringstream ss(line);
+        ss >> ch.index >> ch.identity;
+        std::string pl, ol;
+        std::getline(f, pl);
+        std::getline(f, ol);
+        ch.pos = ints(pl);
+        ch.ore = ints(ol);
+        if (ch.pos.size() != ch.ore.size()) {
+            throw Fail(LDO_E_IO, path + ": positions and orientations differ in length");
+        }
+        out.push_back(ch);
+    }
+    return out;
+}
+
+void set_config_from(Run& run, int r, const std::vector<Chain>& chains) {
+    std::vector<int32_t> idx, idn, len, pos, ore;
+    for (auto& ch: chains) {
+        idx.push_back(ch.index);
+        idn.push_back(ch.identity);
+        len.push_back((int32_t)(ch.pos.size() / 3));
+        pos.insert(pos.end(), ch.pos.begin(), ch.pos.end());
+        ore.insert(ore.end(), ch.ore.begin(), ch.ore.end());
+    }
+    ldo_config c {};
+    c.n_chains = (int32_t)chains.size();
+    c.chain_index = idx.data();
+    c.chain_identity = idn.data();
+    c.chain_length = len.data();
+    c.positions = pos.data();
+    c.orientations = ore.data();
+    run.check(ldo_set_config(run.ctx, r, &c));
+}
+
 int main_run(int argc, char** argv) {
     if (argc < 2 || std::string(argv[1]) == "-h" || std::string(argv[1]) == "--help") {
         std::cout << "usage: ldo_run <parameter file> [key=value ...]\n";
@@ -680,6 +748,38 @@ int main_run(int argc, char** argv) {
         throw Fail(rc, ldo_last_error(nullptr));
     }
 
+    // restarts of parallel-tempering runs (ptmc_simulation.cpp:38-84): each
+    // replica's configuration from frame restart_step of its own trajectory,
+    // and the slot permutation from the last line of the swap file
+    if (pt && p.flag("restart_from_config", false)) {
+        std::string fb = p.str("restart_traj_filebase", "");
+        std::string post = p.str("restart_traj_postfix", ".trj");
+        for (int r = 0; r < run.R; r++) {
+            set_config_from(run, r, read_traj_frame(fb + "-" + std::to_string(r) + post,
+                                                    (int)p.integer("restart_step", 0)));
+        }
+    }
+    if (pt && p.flag("restart_from_swap", false)) {
+        std::string path = p.str("restart_swap_file", "");
+        std::ifstream f(path);
+        if (!f.good()) {
+            throw Fail(LDO_E_IO, "Restart swap file " + path + " does not exist");
+        }
+        std::string line, last;
+        while (std::getline(f, line)) {
+            if (!line.empty()) {
+                last = line;
+            }
+        }
+        std::stringstream ss(last);
+        std::vector<int32_t> q_to_rep(run.R);
+        for (int q = 0; q < run.R; q++) {
+            if (!(ss >> q_to_rep[q])) {
+                throw Fail(LDO_E_IO, path + ": last line is not a permutation");
+            }
+        }
+        run.check(ldo_set_permutation(run.ctx, q_to_rep.data()));
+    }
     std::string base = p.str("output_filebase", "");
     std::vector<std::unique_ptr<Outputs>> outs;
     for (int r = 0; r < run.R; r++) {

@@ -972,6 +972,25 @@ int ldo_get_permutation(void* ctx, int32_t* q_to_rep) {
     });
 }
 
+int ldo_set_permutation(void* ctx, const int32_t* q_to_rep) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        if (!q_to_rep) {
+            throw AbiError {LDO_E_ARG, "permutation required"};
+        }
+        std::vector<int32_t> both(2 * c->R, -1);
+        for (int q = 0; q < c->R; q++) {
+            int r = q_to_rep[q];
+            if (r < 0 || r >= c->R || both[c->R + r] >= 0) {
+                throw AbiError {LDO_E_ARG, "not a permutation of the replicas"};
+            }
+            both[q] = r;
+            both[c->R + r] = q;
+        }
+        upload(*c, c->d_xint, both.data(), both.size() * sizeof(int32_t), "copy permutation");
+    });
+}
+
 void* ldo_get_stream(void* ctx) { return ctx ? (void*)static_cast<Ctx*>(ctx)->stream : nullptr; }
 
 const char* ldo_kernel_name(void* ctx) {

@@ -62,6 +62,10 @@ class Sampler:
         g = (C.c_double * max(1, len(grid)))(*grid)
         self._check(self.lib.ldo_set_windows(self.h, n, arr, g, grid_len))
 
+    def set_permutation(self, q_to_rep):
+        p = (C.c_int32 * self.n)(*q_to_rep)
+        self._check(self.lib.ldo_set_permutation(self.h, p))
+
     def learned_set_network(self, n_ident, params, fallback=0.1):
         """Enable learned domain regrowth with these network parameters
         (ldo_mlp.h layout); `fallback` is the plain CB probability."""

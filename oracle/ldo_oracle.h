@@ -39,6 +39,7 @@ int oracle_set_ladder(void* o, int32_t n, const double* temps,
 int oracle_set_windows(void* o, int32_t n, const ldo_window* windows, const double* grid,
                        int32_t grid_len);
 int oracle_sweep(void* o, int64_t steps);
+int oracle_set_permutation(void* o, const int32_t* q_to_rep);
 /* learned domain regrowth (the sampler's ldo_learned_* restated) */
 int oracle_learned_set_network(void* o, int32_t n_ident, const float* params, double fallback);
 int oracle_learned_sweep(void* o, int64_t attempts);

@@ -252,6 +252,20 @@ int oracle_learned_samples(void* o, int32_t cap, uint8_t* channels, int32_t* ide
     })
 }
 
+int oracle_set_permutation(void* o, const int32_t* q_to_rep) {
+    ORACLE_TRY(o, {
+        int R = (int)O->reps.size();
+        std::vector<int> seen(R, 0);
+        for (int q = 0; q < R; q++) {
+            int r = q_to_rep[q];
+            if (r < 0 || r >= R || seen[r]++) {
+                throw OrigamiError(LDO_E_ARG, "not a permutation of the replicas");
+            }
+        }
+        O->q_to_rep.assign(q_to_rep, q_to_rep + R);
+    })
+}
+
 int oracle_sweep(void* o, int64_t steps) {
     ORACLE_TRY(o, {
         for (auto& r: O->reps) {

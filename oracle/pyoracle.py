@@ -101,6 +101,11 @@ class Oracle:
     def sweep(self, steps):
         _check(self.L, self.h, self.L.oracle_sweep(self.h, steps))
 
+    def set_permutation(self, q_to_rep):
+        p = (C.c_int32 * self.n)(*q_to_rep)
+        self.L.oracle_set_permutation.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+        _check(self.L, self.h, self.L.oracle_set_permutation(self.h, p))
+
     def learned_set_network(self, n_ident, params, fallback=0.1):
         import numpy as np
         self._net = np.ascontiguousarray(params, dtype=np.float32)

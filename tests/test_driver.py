@@ -169,3 +169,38 @@ def test_driver_ptmc_matches_oracle(tmp_path, oracle_mod):
             e, o, _, _ = expect_lines(ref, r, s, 1000 * k)
             assert read(d, f"out-{r}.ene")[k] == e
             assert read(d, f"out-{r}.ops")[k] == o
+
+
+@pytest.mark.gpu
+def test_driver_ptmc_restart_matches_oracle(tmp_path, oracle_mod):
+    """A PTMC run restarted from frame 2 of each replica's trajectory and the
+    last line of the swap file (restart_from_config / restart_from_swap)
+    continues as the oracle does from the same configurations and slots."""
+    s = systems.raster_system(replicas=4)
+    d = str(tmp_path)
+    temps = s.temps
+    common = {"simulation_type": "ut_parallel_tempering", "num_reps": 4,
+              "temps": " ".join(repr(t) for t in temps), "chem_pot_mults": "1 1 1 1",
+              "bias_mults": "1 1 1 1", "stacking_mults": "1 1 1 1", "exchange_interval": 500,
+              "configs_output_freq": 1000, "energies_output_freq": 1000}
+    run_driver(write_inputs(d, s, {**common, "swaps": 6}))
+    first = os.path.join(d, "first")
+    os.mkdir(first)
+    for name in os.listdir(d):
+        if name.startswith("out"):
+            os.rename(os.path.join(d, name), os.path.join(first, name))
+    run_driver(write_inputs(d, s, {**common, "swaps": 4, "restart_from_config": "true",
+                                   "restart_traj_filebase": os.path.join(first, "out"),
+                                   "restart_step": 2, "restart_from_swap": "true",
+                                   "restart_swap_file": os.path.join(first, "out.swp")}))
+    ref = oracle_mod.Oracle(s, temps=temps, exchange_mode=abi.EXCHANGE_TEMPERATURE)
+    ref0 = oracle_mod.Oracle(s, temps=temps, exchange_mode=abi.EXCHANGE_TEMPERATURE)
+    ref0.run(6, 500)  # frame 2 is step 3000, the run's end; the swap file's last line too
+    for r in range(4):
+        ref.set_config(r, ref0.config(r))
+    ref.set_permutation(ref0.permutation())
+    for k in range(1, 3):
+        ref.run(2, 500)
+        for r in range(4):
+            e, _, _, _ = expect_lines(ref, r, s, 1000 * k)
+            assert read(d, f"out-{r}.ene")[k] == e, (k, r)
