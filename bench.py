@@ -388,7 +388,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10, help="timed exchange intervals")
     ap.add_argument("--warmup", type=int, default=2, help="untimed exchange intervals")
     ap.add_argument("--interval", type=int, default=1000, help="MC steps per replica per step")
-    ap.add_argument("--cpu-steps", type=int, default=500,
+    ap.add_argument("--cpu-steps", type=int, default=4000,
                     help="MC steps per replica in each CPU-baseline process")
     ap.add_argument("--cpu-workers", type=int, default=0,
                     help="CPU-baseline processes (0: min(8, host cores))")
@@ -405,9 +405,9 @@ def main():
                     help="batch4096: force a sweep kernel instead of the autotuner's choice")
     ap.add_argument("--cpu-chains", type=int, default=64,
                     help="batch4096: chains per CPU-baseline process")
-    ap.add_argument("--cpu-batch-steps", type=int, default=20000,
+    ap.add_argument("--cpu-batch-steps", type=int, default=60000,
                     help="batch4096: MC steps per chain in the CPU baseline")
-    ap.add_argument("--cpu-swaps", type=int, default=400,
+    ap.add_argument("--cpu-swaps", type=int, default=1000,
                     help="mwus3d: exchange intervals in each CPU-baseline process")
     ap.add_argument("--learned-rounds", type=int, default=200,
                     help="learned: sample-collection rounds (500 steps each)")
