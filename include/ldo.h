@@ -246,6 +246,27 @@ int ldo_set_ladder(void* ctx, int32_t n, const double* temps,
 int ldo_set_windows(void* ctx, int32_t n, const ldo_window* windows, const double* grid,
                     int32_t grid_len);
 
+/* New grid biases for every window (grid[w * grid_len + p], the grid_len of
+ * ldo_set_windows), leaving the windows' stored bias values as they are:
+ * GridBiasFunction::replace_biases (bias_functions.cpp:236-239) as the
+ * adaptive umbrella-sampling loop calls it between iterations
+ * (us_simulation.cpp:385-416).  The next move's bias difference is taken
+ * against the value stored under the old grid, as in the reference. */
+int ldo_set_window_grid(void* ctx, const double* grid);
+
+/* Counting of each window's order parameter after every step of ldo_sweep
+ * and ldo_run (USGCMCSimulation::update_internal, us_simulation.cpp:262-266:
+ * the f_i histogram of the adaptive loop).  on != 0 starts counting from
+ * zero; on == 0 stops and frees the counts.  Requires windows; a later
+ * ldo_set_windows zeroes the counts.  Steps that end in a constraint failure
+ * are not counted. */
+int ldo_set_histogram(void* ctx, int32_t on);
+
+/* counts[w * (grid_len + 1) + p]: steps window w ended with its order
+ * parameter at p (0 <= p < grid_len); bin grid_len counts values off the
+ * grid.  reset != 0 zeroes the counts after the copy. */
+int ldo_get_histogram(void* ctx, int64_t* counts, int32_t reset);
+
 /* Supply a recorded draw sequence for one replica (LDO_RNG_TAPE).  Entry k
  * is the k-th value the reference's RandomGens returned: a uniform_real as
  * its IEEE-754 bit pattern, a uniform_int as the integer. */

@@ -173,6 +173,9 @@ def declare_abi(lib, prefix):
         "get_config": (C.c_int, [v, C.c_int32, C.POINTER(Config), C.c_int32, C.c_int32]),
         "set_config": (C.c_int, [v, C.c_int32, C.POINTER(Config)]),
         "set_windows": (C.c_int, [v, C.c_int32, C.POINTER(Window), _f64p, C.c_int32]),
+        "set_window_grid": (C.c_int, [v, _f64p]),
+        "set_histogram": (C.c_int, [v, C.c_int32]),
+        "get_histogram": (C.c_int, [v, C.POINTER(C.c_int64), C.c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, prefix + name)

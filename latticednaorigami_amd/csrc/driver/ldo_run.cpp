@@ -6,8 +6,11 @@
 // output files (files.cpp:493-791: .vsf .trj .vcf .states .ores .counts
 // .staples .staplestates .times .ene .ops, and .swp for parallel tempering,
 // ptmc_simulation.cpp:91-104, 315-322) while every replica runs on the GPU
-// through ldo.h.  Supported: simulation_type constant_temp and
-// ut_parallel_tempering with the movetypes OrientationRotation,
+// through ldo.h.  Supported: simulation_type constant_temp,
+// ut_parallel_tempering and mw_umbrella_sampling (with the per-window,
+// per-iteration files of us_simulation.cpp:103-248: _iter-<n>.* outputs,
+// -inp.biases / .biases weights and the .out iteration summary) with the
+// movetypes OrientationRotation,
 // MetStapleExchange, CBStapleRegrowth and CTCBScaffoldRegrowth.
 //
 //   ldo_run <parameter file> [key=value ...]
@@ -18,6 +21,7 @@
 // simulate writes them (simulation.cpp:620-650); with parallel tempering the
 // frequencies must be multiples of exchange_interval.  ldo_flags sets
 // ldo_run_params.flags (no reference key).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -33,6 +37,7 @@
 #include <vector>
 
 #include "../../../include/ldo.h"
+#include "us_loop.hpp"
 
 namespace {
 
@@ -592,6 +597,279 @@ void set_config_from(Run& run, int r, const std::vector<Chain>& chains) {
     run.check(ldo_set_config(run.ctx, r, &c));
 }
 
+// ------------------------------------------------ multi-window umbrella sampling
+// MWUSGCMCSimulation (us_simulation.cpp:466-600) with every window a replica
+// of one context: the windows file, the window restraint and the grid bias of
+// the bias-function file (bias_functions.cpp:300-420), the order-parameter
+// file's tags (order_params.cpp), then the equilibration run and the
+// adaptive iterations of USGCMCSimulation::run (93-170).
+struct MwusSetup {
+    int op = LDO_OP_NUMFULLDOMAINS;
+    int grid_len = 0;
+    std::vector<ldo_window> windows;
+    std::vector<std::string> postfix; // MWUSGCMCSimulation::setup_window_variables
+};
+
+MwusSetup parse_mwus(const Run& run) {
+    const Params& p = run.p;
+    MwusSetup m;
+    // order-parameter tags and types
+    std::map<std::string, std::string> op_type;
+    const Json ops = read_json(p.str("order_parameter_file", "")).at("origami").at("order_params");
+    for (auto& o: ops.arr) {
+        op_type[o.at("tag").str] = o.at("type").str;
+    }
+    // windows file: bias tags, then "min, max" per window (parse_windows_file, 552-600)
+    std::string wpath = p.str("windows_file", "");
+    std::ifstream wf(wpath);
+    if (!wf.good()) {
+        throw Fail(LDO_E_IO, "Windows file " + wpath + " does not exist");
+    }
+    std::string line;
+    std::getline(wf, line);
+    std::vector<std::string> wtags;
+    {
+        std::stringstream ss(line);
+        std::string t;
+        while (ss >> t) {
+            wtags.push_back(t);
+        }
+    }
+    if (wtags.size() != 1) {
+        throw Fail(LDO_E_NOTIMPL, "windows over one bias function only (the file names " +
+                                      std::to_string(wtags.size()) + ")");
+    }
+    std::vector<std::pair<int, int>> ranges;
+    while (std::getline(wf, line)) {
+        if (trim(line).empty()) {
+            continue;
+        }
+        size_t comma = line.find(',');
+        if (comma == std::string::npos) {
+            throw Fail(LDO_E_IO, wpath + ": window line without a comma: " + line);
+        }
+        std::stringstream a(line.substr(0, comma)), b(line.substr(comma + 1));
+        int lo, hi;
+        if (!(a >> lo) || !(b >> hi)) {
+            throw Fail(LDO_E_IO, wpath + ": bad window line: " + line);
+        }
+        int extra;
+        if (a >> extra || b >> extra) {
+            throw Fail(LDO_E_NOTIMPL, "windows over one order parameter only");
+        }
+        ranges.push_back({lo, hi});
+    }
+    if (ranges.empty()) {
+        throw Fail(LDO_E_IO, wpath + ": no windows");
+    }
+    // bias functions: the windows' restraint and the grid bias, on one order parameter
+    std::string grid_tag = p.str("us_grid_bias_tag", "");
+    const Json bfs =
+        read_json(p.str("bias_functions_file", "")).at("origami").at("bias_functions");
+    const Json* well = nullptr;
+    const Json* grid = nullptr;
+    for (auto& b: bfs.arr) {
+        const std::string& tag = b.at("tag").str;
+        if (tag == wtags[0]) {
+            well = &b;
+        }
+        else if (tag == grid_tag) {
+            grid = &b;
+        }
+        else {
+            throw Fail(LDO_E_NOTIMPL, "bias function " + tag +
+                                          ": only the window restraint and the grid bias");
+        }
+    }
+    if (!well || !grid) {
+        throw Fail(LDO_E_IO, "bias-function file lacks " + (well ? grid_tag : wtags[0]));
+    }
+    if (grid->at("type").str != "Grid" || grid->at("ops").arr.size() != 1) {
+        throw Fail(LDO_E_NOTIMPL, "us_grid_bias_tag must name a Grid bias over one order parameter");
+    }
+    const std::string& op_tag = grid->at("ops").arr[0].str;
+    if (well->at("ops").arr.size() != 1 || well->at("ops").arr[0].str != op_tag) {
+        throw Fail(LDO_E_NOTIMPL, "the window restraint and the grid bias must share their order parameter");
+    }
+    auto t = op_type.find(op_tag);
+    if (t == op_type.end()) {
+        throw Fail(LDO_E_IO, "order parameter " + op_tag + " not in the order-parameter file");
+    }
+    if (t->second == "NumBoundDomainPairs") {
+        m.op = LDO_OP_NUMFULLDOMAINS;
+    }
+    else if (t->second == "NumStaples") {
+        m.op = LDO_OP_NUMSTAPLES;
+    }
+    else {
+        throw Fail(LDO_E_NOTIMPL, "umbrella sampling over " + t->second + " is not supported");
+    }
+    // every value either parameter can take lies on the grid
+    m.grid_len = run.n_scaffold + run.max_total_staples * run.max_staple_size + 1;
+    const std::string& wtype = well->at("type").str;
+    double mult = p.num("bias_functions_mult", 1);
+    for (auto& r: ranges) {
+        ldo_window w {};
+        w.op = m.op;
+        w.min_op = r.first;
+        w.max_op = r.second;
+        if (wtype == "LinearStepWell") {
+            w.well_bias = well->at("well_bias").num;
+            w.min_bias = well->at("min_bias").num;
+            w.slope = well->at("slope").num;
+        }
+        else if (wtype == "SquareWell") {
+            // outside_bias everywhere outside: a LinearStepWell of slope 0
+            w.well_bias = well->at("well_bias").num;
+            w.min_bias = well->at("outside_bias").num;
+            w.slope = 0;
+        }
+        else {
+            throw Fail(LDO_E_NOTIMPL, "window restraint of type " + wtype);
+        }
+        w.off_grid_bias = 0;
+        w.bias_mult = mult;
+        m.windows.push_back(w);
+        m.postfix.push_back("_win-" + std::to_string(r.first) + "--" + std::to_string(r.second));
+    }
+    for (const char* k: {"restart_us_iter"}) {
+        if (p.flag(k, false)) {
+            throw Fail(LDO_E_NOTIMPL, std::string(k) + " (the iteration archives) is not supported");
+        }
+    }
+    return m;
+}
+
+// USGCMCSimulation::read_weights (us_simulation.cpp:190-204)
+void read_weights(const std::string& path, ldo_us::WindowState& u) {
+    std::ifstream f(path);
+    if (!f.good()) {
+        throw Fail(LDO_E_IO, "Restart bias file " + path + " does not exist");
+    }
+    const Json j = read_json(path);
+    for (auto& e: j.at("biases").arr) {
+        if (e.at("point").arr.size() != 1) {
+            throw Fail(LDO_E_IO, path + ": grid points of one order parameter expected");
+        }
+        u.E_w[(int)e.at("point").arr[0].num] = e.at("bias").num;
+    }
+}
+
+int run_mwus(Run& run, const ldo_system_desc& sd, ldo_run_params rp, const ldo_config& init) {
+    const Params& p = run.p;
+    MwusSetup m = parse_mwus(run);
+    int W = (int)m.windows.size();
+    std::string base = p.str("output_filebase", "");
+    std::vector<ldo_us::WindowState> us(W);
+    std::vector<std::unique_ptr<std::ofstream>> summary;
+    for (int w = 0; w < W; w++) {
+        if (p.flag("read_biases", false)) {
+            read_weights(p.str("biases_filebase", "") + m.postfix[w] + ".biases", us[w]);
+        }
+        summary.push_back(std::make_unique<std::ofstream>(base + m.postfix[w] + ".out"));
+        if (!summary.back()->good()) {
+            throw Fail(LDO_E_IO, base + m.postfix[w] + ".out: output path does not exist");
+        }
+    }
+    run.R = W;
+    run.temps.assign(W, p.num("temp", 300));
+    rp.rng_mode = LDO_RNG_PHILOX;
+    int rc = ldo_create(&sd, &rp, &init, run.R, run.temps.data(), &run.ctx);
+    if (rc != LDO_OK) {
+        throw Fail(rc, ldo_last_error(nullptr));
+    }
+    if (p.flag("restart_from_config", false)) {
+        std::string fb = p.str("restart_traj_filebase", "");
+        std::string post = p.str("restart_traj_postfix", ".trj");
+        for (int w = 0; w < W; w++) {
+            set_config_from(run, w, read_traj_frame(fb + m.postfix[w] + post,
+                                                    (int)p.integer("restart_step", 0)));
+        }
+    }
+    std::vector<double> grid((size_t)W * m.grid_len);
+    auto upload_grid = [&]() {
+        for (int w = 0; w < W; w++) {
+            std::vector<double> g = ldo_us::dense_grid(us[w], m.grid_len);
+            std::copy(g.begin(), g.end(), grid.begin() + (size_t)w * m.grid_len);
+        }
+    };
+    upload_grid();
+    run.check(ldo_set_windows(run.ctx, W, m.windows.data(), grid.data(), m.grid_len));
+    run.check(ldo_set_histogram(run.ctx, 1));
+
+    // one simulate() call of every window: outputs under <window>_iter-<tag>,
+    // steps counted from 1; a duration limit is checked between launches
+    const long long launch_cap = 20000;
+    auto phase = [&](const std::string& tag, long long total, double max_dur) {
+        std::vector<std::unique_ptr<Outputs>> outs;
+        for (int w = 0; w < W; w++) {
+            outs.push_back(setup_outputs(run, base + m.postfix[w] + "_iter-" + tag));
+        }
+        const Outputs& o0 = *outs[0];
+        long long chunk = 0;
+        for (long long f: {o0.f_cfg, o0.f_vtf, o0.f_counts, o0.f_times, o0.f_ene, o0.f_ops}) {
+            if (f > 0) {
+                chunk = gcd_of(f, chunk);
+            }
+        }
+        if (chunk == 0) {
+            chunk = launch_cap;
+        }
+        auto start = std::chrono::steady_clock::now();
+        auto elapsed = [&]() {
+            return std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        };
+        long long step = 0;
+        while (step < total) {
+            long long n = std::min({chunk - step % chunk, total - step, launch_cap});
+            run.check(ldo_sweep(run.ctx, n));
+            step += n;
+            double t = elapsed();
+            for (int w = 0; w < W; w++) {
+                write_outputs(run, *outs[w], w, step, t);
+            }
+            if (t > max_dur) {
+                break;
+            }
+        }
+        return step;
+    };
+    // run_equilibration (103-124): its visits are not used
+    phase("equil", p.integer("equil_steps", 0), p.num("max_equil_dur", 0));
+    std::vector<int64_t> hist((size_t)W * (m.grid_len + 1));
+    run.check(ldo_get_histogram(run.ctx, hist.data(), 1));
+    long long iters = p.integer("max_num_iters", 0);
+    double max_D_bias = p.num("max_D_bias", 0);
+    for (long long n = 0; n < iters; n++) {
+        std::string tag = std::to_string(n);
+        for (int w = 0; w < W; w++) {
+            std::ofstream bf(base + m.postfix[w] + "_iter-" + tag + "-inp.biases");
+            ldo_us::write_weights(bf, us[w]);
+        }
+        long long steps = phase(tag, p.integer("iter_steps", 0), p.num("max_iter_dur", 0));
+        run.check(ldo_get_histogram(run.ctx, hist.data(), 1));
+        for (int w = 0; w < W; w++) {
+            const int64_t* f = hist.data() + (size_t)w * (m.grid_len + 1);
+            std::vector<double> g(grid.begin() + (size_t)w * m.grid_len,
+                                  grid.begin() + (size_t)(w + 1) * m.grid_len);
+            try {
+                // simulate() returns one past its last step
+                ldo_us::process_iteration(us[w], f, m.grid_len, g, steps + 1, max_D_bias);
+            } catch (std::logic_error& e) {
+                throw Fail(LDO_E_MISUSE, e.what());
+            }
+            ldo_us::write_summary(*summary[w], us[w], (int)n);
+            summary[w]->flush();
+            std::ofstream bf(base + m.postfix[w] + "_iter-" + tag + ".biases");
+            ldo_us::write_weights(bf, us[w]);
+        }
+        upload_grid();
+        run.check(ldo_set_window_grid(run.ctx, grid.data()));
+    }
+    ldo_destroy(run.ctx);
+    return 0;
+}
+
 int main_run(int argc, char** argv) {
     if (argc < 2 || std::string(argv[1]) == "-h" || std::string(argv[1]) == "--help") {
         std::cout << "usage: ldo_run <parameter file> [key=value ...]\n";
@@ -720,6 +998,9 @@ int main_run(int argc, char** argv) {
     init.orientations = c_ore.data();
 
     std::string sim = p.str("simulation_type", "constant_temp");
+    if (sim == "mw_umbrella_sampling") {
+        return run_mwus(run, sd, rp, init);
+    }
     bool pt = sim == "ut_parallel_tempering";
     if (!pt && sim != "constant_temp") {
         throw Fail(LDO_E_NOTIMPL, "simulation_type " + sim + " is not supported");

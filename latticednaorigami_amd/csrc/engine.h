@@ -640,6 +640,7 @@ struct Eng {
     DEVNI void simulate(int64_t steps);
     DEVNI void begin_interval(int slot, bool recompute);
     DEVNI int window_op(int staple_offset);
+    DEVNI void count_window_op();
     DEVNI double calc_move_bias(int staple_offset);
 
     // move-local scratch for the CTCB segment (kept in registers)

@@ -1093,7 +1093,19 @@ DEVNI void Eng::simulate(int64_t steps) {
         if (P->check_freq != 0 && S->step % P->check_freq == 0) {
             check_all_constraints();
         }
+        if (P->hist != nullptr && S->status == ST_OK) {
+            count_window_op();
+        }
     }
+}
+
+// USGCMCSimulation::update_internal (us_simulation.cpp:262-266): one visit
+// of the window's order parameter after each step.  A slot belongs to one
+// replica for a whole sweep, so its bins have a single writer.
+DEVNI void Eng::count_window_op() {
+    int p = window_op(0);
+    int b = (p >= 0 && p < P->grid_len) ? p : P->grid_len;
+    P->hist[(size_t)S->temp_slot * (P->grid_len + 1) + b] += 1;
 }
 
 // TPT/UTPTGCMCSimulation::update_control_qs (ptmc_simulation.cpp:651-663):

@@ -173,6 +173,10 @@ struct Params {
     const ldo_window* windows; // [n_slots] umbrella windows, or null
     const double* win_grid;    // [n_slots * grid_len]
     double* win_stored;        // [n_slots * 2] stored restraint and grid bias
+    // visits of each window's order parameter after every step, or null:
+    // [n_slots * (grid_len + 1)], the last bin of a window counting values
+    // off its grid
+    int64_t* hist;
     // learned proposal (ldo_mlp.h layout), or null
     const float* net;
     int64_t net_off[10]; // w1, emb, w1e, b1, w2, b2, w3, b3, w4, b4

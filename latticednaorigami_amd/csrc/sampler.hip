@@ -360,6 +360,7 @@ struct Ctx {
     ldo_window* d_windows = nullptr;
     double* d_grid = nullptr;
     double* d_win_stored = nullptr;
+    int64_t* d_hist = nullptr;        // window order-parameter visits
     std::vector<ldo_window> windows; // host copy
     bool lds_sweep = false;           // sweep with the replica blocks in LDS
     float* d_net = nullptr;           // learned proposal parameters
@@ -598,7 +599,7 @@ void free_ctx(Ctx* c) {
     }
     void* bufs[] = {c->d_blocks, c->d_type_len, c->d_type_ident, c->d_tab, c->d_init,
                     c->d_tapes, c->d_tape_meta, c->d_xint, c->d_xdbl, c->d_xcnt, c->d_cfg,
-                    c->d_windows, c->d_grid, c->d_win_stored, c->d_net};
+                    c->d_windows, c->d_grid, c->d_win_stored, c->d_net, c->d_hist};
     for (void* b: bufs) {
         if (b) {
             (void)hipFree(b);
@@ -608,6 +609,21 @@ void free_ctx(Ctx* c) {
         (void)hipStreamDestroy(c->stream);
     }
     delete c;
+}
+
+size_t hist_bins(const Ctx& c) {
+    return (size_t)c.R * (c.P.grid_len + 1);
+}
+
+// (Re)allocates the window histogram for the current grid, zeroed.
+void reset_histogram(Ctx& c, bool realloc) {
+    if (realloc) {
+        release(c, c.d_hist, "hipFree histogram");
+        hip_check(hipMalloc(&c.d_hist, hist_bins(c) * sizeof(int64_t)), "hipMalloc histogram");
+    }
+    std::vector<int64_t> zero(hist_bins(c), 0);
+    upload(c, c.d_hist, zero.data(), zero.size() * sizeof(int64_t), "clear histogram");
+    c.P.hist = c.d_hist;
 }
 
 } // namespace
@@ -765,6 +781,53 @@ int ldo_set_windows(void* ctx, int32_t n, const ldo_window* windows, const doubl
         c->P.win_grid = c->d_grid;
         c->P.grid_len = grid_len;
         c->P.win_stored = c->d_win_stored;
+        if (c->d_hist) {
+            reset_histogram(*c, true);
+        }
+    });
+}
+
+int ldo_set_window_grid(void* ctx, const double* grid) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        if (!c->P.windows) {
+            throw AbiError {LDO_E_ARG, "no umbrella windows set"};
+        }
+        if (!grid && c->P.grid_len > 0) {
+            throw AbiError {LDO_E_ARG, "grid values required"};
+        }
+        upload(*c, c->d_grid, grid, (size_t)c->R * c->P.grid_len * sizeof(double), "copy grid");
+    });
+}
+
+int ldo_set_histogram(void* ctx, int32_t on) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        if (!on) {
+            c->P.hist = nullptr;
+            release(*c, c->d_hist, "hipFree histogram");
+            return;
+        }
+        if (!c->P.windows) {
+            throw AbiError {LDO_E_ARG, "the histogram counts a window's order parameter: set windows first"};
+        }
+        reset_histogram(*c, true);
+    });
+}
+
+int ldo_get_histogram(void* ctx, int64_t* counts, int32_t reset) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        if (!c->d_hist) {
+            throw AbiError {LDO_E_ARG, "histogram not enabled"};
+        }
+        if (!counts) {
+            throw AbiError {LDO_E_ARG, "output required"};
+        }
+        download(*c, counts, c->d_hist, hist_bins(*c) * sizeof(int64_t), "copy histogram");
+        if (reset) {
+            reset_histogram(*c, false);
+        }
     });
 }
 

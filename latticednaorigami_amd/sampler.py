@@ -61,6 +61,30 @@ class Sampler:
         arr = (abi.Window * n)(*windows)
         g = (C.c_double * max(1, len(grid)))(*grid)
         self._check(self.lib.ldo_set_windows(self.h, n, arr, g, grid_len))
+        self._grid_len = grid_len
+
+    def _fn(self, name):
+        return getattr(self.lib, "ldo_" + name)
+
+    def _check_call(self, rc):
+        self._check(rc)
+
+    def set_window_grid(self, grid):
+        """New grid biases for every window; the stored bias values stay."""
+        g = (C.c_double * max(1, len(grid)))(*grid)
+        self._check_call(self._fn("set_window_grid")(self.h, g))
+
+    def set_histogram(self, on=True):
+        self._check_call(self._fn("set_histogram")(self.h, 1 if on else 0))
+
+    def histogram(self, reset=False):
+        """Visits [n_windows, grid_len + 1] of each window's order parameter
+        (last column: off the grid) since counting started or was reset."""
+        import numpy as np
+        out = np.zeros((self.n, self._grid_len + 1), dtype=np.int64)
+        self._check_call(self._fn("get_histogram")(
+            self.h, out.ctypes.data_as(C.POINTER(C.c_int64)), 1 if reset else 0))
+        return out
 
     def set_permutation(self, q_to_rep):
         p = (C.c_int32 * self.n)(*q_to_rep)

@@ -40,6 +40,10 @@ int oracle_set_windows(void* o, int32_t n, const ldo_window* windows, const doub
                        int32_t grid_len);
 int oracle_sweep(void* o, int64_t steps);
 int oracle_set_permutation(void* o, const int32_t* q_to_rep);
+/* ldo_set_window_grid, ldo_set_histogram, ldo_get_histogram */
+int oracle_set_window_grid(void* o, const double* grid);
+int oracle_set_histogram(void* o, int32_t on);
+int oracle_get_histogram(void* o, int64_t* counts, int32_t reset);
 /* learned domain regrowth (the sampler's ldo_learned_* restated) */
 int oracle_learned_set_network(void* o, int32_t n_ident, const float* params, double fallback);
 int oracle_learned_sweep(void* o, int64_t attempts);

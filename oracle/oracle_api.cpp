@@ -33,6 +33,10 @@ struct Oracle {
     // umbrella windows, one per ladder slot, and their stored bias values
     std::vector<WindowBias> windows;
     std::vector<WindowState> win_state;
+    int grid_len {0};
+    // per-window visit counts [n_windows * (grid_len + 1)], when counted
+    bool hist_on {false};
+    std::vector<int64_t> hist;
     std::string err;
 };
 
@@ -45,6 +49,7 @@ static void assign_windows(Oracle* O) {
         Replica& rep = *O->reps[O->q_to_rep[q]];
         rep.win = &O->windows[q];
         rep.win_state = &O->win_state[q];
+        rep.hist = O->hist_on ? O->hist.data() + q * (O->grid_len + 1) : nullptr;
     }
 }
 
@@ -356,6 +361,10 @@ int oracle_set_windows(void* o, int32_t n, const ldo_window* windows, const doub
             b.mult = windows[w].bias_mult;
             b.grid.assign(grid + (size_t)w * grid_len, grid + (size_t)(w + 1) * grid_len);
         }
+        O->grid_len = grid_len;
+        if (O->hist_on) {
+            O->hist.assign((size_t)n * (grid_len + 1), 0);
+        }
         assign_windows(O);
         // LinearStepWellBiasFunction stores its value when constructed
         // (bias_functions.cpp:107-122); a GridBiasFunction starts from 0
@@ -364,6 +373,47 @@ int oracle_set_windows(void* o, int32_t n, const ldo_window* windows, const doub
             Replica& rep = *O->reps[O->q_to_rep[w]];
             O->win_state[w].restraint = O->windows[w].restraint(rep.window_op(0));
             O->win_state[w].grid = 0;
+        }
+    })
+}
+
+// GridBiasFunction::replace_biases (bias_functions.cpp:236-239): the grid
+// changes, the stored values of the windows' bias functions do not.
+int oracle_set_window_grid(void* o, const double* grid) {
+    ORACLE_TRY(o, {
+        if (O->windows.empty()) {
+            throw OrigamiError(LDO_E_ARG, "no umbrella windows set");
+        }
+        for (size_t w = 0; w < O->windows.size(); w++) {
+            O->windows[w].grid.assign(grid + w * O->grid_len, grid + (w + 1) * O->grid_len);
+        }
+    })
+}
+
+int oracle_set_histogram(void* o, int32_t on) {
+    ORACLE_TRY(o, {
+        if (on && O->windows.empty()) {
+            throw OrigamiError(LDO_E_ARG, "set windows first");
+        }
+        O->hist_on = on != 0;
+        O->hist.assign(on ? O->windows.size() * (O->grid_len + 1) : 0, 0);
+        assign_windows(O);
+        if (!on) {
+            for (auto& r: O->reps) {
+                r->hist = nullptr;
+            }
+        }
+    })
+}
+
+int oracle_get_histogram(void* o, int64_t* counts, int32_t reset) {
+    ORACLE_TRY(o, {
+        if (!O->hist_on) {
+            throw OrigamiError(LDO_E_ARG, "histogram not enabled");
+        }
+        std::copy(O->hist.begin(), O->hist.end(), counts);
+        if (reset) {
+            std::fill(O->hist.begin(), O->hist.end(), 0);
         }
     })
 }

@@ -384,6 +384,9 @@ class Replica {
     // bias values (both null without biases)
     const WindowBias* win {nullptr};
     WindowState* win_state {nullptr};
+    // the window's visit counts ([grid size + 1], the last bin off the grid),
+    // or null when not counted (USGCMCSimulation::update_internal)
+    int64_t* hist {nullptr};
 
     void simulate(int64_t steps); // simulation.cpp:568-653
     int window_op(int staple_offset) const;

@@ -318,6 +318,12 @@ void Replica::simulate(int64_t steps) {
                 throw;
             }
         }
+        // update_internal (us_simulation.cpp:262-266): f_i[point]++
+        if (hist != nullptr) {
+            int p = window_op(0);
+            int glen = (int)win->grid.size();
+            hist[(p >= 0 && p < glen) ? p : glen]++;
+        }
     }
 }
 

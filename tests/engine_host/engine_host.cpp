@@ -39,6 +39,7 @@ struct HostCtx {
     std::vector<std::vector<uint64_t>> tapes;
     std::vector<ldo_window> windows;
     std::vector<double> grid, win_stored;
+    std::vector<int64_t> hist;
     std::vector<float> net;
     ~HostCtx() { std::free(blocks); }
 };
@@ -237,6 +238,49 @@ int eh_set_windows(void* ctx, int32_t n, const ldo_window* windows, const double
         c->P.win_grid = c->grid.data();
         c->P.grid_len = grid_len;
         c->P.win_stored = c->win_stored.data();
+        if (c->P.hist) {
+            c->hist.assign((size_t)c->R * (grid_len + 1), 0);
+            c->P.hist = c->hist.data();
+        }
+    });
+}
+
+int eh_set_window_grid(void* ctx, const double* grid) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        if (!c->P.windows) {
+            throw ldo::AbiError {LDO_E_ARG, "no umbrella windows set"};
+        }
+        std::copy(grid, grid + (size_t)c->R * c->P.grid_len, c->grid.begin());
+    });
+}
+
+int eh_set_histogram(void* ctx, int32_t on) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        if (!on) {
+            c->P.hist = nullptr;
+            c->hist.clear();
+            return;
+        }
+        if (!c->P.windows) {
+            throw ldo::AbiError {LDO_E_ARG, "set windows first"};
+        }
+        c->hist.assign((size_t)c->R * (c->P.grid_len + 1), 0);
+        c->P.hist = c->hist.data();
+    });
+}
+
+int eh_get_histogram(void* ctx, int64_t* counts, int32_t reset) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        if (!c->P.hist) {
+            throw ldo::AbiError {LDO_E_ARG, "histogram not enabled"};
+        }
+        std::copy(c->hist.begin(), c->hist.end(), counts);
+        if (reset) {
+            std::fill(c->hist.begin(), c->hist.end(), 0);
+        }
     });
 }
 

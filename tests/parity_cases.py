@@ -172,6 +172,51 @@ def case_windows_exchange_matches_oracle(sampler_cls, oracle_mod):
     assert e.value.code == abi.LDO_E_ARG
 
 
+def case_window_histogram_and_grid_replacement(sampler_cls, oracle_mod):
+    say("case: 64-domain raster, 8 numstaples windows at 330 K with a 3-point grid "
+        "(visits at 3-4 staples fall off it), per-step histograms over 2 x 2000 steps "
+        "with the grid replaced in between, then 8 window swaps x 250 steps")
+    import numpy as np
+    s = systems.raster_system(replicas=8)
+    temps = [330.0] * 8
+    wins, grid, _ = _raster_windows(8)
+    glen = 3
+    grid = [g for w in range(8) for g in grid[41 * w:41 * w + glen]]
+    grid2 = [0.7 * g - 0.3 for g in grid]
+    for mode in (None, abi.EXCHANGE_WINDOWS):
+        kw = {} if mode is None else {"exchange_mode": mode}
+        gpu = sampler_cls(s, temps=temps, **kw)
+        ref = oracle_mod.Oracle(s, temps=temps, **kw)
+        for x in (gpu, ref):
+            x.set_windows(wins, grid, glen)
+            x.set_histogram(True)
+        if mode is None:
+            for g in (grid, grid2):
+                for x in (gpu, ref):
+                    x.set_window_grid(g)
+                    x.sweep(2000)
+                hg, hr = gpu.histogram(reset=True), ref.histogram(reset=True)
+                assert np.array_equal(hg, hr)
+                assert (hg.sum(axis=1) == 2000).all()
+                assert hg[:, glen].sum() > 0  # some steps ended off the grid
+                for r in range(8):
+                    assert_same(gpu, ref, r)
+                    assert gpu.observables(r)["bias"] == ref.observables(r)["bias"], r
+            assert not gpu.histogram().any()
+        else:
+            for x in (gpu, ref):
+                x.run(8, 250)
+            hg, hr = gpu.histogram(), ref.histogram()
+            assert np.array_equal(hg, hr)
+            assert (hg.sum(axis=1) == 2000).all()
+            assert gpu.permutation() == ref.permutation()
+        assert_counters(gpu, ref)
+        gpu.set_histogram(False)
+        with pytest.raises(abi.LdoError) as e:
+            gpu.histogram()
+        assert e.value.code == abi.LDO_E_ARG
+
+
 def case_three_quarter_turn_and_three_domain_staples(sampler_cls, oracle_mod):
     say("case: ThreeQuarterTurn four-domain system (8 replicas, 340-354 K) and the "
         "three-domain staple system with both CTCB endpoint rules (8 replicas, 345-352 K), "

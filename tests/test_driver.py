@@ -3,6 +3,7 @@ replica on the GPU.  The expected lines are built from the CPU oracle run on
 the same system and seed, in the reference's formats (files.cpp:493-791,
 ptmc_simulation.cpp:91-104, 315-322)."""
 import json
+import sys
 import os
 import subprocess
 
@@ -204,3 +205,114 @@ def test_driver_ptmc_restart_matches_oracle(tmp_path, oracle_mod):
         for r in range(4):
             e, _, _, _ = expect_lines(ref, r, s, 1000 * k)
             assert read(d, f"out-{r}.ene")[k] == e, (k, r)
+
+
+def write_mwus_inputs(d, s, windows, extra):
+    """The reference's order-parameter, bias-function and windows files for
+    numstaples windows with a LinearStepWell restraint and a Grid bias."""
+    with open(os.path.join(d, "ops.json"), "w") as f:
+        json.dump({"origami": {"order_params": [
+            {"label": "Number of bound staples", "tag": "numstaples", "type": "NumStaples",
+             "level": 0},
+            {"label": "Number of fully bound domain pairs", "tag": "numfulldomains",
+             "type": "NumBoundDomainPairs", "level": 0}]}}, f)
+    with open(os.path.join(d, "biases.json"), "w") as f:
+        json.dump({"origami": {"bias_functions": [
+            {"label": "well", "tag": "lswnumstaples", "type": "LinearStepWell", "min_op": 0,
+             "max_op": 24, "well_bias": 0, "min_bias": 2, "slope": 1.5, "ops": ["numstaples"],
+             "update_per_domain": False},
+            {"label": "grid", "tag": "grid", "type": "Grid", "ops": ["numstaples"],
+             "update_per_domain": False}]}}, f)
+    with open(os.path.join(d, "run.windows"), "w") as f:
+        f.write("lswnumstaples\n" + "".join(f"{lo}, {hi}\n" for lo, hi in windows))
+    return write_inputs(d, s, {
+        "simulation_type": "mw_umbrella_sampling", "temp": "330",
+        "order_parameter_file": os.path.join(d, "ops.json"),
+        "bias_functions_file": os.path.join(d, "biases.json"),
+        "windows_file": os.path.join(d, "run.windows"), "us_grid_bias_tag": "grid",
+        "multi_window": "true", "max_equil_dur": 1000000000, "max_iter_dur": 1000000000,
+        **extra})
+
+
+def test_driver_rejects_unsupported_umbrella_inputs(tmp_path):
+    """Umbrella inputs the driver cannot honour stop the run before the GPU
+    is touched: a bias function besides the window restraint and the grid,
+    and windows over two order parameters."""
+    if not os.path.exists(DRIVER):
+        pytest.skip("driver not built")
+    d = str(tmp_path)
+    s = systems.raster_system(replicas=2)
+    inp = write_mwus_inputs(d, s, [(0, 2), (1, 3)], {})
+    with open(os.path.join(d, "biases.json")) as f:
+        b = json.load(f)
+    b["origami"]["bias_functions"].append({"label": "x", "tag": "extra", "type": "SquareWell",
+                                           "ops": ["numfulldomains"], "min_op": 0, "max_op": 1,
+                                           "well_bias": 0, "outside_bias": 1})
+    with open(os.path.join(d, "biases.json"), "w") as f:
+        json.dump(b, f)
+    p = subprocess.run([DRIVER, inp], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 1 and "bias function extra" in p.stderr, p.stderr
+    inp = write_mwus_inputs(d, s, [(0, 2), (1, 3)], {})
+    with open(os.path.join(d, "run.windows"), "w") as f:
+        f.write("lswnumstaples\n0 1, 2 3\n")
+    p = subprocess.run([DRIVER, inp], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 1 and "one order parameter" in p.stderr, p.stderr
+
+
+@pytest.mark.gpu
+def test_driver_mwus_iterations_match_oracle(tmp_path, oracle_mod):
+    """Three numstaples windows, an equilibration run and three adaptive
+    iterations (max_D_bias 1): every window's per-iteration energies, its
+    -inp.biases and .biases weights and its .out summary equal the oracle's
+    run with the same histograms fed through the restated update
+    (oracle/us_ref.py)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "oracle"))
+    import us_ref
+    s = systems.raster_system(replicas=3)
+    d = str(tmp_path)
+    windows = [(0, 2), (1, 3), (2, 4)]
+    run_driver(write_mwus_inputs(d, s, windows, {
+        "equil_steps": 1000, "iter_steps": 2000, "max_num_iters": 3, "max_D_bias": 1.0,
+        "energies_output_freq": 500}))
+    W = len(windows)
+    glen = 64 + s.max_total_staples * s.max_staple_size + 1
+    wins = [abi.Window(abi.OP_NUMSTAPLES, lo, hi, 0, 0.0, 2.0, 1.5, 0.0, 1.0)
+            for lo, hi in windows]
+    ref = oracle_mod.Oracle(s, temps=[330.0] * W)
+    ref.set_windows(wins, [0.0] * (W * glen), glen)
+    ref.set_histogram(True)
+    post = [f"out_win-{lo}--{hi}" for lo, hi in windows]
+    for k in (1, 2):
+        ref.sweep(500)
+        for w in range(W):
+            e, _, _, _ = expect_lines(ref, w, s, 500 * k)
+            assert read(d, post[w] + "_iter-equil.ene")[k] == e, (k, w)
+    ref.histogram(reset=True)
+    us = [us_ref.WindowState() for _ in range(W)]
+    summaries = [""] * W
+    visited = 0
+    for n in range(3):
+        for w in range(W):
+            with open(os.path.join(d, f"{post[w]}_iter-{n}-inp.biases")) as f:
+                assert f.read() == us_ref.weights_json(us[w]), (n, w)
+        for k in range(1, 5):
+            ref.sweep(500)
+            for w in range(W):
+                e, _, _, _ = expect_lines(ref, w, s, 500 * k)
+                assert read(d, f"{post[w]}_iter-{n}.ene")[k] == e, (n, k, w)
+        h = ref.histogram(reset=True)
+        assert (h[:, glen] == 0).all() and (h.sum(axis=1) == 2000).all()
+        for w in range(W):
+            us_ref.process_iteration(us[w], [int(x) for x in h[w, :glen]], us[w].grid(glen),
+                                     2001, 1.0)
+            summaries[w] += us_ref.summary(us[w], n)
+            with open(os.path.join(d, f"{post[w]}_iter-{n}.biases")) as f:
+                assert f.read() == us_ref.weights_json(us[w]), (n, w)
+            visited += len(us[w].S_n)
+        ref.set_window_grid(np.concatenate([us[w].grid(glen) for w in range(W)]).tolist())
+    for w in range(W):
+        with open(os.path.join(d, post[w] + ".out")) as f:
+            assert f.read() == summaries[w], w
+    assert visited > 3 * W  # the windows sampled more than one point each

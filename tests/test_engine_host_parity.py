@@ -19,7 +19,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 HOST_DIR = os.path.join(HERE, "engine_host")
 HOST_LIB = os.path.join(HOST_DIR, "_build", "libldo_engine_host.so")
 NAMES = ["create", "destroy", "last_error", "set_config", "get_config", "set_ladder", "set_tape",
-         "set_windows", "sweep", "run", "get_observables", "get_counters", "get_permutation"]
+         "set_windows", "sweep", "run", "get_observables", "get_counters", "get_permutation",
+         "set_window_grid", "set_histogram", "get_histogram"]
 
 
 class HostLib:
@@ -72,6 +73,10 @@ def test_ptmc_exchange_matches_oracle(sampler_cls, oracle_mod):
 
 def test_windows_exchange_matches_oracle(sampler_cls, oracle_mod):
     pc.case_windows_exchange_matches_oracle(sampler_cls, oracle_mod)
+
+
+def test_window_histogram_and_grid_replacement(sampler_cls, oracle_mod):
+    pc.case_window_histogram_and_grid_replacement(sampler_cls, oracle_mod)
 
 
 def test_reference_rng_tape_replay(sampler_cls, oracle_mod):

@@ -37,6 +37,10 @@ def test_windows_exchange_matches_oracle(sampler_cls, oracle_mod):
     pc.case_windows_exchange_matches_oracle(sampler_cls, oracle_mod)
 
 
+def test_window_histogram_and_grid_replacement(sampler_cls, oracle_mod):
+    pc.case_window_histogram_and_grid_replacement(sampler_cls, oracle_mod)
+
+
 def test_mwus3d_window_ladders_by_temperature(sampler_cls, oracle_mod):
     pc.case_mwus3d_window_ladders_by_temperature(sampler_cls, oracle_mod)
 
