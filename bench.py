@@ -225,14 +225,35 @@ def gpu_leg_mwus3d(args):
     from latticednaorigami_amd import abi, systems
     from latticednaorigami_amd.sampler import Sampler
     s, wins, grid, glen = systems.mwus3d_system()
+    n_learned = args.mwus3d_learned
+    train_info = None
+    if n_learned > 0:
+        # the learned proposal: samples from a plain run of every eighth
+        # replica's temperature, a network over the 200 scaffold identities
+        from latticednaorigami_amd import learned
+        t0 = time.perf_counter()
+        data, n_cfg = learned.collect_samples(s, s.temps[::8], 8, 500, seed_steps=5000)
+        params, hist = learned.train(data, 200, args.learned_train_steps, batch=256, lr=1e-3)
+        train_info = {"samples": int(len(data[1])), "train_steps": args.learned_train_steps,
+                      "final_loss": float(hist[-1]) if len(hist) else None,
+                      "collect_and_train_s": time.perf_counter() - t0}
+        print(f"mwus3d: network trained on {len(data[1])} samples", file=sys.stderr, flush=True)
     smp = Sampler(s, temps=s.temps, exchange_mode=abi.EXCHANGE_WINDOWS)
     smp.set_windows(wins, grid, glen)
+    if n_learned > 0:
+        smp.learned_set_network(200, params, 0.1)
+
+    def interval_round(n):
+        for _ in range(n):
+            if n_learned > 0:
+                smp.learned_sweep(n_learned)
+            smp.run(1, args.interval)
     if args.warmup > 0:
-        smp.run(args.warmup, args.interval)
+        interval_round(args.warmup)
     c0 = smp.counters()
     smp.set_profiling(True)
     t0 = time.perf_counter()
-    smp.run(args.steps, args.interval)
+    interval_round(args.steps)
     wall = time.perf_counter() - t0
     prof = smp.profile()
     smp.set_profiling(False)
@@ -262,6 +283,11 @@ def gpu_leg_mwus3d(args):
         "exchange_accepts": c1.exchange_accepts - c0.exchange_accepts,
         "exchange_kernel_ms": prof["exchange_ms"],
     }
+    if n_learned > 0:
+        # counter slots 5, 6, 7: learned staple exchange, CB fallback, learned regrowth
+        extra["learned_attempts_by_slot"] = [c1.attempts[k] - c0.attempts[k] for k in (5, 6, 7)]
+        extra["learned_accepts_by_slot"] = [c1.accepts[k] - c0.accepts[k] for k in (5, 6, 7)]
+        extra["learned_training"] = train_info
     smp.close()
     return accepted / wall, wall, roofline, extra
 
@@ -378,6 +404,11 @@ def gpu_leg(args, system):
         "attempts_by_move": list(c1.attempts)[:4],
         "exchange_kernel_ms": prof["exchange_ms"],
     }
+    if n_learned > 0:
+        # counter slots 5, 6, 7: learned staple exchange, CB fallback, learned regrowth
+        extra["learned_attempts_by_slot"] = [c1.attempts[k] - c0.attempts[k] for k in (5, 6, 7)]
+        extra["learned_accepts_by_slot"] = [c1.accepts[k] - c0.accepts[k] for k in (5, 6, 7)]
+        extra["learned_training"] = train_info
     smp.close()
     return accepted / wall, wall, roofline, extra
 
@@ -416,6 +447,9 @@ def main():
     ap.add_argument("--learned-eval-temp", type=float, default=350.0,
                     help="learned: temperature of every replica in the evaluation")
     ap.add_argument("--learned-train-steps", type=int, default=3000)
+    ap.add_argument("--mwus3d-learned", type=int, default=0,
+                    help="mwus3d: learned attempts per replica before each exchange interval "
+                         "(0: the plain move set)")
     ap.add_argument("--learned-eval-rounds", type=int, default=200,
                     help="learned: evaluation rounds (--interval plain steps each)")
     ap.add_argument("--learned-attempts", type=int, default=100,
@@ -546,8 +580,9 @@ def main_mwus3d(args):
             "temps_K": [310.0, 315.0, 320.0, 325.0],
             "exchange_interval": args.interval,
             "hybridization": "NearestNeighbour",
-            "learned_proposal": False,
-            "step": "one exchange interval on every replica",
+            "learned_proposal": args.mwus3d_learned > 0,
+            "learned_attempts_per_interval": args.mwus3d_learned,
+            "step": "one exchange interval on every replica (after its learned attempts)",
         },
         "roofline": None,
         "cpu_baseline": None,

@@ -320,6 +320,36 @@ def case_learned_move_matches_oracle(sampler_cls, oracle_mod):
         assert np.array_equal(x, y)
 
 
+def case_learned_moves_in_3d_umbrella_windows(sampler_cls, oracle_mod):
+    say("case: learned moves (random network over 200 identities, CB fallback 0.1) in the "
+        "200-domain 3/4-turn 5x5x8 umbrella run with three-domain staples: 8 numfulldomains "
+        "windows at 318 K, 3 x (100 learned attempts + 2 window swaps x 500 steps)")
+    s, wins, grid, glen = systems.mwus3d_system(windows=64, temps=(318.0,))
+    wins = [abi.Window(w.op, 0, w.max_op + 4 * i, 0, 0.0, 0.0, 1.0, 0.0, 1.0)
+            for i, w in enumerate(wins[:8])]
+    temps = [318.0] * 8
+    s.temps = temps
+    gpu = sampler_cls(s, temps=temps, exchange_mode=abi.EXCHANGE_WINDOWS)
+    ref = oracle_mod.Oracle(s, temps=temps, exchange_mode=abi.EXCHANGE_WINDOWS)
+    net = random_network(200, 23)
+    for x in (gpu, ref):
+        x.set_windows(wins, grid[:8 * glen], glen)
+        x.run(2, 500)
+        x.learned_set_network(200, net, 0.1)
+    for _ in range(3):
+        for x in (gpu, ref):
+            x.learned_sweep(100)
+            x.run(2, 500)
+        assert gpu.permutation() == ref.permutation()
+        for r in range(8):
+            assert_same(gpu, ref, r)
+            assert gpu.observables(r)["bias"] == ref.observables(r)["bias"], r
+    assert_counters(gpu, ref)
+    c = gpu.counters()
+    assert c.attempts[5] + c.attempts[6] + c.attempts[7] == 8 * 300
+    assert c.accepts[7] > 0
+
+
 def case_batch_matches_oracle(batch_cls, oracle_mod):
     say("case: batch of 12 chains (8-40 domains, 310-350 K), 3 x 500 steps, each segment on "
         "a different sweep kernel (LDS, wavefront, lane)")

@@ -99,6 +99,10 @@ def test_learned_move_matches_oracle(sampler_cls, oracle_mod):
     pc.case_learned_move_matches_oracle(sampler_cls, oracle_mod)
 
 
+def test_learned_moves_in_3d_umbrella_windows(sampler_cls, oracle_mod):
+    pc.case_learned_moves_in_3d_umbrella_windows(sampler_cls, oracle_mod)
+
+
 def test_reference_ctcb_rule_fails_alike_on_3d_system(sampler_cls, oracle_mod):
     """Under the reference's endpoint rule the 3D three-domain-staple system
     corrupts its scaffold (a dangling staple domain blocks the rejected

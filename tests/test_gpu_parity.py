@@ -53,6 +53,10 @@ def test_learned_move_matches_oracle(sampler_cls, oracle_mod):
     pc.case_learned_move_matches_oracle(sampler_cls, oracle_mod)
 
 
+def test_learned_moves_in_3d_umbrella_windows(sampler_cls, oracle_mod):
+    pc.case_learned_moves_in_3d_umbrella_windows(sampler_cls, oracle_mod)
+
+
 def test_batch_matches_oracle(oracle_mod):
     from latticednaorigami_amd.sampler import BatchSampler
     pc.case_batch_matches_oracle(BatchSampler, oracle_mod)

@@ -371,6 +371,41 @@ def test_learned_move_matches_enumeration(oracle_mod):
         assert abs(m - p) <= 4 * se + 0.005, (key, m, p, se)
 
 
+def test_learned_move_matches_enumeration_three_domain_staples(oracle_mod):
+    """Learned end-domain regrowth of a three-domain staple (the anchor is
+    its middle domain; the staple stays connected through the other end)
+    with the CB fallback, interleaved with the plain moves, samples the
+    enumerated ensemble at 349 K (the learned exchange rejects staple types
+    of other than two domains in both directions, which keeps balance)."""
+    from latticednaorigami_amd import abi
+    import parity_cases as pc
+    s = systems.three_domain_system(flags=abi.FLAG_CTCB_SCAFFOLD_ENDPOINTS)
+    n_ident = len(s.identities[0])
+    o = oracle_mod.Oracle(s, temps=[349.0] * 8)
+    exact, _, _ = o.enumerate()
+    o.learned_set_network(n_ident, pc.random_network(n_ident, 5), 0.1)
+    o.sweep(20000)
+    counts = [dict() for _ in range(8)]
+    n = 1500
+    for _ in range(n):
+        o.learned_sweep(10)
+        o.sweep(100)
+        for r in range(8):
+            ob = o.observables(r)
+            k = (ob["num_staples"], ob["num_fully_bound_domain_pairs"],
+                 ob["num_misbound_domain_pairs"], ob["num_stacked_domain_pairs"])
+            counts[r][k] = counts[r].get(k, 0) + 1
+    acc = o.counters().accepts
+    att = o.counters().attempts
+    slot = 7  # the learned regrowth's counters (the last of LDO_MAX_MOVES = 8)
+    assert att[slot] > 1000 and acc[slot] > 100, (att, acc)
+    for key, p in exact.items():
+        if p < 0.01:
+            continue
+        m, se = _mean_se(counts, key, n)
+        assert abs(m - p) <= 4 * se + 0.005, (key, m, p, se)
+
+
 def test_learned_exchange_matches_enumeration(oracle_mod):
     """The learned staple exchange alone changes the staple number (Met is
     taken out of the move set): a staple [1, 5] whose domain 5 has no
