@@ -7,7 +7,8 @@
 // .staples .staplestates .times .ene .ops, and .swp for parallel tempering,
 // ptmc_simulation.cpp:91-104, 315-322) while every replica runs on the GPU
 // through ldo.h.  Supported: simulation_type constant_temp,
-// ut_parallel_tempering and mw_umbrella_sampling (with the per-window,
+// ut_parallel_tempering, mw_umbrella_sampling and ptmw_umbrella_sampling
+// (with the per-window,
 // per-iteration files of us_simulation.cpp:103-248: _iter-<n>.* outputs,
 // -inp.biases / .biases weights and the .out iteration summary) with the
 // movetypes OrientationRotation,
@@ -413,7 +414,11 @@ std::unique_ptr<Outputs> setup_outputs(const Run& run, const std::string& base) 
 
 bool due(long long freq, long long step) { return freq != 0 && step % freq == 0; }
 
-void write_outputs(Run& run, Outputs& o, int r, long long step, double seconds) {
+// Outputs of replica r at `step`; bias_rep >= 0 takes the bias column from
+// that replica instead (the replica that now holds the window whose stored
+// bias the record must show, after a window exchange)
+void write_outputs(Run& run, Outputs& o, int r, long long step, double seconds,
+                   int bias_rep = -1) {
     bool need_cfg = due(o.f_cfg, step) || due(o.f_vtf, step) || due(o.f_counts, step);
     std::vector<Chain> chains;
     std::vector<std::vector<int>> st;
@@ -472,6 +477,11 @@ void write_outputs(Run& run, Outputs& o, int r, long long step, double seconds) 
     ldo_obs ob {};
     if (due(o.f_counts, step) || due(o.f_ene, step) || due(o.f_ops, step)) {
         run.check(ldo_get_observables(run.ctx, r, &ob));
+        if (bias_rep >= 0) {
+            ldo_obs ob2 {};
+            run.check(ldo_get_observables(run.ctx, bias_rep, &ob2));
+            ob.bias = ob2.bias;
+        }
     }
     if (due(o.f_counts, step)) {
         int n_types = (int)run.identities.size();
@@ -755,9 +765,30 @@ void read_weights(const std::string& path, ldo_us::WindowState& u) {
     }
 }
 
-int run_mwus(Run& run, const ldo_system_desc& sd, ldo_run_params rp, const ldo_config& init) {
+int run_mwus(Run& run, const ldo_system_desc& sd, ldo_run_params rp, const ldo_config& init,
+             bool pt) {
     const Params& p = run.p;
     MwusSetup m = parse_mwus(run);
+    // PTMWUS (us_simulation.cpp:602-919): windows swap configurations every
+    // exchange_interval steps; the swap file records which configuration
+    // each window holds whenever the iteration's step count is a multiple
+    // of configs_output_freq
+    long long interval = p.integer("exchange_interval", 0);
+    long long cfg_freq = p.integer("configs_output_freq", 0);
+    if (pt) {
+        if (interval < 1 || cfg_freq < 1) {
+            throw Fail(LDO_E_ARG, "ptmw_umbrella_sampling needs exchange_interval and "
+                                  "configs_output_freq");
+        }
+        for (const char* k: {"configs_output_freq", "vtf_output_freq", "counts_output_freq",
+                             "times_output_freq", "energies_output_freq",
+                             "order_params_output_freq"}) {
+            if (p.integer(k, 0) % interval != 0) {
+                throw Fail(LDO_E_NOTIMPL, "output frequencies must be multiples of exchange_interval");
+            }
+        }
+        rp.exchange_mode = LDO_EXCHANGE_WINDOWS;
+    }
     int W = (int)m.windows.size();
     std::string base = p.str("output_filebase", "");
     std::vector<ldo_us::WindowState> us(W);
@@ -834,8 +865,56 @@ int run_mwus(Run& run, const ldo_system_desc& sd, ldo_run_params rp, const ldo_c
         }
         return step;
     };
-    // run_equilibration (103-124): its visits are not used
-    phase("equil", p.integer("equil_steps", 0), p.num("max_equil_dur", 0));
+    // one PTMWUS iteration (run_swaps, 658-694): exchange rounds with the
+    // outputs of each window taken before the round's swap
+    std::vector<int32_t> q_before(W), q_after(W);
+    auto swap_phase = [&](const std::string& tag, long long swaps, double max_dur) {
+        std::vector<std::unique_ptr<Outputs>> outs;
+        for (int w = 0; w < W; w++) {
+            outs.push_back(setup_outputs(run, base + m.postfix[w] + "_iter-" + tag));
+        }
+        std::ofstream swp;
+        Outputs::open(swp, base + "_iter-" + tag + ".swp");
+        run.check(ldo_get_permutation(run.ctx, q_before.data()));
+        for (int w = 0; w < W; w++) {
+            swp << q_before[w] << " ";
+        }
+        swp << "\n";
+        auto start = std::chrono::steady_clock::now();
+        long long m_steps = 0;
+        for (long long s = 1; s <= swaps; s++) {
+            run.check(ldo_get_permutation(run.ctx, q_before.data()));
+            run.check(ldo_run(run.ctx, 1, interval));
+            m_steps += interval;
+            run.check(ldo_get_permutation(run.ctx, q_after.data()));
+            double t =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+            for (int w = 0; w < W; w++) {
+                write_outputs(run, *outs[w], q_before[w], m_steps, t, q_after[w]);
+            }
+            if (m_steps % cfg_freq == 0) {
+                for (int w = 0; w < W; w++) {
+                    swp << q_before[w] << " ";
+                }
+                swp << "\n";
+            }
+            if (t > max_dur) {
+                break;
+            }
+        }
+        if (m_steps % cfg_freq == 0) {
+            for (int w = 0; w < W; w++) {
+                swp << q_after[w] << " ";
+            }
+            swp << "\n";
+        }
+        return m_steps;
+    };
+    // run_equilibration (103-124): its visits are not used; PTMWUS skips it
+    // when the biases are read in (PTMWUSGCMCSimulation::run, 646-648)
+    if (!(pt && p.flag("read_biases", false))) {
+        phase("equil", p.integer("equil_steps", 0), p.num("max_equil_dur", 0));
+    }
     std::vector<int64_t> hist((size_t)W * (m.grid_len + 1));
     run.check(ldo_get_histogram(run.ctx, hist.data(), 1));
     long long iters = p.integer("max_num_iters", 0);
@@ -846,15 +925,19 @@ int run_mwus(Run& run, const ldo_system_desc& sd, ldo_run_params rp, const ldo_c
             std::ofstream bf(base + m.postfix[w] + "_iter-" + tag + "-inp.biases");
             ldo_us::write_weights(bf, us[w]);
         }
-        long long steps = phase(tag, p.integer("iter_steps", 0), p.num("max_iter_dur", 0));
+        // the step count the visits are divided by: simulate()'s return
+        // value (one past the last step) for MWUS, the summed exchange
+        // intervals for PTMWUS
+        long long m_steps =
+            pt ? swap_phase(tag, p.integer("iter_swaps", 0), p.num("max_iter_dur", 0))
+               : phase(tag, p.integer("iter_steps", 0), p.num("max_iter_dur", 0)) + 1;
         run.check(ldo_get_histogram(run.ctx, hist.data(), 1));
         for (int w = 0; w < W; w++) {
             const int64_t* f = hist.data() + (size_t)w * (m.grid_len + 1);
             std::vector<double> g(grid.begin() + (size_t)w * m.grid_len,
                                   grid.begin() + (size_t)(w + 1) * m.grid_len);
             try {
-                // simulate() returns one past its last step
-                ldo_us::process_iteration(us[w], f, m.grid_len, g, steps + 1, max_D_bias);
+                ldo_us::process_iteration(us[w], f, m.grid_len, g, m_steps, max_D_bias);
             } catch (std::logic_error& e) {
                 throw Fail(LDO_E_MISUSE, e.what());
             }
@@ -998,8 +1081,8 @@ int main_run(int argc, char** argv) {
     init.orientations = c_ore.data();
 
     std::string sim = p.str("simulation_type", "constant_temp");
-    if (sim == "mw_umbrella_sampling") {
-        return run_mwus(run, sd, rp, init);
+    if (sim == "mw_umbrella_sampling" || sim == "ptmw_umbrella_sampling") {
+        return run_mwus(run, sd, rp, init, sim == "ptmw_umbrella_sampling");
     }
     bool pt = sim == "ut_parallel_tempering";
     if (!pt && sim != "constant_temp") {

@@ -316,3 +316,64 @@ def test_driver_mwus_iterations_match_oracle(tmp_path, oracle_mod):
         with open(os.path.join(d, post[w] + ".out")) as f:
             assert f.read() == summaries[w], w
     assert visited > 3 * W  # the windows sampled more than one point each
+
+
+@pytest.mark.gpu
+def test_driver_ptmwus_iterations_match_oracle(tmp_path, oracle_mod):
+    """Three numstaples windows exchanging configurations every 500 steps
+    (ptmw_umbrella_sampling): equilibration, then two iterations of 4 swaps.
+    Each window's energies come from the configuration it held before the
+    round's swap, its bias column from its own stored value; the swap file
+    lists each window's configuration; the weights divide the visits by the
+    2000 steps of the iteration (run_swaps counts no extra step)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "oracle"))
+    import us_ref
+    s = systems.raster_system(replicas=3)
+    d = str(tmp_path)
+    windows = [(0, 2), (0, 3), (1, 4)]
+    run_driver(write_mwus_inputs(d, s, windows, {
+        "simulation_type": "ptmw_umbrella_sampling", "equil_steps": 1000, "iter_swaps": 4,
+        "exchange_interval": 500, "max_num_iters": 2, "max_D_bias": 1.0,
+        "energies_output_freq": 500, "configs_output_freq": 1000}))
+    W = len(windows)
+    glen = 64 + s.max_total_staples * s.max_staple_size + 1
+    wins = [abi.Window(abi.OP_NUMSTAPLES, lo, hi, 0, 0.0, 2.0, 1.5, 0.0, 1.0)
+            for lo, hi in windows]
+    ref = oracle_mod.Oracle(s, temps=[330.0] * W, exchange_mode=abi.EXCHANGE_WINDOWS)
+    ref.set_windows(wins, [0.0] * (W * glen), glen)
+    ref.set_histogram(True)
+    post = [f"out_win-{lo}--{hi}" for lo, hi in windows]
+    ref.sweep(1000)
+    ref.histogram(reset=True)
+    us = [us_ref.WindowState() for _ in range(W)]
+    summaries = [""] * W
+    swapped = False
+    for n in range(2):
+        swp = [" ".join(str(q) for q in ref.permutation()) + " "]
+        for k in range(1, 5):
+            before = ref.permutation()
+            ref.run(1, 500)
+            after = ref.permutation()
+            swapped = swapped or before != after
+            for w in range(W):
+                e, _, _, _ = expect_lines(ref, before[w], s, 500 * k)
+                e = e.rsplit(" ", 2)[0] + " " + g10(ref.observables(after[w])["bias"]) + " "
+                assert read(d, f"{post[w]}_iter-{n}.ene")[k] == e, (n, k, w)
+            if k % 2 == 0:
+                swp.append(" ".join(str(q) for q in before) + " ")
+        swp.append(" ".join(str(q) for q in ref.permutation()) + " ")
+        assert read(d, f"out_iter-{n}.swp")[:-1] == swp, n
+        h = ref.histogram(reset=True)
+        for w in range(W):
+            us_ref.process_iteration(us[w], [int(x) for x in h[w, :glen]], us[w].grid(glen),
+                                     2000, 1.0)
+            summaries[w] += us_ref.summary(us[w], n)
+            with open(os.path.join(d, f"{post[w]}_iter-{n}.biases")) as f:
+                assert f.read() == us_ref.weights_json(us[w]), (n, w)
+        ref.set_window_grid(np.concatenate([us[w].grid(glen) for w in range(W)]).tolist())
+    for w in range(W):
+        with open(os.path.join(d, post[w] + ".out")) as f:
+            assert f.read() == summaries[w], w
+    assert swapped
