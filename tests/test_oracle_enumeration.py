@@ -302,13 +302,24 @@ def _learned_system():
     return s
 
 
-def _dangling_orientations(oracle_mod, network, n_samples=500):
+def _learned_system_three_domain():
+    """The same with a three-domain staple [1, 2, 5]: bound through domains
+    1 and 2, its end domain 5 dangles from the middle domain."""
+    from latticednaorigami_amd import abi
+    s = _learned_system()
+    s.identities = [[-1, -2, -3, -4], [1, 2, 5], [3, 4]]
+    s.max_staple_size = 3
+    s.flags = abi.FLAG_CTCB_SCAFFOLD_ENDPOINTS
+    return s
+
+
+def _dangling_orientations(oracle_mod, network, n_samples=500, system=None):
     """Per-replica frequencies of the six orientations of unbound staple
     domains, sampled between rounds of 20 learned attempts and 5 plain steps
     at 300 K."""
     import numpy as np
     import parity_cases as pc
-    o = oracle_mod.Oracle(_learned_system(), temps=[300.0] * 8)
+    o = oracle_mod.Oracle(system or _learned_system(), temps=[300.0] * 8)
     o.learned_set_network(5, pc.random_network(5, network), 0.0)
     o.sweep(5000)
     units = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
@@ -339,6 +350,20 @@ def test_learned_move_keeps_unbound_orientations_uniform(oracle_mod):
     frequencies are 0.12-0.24, 2.8 times the tolerance."""
     import numpy as np
     fr, c = _dangling_orientations(oracle_mod, 5)
+    assert c.accepts[7] > 1000
+    m = fr.mean(axis=0)
+    se = fr.std(axis=0, ddof=1) / np.sqrt(len(fr))
+    assert np.all(np.abs(m - 1 / 6) <= 4 * se + 0.005), (m, se)
+
+
+def test_learned_move_keeps_three_domain_dangling_orientations_uniform(oracle_mod):
+    """The uniform-orientation check for the end domain of a three-domain
+    staple, which the learned move places next to the staple's middle
+    domain.  Without the q(old)/q(new) factor (a planted mistake) the
+    frequencies are 0.10-0.29, well outside the tolerance; the enumeration
+    test of the three-domain system does not see that mistake."""
+    import numpy as np
+    fr, c = _dangling_orientations(oracle_mod, 5, system=_learned_system_three_domain())
     assert c.accepts[7] > 1000
     m = fr.mean(axis=0)
     se = fr.std(axis=0, ddof=1) / np.sqrt(len(fr))
