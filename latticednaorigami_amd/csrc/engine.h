@@ -18,7 +18,12 @@
 // Engine functions compile for both the device (the kernels) and the host,
 // where a test build runs the same code against the CPU oracle.
 #define DEV __host__ __device__ __forceinline__
+// Call boundaries: DEVNI functions are compiled once and called; DEVIN
+// functions (the FourBody walk's parts, the occupancy updates) are inlined
+// into the DEVNI function that calls them, so a binding evaluation runs as
+// a few large functions with its operands in registers.
 #define DEVNI __host__ __device__ __noinline__
+#define DEVIN __host__ __device__ __forceinline__
 
 struct I3 {
     int x, y, z;
@@ -141,10 +146,12 @@ struct Eng {
     NetFrame* frames;
     int8_t* part;
     NetFrame* sframes;
+    GrowFrame* gstack;
+    int16_t* gpl;
     const double* t_hyb;
     const double* t_stack;
     double init_e;
-    int ns, lmax, nslots, nchain, hmask, hbits, n_ident;
+    int ns, lshift, lmask, nslots, nchain, hmask, hbits, n_ident;
     // FourBody accumulators (BindingPotential::m_delta_config)
     double dc_e;
     int dc_st;
@@ -205,8 +212,11 @@ struct Eng {
         frames = (NetFrame*)(blk + L.off_frames);
         part = (int8_t*)(blk + L.off_part);
         sframes = (NetFrame*)(blk + L.off_sframes);
+        gstack = (GrowFrame*)(blk + L.off_gstack);
+        gpl = (int16_t*)(blk + L.off_gpl);
         ns = L.ns;
-        lmax = L.lmax;
+        lshift = L.lshift;
+        lmask = (1 << L.lshift) - 1;
         nslots = L.nslots;
         nchain = L.nchain;
         hbits = L.hbits;
@@ -301,9 +311,9 @@ struct Eng {
     }
 
     // ------------------------------------------------------------ slots
-    DEV int cslot_of(int s) const { return s < ns ? 0 : 1 + (s - ns) / lmax; }
-    DEV int d_of(int s) const { return s < ns ? s : (s - ns) % lmax; }
-    DEV int slot(int cs, int d) const { return cs == 0 ? d : ns + (cs - 1) * lmax + d; }
+    DEV int cslot_of(int s) const { return s < ns ? 0 : 1 + ((s - ns) >> lshift); }
+    DEV int d_of(int s) const { return s < ns ? s : ((s - ns) & lmask); }
+    DEV int slot(int cs, int d) const { return cs == 0 ? d : ns + ((cs - 1) << lshift) + d; }
     DEV int len_of(int cs) const { return ch[cs].len; }
 
     // Domain::m_forward_domain / m_backward_domain (origami_system.cpp:427-433, 686-692)
@@ -502,30 +512,30 @@ struct Eng {
     }
 
     // ------------------------------------------------------------ FourBody
-    DEVNI void fb_single(int h1, int h2, int h3);
-    DEVNI void fb_double(int h1, int h2, int h3);
-    DEVNI void fb_triply(int h1, int h2, int h3);
+    DEVIN void fb_single(int h1, int h2, int h3);
+    DEVIN void fb_double(int h1, int h2, int h3);
+    DEVIN void fb_triply(int h1, int h2, int h3);
     DEVNI void fb_junction(int j1, int j2, int j3, int j4, int k1, int k2);
-    DEVNI void fb_junction_pair(int a, int b, int j);
-    DEVNI void fb_backward_triplets(int a, int b);
-    DEVNI void fb_forward_triplets(int a, int b);
-    DEVNI void fb_central_triplets(int di, int dj);
-    DEVNI void fb_backward_single_junction(int a, int b);
-    DEVNI void fb_backward_outer(int j3, int j4, int k2, int k1);
-    DEVNI void fb_forward_outer(int j1, int j2, int k1, int k2);
-    DEVNI void fb_central_outer(int j1, int j2, int k1, int k2, int k2b);
-    DEVNI void fb_forward_single_junction(int a, int b);
-    DEVNI void fb_central_single_junction(int a, int b);
-    DEVNI void fb_regular_pair(int a, int b, int i);
-    DEVNI void fb_helix_pair(int a, int b, int i, int j);
+    DEVIN void fb_junction_pair(int a, int b, int j);
+    DEVIN void fb_backward_triplets(int a, int b);
+    DEVIN void fb_forward_triplets(int a, int b);
+    DEVIN void fb_central_triplets(int di, int dj);
+    DEVIN void fb_backward_single_junction(int a, int b);
+    DEVIN void fb_backward_outer(int j3, int j4, int k2, int k1);
+    DEVIN void fb_forward_outer(int j1, int j2, int k1, int k2);
+    DEVIN void fb_central_outer(int j1, int j2, int k1, int k2, int k2b);
+    DEVIN void fb_forward_single_junction(int a, int b);
+    DEVIN void fb_central_single_junction(int a, int b);
+    DEVIN void fb_regular_pair(int a, int b, int i);
+    DEVIN void fb_helix_pair(int a, int b, int i, int j);
     DEVNI void fb_around(int cd, int j);
     DEVNI void fb_terms(int di, int dj);
-    DEVNI Delta bind_domain(int di);
+    DEVIN Delta bind_domain(int di);
 
     // ------------------------------------------------------------ system
-    DEVNI void update_occupancies(int s, I3 p);
-    DEVNI Delta internal_check(int s, I3 p, int o);
-    DEVNI Delta internal_unassign(int s, bool need_delta);
+    DEVIN void update_occupancies(int s, I3 p);
+    DEVIN Delta internal_check(int s, I3 p, int o);
+    DEVIN Delta internal_unassign(int s, bool need_delta);
     DEVNI double check_domain_constraints(int s, I3 p, int o);
     DEVNI double unassign_domain(int s);
     DEVNI double set_checked_domain_config(int s, I3 p, int o);
@@ -548,37 +558,37 @@ struct Eng {
     DEVNI void update_enthalpy_and_entropy();
 
     // ------------------------------------------------------------ moves
-    DEVNI int select_random_domain();
-    DEVNI bool test_acceptance(double p);
+    DEVIN int select_random_domain();
+    DEVIN bool test_acceptance(double p);
     DEVNI bool staple_is_connector(int cs);
     DEVNI bool scan_for_scaffold(int dom0, int excl_cs);
-    DEVNI int num_bound_staple_domains(int cs);
+    DEVIN int num_bound_staple_domains(int cs);
     DEVNI void reset_origami();
-    DEVNI double set_growth_point(int gnew, int gold);
+    DEVIN double set_growth_point(int gnew, int gold);
     DEVNI void grow_staple(int cs, int d_i);
     DEVNI void grow_chain_seq(int cs, int from, int to);
     DEVNI bool orientation_move();
     DEVNI bool insert_staple();
     DEVNI bool delete_staple();
-    DEVNI void cb_unassign_chain(int cs, int from, int to);
+    DEVIN void cb_unassign_chain(int cs, int from, int to);
     DEVNI void cb_select_and_set(int d, int prev_d);
-    DEVNI void cb_setup_regrow_old();
+    DEVIN void cb_setup_regrow_old();
     DEVNI bool cb_test_acceptance();
-    DEVNI void cb_set_old_growth_point(int gnew, int gold);
+    DEVIN void cb_set_old_growth_point(int gnew, int gold);
     DEVNI bool cb_staple_move();
     // constraint points
     DEVNI void cp_reset();
-    DEVNI void cp_add_active(int s, I3 p, int sg);
-    DEVNI void cp_remove_active(int s);
-    DEVNI void cp_update_endpoints(int s);
-    DEVNI bool cp_endpoint_reached(int s, I3 p);
-    DEVNI bool cp_walks_remain(int s, I3 p);
+    DEVIN void cp_add_active(int s, I3 p, int sg);
+    DEVIN void cp_remove_active(int s);
+    DEVIN void cp_update_endpoints(int s);
+    DEVIN bool cp_endpoint_reached(int s, I3 p);
+    DEVIN bool cp_walks_remain(int s, I3 p);
     DEVNI void cp_scan_network(int d);
     DEVNI void cp_find(int first, int count, int dir);
     DEVNI void ctcb_grow_chain_scaffold(int first, int count, int dir);
     DEVNI void ctcb_grow_staple_from(int gold);
     DEVNI bool ctcb_scaffold_move(int max_regrowth);
-    DEVNI bool attempt(int m);
+    DEVIN bool attempt(int m);
     DEVNI bool attempt_kind(int kind, int max_regrowth);
     // learned domain regrowth (lane 0 parts)
     DEVNI bool chain_connected(int cs);

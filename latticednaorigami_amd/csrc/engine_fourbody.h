@@ -11,7 +11,7 @@
 
 #include "engine.h"
 
-DEVNI void Eng::fb_single(int h1, int h2, int h3) {
+DEVIN void Eng::fb_single(int h1, int h2, int h3) {
     I3 n1 = sub3(pos(h2), pos(h1));
     if (eq3(n1, ore(h1))) {
         return;
@@ -23,7 +23,7 @@ DEVNI void Eng::fb_single(int h1, int h2, int h3) {
     }
 }
 
-DEVNI void Eng::fb_double(int h1, int h2, int h3) {
+DEVIN void Eng::fb_double(int h1, int h2, int h3) {
     I3 n1 = sub3(pos(h2), pos(h1));
     I3 n2 = sub3(pos(h3), pos(h2));
     if (!eq3(n1, n2)) {
@@ -33,7 +33,7 @@ DEVNI void Eng::fb_double(int h1, int h2, int h3) {
     }
 }
 
-DEVNI void Eng::fb_triply(int h1, int h2, int h3) {
+DEVIN void Eng::fb_triply(int h1, int h2, int h3) {
     I3 n1 = sub3(pos(h2), pos(h1));
     I3 n2 = sub3(pos(h3), pos(h2));
     if (!eq3(n1, n2)) {
@@ -88,7 +88,7 @@ DEVNI void Eng::fb_junction(int j1, int j2, int j3, int j4, int k1, int k2) {
 }
 
 // check_doubly_contig_junction_pair (:359-417)
-DEVNI void Eng::fb_junction_pair(int a, int b, int j) {
+DEVIN void Eng::fb_junction_pair(int a, int b, int j) {
     int k1 = a, k2 = b;
     I3 ndr = sub3(pos(k2), pos(k1));
     if (!eq3(ore(k1), ndr)) {
@@ -116,7 +116,7 @@ DEVNI void Eng::fb_junction_pair(int a, int b, int j) {
 }
 
 // check_backward_triplet_stacking_combos (:464-519)
-DEVNI void Eng::fb_backward_triplets(int a, int b) {
+DEVIN void Eng::fb_backward_triplets(int a, int b) {
     int h2 = a, h3 = b;
     if (!pair_stacked(h2, h3)) {
         return;
@@ -158,7 +158,7 @@ DEVNI void Eng::fb_backward_triplets(int a, int b) {
 }
 
 // check_forward_triplet_stacking_combos (:521-585)
-DEVNI void Eng::fb_forward_triplets(int a, int b) {
+DEVIN void Eng::fb_forward_triplets(int a, int b) {
     int h2 = b, h1 = a;
     bool first = pair_stacked(h1, h2);
     int h3 = walk(b, 1);
@@ -209,7 +209,7 @@ DEVNI void Eng::fb_forward_triplets(int a, int b) {
 }
 
 // check_central_triplet_stacking_combos (:587-652)
-DEVNI void Eng::fb_central_triplets(int di, int dj) {
+DEVIN void Eng::fb_central_triplets(int di, int dj) {
     int h1 = walk(di, -1);
     int h2 = di;
     int h3 = walk(dj, 1);
@@ -270,7 +270,7 @@ DEVNI void Eng::fb_central_triplets(int di, int dj) {
 // of the fb_junction calls, and of every read before them, is the reference's.
 
 // inner candidates of one outer pair (k2, k1) of check_backward_single_junction
-DEVNI void Eng::fb_backward_outer(int j3, int j4, int k2, int k1) {
+DEVIN void Eng::fb_backward_outer(int j3, int j4, int k2, int k1) {
     if (!eb(k1) || pair_stacked(k1, k2)) {
         return;
     }
@@ -300,7 +300,7 @@ DEVNI void Eng::fb_backward_outer(int j3, int j4, int k2, int k1) {
 }
 
 // check_backward_single_junction (:654-748)
-DEVNI void Eng::fb_backward_single_junction(int a, int b) {
+DEVIN void Eng::fb_backward_single_junction(int a, int b) {
     int j3 = a, j4 = b;
     int j3_bac = bwd(j3);
     int j3b = bd(j3);
@@ -326,7 +326,7 @@ DEVNI void Eng::fb_backward_single_junction(int a, int b) {
 }
 
 // inner candidates of one outer pair (k1, k2) of check_forward_single_junction
-DEVNI void Eng::fb_forward_outer(int j1, int j2, int k1, int k2) {
+DEVIN void Eng::fb_forward_outer(int j1, int j2, int k1, int k2) {
     if (!eb(k2) || pair_stacked(k1, k2)) {
         return;
     }
@@ -356,7 +356,7 @@ DEVNI void Eng::fb_forward_outer(int j1, int j2, int k1, int k2) {
 }
 
 // check_forward_single_junction (:750-844)
-DEVNI void Eng::fb_forward_single_junction(int a, int b) {
+DEVIN void Eng::fb_forward_single_junction(int a, int b) {
     int j1 = a, j2 = b;
     int j2_for = fwd(j2);
     int j1b = bd(j1);
@@ -383,7 +383,7 @@ DEVNI void Eng::fb_forward_single_junction(int a, int b) {
 }
 
 // inner candidates of one outer pair (j2, j1) of check_central_single_junction
-DEVNI void Eng::fb_central_outer(int j1, int j2, int k1, int k2, int k2b) {
+DEVIN void Eng::fb_central_outer(int j1, int j2, int k1, int k2, int k2b) {
     if (!eb(j1)) {
         return;
     }
@@ -412,7 +412,7 @@ DEVNI void Eng::fb_central_outer(int j1, int j2, int k1, int k2, int k2b) {
 }
 
 // check_central_single_junction (:846-934)
-DEVNI void Eng::fb_central_single_junction(int a, int b) {
+DEVIN void Eng::fb_central_single_junction(int a, int b) {
     int k1 = a, k2 = b;
     int k1b = bd(k1), k2b = bd(k2);
     if (cslot_of(k1b) == cslot_of(k2b)) {
@@ -440,7 +440,7 @@ DEVNI void Eng::fb_central_single_junction(int a, int b) {
 }
 
 // check_regular_pair_constraints (:288-320)
-DEVNI void Eng::fb_regular_pair(int a, int b, int i) {
+DEVIN void Eng::fb_regular_pair(int a, int b, int i) {
     I3 ndr = sub3(pos(b), pos(a));
     if (!kink_ok(a, ndr, b)) {
         fb_viol = true;
@@ -468,7 +468,7 @@ DEVNI void Eng::fb_regular_pair(int a, int b, int i) {
 }
 
 // check_doubly_contig_helix_pair (:322-357)
-DEVNI void Eng::fb_helix_pair(int a, int b, int i, int j) {
+DEVIN void Eng::fb_helix_pair(int a, int b, int i, int j) {
     if (j == 1) {
         return;
     }
@@ -558,7 +558,7 @@ DEVNI void Eng::fb_terms(int di, int dj) {
 
 // OrigamiPotential::bind_domain (:1282-1312), BindingPotential::bind_domains
 // (:131-147) and OpposingMisbindingPotential / Disallowed (:938-950)
-DEVNI Delta Eng::bind_domain(int di) {
+DEVIN Delta Eng::bind_domain(int di) {
     Delta r;
     r.e = 0;
     r.st = 0;

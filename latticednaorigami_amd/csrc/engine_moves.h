@@ -11,7 +11,7 @@
 #include "engine.h"
 
 // MCMovetype::select_random_domain (movetypes.cpp:98-113)
-DEVNI int Eng::select_random_domain() {
+DEVIN int Eng::select_random_domain() {
     int idx = uniform_int(0, S->num_domains - 1);
     for (int w = 0; w < S->n_chains; w++) {
         int cs = order[w];
@@ -25,7 +25,7 @@ DEVNI int Eng::select_random_domain() {
 }
 
 // MCMovetype::test_acceptance (movetypes.cpp:142-158)
-DEVNI bool Eng::test_acceptance(double p_ratio) {
+DEVIN bool Eng::test_acceptance(double p_ratio) {
     double p_accept = fmin(1.0, p_ratio) * S->modifier;
     if (p_accept == 1) {
         return true;
@@ -103,7 +103,7 @@ DEVNI bool Eng::staple_is_connector(int cs) {
 }
 
 // RegrowthMCMovetype::num_bound_staple_domains (movetypes.cpp:395-405)
-DEVNI int Eng::num_bound_staple_domains(int cs) {
+DEVIN int Eng::num_bound_staple_domains(int cs) {
     int n = 0;
     for (int d = 0; d < ch[cs].len; d++) {
         int stt = dom[slot(cs, d)].state;
@@ -131,7 +131,7 @@ DEVNI void Eng::reset_origami() {
 }
 
 // RegrowthMCMovetype::set_growth_point (movetypes.cpp:322-341)
-DEVNI double Eng::set_growth_point(int gnew, int gold) {
+DEVIN double Eng::set_growth_point(int gnew, int gold) {
     double de = set_domain_config(gnew, pos(gold), dom[gold].ore ^ 1);
     if (S->violated) {
         S->rejected = 1;
@@ -308,13 +308,22 @@ DEVNI bool Eng::delete_staple() {
 // (cb_movetypes.cpp:58-166, 363-387, 486-539)
 DEVNI void Eng::cb_select_and_set(int d, int prev_d) {
     I3 pp = pos(prev_d);
-    I3 cpos[6];
+    // Per trial site v (the six unit vectors in order): whether it is a
+    // candidate, its weight, the orientation it forces (-1: drawn after the
+    // selection) and its unbound occupant (LDO_NONE: an empty site).  The
+    // loops over v are unrolled, so these stay in registers; visiting the
+    // candidates in v order sums and selects in the reference's order.
+    bool cand[6];
     int core[6];
-    int cub[6]; // unbound domain at the trial site, or LDO_NONE for an empty site
+    int cub[6];
     double w[6];
-    int n = 0;
+#pragma unroll
     for (int v = 0; v < 6; v++) {
         I3 pn = add3(pp, unit3(v));
+        cand[v] = false;
+        core[v] = -1;
+        cub[v] = LDO_NONE;
+        w[v] = 0;
         int occ = occupancy(pn);
         if (occ == OCC_BOUND || occ == OCC_MISBOUND) {
             continue;
@@ -324,20 +333,17 @@ DEVNI void Eng::cb_select_and_set(int d, int prev_d) {
             int on = dom[ub].ore ^ 1;
             double de = check_domain_constraints(d, pn, on);
             if (!S->violated) {
-                cpos[n] = pn;
-                core[n] = on;
-                cub[n] = ub;
-                w[n] = exp(-de);
-                n++;
+                cand[v] = true;
+                core[v] = on;
+                cub[v] = ub;
+                w[v] = exp(-de);
             }
             else {
                 S->violated = 0;
             }
         }
         else {
-            cpos[n] = pn;
-            core[n] = -1;
-            cub[n] = LDO_NONE;
+            cand[v] = true;
             if (dom[d].state == OCC_UNASSIGNED) {
                 // check_domain_constraints at an empty site, with the
                 // occupancy already known: the domain is left unassigned at
@@ -347,33 +353,39 @@ DEVNI void Eng::cb_select_and_set(int d, int prev_d) {
                 dom[d].z = pn.z;
                 dom[d].ore = 0;
                 S->site_empty++;
-                w[n] = 6.0;
+                w[v] = 6.0;
             }
             else {
                 double de = check_domain_constraints(d, pn, 0);
-                w[n] = 6 * exp(-de);
+                w[v] = 6 * exp(-de);
             }
-            n++;
         }
     }
     double sum = 0;
     if (S->kind == LDO_MOVE_KIND_CTCB) {
-        for (int i = 0; i < n; i++) {
-            if (cub[i] >= 0) {
-                int occd = cub[i];
-                bool same_chain = cslot_of(occd) == cslot_of(d);
-                bool endpoint = cp_endpoint_reached(d, cpos[i]);
+#pragma unroll
+        for (int v = 0; v < 6; v++) {
+            if (!cand[v]) {
+                continue;
+            }
+            I3 pn = add3(pp, unit3(v));
+            if (cub[v] >= 0) {
+                bool same_chain = cslot_of(cub[v]) == cslot_of(d);
+                bool endpoint = cp_endpoint_reached(d, pn);
                 if (!(same_chain || endpoint)) {
-                    w[i] = 0;
+                    w[v] = 0;
                 }
             }
-            if (!cp_walks_remain(d, cpos[i])) {
-                w[i] = 0;
+            if (!cp_walks_remain(d, pn)) {
+                w[v] = 0;
             }
         }
     }
-    for (int i = 0; i < n; i++) {
-        sum += w[i];
+#pragma unroll
+    for (int v = 0; v < 6; v++) {
+        if (cand[v]) {
+            sum += w[v];
+        }
     }
     if (sum == 0) {
         S->rejected = 1;
@@ -384,15 +396,18 @@ DEVNI void Eng::cb_select_and_set(int d, int prev_d) {
         double u = uniform_real();
         double cum = 0;
         int sel = -1;
-        for (int i = 0; i < n; i++) {
-            cum += w[i] / sum;
-            if (u < cum) {
-                sel = i;
-                break;
+        int o = -1;
+#pragma unroll
+        for (int v = 0; v < 6; v++) {
+            if (sel < 0 && cand[v]) {
+                cum += w[v] / sum;
+                if (u < cum) {
+                    sel = v;
+                    o = core[v];
+                }
             }
         }
-        I3 p = sel >= 0 ? cpos[sel] : i3(0, 0, 0);
-        int o = sel >= 0 ? core[sel] : -1;
+        I3 p = sel >= 0 ? add3(pp, unit3(sel)) : i3(0, 0, 0);
         if (o < 0) {
             o = uniform_int(0, 5);
         }
@@ -406,7 +421,7 @@ DEVNI void Eng::cb_select_and_set(int d, int prev_d) {
 
 // CBMCMovetype::unassign_domains (cb_movetypes.cpp:200-213) over the domains
 // of chain cs with d in [from, to)
-DEVNI void Eng::cb_unassign_chain(int cs, int from, int to) {
+DEVIN void Eng::cb_unassign_chain(int cs, int from, int to) {
     for (int d = from; d < to; d++) {
         int s = slot(cs, d);
         prev[s].x = dom[s].x;
@@ -419,7 +434,7 @@ DEVNI void Eng::cb_unassign_chain(int cs, int from, int to) {
 }
 
 // CBMCMovetype::setup_for_regrow_old (cb_movetypes.cpp:215-225)
-DEVNI void Eng::cb_setup_regrow_old() {
+DEVIN void Eng::cb_setup_regrow_old() {
     S->regrow_old = 1;
     S->new_bias = S->bias;
     S->new_modifier = S->modifier;
@@ -446,7 +461,7 @@ DEVNI bool Eng::cb_test_acceptance() {
 }
 
 // CBMCMovetype::set_old_growth_point (cb_movetypes.cpp:168-180)
-DEVNI void Eng::cb_set_old_growth_point(int gnew, int gold) {
+DEVIN void Eng::cb_set_old_growth_point(int gnew, int gold) {
     double de = set_checked_domain_config(gnew, pos(gold), old[gnew].ore);
     S->bias *= exp(-de);
     push_assigned(gnew);
@@ -463,14 +478,17 @@ DEVNI bool Eng::cb_staple_move() {
         return false;
     }
     int len = ch[cs].len;
-    int gnew_l[8], gold_l[8];
+    // (staple domain, partner) pairs of the old configuration, in the
+    // replica block: a draw picks one by index
+    int16_t* gnew_l = gpl;
+    int16_t* gold_l = gpl + LDO_MAX_STAPLE_LEN;
     int nb = 0;
     for (int d = 0; d < len; d++) {
         int s = slot(cs, d);
         int b = dom[s].bound;
         if (b >= 0 && cslot_of(b) != cs) {
-            gnew_l[nb] = s;
-            gold_l[nb] = b;
+            gnew_l[nb] = (int16_t)s;
+            gold_l[nb] = (int16_t)b;
             nb++;
         }
     }
@@ -523,7 +541,7 @@ DEVNI void Eng::cp_reset() {
 }
 
 // Constraintpoints::add_active_endpoint (:264-278)
-DEVNI void Eng::cp_add_active(int s, I3 p, int sg) {
+DEVIN void Eng::cp_add_active(int s, I3 p, int sg) {
     if (S->n_active >= P->L.ecap) {
         fail(ST_CAPACITY);
         return;
@@ -538,7 +556,7 @@ DEVNI void Eng::cp_add_active(int s, I3 p, int sg) {
 }
 
 // Constraintpoints::remove_active_endpoint (:299-320)
-DEVNI void Eng::cp_remove_active(int s) {
+DEVIN void Eng::cp_remove_active(int s) {
     int sg = seg[s];
     int cs = cslot_of(s), dd = d_of(s);
     int k = 0;
@@ -553,7 +571,7 @@ DEVNI void Eng::cp_remove_active(int s) {
 }
 
 // Constraintpoints::update_endpoints (:340-349)
-DEVNI void Eng::cp_update_endpoints(int s) {
+DEVIN void Eng::cp_update_endpoints(int s) {
     cp_remove_active(s);
     int e = inact[s];
     if (e >= 0) {
@@ -562,7 +580,7 @@ DEVNI void Eng::cp_update_endpoints(int s) {
 }
 
 // Constraintpoints::endpoint_reached (:359-372)
-DEVNI bool Eng::cp_endpoint_reached(int s, I3 p) {
+DEVIN bool Eng::cp_endpoint_reached(int s, I3 p) {
     int sg = seg[s];
     int cs = cslot_of(s), dd = d_of(s);
     for (int i = 0; i < S->n_active; i++) {
@@ -580,7 +598,7 @@ DEVNI bool Eng::cp_endpoint_reached(int s, I3 p) {
 // distance exceeds the remaining steps or differs from them in parity
 // (ideal_random_walk.cpp:35-41).  Stem domains look up their segment pair in
 // a map the contiguous move never fills, so they have no walk constraint.
-DEVNI bool Eng::cp_walks_remain(int s, I3 p) {
+DEVIN bool Eng::cp_walks_remain(int s, I3 p) {
     if (stem[s] >= 0) {
         return true;
     }
@@ -797,18 +815,12 @@ DEVNI void Eng::cp_find(int first, int count, int dir) {
     S->n_active0 = S->n_active;
 }
 
-struct GrowFrame {
-    int16_t first, count, i, prev;
-    int16_t gold, gnew;
-    int8_t dir, kind, phase, pad;
-};
-
 // CTCBRegrowthMCMovetype::grow_chain (cb_movetypes.cpp:463-484) and
 // grow_staple_and_update_endpoints (:541-562), interleaved through an
 // explicit stack.  Growing starts with the scaffold segment and descends
 // into staples at growthpoints; any rejection unwinds the whole growth.
 DEVNI void Eng::ctcb_grow_chain_scaffold(int first, int count, int dir) {
-    GrowFrame stk[32];
+    GrowFrame* stk = gstack;
     int sp = 0;
     if (count > 1) {
         GrowFrame f;
@@ -847,7 +859,7 @@ DEVNI void Eng::ctcb_grow_chain_scaffold(int first, int count, int dir) {
             F.prev = (int16_t)d;
             F.i++;
             if (gp[d] >= 0) {
-                if (sp >= 32) {
+                if (sp >= LDO_GROW_DEPTH) {
                     fail(ST_CAPACITY);
                     break;
                 }
@@ -882,7 +894,7 @@ DEVNI void Eng::ctcb_grow_chain_scaffold(int first, int count, int dir) {
             F.phase = 1;
             int len = ch[cs].len, di = d_of(gnew);
             if (len - di > 1) {
-                if (sp >= 32) {
+                if (sp >= LDO_GROW_DEPTH) {
                     fail(ST_CAPACITY);
                     break;
                 }
@@ -902,7 +914,7 @@ DEVNI void Eng::ctcb_grow_chain_scaffold(int first, int count, int dir) {
             F.phase = 2;
             int di = d_of(F.gnew);
             if (di + 1 > 1) {
-                if (sp >= 32) {
+                if (sp >= LDO_GROW_DEPTH) {
                     fail(ST_CAPACITY);
                     break;
                 }
@@ -1026,7 +1038,7 @@ DEVNI bool Eng::ctcb_scaffold_move(int max_regrowth) {
 
 // MCMovetype::attempt_move and the reset_internal chain (movetypes.cpp:41-51,
 // 242-250; met_movetypes.cpp:45-48, 139-143; cb_movetypes.cpp:44-50, 457-461)
-DEVNI bool Eng::attempt(int m) {
+DEVIN bool Eng::attempt(int m) {
     return attempt_kind(P->move_kind[m], P->move_maxreg[m]);
 }
 

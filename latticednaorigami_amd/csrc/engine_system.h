@@ -10,7 +10,7 @@
 #include "engine.h"
 
 // OrigamiSystem::update_occupancies (origami_system.cpp:770-806)
-DEVNI void Eng::update_occupancies(int s, I3 p) {
+DEVIN void Eng::update_occupancies(int s, I3 p) {
     int occ = occupancy(p);
     if (occ == OCC_UNBOUND) {
         int dj = unbound_at(p);
@@ -44,7 +44,7 @@ DEVNI void Eng::update_occupancies(int s, I3 p) {
 }
 
 // OrigamiSystem::internal_check_domain_constraints (:828-871)
-DEVNI Delta Eng::internal_check(int s, I3 p, int o) {
+DEVIN Delta Eng::internal_check(int s, I3 p, int o) {
     Delta dc;
     dc.e = 0;
     dc.st = 0;
@@ -76,7 +76,7 @@ DEVNI Delta Eng::internal_check(int s, I3 p, int o) {
 // OrigamiSystem::internal_unassign_domain (:695-760).  need_delta=false is
 // used where the caller discards the returned change (a peek, or undoing a
 // violating placement); the stacking walk is then skipped.
-DEVNI Delta Eng::internal_unassign(int s, bool need_delta) {
+DEVIN Delta Eng::internal_unassign(int s, bool need_delta) {
     Delta dc;
     dc.e = 0;
     dc.st = 0;

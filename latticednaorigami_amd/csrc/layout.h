@@ -58,6 +58,17 @@ struct NetFrame {
     int16_t idx;    // position in that staple's domain stack
 };
 
+// One level of the CTCB growth recursion (scaffold segment or staple grown
+// from a growthpoint), kept on an explicit stack in the replica block.
+struct GrowFrame {
+    int16_t first, count, i, prev;
+    int16_t gold, gnew;
+    int8_t dir, kind, phase, pad;
+};
+#define LDO_GROW_DEPTH 32
+// growth-point pairs of the staple a CB staple regrowth moves
+#define LDO_MAX_STAPLE_LEN 8
+
 #define LDO_MAX_MOVES 8
 
 // Per-replica scalars at the head of a replica block (kept 8-byte aligned).
@@ -113,20 +124,22 @@ struct RepScalars {
 struct Layout {
     int32_t ns;        // scaffold domains
     int32_t lmax;      // longest staple
+    int32_t lshift;    // log2 of the slot stride of a staple chain
     int32_t smax;      // max_total_staples
-    int32_t nslots;    // ns + smax*lmax
+    int32_t nslots;    // ns + smax << lshift
     int32_t nchain;    // 1 + smax chain slots
     int32_t ntypes;    // chain identities (scaffold + staple types)
     int32_t tcap;      // per-type list capacity
     int32_t hcap;      // hash capacity (power of two)
     int32_t hbits;
     int32_t ecap;      // endpoint list capacity
+    int32_t pad_layout;
     int64_t off_dom, off_dident, off_chain, off_order, off_free, off_tlist, off_tcount;
     int64_t off_hkey, off_hval, off_modified, off_assigned, off_prev, off_old;
     int64_t off_seg, off_gp, off_stem, off_inact, off_netseg, off_netds_flag;
     int64_t off_checked, off_netcs, off_regrow, off_dir0, off_dir1;
     int64_t off_active, off_active0, off_gps, off_iaes, off_netds, off_frames;
-    int64_t off_part, off_sframes;
+    int64_t off_part, off_sframes, off_gstack, off_gpl;
     int64_t bytes; // block size (multiple of 256)
 };
 

@@ -38,8 +38,15 @@ Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap) {
     std::memset(&L, 0, sizeof L);
     L.ns = ns;
     L.lmax = lmax;
+    // staple chain k owns the slots ns + (k-1)*2^lshift + [0, lmax): a
+    // power-of-two stride turns the slot -> (chain, domain) maps into a
+    // shift and a mask
+    L.lshift = 0;
+    while ((1 << L.lshift) < lmax) {
+        L.lshift++;
+    }
     L.smax = smax;
-    L.nslots = ns + smax * lmax;
+    L.nslots = ns + (smax << L.lshift);
     L.nchain = 1 + smax;
     L.ntypes = ntypes;
     L.tcap = tcap;
@@ -93,6 +100,8 @@ Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap) {
     take(L.off_frames, C * (int64_t)sizeof(NetFrame));
     take(L.off_part, C);
     take(L.off_sframes, C * (int64_t)sizeof(NetFrame));
+    take(L.off_gstack, LDO_GROW_DEPTH * (int64_t)sizeof(GrowFrame));
+    take(L.off_gpl, 2 * LDO_MAX_STAPLE_LEN * 2);
     L.bytes = round_up(off, 256);
     return L;
 }
@@ -170,7 +179,11 @@ void make_plan(const ldo_system_desc* sys, const ldo_run_params* run, int32_t n_
     if (smax < 1 || smax > 4000) {
         throw AbiError {LDO_E_ARG, "max_total_staples must be in 1..4000"};
     }
-    if (ns + (int64_t)smax * lmax > 30000) {
+    int lstride = 1;
+    while (lstride < lmax) {
+        lstride *= 2;
+    }
+    if (ns + (int64_t)smax * lstride > 30000) {
         throw AbiError {LDO_E_ARG, "too many domain slots for 16-bit slot indices"};
     }
     int tcap = std::min(std::max(run->max_type_staples, 1), smax);
@@ -335,7 +348,7 @@ void read_config(const Plan& p, const char* block, ldo_config* cfg, int32_t cap_
             if (k >= cap_domains) {
                 throw AbiError {LDO_E_ARG, "domain capacity too small"};
             }
-            int s = cs == 0 ? d : L.ns + (cs - 1) * L.lmax + d;
+            int s = cs == 0 ? d : L.ns + ((cs - 1) << L.lshift) + d;
             cfg->positions[3 * k] = dom[s].x;
             cfg->positions[3 * k + 1] = dom[s].y;
             cfg->positions[3 * k + 2] = dom[s].z;
