@@ -159,6 +159,63 @@ struct Eng {
     bool pot_viol;
     int n_gps, n_iaes, n_netds;
     bool net_external;
+    // par = 1: the 64 lanes of a wavefront run this engine together on
+    // identical values (every lane computes and stores the same thing), and
+    // the lane-parallel sections split independent reads over the lanes and
+    // combine them with wave votes and shuffles.  par = 0: one lane runs the
+    // engine and the sections run serially.
+    int par;
+
+    // ------------------------------------------------------------ lanes
+    DEV int plane() const {
+#ifdef __HIP_DEVICE_COMPILE__
+        return par ? (int)(threadIdx.x & 63u) : 0;
+#else
+        return 0;
+#endif
+    }
+    DEV int pwidth() const { return par ? 64 : 1; }
+    // whether b holds on any lane of the section
+    DEV bool pany(bool b) const {
+#ifdef __HIP_DEVICE_COMPILE__
+        return par ? __any((int)b) != 0 : b;
+#else
+        return b;
+#endif
+    }
+    // lanes where b holds (bit l: lane l)
+    DEV uint64_t pballot(bool b) const {
+#ifdef __HIP_DEVICE_COMPILE__
+        return par ? (uint64_t)__ballot((int)b) : (b ? 1ull : 0ull);
+#else
+        return b ? 1ull : 0ull;
+#endif
+    }
+    // value v of lane src
+    DEV int pshfl(int v, int src) const {
+#ifdef __HIP_DEVICE_COMPILE__
+        return par ? __shfl(v, src) : v;
+#else
+        (void)src;
+        return v;
+#endif
+    }
+    // inclusive prefix sum of v over lanes 0..plane()
+    DEV int pscan(int v) const {
+#ifdef __HIP_DEVICE_COMPILE__
+        if (par) {
+            int lane = plane();
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                int t = __shfl_up(v, off);
+                if (lane >= off) {
+                    v += t;
+                }
+            }
+        }
+#endif
+        return v;
+    }
 
     // ------------------------------------------------------------ wiring
     // Fills a block section of `bytes` bytes with a 32-bit pattern in 16-byte
@@ -170,14 +227,15 @@ struct Eng {
         };
         Quad q {pattern, pattern, pattern, pattern};
         Quad* dst = (Quad*)base;
-        for (int64_t i = 0, n = (bytes + 15) / 16; i < n; i++) {
+        for (int64_t i = plane(), n = (bytes + 15) / 16; i < n; i += pwidth()) {
             dst[i] = q;
         }
     }
 
-    DEV void attach(const Params* p, char* blk, int r) {
+    DEV void attach(const Params* p, char* blk, int r, int par_ = 0) {
         P = p;
         rep = r;
+        par = par_;
         const Layout& L = p->L;
         S = (RepScalars*)blk;
         dom = (DomRec*)(blk + L.off_dom);

@@ -13,13 +13,34 @@
 // MCMovetype::select_random_domain (movetypes.cpp:98-113)
 DEVIN int Eng::select_random_domain() {
     int idx = uniform_int(0, S->num_domains - 1);
-    for (int w = 0; w < S->n_chains; w++) {
-        int cs = order[w];
-        int len = ch[cs].len;
-        if (idx < len) {
-            return slot(cs, idx);
+    int n = S->n_chains;
+    if (!par) {
+        for (int w = 0; w < n; w++) {
+            int cs = order[w];
+            int len = ch[cs].len;
+            if (idx < len) {
+                return slot(cs, idx);
+            }
+            idx -= len;
         }
-        idx -= len;
+        return LDO_NONE;
+    }
+    // lanes: 64 chains of the working order at a time; the chain holding
+    // domain idx is the first whose running length total exceeds idx
+    for (int base = 0; base < n; base += 64) {
+        int w = base + plane();
+        int cs = 0, len = 0;
+        if (w < n) {
+            cs = order[w];
+            len = ch[cs].len;
+        }
+        int incl = pscan(len);
+        uint64_t hit = pballot(w < n && idx < incl && idx >= incl - len);
+        if (hit) {
+            int src = __builtin_ctzll(hit);
+            return slot(pshfl(cs, src), idx - (pshfl(incl, src) - pshfl(len, src)));
+        }
+        idx -= pshfl(incl, 63);
     }
     return LDO_NONE;
 }
@@ -36,7 +57,7 @@ DEVIN bool Eng::test_acceptance(double p_ratio) {
 // MCMovetype::scan_for_scaffold_domain (movetypes.cpp:191-232) as an
 // explicit depth-first search; part[] holds the participating chains
 DEVNI bool Eng::scan_for_scaffold(int dom0, int excl_cs) {
-    for (int c = 0; c < nchain; c++) {
+    for (int c = plane(); c < nchain; c += pwidth()) {
         part[c] = 0;
     }
     if (excl_cs >= 0) {
@@ -317,6 +338,19 @@ DEVNI void Eng::cb_select_and_set(int d, int prev_d) {
     int core[6];
     int cub[6];
     double w[6];
+    // lanes: lane v < 6 looks up trial site v.  The checks below place d
+    // and take it off again, which leaves every site's occupant as it was,
+    // so the six lookups can all be made first (d is unassigned here; an
+    // assigned d is looked up site by site).
+    bool looked = par && dom[d].state == OCC_UNASSIGNED;
+    int occ_l = OCC_UNASSIGNED, ub_l = -1;
+    if (looked && plane() < 6) {
+        int hi = hfind(pkey(add3(pp, unit3(plane()))));
+        if (hi >= 0) {
+            occ_l = (int)((hval[hi] >> 16) & 3u);
+            ub_l = (int)(int16_t)(hval[hi] & 0xFFFFu);
+        }
+    }
 #pragma unroll
     for (int v = 0; v < 6; v++) {
         I3 pn = add3(pp, unit3(v));
@@ -324,12 +358,23 @@ DEVNI void Eng::cb_select_and_set(int d, int prev_d) {
         core[v] = -1;
         cub[v] = LDO_NONE;
         w[v] = 0;
-        int occ = occupancy(pn);
+        int occ = looked ? pshfl(occ_l, v) : occupancy(pn);
         if (occ == OCC_BOUND || occ == OCC_MISBOUND) {
             continue;
         }
         if (occ == OCC_UNBOUND) {
-            int ub = unbound_at(pn);
+            int ub;
+            if (looked) {
+                // unbound_at's check of the occupant slot
+                ub = pshfl(ub_l, v);
+                if (ub < 0 || ub >= nslots) {
+                    fail(ST_CONSTRAINT);
+                    ub = 0;
+                }
+            }
+            else {
+                ub = unbound_at(pn);
+            }
             int on = dom[ub].ore ^ 1;
             double de = check_domain_constraints(d, pn, on);
             if (!S->violated) {
@@ -559,13 +604,23 @@ DEVIN void Eng::cp_add_active(int s, I3 p, int sg) {
 DEVIN void Eng::cp_remove_active(int s) {
     int sg = seg[s];
     int cs = cslot_of(s), dd = d_of(s);
+    int n = S->n_active;
     int k = 0;
-    for (int i = 0; i < S->n_active; i++) {
-        Endpoint e = active[i];
-        if (e.cslot == cs && e.seg == sg && e.d == dd) {
-            continue;
+    // lanes: order-preserving compaction, 64 entries at a time (an entry
+    // moves to an index at or below its own, after every lane has read)
+    for (int base = 0; base < n; base += pwidth()) {
+        int i = base + plane();
+        bool keep = false;
+        Endpoint e;
+        if (i < n) {
+            e = active[i];
+            keep = !(e.cslot == cs && e.seg == sg && e.d == dd);
         }
-        active[k++] = e;
+        uint64_t m = pballot(keep);
+        if (keep) {
+            active[k + __builtin_popcountll(m & ((1ull << plane()) - 1))] = e;
+        }
+        k += __builtin_popcountll(m);
     }
     S->n_active = k;
 }
@@ -583,14 +638,16 @@ DEVIN void Eng::cp_update_endpoints(int s) {
 DEVIN bool Eng::cp_endpoint_reached(int s, I3 p) {
     int sg = seg[s];
     int cs = cslot_of(s), dd = d_of(s);
-    for (int i = 0; i < S->n_active; i++) {
+    int n = S->n_active;
+    bool hit = false;
+    for (int i = plane(); i < n && !(hit && !par); i += pwidth()) {
         const Endpoint& e = active[i];
         if (e.cslot == cs && e.seg == sg && e.d == dd && e.x == p.x && e.y == p.y &&
             e.z == p.z) {
-            return true;
+            hit = true;
         }
     }
-    return false;
+    return pany(hit);
 }
 
 // Constraintpoints::walks_remain (:397-442, 571-601).  The ideal-walk count
@@ -605,7 +662,9 @@ DEVIN bool Eng::cp_walks_remain(int s, I3 p) {
     int sg = seg[s];
     int cs = cslot_of(s), dd = d_of(s);
     int dir = sg == 0 ? dir0[cs] : dir1[cs];
-    for (int i = 0; i < S->n_active; i++) {
+    int n = S->n_active;
+    bool bad = false;
+    for (int i = plane(); i < n && !(bad && !par); i += pwidth()) {
         const Endpoint& e = active[i];
         if (e.cslot != cs || e.seg != sg) {
             continue;
@@ -632,10 +691,10 @@ DEVIN bool Eng::cp_walks_remain(int s, I3 p) {
         int ax = e.x - p.x, ay = e.y - p.y, az = e.z - p.z;
         int l1 = (ax < 0 ? -ax : ax) + (ay < 0 ? -ay : ay) + (az < 0 ? -az : az);
         if (l1 > steps || ((steps - l1) & 1)) {
-            return false;
+            bad = true;
         }
     }
-    return true;
+    return !pany(bad);
 }
 
 // StapleNetwork::scan_network + scan_staple_topology + make_staple_stack
@@ -809,7 +868,7 @@ DEVNI void Eng::cp_find(int first, int count, int dir) {
         }
         checked[0] = 1;
     }
-    for (int i = 0; i < S->n_active; i++) {
+    for (int i = plane(), n = S->n_active; i < n; i += pwidth()) {
         active0[i] = active[i];
     }
     S->n_active0 = S->n_active;
@@ -1003,7 +1062,7 @@ DEVNI bool Eng::ctcb_scaffold_move(int max_regrowth) {
     for (int phase = 0; phase < 2; phase++) {
         if (phase == 1) {
             cb_setup_regrow_old();
-            for (int i = 0; i < S->n_active0; i++) {
+            for (int i = plane(), n = S->n_active0; i < n; i += pwidth()) {
                 active[i] = active0[i];
             }
             S->n_active = S->n_active0;

@@ -92,9 +92,11 @@ DEV void init_replica(const Params& P, int r, const int32_t* c_idx, const int32_
 // dependent quantities.
 // One sweep of replica r on a caller-provided engine object and replica block
 // (the block in device memory, or a copy of it held closer to the lane).
+// par = 1: the 64 lanes of the calling wavefront all make this call (see
+// Eng::par).
 DEV void sweep_replica_on(Eng& e, const Params& P, int r, char* blk, int64_t steps,
-                          const int32_t* rep_slot) {
-    e.attach(&P, blk, r);
+                          const int32_t* rep_slot, int par = 0) {
+    e.attach(&P, blk, r, par);
     if (e.S->status != ST_OK) {
         return;
     }
@@ -107,9 +109,10 @@ DEV void sweep_replica_on(Eng& e, const Params& P, int r, char* blk, int64_t ste
     }
 }
 
-DEV void sweep_replica(const Params& P, int r, int64_t steps, const int32_t* rep_slot) {
+DEV void sweep_replica(const Params& P, int r, int64_t steps, const int32_t* rep_slot,
+                       int par = 0) {
     Eng e;
-    sweep_replica_on(e, P, r, P.blocks + (size_t)r * P.L.bytes, steps, rep_slot);
+    sweep_replica_on(e, P, r, P.blocks + (size_t)r * P.L.bytes, steps, rep_slot, par);
 }
 
 // OrigamiSystem::update_enthalpy_and_entropy (origami_system.cpp:204-246)

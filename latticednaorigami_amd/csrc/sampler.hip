@@ -13,10 +13,13 @@
 //                        chain there and writes the block back
 //   ldo_exchange_kernel  one PTMC exchange round over even/odd ladder pairs
 //                        (OneDPTGCMCSimulation::attempt_exchange)
-// A replica is driven by one lane.  With lanes_per_replica = 64 every
-// replica has a wavefront of its own (lane 0 works), which keeps replicas on
-// separate CUs when there are few of them, and its block is held in that
-// CU's LDS when it fits; with 1, 64 replicas share a wavefront.
+// With lanes_per_replica = 64 every replica has a wavefront of its own, which
+// keeps replicas on separate CUs when there are few of them; its 64 lanes run
+// the chain together on identical values (any lane's store is every lane's
+// store) and split the independent reads of the lane-parallel sections
+// (endpoint scans, trial-site lookups, the random domain draw) between them.
+// Its block is held in that CU's LDS when it fits.  With 1, 64 replicas
+// share a wavefront, one lane each.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -60,11 +63,20 @@ extern "C" __global__ void ldo_init_kernel(Params P, const int32_t* c_idx, const
     init_replica(P, r, c_idx, c_ident, c_pos, c_ore, n_chains, mode, rep_slot);
 }
 
+// With a wavefront per replica, its 64 lanes run the replica's chain
+// together (lane-parallel sections spread independent reads over them).
 extern "C" __global__ void ldo_sweep_kernel(Params P, int64_t steps, const int32_t* rep_slot) {
-    bool active;
-    int r = replica_of_thread(P, active);
-    if (active) {
-        sweep_replica(P, r, steps, rep_slot);
+    if (P.lanes_per_replica == 1) {
+        bool active;
+        int r = replica_of_thread(P, active);
+        if (active) {
+            sweep_replica(P, r, steps, rep_slot);
+        }
+        return;
+    }
+    int r = blockIdx.x;
+    if (r < P.n_replicas) {
+        sweep_replica(P, r, steps, rep_slot, 1);
     }
 }
 
@@ -91,9 +103,7 @@ extern "C" __global__ void __launch_bounds__(64)
         lds_params = P;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        sweep_replica_on(lds_engine, lds_params, r, (char*)lds_block, steps, rep_slot);
-    }
+    sweep_replica_on(lds_engine, lds_params, r, (char*)lds_block, steps, rep_slot, 1);
     __syncthreads();
     for (size_t i = threadIdx.x; i < n16; i += blockDim.x) {
         g[i] = lds_block[i];
@@ -278,9 +288,7 @@ extern "C" __global__ void __launch_bounds__(64)
         lds_block[i] = g[i];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        sweep_replica_on(lds_engine, lds_params, 0, (char*)lds_block, steps, nullptr);
-    }
+    sweep_replica_on(lds_engine, lds_params, 0, (char*)lds_block, steps, nullptr, 1);
     __syncthreads();
     for (size_t i = threadIdx.x; i < n16; i += blockDim.x) {
         g[i] = lds_block[i];
@@ -292,17 +300,16 @@ extern "C" __global__ void __launch_bounds__(64)
 extern "C" __global__ void ldo_batch_sweep_kernel(const Params* plist, int32_t n, int64_t steps,
                                                   int32_t per_lane) {
     int k;
-    bool active;
     if (per_lane) {
         k = blockIdx.x * blockDim.x + threadIdx.x;
-        active = k < n;
+        if (k < n) {
+            sweep_replica(plist[k], 0, steps, nullptr);
+        }
+        return;
     }
-    else {
-        k = blockIdx.x;
-        active = threadIdx.x == 0 && k < n;
-    }
-    if (active) {
-        sweep_replica(plist[k], 0, steps, nullptr);
+    k = blockIdx.x;
+    if (k < n) {
+        sweep_replica(plist[k], 0, steps, nullptr, 1);
     }
 }
 
