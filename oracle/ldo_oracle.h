@@ -59,6 +59,8 @@ int oracle_set_config(void* o, int32_t replica, const ldo_config* cfg);
 
 /* Tape recording (mt19937_64 mode): every draw of replica r is appended to
  * its tape; the tape is what ldo_set_tape replays on the GPU. */
+// replica `replica` replays these draws (as ldo_set_tape does)
+int oracle_set_tape(void* o, int32_t replica, const uint64_t* values, int64_t n);
 int oracle_record_tape(void* o, int32_t replica, int32_t on);
 int64_t oracle_tape_length(void* o, int32_t replica);
 int oracle_get_tape(void* o, int32_t replica, uint64_t* out, int64_t cap);

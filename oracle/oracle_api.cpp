@@ -561,6 +561,10 @@ int oracle_record_tape(void* o, int32_t r, int32_t on) {
     ORACLE_TRY(o, { O->reps.at(r)->rng.set_record(on != 0); })
 }
 
+int oracle_set_tape(void* o, int32_t r, const uint64_t* values, int64_t n) {
+    ORACLE_TRY(o, { O->reps.at(r)->rng.set_replay(values, n); })
+}
+
 int64_t oracle_tape_length(void* o, int32_t r) {
     return (int64_t)static_cast<Oracle*>(o)->reps.at(r)->rng.tape.size();
 }

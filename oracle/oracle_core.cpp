@@ -126,9 +126,20 @@ uint64_t Rng::philox64() {
     return (uint64_t)out[0] | ((uint64_t)out[1] << 32);
 }
 
+uint64_t Rng::replay_next() {
+    if (m_replay_pos >= m_replay.size()) {
+        throw OrigamiError(LDO_E_MISUSE, "replayed tape exhausted");
+    }
+    return m_replay[m_replay_pos++];
+}
+
 double Rng::uniform_real() {
     double u;
-    if (m_kind == 1) {
+    if (m_kind == 2) {
+        uint64_t b = replay_next();
+        std::memcpy(&u, &b, sizeof u);
+    }
+    else if (m_kind == 1) {
         std::uniform_real_distribution<double> dist;
         u = dist(m_mt);
     }
@@ -148,7 +159,10 @@ int Rng::uniform_int(int lo, int hi) {
         throw OrigamiError(LDO_E_MISUSE, "uniform_int with empty range");
     }
     int v;
-    if (m_kind == 1) {
+    if (m_kind == 2) {
+        v = (int)(int64_t)replay_next();
+    }
+    else if (m_kind == 1) {
         std::uniform_int_distribution<int> dist {lo, hi};
         v = dist(m_mt);
     }

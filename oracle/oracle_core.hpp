@@ -94,10 +94,20 @@ class Rng {
     double uniform_real();
     int uniform_int(int lo, int hi);
     void set_record(bool on) { m_record = on; }
+    // replay recorded draws instead of generating them (doubles as IEEE-754
+    // bit patterns, integers as integers, in draw order)
+    void set_replay(const uint64_t* v, int64_t n) {
+        m_replay.assign(v, v + n);
+        m_replay_pos = 0;
+        m_kind = 2;
+    }
     std::vector<uint64_t> tape;
 
   private:
     uint64_t philox64();
+    uint64_t replay_next();
+    std::vector<uint64_t> m_replay;
+    size_t m_replay_pos {0};
     int m_kind {0};
     uint64_t m_seed {0};
     uint32_t m_stream {0};

@@ -213,6 +213,12 @@ class Oracle:
         cfg = config_to_abi(chains, keep)
         _check(self.L, self.h, self.L.oracle_set_config(self.h, r, C.byref(cfg)))
 
+    def set_tape(self, r, values):
+        """Replica r replays these draws (doubles as bit patterns)."""
+        buf = (C.c_uint64 * max(1, len(values)))(*values)
+        self.L.oracle_set_tape.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64), C.c_int64]
+        _check(self.L, self.h, self.L.oracle_set_tape(self.h, r, buf, len(values)))
+
     def record_tape(self, r, on=True):
         _check(self.L, self.h, self.L.oracle_record_tape(self.h, r, int(on)))
 

@@ -415,3 +415,114 @@ def case_config_roundtrip_and_errors(sampler_cls, oracle_mod):
     with pytest.raises(abi.LdoError) as e:
         gpu.set_config(0, bad)
     assert e.value.code == abi.LDO_E_CONSTRAINT
+
+
+# ------------------------------------------------ hand-built pins (pin_cases)
+def case_hand_built_fourbody_configurations(sampler_cls, oracle_mod):
+    """Every hand-built FourBody configuration gives the oracle's result
+    (which test_oracle_pins.py holds to the hand-derived stacked-pair count
+    and energy): the same observables, or the same constraint error."""
+    import pin_cases as pin
+    say("case: hand-built FourBody configurations (set_config)")
+    for name, (pos, ore, dtype, n_stack, why) in sorted(pin.FOURBODY.items()):
+        s = pin.fourbody_system(dtype)
+        gpu = sampler_cls(s, temps=[330.0])
+        ref = oracle_mod.Oracle(s, temps=[330.0])
+        if n_stack is None:
+            for x in (gpu, ref):
+                with pytest.raises(abi.LdoError) as e:
+                    x.set_config(0, pin.fourbody_chains(name))
+                assert e.value.code == abi.LDO_E_CONSTRAINT, (name, why)
+            continue
+        gpu.set_config(0, pin.fourbody_chains(name))
+        ref.set_config(0, pin.fourbody_chains(name))
+        assert_same(gpu, ref, 0)
+        ob = gpu.observables(0)
+        assert ob["num_stacked_domain_pairs"] == n_stack, (name, why)
+        assert ob["energy"] == pytest.approx(pin.uniform_energy(s, 330.0, 4, 2, n_stack),
+                                             rel=ENERGY_RTOL, abs=1e-12), name
+    for name, (pos, n_stack, why) in sorted(pin.THREE_DOMAIN.items()):
+        s = systems.three_domain_system()
+        gpu = sampler_cls(s, temps=[349.0])
+        ref = oracle_mod.Oracle(s, temps=[349.0])
+        if n_stack is None:
+            for x in (gpu, ref):
+                with pytest.raises(abi.LdoError) as e:
+                    x.set_config(0, pin.three_domain_chains(name))
+                assert e.value.code == abi.LDO_E_CONSTRAINT, (name, why)
+            continue
+        gpu.set_config(0, pin.three_domain_chains(name))
+        ref.set_config(0, pin.three_domain_chains(name))
+        assert_same(gpu, ref, 0)
+        assert gpu.observables(0)["num_stacked_domain_pairs"] == n_stack, name
+
+
+def case_fixed_configuration_exchange_chains(sampler_cls, oracle_mod, rounds=2000):
+    """PTMC and window exchange over fixed configurations (0 MC steps per
+    round): the device exchange kernel makes the oracle's decisions round by
+    round, and its visit frequencies approach the exact stationary
+    distribution that test_oracle_pins.py holds the oracle to."""
+    import pin_cases as pin
+    s = pin.exchange_system()
+    cfgs, energies = pin.straight_configs()
+    for mode in (abi.EXCHANGE_TEMPERATURE, abi.EXCHANGE_WINDOWS):
+        say(f"case: fixed-configuration exchange chain, mode {mode}, {rounds} rounds")
+        temps = pin.PT_TEMPS if mode == abi.EXCHANGE_TEMPERATURE else [pin.WIN_TEMP] * 4
+        gpu = sampler_cls(s, temps=temps, exchange_mode=mode)
+        ref = oracle_mod.Oracle(s, temps=temps, exchange_mode=mode)
+        if mode == abi.EXCHANGE_WINDOWS:
+            for x in (gpu, ref):
+                x.set_windows(pin.windows(), sum(pin.WIN_GRIDS, []), len(pin.WIN_GRIDS[0]))
+        for r, c in enumerate(cfgs):
+            gpu.set_config(r, c)
+            ref.set_config(r, c)
+        if mode == abi.EXCHANGE_TEMPERATURE:
+            exact = pin.ptmc_stationary(s, energies)
+        else:
+            exact = pin.window_stationary([e[0] for e in energies], ref.permutation())
+        counts = {}
+        for _ in range(rounds):
+            gpu.run(1, 0)
+            ref.run(1, 0)
+            p = gpu.permutation()
+            assert p == ref.permutation()
+            counts[tuple(p)] = counts.get(tuple(p), 0) + 1
+        assert set(counts) <= set(exact)
+        # a loose bound at this length: the stationary law itself is pinned
+        # on the oracle with 60 000 rounds
+        for p, q in exact.items():
+            assert abs(counts.get(p, 0) / rounds - q) < 0.08, (mode, p, q)
+        for r in range(4):
+            assert_same(gpu, ref, r)
+
+
+def case_met_known_answer_tapes(sampler_cls, oracle_mod):
+    """Met staple insertion and deletion replayed from hand-written tapes of
+    the reference's draws, with the acceptance uniform just below and just
+    above the hand-computed ratio (test_oracle_pins.py)."""
+    import pin_cases as pin
+    say("case: Met insertion/deletion known-answer tapes")
+    p = 0.4
+    for kind in ("insert", "delete"):
+        for below in (True, False):
+            u = p * (1 - 1e-9) if below else p * (1 + 1e-9)
+            if kind == "insert":
+                s = pin.met_system(pin.met_insert_concentration(p))
+                tape = pin.met_insert_tape(u)
+            else:
+                s = pin.met_system(pin.met_delete_concentration(p))
+                tape = pin.met_delete_tape(u)
+            gpu = sampler_cls(s, temps=[pin.MET_TEMP], rng_mode=abi.RNG_TAPE)
+            ref = oracle_mod.Oracle(s, temps=[pin.MET_TEMP])
+            if kind == "delete":
+                gpu.set_config(0, pin.met_full_chains())
+                ref.set_config(0, pin.met_full_chains())
+            gpu.set_tape(0, tape)
+            ref.set_tape(0, tape)
+            gpu.sweep(1)
+            ref.sweep(1)
+            assert_same(gpu, ref, 0)
+            assert_counters(gpu, ref)
+            staples = gpu.observables(0)["num_staples"]
+            expect = (1 if below else 0) if kind == "insert" else (0 if below else 1)
+            assert staples == expect, (kind, below)

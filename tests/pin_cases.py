@@ -1,0 +1,284 @@
+"""Hand-built inputs whose outcomes are derived by hand from the reference's
+rules, shared by the oracle pins (test_oracle_pins.py) and the engine parity
+cases (parity_cases.py).
+
+FourBody configurations (origami_potential.cpp:25-127, 158-934; domain.cpp:33-118)
+use the four-domain system: scaffold s0..s3 (identities -1..-4), staple A =
+[1, 2] on s0, s1 and staple B = [3, 4] on s2, s3, every staple domain at its
+scaffold partner's site with the opposite orientation.  With Uniform
+hybridization and constant stacking the total energy is
+
+    E = 4 E_bind + 2 E_init + n_stack E_stack,
+
+with E_bind = binding_h / T - (binding_s + ln 6), E_init = (0.2 kcal/mol) / (R T)
++ 0.0057 kcal/mol/K / R and E_stack = stacking_ene / T, and n_stack follows
+from the geometry as written next to each case.  A HalfTurn pair (a, b) at
+chain-adjacent sites with step n = pos(b) - pos(a) is stacked when n is not
+along +-ore(a) and ore(b) = ore(a) turned half about n (domain.cpp:33-40,
+origami_potential.cpp:70-83); a pair bound to consecutive domains of one
+other chain counts once.
+"""
+import math
+import struct
+
+from latticednaorigami_amd import abi, systems
+
+X, Y, Z = (1, 0, 0), (0, 1, 0), (0, 0, 1)
+
+
+def neg(v):
+    return tuple(-a for a in v)
+
+
+def bits(x):
+    """IEEE-754 bit pattern of a double (a recorded uniform_real draw)."""
+    return struct.unpack("<Q", struct.pack("<d", x))[0]
+
+
+# name: (scaffold positions, scaffold orientations, domain type, expected
+# stacked pairs or None for a constraint violation, derivation)
+FOURBODY = {
+    "straight_nicked_helix": (
+        [(0, 0, 0), (1, 0, 0), (2, 0, 0), (3, 0, 0)], [Y, neg(Y), Y, neg(Y)], abi.HALFTURN, 3,
+        "one helix along +x: s0-s1 and s2-s3 are helix pairs (1 each), the nick pair s1-s2 "
+        "(partners on different staples) is stacked (1); both triplets straight: 3"),
+    "bent_at_nick_stacked": (
+        [(0, 0, 0), (1, 0, 0), (1, 0, 1), (1, 0, 2)], [Y, neg(Y), Y, neg(Y)], abi.HALFTURN, 2,
+        "s1-s2 steps +z and is stacked (ore turns -y -> +y about z), so three stacked pairs; "
+        "triplet s0-s1-s2 is doubly stacked but bent (+x then +z): -1/2 E - 1/2 E, -1: 2"),
+    "kink_at_nick_continues": (
+        [(0, 0, 0), (1, 0, 0), (1, 0, 1), (1, 0, 2)], [Y, neg(Y), X, neg(X)], abi.HALFTURN, 2,
+        "s1-s2 steps +z with ore(s2) = +x: not stacked, kink rule holds; helix pairs s0-s1, "
+        "s2-s3 (1 each); single-stacked triplet s1-s2-s3 keeps the +z direction: no penalty: 2"),
+    "kink_at_nick_turns": (
+        [(0, 0, 0), (1, 0, 0), (1, 0, 1), (1, 1, 1)], [Y, neg(Y), X, neg(X)], abi.HALFTURN, 1,
+        "as above but s2-s3 steps +y: the single-stacked triplet s1-s2-s3 turns (+z then +y) "
+        "and s1 -> s2 is not along ore(s1), so the s2-s3 stack is lost: 1"),
+    "crossover_antiparallel": (
+        [(0, 0, 0), (1, 0, 0), (1, -1, 0), (0, -1, 0)], [Y, neg(Y), Z, neg(Z)], abi.HALFTURN, 2,
+        "s1 -> s2 steps along ore(s1) (crossover); flanks s0->s1 = +x and s2->s3 = -x are "
+        "antiparallel: junction penalty 0; helix pairs 1 + 1: 2"),
+    "crossover_parallel": (
+        [(0, 0, 0), (1, 0, 0), (1, -1, 0), (2, -1, 0)], [Y, neg(Y), Z, neg(Z)], abi.HALFTURN, 0,
+        "flanks +x and +x are parallel: junction penalty 2 removes both stacks: 0"),
+    "crossover_perpendicular": (
+        [(0, 0, 0), (1, 0, 0), (1, -1, 0), (1, -1, 1)], [Y, neg(Y), X, neg(X)], abi.HALFTURN, 1,
+        "flanks +x and +z are perpendicular: penalty 1 removes 1/2 E + 1/2 E and one pair: 1"),
+    "kink_rule_step_against_ore": (
+        [(0, 0, 0), (1, 0, 0), (1, 1, 0), (2, 1, 0)], [Y, neg(Y), Z, neg(Z)], abi.HALFTURN, None,
+        "s1 -> s2 steps along -ore(s1): the kink rule fails (domain.cpp:42-46)"),
+    "crossover_opposing_ores": (
+        [(0, 0, 0), (1, 0, 0), (1, -1, 0), (0, -1, 0)], [Y, neg(Y), Y, neg(Y)], abi.HALFTURN, None,
+        "s1 -> s2 steps along ore(s1) with ore(s2) = -ore(s1): the kink rule fails "
+        "(domain.cpp:47-50)"),
+    "helix_pair_untwisted": (
+        [(0, 0, 0), (1, 0, 0), (2, 0, 0), (3, 0, 0)], [Y, Z, Y, neg(Y)], abi.HALFTURN, None,
+        "s0, s1 bound to consecutive domains of staple A but ore(s1) is not ore(s0) turned "
+        "half about +x: a doubly contiguous helix pair must be twisted (origami_potential.cpp:"
+        "322-357)"),
+    "three_quarter_crossover_twisted": (
+        [(0, 0, 0), (1, 0, 0), (1, 0, -1), (0, 0, -1)], [Y, neg(Z), neg(Y), neg(Z)],
+        abi.THREEQUARTERTURN, 2,
+        "3/4-turn twist: ore(b) = ore(a) turned a quarter (-1) about the step; the crossover "
+        "s1 -> s2 = -z = ore(s1) with ore(s2) = ore(s1) turned (-1) about the flank step +x "
+        "= -y, as the junction rule requires (domain.cpp:99-118); antiparallel flanks: 2"),
+    "three_quarter_crossover_untwisted": (
+        [(0, 0, 0), (1, 0, 0), (1, 0, -1), (0, 0, -1)], [Y, neg(Z), Y, Z], abi.THREEQUARTERTURN,
+        None,
+        "the same crossover with ore(s2) = +y: every pair rule holds but the 3/4-turn junction "
+        "rule does not"),
+}
+
+# three-domain scaffold [-1,-2,-3] with one staple [1,2,3] bound along it
+THREE_DOMAIN = {
+    "triply_contiguous_straight": ([(0, 0, 0), (1, 0, 0), (2, 0, 0)], 2,
+                                   "two helix pairs, straight triple: 2"),
+    "triply_contiguous_bent": ([(0, 0, 0), (1, 0, 0), (1, 0, 1)], None,
+                               "a triply contiguous helix must be straight "
+                               "(origami_potential.cpp:188-198)"),
+}
+
+
+def fourbody_system(domain_type):
+    s = systems.four_domain_system()
+    s.domain_type = domain_type
+    return s
+
+
+def fourbody_chains(name):
+    pos, ore, _, _, _ = FOURBODY[name]
+    # staple A's domains sit on s0, s1 and staple B's on s2, s3, facing them
+    return [systems.Chain(0, 0, pos, ore),
+            systems.Chain(1, 1, pos[0:2], [neg(o) for o in ore[0:2]]),
+            systems.Chain(2, 2, pos[2:4], [neg(o) for o in ore[2:4]])]
+
+
+def three_domain_chains(name):
+    pos, _, _ = THREE_DOMAIN[name]
+    ore = [Y, neg(Y), Y]
+    return [systems.Chain(0, 0, pos, ore), systems.Chain(1, 1, pos, [neg(o) for o in ore])]
+
+
+def uniform_energy(system, temp, n_pairs, n_staples, n_stack):
+    """Closed-form total energy (kT) of fully bound pairs, staples and
+    stacked pairs under Uniform hybridization and constant stacking."""
+    k = systems.KCAL_PER_MOL_TO_K
+    e_bind = system.binding_h / temp - (system.binding_s + math.log(6))
+    e_init = 0.2 * k / temp + 0.0057 * k
+    return n_pairs * e_bind + n_staples * e_init + n_stack * system.stacking_ene / temp
+
+
+# ---------------------------------------------------------------- exchange
+# Replicas whose configurations stay fixed (no MC steps between rounds): the
+# exchange step alone is then a Markov chain over permutations whose
+# stationary distribution follows from the configurations' energies.
+
+def straight_configs():
+    """Four configurations of the four-domain system on the straight scaffold
+    of "straight_nicked_helix", and their (fully bound pairs, staples,
+    stacked pairs): two staples (3 stacked pairs), two staples bent at the
+    nick (2), one staple (1), none."""
+    st = FOURBODY["straight_nicked_helix"]
+    bent = FOURBODY["bent_at_nick_stacked"]
+
+    def cfg(spec, staples):
+        pos, ore = spec[0], spec[1]
+        ch = [systems.Chain(0, 0, pos, ore)]
+        if "A" in staples:
+            ch.append(systems.Chain(1, 1, pos[0:2], [neg(o) for o in ore[0:2]]))
+        if "B" in staples:
+            ch.append(systems.Chain(2, 2, pos[2:4], [neg(o) for o in ore[2:4]]))
+        return ch
+    return ([cfg(st, "AB"), cfg(bent, "AB"), cfg(st, "A"), cfg(st, "")],
+            [(4, 2, 3), (4, 2, 2), (2, 1, 1), (0, 0, 0)])
+
+
+def ptmc_enthalpy_K(system, n_pairs, n_staples, n_stack):
+    """Enthalpy in K of a configuration (hybridization enthalpy + initiation
+    enthalpy + stacking), the quantity PTMC exchange weighs by 1/T
+    (ptmc_simulation.cpp:152-161, 275-313)."""
+    k = systems.KCAL_PER_MOL_TO_K
+    return n_pairs * system.binding_h + n_staples * 0.2 * k + n_stack * system.stacking_ene
+
+
+def exchange_system():
+    """The four-domain system with a large stacking energy, so that the two
+    configurations with two staples differ by 1 stacked pair = 30 000 K."""
+    s = systems.four_domain_system(max_total_staples=2)
+    s.max_type_staples = 2
+    s.stacking_ene = -30000.0
+    return s
+
+
+PT_TEMPS = [342.0, 343.0, 344.0, 345.0]
+
+# window exchange: numfulldomains windows at one temperature; window 2
+# excludes 0 fully bound pairs, so the bare scaffold cannot enter it
+WIN_TEMP = 345.0
+WIN_LIMITS = [(0, 4), (0, 4), (1, 4), (0, 4)]
+WIN_GRIDS = [[0.0, 0.3, -0.5, 0.8, 1.1], [0.2, -0.4, 0.9, 0.0, -1.3],
+             [0.7, 0.1, -0.2, 0.6, 0.4], [-0.8, 0.5, 0.3, -0.6, 0.0]]
+
+
+def windows():
+    return [abi.Window(abi.OP_NUMFULLDOMAINS, lo, hi, 0, 0.0, 0.0, 1.0, 0.0, 1.0)
+            for lo, hi in WIN_LIMITS]
+
+
+def window_stationary(ops, start):
+    """Exact stationary distribution of the window-exchange chain: every
+    permutation reachable from `start` by swaps of adjacent windows whose
+    order-parameter values lie inside each other's window
+    (us_simulation.cpp:790-815), weighted by prod_w exp(-G_w(op at w))."""
+    n = len(WIN_LIMITS)
+
+    def allowed(perm, i):
+        p1, p2 = ops[perm[i]], ops[perm[i + 1]]
+        return (WIN_LIMITS[i][0] <= p2 <= WIN_LIMITS[i][1]
+                and WIN_LIMITS[i + 1][0] <= p1 <= WIN_LIMITS[i + 1][1])
+    seen = {tuple(start)}
+    todo = [tuple(start)]
+    while todo:
+        p = todo.pop()
+        for i in range(n - 1):
+            if allowed(p, i):
+                q = list(p)
+                q[i], q[i + 1] = q[i + 1], q[i]
+                q = tuple(q)
+                if q not in seen:
+                    seen.add(q)
+                    todo.append(q)
+    lw = {p: -sum(WIN_GRIDS[w][ops[p[w]]] for w in range(n)) for p in seen}
+    m = max(lw.values())
+    z = sum(math.exp(v - m) for v in lw.values())
+    return {p: math.exp(v - m) / z for p, v in lw.items()}
+
+
+def ptmc_stationary(system, energies):
+    """Exact stationary distribution of the PTMC exchange chain over slot ->
+    replica permutations: prod_q exp(-H_{perm[q]} / T_q)."""
+    import itertools
+    n = len(PT_TEMPS)
+    lw = {p: -sum(ptmc_enthalpy_K(system, *energies[p[q]]) / PT_TEMPS[q] for q in range(n))
+          for p in itertools.permutations(range(n))}
+    m = max(lw.values())
+    z = sum(math.exp(v - m) for v in lw.values())
+    return {p: math.exp(v - m) / z for p, v in lw.items()}
+
+
+# ------------------------------------------------------ Met known answers
+# A two-domain scaffold (s0 at the origin with ore +z, s1 at +x with ore -z,
+# a stacked pair) and one staple type [1, 2].  Moveset: orientation 0.2,
+# staple exchange 0.8.  The tapes are the reference's RandomGens draws in
+# its order (simulation.cpp:655-665, met_movetypes.cpp:303-403,
+# movetypes.cpp:98-130, 343-370): move type, insert (< 0.5) or delete,
+# staple identity, growth domain, system domain, step direction and
+# orientation of the grown domain, acceptance uniform.
+
+MET_TEMP = 345.0
+
+
+def met_system(staple_M):
+    s = systems.four_domain_system(max_total_staples=1)
+    s.identities = [[-1, -2], [1, 2]]
+    s.chains = [systems.Chain(0, 0, [(0, 0, 0), (1, 0, 0)], [Z, neg(Z)])]
+    s.max_type_staples = 1
+    s.staple_M = staple_M
+    s.moves = [systems.Move(abi.MOVE_ORIENTATION_ROTATION, 0.2),
+               systems.Move(abi.MOVE_MET_STAPLE_EXCHANGE, 0.8)]
+    s.temps = [MET_TEMP]
+    return s
+
+
+def met_full_chains():
+    return [systems.Chain(0, 0, [(0, 0, 0), (1, 0, 0)], [Z, neg(Z)]),
+            systems.Chain(1, 1, [(0, 0, 0), (1, 0, 0)], [neg(Z), Z])]
+
+
+def met_delta_e():
+    """Energy of inserting the staple fully bound and stacked (kT)."""
+    return uniform_energy(met_system(1.0), MET_TEMP, 2, 1, 1)
+
+
+# insertion ratio L/6/N_new exp(-dE) sites c / n_bd (met_movetypes.cpp:212-253)
+# with L = 2, N_new = 1, sites = 2 (domains before insertion), n_bd = 2;
+# deletion ratio N 6/L exp(-dE_del) / (c (sites - L)) n_bd (:255-301) with
+# N = 1, sites = 4, dE_del = -dE.  staple_M is chosen so that the ratio is
+# `p` and a missing or doubled factor moves it across the acceptance draw.
+def met_insert_concentration(p):
+    return p / (2 / 6 / 1 * math.exp(-met_delta_e()) * 2 / 2)
+
+
+def met_delete_concentration(p):
+    return 1 * 6 / 2 * math.exp(met_delta_e()) / (4 - 2) * 2 / p
+
+
+def met_insert_tape(u):
+    # Met (0.5 in [0.2, 1)), insert (0.25), identity 1, staple domain 0 grows
+    # on system domain 0 (s0), the other domain steps +x (0) with ore +z (4)
+    return [bits(0.5), bits(0.25), 1, 0, 0, 0, 4, bits(u)]
+
+
+def met_delete_tape(u):
+    # Met, delete (0.75), identity 1, its staple 0, acceptance
+    return [bits(0.5), bits(0.75), 1, 0, bits(u)]

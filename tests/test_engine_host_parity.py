@@ -119,3 +119,15 @@ def test_reference_ctcb_rule_fails_alike_on_3d_system(sampler_cls, oracle_mod):
         with pytest.raises(abi.LdoError) as e:
             x.run(3, 500)
         assert e.value.code == abi.LDO_E_CONSTRAINT
+
+
+def test_hand_built_fourbody_configurations(sampler_cls, oracle_mod):
+    pc.case_hand_built_fourbody_configurations(sampler_cls, oracle_mod)
+
+
+def test_fixed_configuration_exchange_chains(sampler_cls, oracle_mod):
+    pc.case_fixed_configuration_exchange_chains(sampler_cls, oracle_mod)
+
+
+def test_met_known_answer_tapes(sampler_cls, oracle_mod):
+    pc.case_met_known_answer_tapes(sampler_cls, oracle_mod)

@@ -68,3 +68,15 @@ def test_reference_rng_tape_replay(sampler_cls, oracle_mod):
 
 def test_config_roundtrip_and_errors(sampler_cls, oracle_mod):
     pc.case_config_roundtrip_and_errors(sampler_cls, oracle_mod)
+
+
+def test_hand_built_fourbody_configurations(sampler_cls, oracle_mod):
+    pc.case_hand_built_fourbody_configurations(sampler_cls, oracle_mod)
+
+
+def test_fixed_configuration_exchange_chains(sampler_cls, oracle_mod):
+    pc.case_fixed_configuration_exchange_chains(sampler_cls, oracle_mod)
+
+
+def test_met_known_answer_tapes(sampler_cls, oracle_mod):
+    pc.case_met_known_answer_tapes(sampler_cls, oracle_mod)
