@@ -526,3 +526,82 @@ def case_met_known_answer_tapes(sampler_cls, oracle_mod):
             staples = gpu.observables(0)["num_staples"]
             expect = (1 if below else 0) if kind == "insert" else (0 if below else 1)
             assert staples == expect, (kind, below)
+
+
+# ------------------------------------------------ batches at the workload size
+BATCH_LENGTHS = [8, 9, 16, 24, 31, 40, 48, 57, 64, 72, 85, 96, 101, 113, 128, 128]
+
+
+def _batch_systems(lengths, t_lo=310.0, t_hi=350.0, seed=20261202):
+    import dataclasses
+    cache = {}
+    ss, ts = [], []
+    for k, L in enumerate(lengths):
+        if L not in cache:
+            cache[L] = systems.batch_chain_system(L)
+        t = t_lo + (t_hi - t_lo) * k / max(1, len(lengths) - 1)
+        ss.append(dataclasses.replace(cache[L], seed=seed + 7 * k, temps=[t]))
+        ts.append(t)
+    return ss, ts
+
+
+def _assert_chain(obs, ref, chain, ref_cfg, k, tag):
+    r = ref.observables(0)
+    for key in EXACT_KEYS:
+        assert obs[key] == r[key], (tag, k, key, obs[key], r[key])
+    for key in ("energy", "enthalpy", "entropy", "stacking_energy"):
+        assert obs[key] == pytest.approx(r[key], rel=ENERGY_RTOL, abs=1e-12), (tag, k, key)
+    assert chain == ref_cfg, (tag, k)
+
+
+def case_batch_lengths_8_to_128(batch_cls, oracle_mod, steps=2000):
+    """Chains across the workload's length range, two of them at L = 128 (the
+    LDS kernel then gives every workgroup the largest block of the
+    batch), each sweep kernel for `steps` steps against per-chain oracles."""
+    say(f"case: batch of {len(BATCH_LENGTHS)} chains, L = 8..128, {steps} steps on each of the "
+        "LDS, wavefront and lane kernels")
+    ss, ts = _batch_systems(BATCH_LENGTHS)
+    b = batch_cls(ss, ts)
+    refs = [oracle_mod.Oracle(s, temps=[t]) for s, t in zip(ss, ts)]
+    for kernel in (abi.BATCH_LDS, abi.BATCH_WAVE, abi.BATCH_LANE):
+        say(f"  kernel {kernel}")
+        b.set_kernel(kernel)
+        b.sweep(steps)
+        obs = b.observables()
+        for k, ref in enumerate(refs):
+            ref.sweep(steps)
+            _assert_chain(obs[k], ref, b.config(k), ref.config(0), k, kernel)
+    tot = b.counters()
+    sums = [ref.counters() for ref in refs]
+    assert list(tot.accepts) == [sum(c.accepts[i] for c in sums) for i in range(len(tot.accepts))]
+    for key in ("site_empty", "site_bind", "commits"):
+        assert getattr(tot, key) == sum(getattr(c, key) for c in sums), key
+
+
+def case_batch_full_workload(batch_cls, oracle_mod, steps=1000):
+    """The batching workload at its size: 4096 chains, L ~ U{8..128},
+    T ~ U(310, 350) K, autotuned kernel.  Every chain ends OK with the
+    moveset's attempt count; 16 chains spread over the batch (the longest
+    among them) are replayed on the oracle and must agree bit for bit."""
+    say(f"case: batch workload, 4096 chains, tune + {steps} steps, 16 chains replayed on the oracle")
+    ss, ts = systems.batch_systems()
+    b = batch_cls(ss, ts)
+    chosen, ms = b.tune(20)
+    say(f"  tuned kernel {chosen}: {ms}")
+    b.sweep(steps)
+    obs = b.observables()
+    assert all(o["status"] == 0 for o in obs)
+    n_steps = [o["step"] for o in obs]
+    assert min(n_steps) == max(n_steps) and n_steps[0] >= steps
+    tot = b.counters()
+    assert sum(tot.attempts) == sum(n_steps)
+    for o, s in zip(obs, ss):
+        assert 0 <= o["num_staples"] <= s.max_total_staples
+        assert o["num_fully_bound_domain_pairs"] + o["num_misbound_domain_pairs"] == \
+            o["num_bound_domain_pairs"]
+    longest = max(range(len(ss)), key=lambda k: len(ss[k].identities[0]))
+    sample = sorted(set(list(range(0, 4096, 273)) + [longest]))[:16]
+    for k in sample:
+        ref = oracle_mod.Oracle(ss[k], temps=[ts[k]])
+        ref.sweep(n_steps[k])
+        _assert_chain(obs[k], ref, b.config(k), ref.config(0), k, "batch4096")

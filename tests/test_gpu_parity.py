@@ -80,3 +80,13 @@ def test_fixed_configuration_exchange_chains(sampler_cls, oracle_mod):
 
 def test_met_known_answer_tapes(sampler_cls, oracle_mod):
     pc.case_met_known_answer_tapes(sampler_cls, oracle_mod)
+
+
+def test_batch_lengths_8_to_128(oracle_mod):
+    from latticednaorigami_amd.sampler import BatchSampler
+    pc.case_batch_lengths_8_to_128(BatchSampler, oracle_mod)
+
+
+def test_batch_full_workload(oracle_mod):
+    from latticednaorigami_amd.sampler import BatchSampler
+    pc.case_batch_full_workload(BatchSampler, oracle_mod)
