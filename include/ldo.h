@@ -224,6 +224,16 @@ const char* ldo_last_error(void* ctx);
  * origami_system.cpp:327-341). */
 int ldo_set_config(void* ctx, int32_t replica, const ldo_config* cfg);
 
+/* ldo_set_config without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops).  The configuration is staged in a static
+ * device array of LDO_NOALLOC_CONFIG_INTS int32 values per GPU (2 per chain
+ * + 6 per domain; 256 KiB, held until the process ends); a larger
+ * configuration returns LDO_E_ARG.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU race. */
+#define LDO_NOALLOC_CONFIG_INTS 65536
+int ldo_set_config_noalloc(void* ctx, int32_t replica, const ldo_config* cfg);
+
 /* Read one replica's configuration (OrigamiSystem::chains(),
  * origami_system.cpp:175-191).  cfg arrays must hold cap_chains chains and
  * cap_domains domains. */

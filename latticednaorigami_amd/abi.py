@@ -184,6 +184,8 @@ def declare_abi(lib, prefix):
 
 
 BATCH_LDS, BATCH_WAVE, BATCH_LANE = 0, 1, 2
+# int32 values staged by ldo_set_config_noalloc (2 per chain + 6 per domain)
+NOALLOC_CONFIG_INTS = 65536
 BATCH_KERNEL_NAMES = {BATCH_LDS: "ldo_batch_sweep_lds_kernel",
                       BATCH_WAVE: "ldo_batch_sweep_kernel (one chain per wavefront)",
                       BATCH_LANE: "ldo_batch_sweep_kernel (one chain per lane)"}
@@ -270,6 +272,8 @@ def load_library(path=None):
             "'import __graft_entry__ as g; g.build()')")
     lib = C.CDLL(p)
     declare_abi(lib, "ldo_")
+    lib.ldo_set_config_noalloc.restype = C.c_int
+    lib.ldo_set_config_noalloc.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Config)]
     lib.ldo_create.restype = C.c_int
     lib.ldo_create.argtypes = [C.POINTER(SystemDesc), C.POINTER(RunParams),
                                C.POINTER(Config), C.c_int32, _f64p,

@@ -294,15 +294,14 @@ std::vector<double> ladder_values(const Plan& p) {
     return xd;
 }
 
-void pack_config(const Plan& p, const ldo_config* cfg, std::vector<int32_t>& packed,
-                 int& n_domains) {
+int config_domains(const Plan& p, const ldo_config* cfg) {
     if (!cfg || cfg->n_chains < 1 || cfg->chain_identity[0] != 0) {
         throw AbiError {LDO_E_ARG, "configuration must start with the scaffold chain"};
     }
     if (cfg->n_chains - 1 > p.P.max_total_staples) {
         throw AbiError {LDO_E_ARG, "configuration holds more staples than max_total_staples"};
     }
-    n_domains = 0;
+    int n_domains = 0;
     for (int i = 0; i < cfg->n_chains; i++) {
         int id = cfg->chain_identity[i];
         if (id < 0 || id >= p.ntypes) {
@@ -316,6 +315,12 @@ void pack_config(const Plan& p, const ldo_config* cfg, std::vector<int32_t>& pac
             throw AbiError {LDO_E_ARG, "orientation is not a unit vector"};
         }
     }
+    return n_domains;
+}
+
+void pack_config(const Plan& p, const ldo_config* cfg, std::vector<int32_t>& packed,
+                 int& n_domains) {
+    n_domains = config_domains(p, cfg);
     size_t need = (size_t)2 * cfg->n_chains + (size_t)6 * n_domains;
     packed.assign(need, 0);
     std::copy(cfg->chain_index, cfg->chain_index + cfg->n_chains, packed.begin());

@@ -52,6 +52,8 @@ void slot_tables(const Plan& p, std::vector<double>& tab, std::vector<double>& i
 // ladder_t | ladder_u | ladder_s | reduced_u, as the exchange reads them
 std::vector<double> ladder_values(const Plan& p);
 
+// Checks a configuration against the plan and returns its domain count.
+int config_domains(const Plan& p, const ldo_config* cfg);
 // Validates a configuration and packs it as chain_index | chain_identity |
 // positions | orientations (three ints per domain).
 void pack_config(const Plan& p, const ldo_config* cfg, std::vector<int32_t>& packed,

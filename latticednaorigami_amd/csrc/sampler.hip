@@ -28,6 +28,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -325,6 +326,12 @@ extern "C" __global__ void ldo_batch_scalars_kernel(const Params* plist, int32_t
     out[k] = *(const RepScalars*)plist[k].blocks;
 }
 
+// Staging of ldo_set_config_noalloc: the packed configuration (chain
+// indices | identities | positions | orientations) of the latest call.  One
+// array per GPU, held until the process ends; host code reaches it through
+// the address hipGetSymbolAddress returns on that device, never by name.
+static __device__ __align__(256) int32_t scratch[LDO_NOALLOC_CONFIG_INTS];
+
 // ------------------------------------------------------------------- host
 namespace {
 
@@ -364,6 +371,7 @@ struct Ctx {
     int64_t* d_xcnt = nullptr;      // exchange attempts, accepts
     int32_t* d_cfg = nullptr;
     size_t cfg_cap = 0;
+    int32_t* scratch = nullptr;       // this device's `scratch` (ldo_set_config_noalloc)
     ldo_window* d_windows = nullptr;
     double* d_grid = nullptr;
     double* d_win_stored = nullptr;
@@ -546,6 +554,34 @@ void upload_config(Ctx& c, const ldo_config* cfg, int& n_domains) {
     upload(c, c.d_cfg, h.data(), h.size() * sizeof(int32_t), "copy cfg");
 }
 
+// Device address of `scratch` on the current device, looked up once per
+// device (at ldo_create, with that device current).
+std::mutex g_scratch_mu;
+int32_t* g_scratch_ptr[64] = {};
+
+int32_t* scratch_address(int dev) {
+    if (dev < 0 || dev >= 64) {
+        throw AbiError {LDO_E_HIP, "device index out of range for the no-alloc staging"};
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress scratch");
+        g_scratch_ptr[dev] = static_cast<int32_t*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Status word of one replica (no host allocation).
+void check_replica_status(Ctx& c, int replica) {
+    hipLaunchKernelGGL(ldo_status_kernel, dim3((c.R + 255) / 256), dim3(256), 0, c.stream, c.P,
+                       c.d_xint + 2 * c.R);
+    hip_check(hipGetLastError(), "launch status");
+    int32_t st = 0;
+    download(c, &st, c.d_xint + 2 * c.R + replica, sizeof st, "copy status");
+    ldo::check_status(st, replica);
+}
+
 void check_statuses(Ctx& c) {
     hipLaunchKernelGGL(ldo_status_kernel, dim3((c.R + 255) / 256), dim3(256), 0, c.stream, c.P,
                        c.d_xint + 2 * c.R);
@@ -649,6 +685,9 @@ int ldo_create(const ldo_system_desc* sys, const ldo_run_params* run, const ldo_
         c->R = n_replicas;
         Params& P = c->P;
         hip_check(hipStreamCreate(&c->stream), "stream");
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "get device");
+        c->scratch = scratch_address(dev);
         hip_check(hipMalloc(&c->d_type_len, c->plan.type_len.size() * sizeof(int32_t)),
                   "hipMalloc type_len");
         upload(*c, c->d_type_len, c->plan.type_len.data(),
@@ -708,6 +747,35 @@ const char* ldo_last_error(void* ctx) {
         return g_create_error.c_str();
     }
     return static_cast<Ctx*>(ctx)->err.c_str();
+}
+
+// The configuration goes into this device's `scratch` array instead of a
+// buffer of the context; no device memory is allocated or freed.
+int ldo_set_config_noalloc(void* ctx, int32_t replica, const ldo_config* cfg) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        if (replica < 0 || replica >= c->R) {
+            throw AbiError {LDO_E_ARG, "replica out of range"};
+        }
+        int nd = ldo::config_domains(c->plan, cfg);
+        int64_t n = cfg->n_chains;
+        if (2 * n + 6 * (int64_t)nd > LDO_NOALLOC_CONFIG_INTS) {
+            throw AbiError {LDO_E_ARG, "configuration larger than the no-alloc staging "
+                                       "(2 per chain + 6 per domain > LDO_NOALLOC_CONFIG_INTS)"};
+        }
+        int32_t* cf = c->scratch;
+        upload(*c, cf, cfg->chain_index, n * sizeof(int32_t), "stage chain indices");
+        upload(*c, cf + n, cfg->chain_identity, n * sizeof(int32_t), "stage identities");
+        upload(*c, cf + 2 * n, cfg->positions, 3 * (size_t)nd * sizeof(int32_t),
+               "stage positions");
+        upload(*c, cf + 2 * n + 3 * nd, cfg->orientations, 3 * (size_t)nd * sizeof(int32_t),
+               "stage orientations");
+        hipLaunchKernelGGL(ldo_init_kernel, grid_of(*c), dim3(64), 0, c->stream, c->P, cf, cf + n,
+                           cf + 2 * n, cf + 2 * n + 3 * nd, (int32_t)n, replica, 1,
+                           (const int32_t*)nullptr);
+        hip_check(hipGetLastError(), "launch set_config");
+        check_replica_status(*c, replica);
+    });
 }
 
 int ldo_set_config(void* ctx, int32_t replica, const ldo_config* cfg) {

@@ -149,6 +149,12 @@ class Sampler:
         cfg = config_to_abi(chains, keep)
         self._check(self.lib.ldo_set_config(self.h, r, C.byref(cfg)))
 
+    def set_config_noalloc(self, r, chains):
+        """set_config through ldo_set_config_noalloc (no device allocation)."""
+        keep = []
+        cfg = config_to_abi(chains, keep)
+        self._check(self.lib.ldo_set_config_noalloc(self.h, r, C.byref(cfg)))
+
     def stream(self):
         return self.lib.ldo_get_stream(self.h)
 
