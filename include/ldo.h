@@ -327,6 +327,12 @@ int ldo_learned_samples(void* ctx, int32_t cap, uint8_t* channels, int32_t* iden
  * 770-864): adjacent windows swap configurations. */
 int ldo_run(void* ctx, int64_t n_swaps, int64_t exchange_interval);
 
+/* The next exchange round pairs as the first of a run (swap_i = 1, odd
+ * pairs first), as PTMWUSGCMCSimulation::run_swaps restarts swap_i every
+ * iteration (us_simulation.cpp:662-706).  The exchange's random stream
+ * continues. */
+int ldo_restart_swaps(void* ctx);
+
 int ldo_get_observables(void* ctx, int32_t replica, ldo_obs* out);
 int ldo_get_counters(void* ctx, ldo_counters* out);
 

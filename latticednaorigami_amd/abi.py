@@ -272,6 +272,8 @@ def load_library(path=None):
             "'import __graft_entry__ as g; g.build()')")
     lib = C.CDLL(p)
     declare_abi(lib, "ldo_")
+    lib.ldo_restart_swaps.restype = C.c_int
+    lib.ldo_restart_swaps.argtypes = [C.c_void_p]
     lib.ldo_set_config_noalloc.restype = C.c_int
     lib.ldo_set_config_noalloc.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Config)]
     lib.ldo_create.restype = C.c_int

@@ -875,6 +875,9 @@ int run_mwus(Run& run, const ldo_system_desc& sd, ldo_run_params rp, const ldo_c
         }
         std::ofstream swp;
         Outputs::open(swp, base + "_iter-" + tag + ".swp");
+        // run_swaps counts swap_i from 1 in every iteration
+        // (us_simulation.cpp:662-706)
+        run.check(ldo_restart_swaps(run.ctx));
         run.check(ldo_get_permutation(run.ctx, q_before.data()));
         for (int w = 0; w < W; w++) {
             swp << q_before[w] << " ";
@@ -1087,6 +1090,15 @@ int main_run(int argc, char** argv) {
     bool pt = sim == "ut_parallel_tempering";
     if (!pt && sim != "constant_temp") {
         throw Fail(LDO_E_NOTIMPL, "simulation_type " + sim + " is not supported");
+    }
+    // The reference builds SystemBiases from the bias-function file for
+    // every simulation type (bias_functions.cpp:304-305) and every move adds
+    // them; here biases exist only as the umbrella windows of the MWUS
+    // drivers, so a biased constant-T or PTMC run is refused rather than run
+    // unbiased.
+    if (!p.str("bias_functions_file", "").empty()) {
+        throw Fail(LDO_E_NOTIMPL, "bias_functions_file is supported only with "
+                                  "mw_umbrella_sampling and ptmw_umbrella_sampling");
     }
     if (pt) {
         run.temps = p.list("temps");

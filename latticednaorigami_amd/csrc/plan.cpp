@@ -294,6 +294,14 @@ std::vector<double> ladder_values(const Plan& p) {
     return xd;
 }
 
+void check_learned_types(const Plan& p) {
+    for (int t = 1; t < p.ntypes; t++) {
+        if (p.type_len[t] < 2) {
+            throw AbiError {LDO_E_NOTIMPL, "the learned moves need staples of two or more domains"};
+        }
+    }
+}
+
 int config_domains(const Plan& p, const ldo_config* cfg) {
     if (!cfg || cfg->n_chains < 1 || cfg->chain_identity[0] != 0) {
         throw AbiError {LDO_E_ARG, "configuration must start with the scaffold chain"};

@@ -52,6 +52,9 @@ void slot_tables(const Plan& p, std::vector<double>& tab, std::vector<double>& i
 // ladder_t | ladder_u | ladder_s | reduced_u, as the exchange reads them
 std::vector<double> ladder_values(const Plan& p);
 
+// The learned moves place a staple end next to its chain neighbour: every
+// staple type needs at least two domains (LDO_E_NOTIMPL otherwise).
+void check_learned_types(const Plan& p);
 // Checks a configuration against the plan and returns its domain count.
 int config_domains(const Plan& p, const ldo_config* cfg);
 // Validates a configuration and packs it as chain_index | chain_identity |

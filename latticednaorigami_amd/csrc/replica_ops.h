@@ -138,8 +138,9 @@ struct XState {
 };
 
 // The exchange stream: Philox4x32-10 keyed by (seed, 0xFFFFFFFF), counter
-// (draw, 0, swap index); or the recorded draws of replica 0's tape.
-DEV double exchange_uniform(const Params& P, int64_t swap_i, uint32_t& idx) {
+// (draw, 0, exchange round of the context); or the recorded draws of replica
+// 0's tape.
+DEV double exchange_uniform(const Params& P, int64_t rng_ctr, uint32_t& idx) {
     if (P.rng_mode == 1) {
         RepScalars* S0 = (RepScalars*)P.blocks;
         int64_t pos = S0->tape_pos;
@@ -150,8 +151,8 @@ DEV double exchange_uniform(const Params& P, int64_t swap_i, uint32_t& idx) {
         S0->tape_pos = pos + 1;
         return bits_to_double(P.tapes[P.tape_off[0] + pos]);
     }
-    uint32_t c0 = idx++, c1 = 0u, c2 = (uint32_t)(uint64_t)swap_i,
-             c3 = (uint32_t)((uint64_t)swap_i >> 32);
+    uint32_t c0 = idx++, c1 = 0u, c2 = (uint32_t)(uint64_t)rng_ctr,
+             c3 = (uint32_t)((uint64_t)rng_ctr >> 32);
     uint32_t k0 = (uint32_t)P.seed, k1 = 0xFFFFFFFFu;
     for (int r = 0; r < 10; r++) {
         uint64_t p0 = (uint64_t)0xD2511F53u * c0;
@@ -173,8 +174,9 @@ DEV double exchange_uniform(const Params& P, int64_t swap_i, uint32_t& idx) {
 
 // OneDPTGCMCSimulation::attempt_exchange + calc_acceptance_p
 // (ptmc_simulation.cpp:255-313, 360-412).  The per-staple chemical
-// potential term runs over the staple types.
-DEV void exchange_round(const Params& P, const XState& X, int64_t swap_i) {
+// potential term runs over the staple types.  swap_i picks the even or odd
+// pairs; rng_ctr is the round's exchange-stream counter.
+DEV void exchange_round(const Params& P, const XState& X, int64_t swap_i, int64_t rng_ctr) {
     int R = P.n_replicas;
     int nst = P.L.ntypes - 1;
     uint32_t idx = 0;
@@ -199,7 +201,7 @@ DEV void exchange_round(const Params& P, const XState& X, int64_t swap_i) {
         double DBM = sm2 / t2 - sm1 / t1;
         double DBias = 0.0 * t2 - 0.0 * t1;
         double p = fmin(1.0, exp(DB * (DH + DBias) + DBM * Dst - dbu_dn));
-        bool accept = (p == 1) ? true : (p > exchange_uniform(P, swap_i, idx));
+        bool accept = (p == 1) ? true : (p > exchange_uniform(P, rng_ctr, idx));
         if (accept) {
             X.counts[1]++;
             X.q_to_rep[i] = r2;
@@ -226,8 +228,8 @@ DEV double exchange_grid(const Params& P, int q, int p) {
 // adjacent windows swap configurations when each window's order parameter
 // lies inside the other window.  Windows are ladder slots here, so a swap
 // exchanges the two replicas' slots; each window keeps its stored biases.
-DEV void exchange_window_group(const Params& P, const XState& X, int64_t swap_i, int g0, int g1,
-                               uint32_t& idx) {
+DEV void exchange_window_group(const Params& P, const XState& X, int64_t swap_i,
+                               int64_t rng_ctr, int g0, int g1, uint32_t& idx) {
     for (int i = g0 + (int)(swap_i % 2); i < g1 - 1; i += 2) {
         X.counts[0]++;
         const ldo_window& w1 = P.windows[i];
@@ -243,7 +245,7 @@ DEV void exchange_window_group(const Params& P, const XState& X, int64_t swap_i,
             double d1 = exchange_grid(P, i, p1) - exchange_grid(P, i + 1, p1);
             double d2 = exchange_grid(P, i + 1, p2) - exchange_grid(P, i, p2);
             double p = fmin(1.0, exp(d1 + d2));
-            accept = (p == 1) ? true : (p > exchange_uniform(P, swap_i, idx));
+            accept = (p == 1) ? true : (p > exchange_uniform(P, rng_ctr, idx));
         }
         if (accept) {
             X.counts[1]++;
@@ -256,7 +258,8 @@ DEV void exchange_window_group(const Params& P, const XState& X, int64_t swap_i,
 }
 
 // One exchange round over every window ladder.
-DEV void exchange_windows_round(const Params& P, const XState& X, int64_t swap_i) {
+DEV void exchange_windows_round(const Params& P, const XState& X, int64_t swap_i,
+                                int64_t rng_ctr) {
     int W = P.n_replicas;
     uint32_t idx = 0;
     // each run of consecutive slots at one temperature is a window ladder of
@@ -264,7 +267,7 @@ DEV void exchange_windows_round(const Params& P, const XState& X, int64_t swap_i
     for (int g0 = 0, g1; g0 < W; g0 = g1) {
         for (g1 = g0 + 1; g1 < W && X.ladder_t[g1] == X.ladder_t[g0]; g1++) {
         }
-        exchange_window_group(P, X, swap_i, g0, g1, idx);
+        exchange_window_group(P, X, swap_i, rng_ctr, g0, g1, idx);
     }
 }
 

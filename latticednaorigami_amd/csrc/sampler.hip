@@ -119,13 +119,14 @@ extern "C" __global__ void ldo_dependent_kernel(Params P) {
     }
 }
 
-extern "C" __global__ void ldo_exchange_kernel(Params P, XState X, int64_t swap_i) {
+extern "C" __global__ void ldo_exchange_kernel(Params P, XState X, int64_t swap_i,
+                                               int64_t rng_ctr) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         if (P.exchange_mode == LDO_EXCHANGE_WINDOWS) {
-            exchange_windows_round(P, X, swap_i);
+            exchange_windows_round(P, X, swap_i, rng_ctr);
         }
         else {
-            exchange_round(P, X, swap_i);
+            exchange_round(P, X, swap_i, rng_ctr);
         }
     }
 }
@@ -345,7 +346,8 @@ struct Ctx {
     int R = 0;
     std::vector<std::vector<uint64_t>> tapes;
     hipStream_t stream = nullptr;
-    int64_t swap_counter = 0;
+    int64_t swap_counter = 0; // exchange rounds so far (the exchange stream's counter)
+    int64_t swap_base = 0;    // swap_counter when the pairing index last restarted
     std::string err;
     // kernel timing (ldo_set_profiling): event pairs awaiting readout and a
     // pool of spent events for reuse
@@ -928,6 +930,7 @@ int ldo_learned_set_network(void* ctx, int32_t n_ident, const float* params, dou
         if (!params || n_ident < 1) {
             throw AbiError {LDO_E_ARG, "parameters and n_ident >= 1 required"};
         }
+        ldo::check_learned_types(c->plan);
         ldo_mlp_layout L;
         ldo::mlp_layout(n_ident, L);
         if (!(fallback >= 0.0 && fallback <= 1.0)) {
@@ -979,6 +982,7 @@ int ldo_learned_samples(void* ctx, int32_t cap, uint8_t* channels, int32_t* iden
         if (cap < 0 || !channels || !ident || !legal || !target || !n_out) {
             throw AbiError {LDO_E_ARG, "buffers and cap >= 0 required"};
         }
+        ldo::check_learned_types(c->plan);
         int per = 2 * c->P.max_total_staples;
         size_t n = (size_t)c->R * per;
         uint8_t *d_ch = nullptr, *d_lg = nullptr;
@@ -1051,16 +1055,22 @@ int ldo_run(void* ctx, int64_t n_swaps, int64_t interval) {
         X.reduced_u = c->d_xdbl + 3 * c->R;
         X.counts = c->d_xcnt;
         for (int64_t s = 0; s < n_swaps; s++) {
-            int64_t swap_i = ++c->swap_counter;
+            int64_t rng_ctr = ++c->swap_counter;
+            int64_t swap_i = rng_ctr - c->swap_base;
             launch_sweep(*c, interval, X.rep_slot);
             launch_timed(*c, 1, [&]() {
                 hipLaunchKernelGGL(ldo_exchange_kernel, dim3(1), dim3(64), 0, c->stream, c->P, X,
-                                   swap_i);
+                                   swap_i, rng_ctr);
             });
             hip_check(hipGetLastError(), "launch exchange");
         }
         check_statuses(*c);
     });
+}
+
+int ldo_restart_swaps(void* ctx) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() { c->swap_base = c->swap_counter; });
 }
 
 int ldo_get_observables(void* ctx, int32_t replica, ldo_obs* out) {

@@ -48,6 +48,10 @@ class Sampler:
     def run(self, n_swaps, interval):
         self._check(self.lib.ldo_run(self.h, n_swaps, interval))
 
+    def restart_swaps(self):
+        """The next exchange round pairs as the first of a run (swap_i = 1)."""
+        self._check(self.lib.ldo_restart_swaps(self.h))
+
     def set_ladder(self, temps, u=None, b=None, s=None):
         n = len(temps)
         one = [1.0] * n

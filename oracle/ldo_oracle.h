@@ -59,6 +59,8 @@ int oracle_set_config(void* o, int32_t replica, const ldo_config* cfg);
 
 /* Tape recording (mt19937_64 mode): every draw of replica r is appended to
  * its tape; the tape is what ldo_set_tape replays on the GPU. */
+// the next exchange round pairs as the first of a run (ldo_restart_swaps)
+int oracle_restart_swaps(void* o);
 // replica `replica` replays these draws (as ldo_set_tape does)
 int oracle_set_tape(void* o, int32_t replica, const uint64_t* values, int64_t n);
 int oracle_record_tape(void* o, int32_t replica, int32_t on);

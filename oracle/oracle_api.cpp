@@ -26,7 +26,8 @@ struct Oracle {
     uint64_t seed {0};
     std::vector<float> net; // learned proposal parameters
     Rng exchange_rng;
-    int64_t swap_counter {0};
+    int64_t swap_counter {0}; // exchange rounds so far (the exchange stream's counter)
+    int64_t swap_base {0};    // swap_counter when the pairing index last restarted
     ldo_counters counters {};
     Limits lim {0, 0};
     int exchange_mode {0};
@@ -291,13 +292,14 @@ int oracle_sweep(void* o, int64_t steps) {
 static void run_windows(Oracle* O, int64_t n_swaps, int64_t interval) {
     int W = (int)O->reps.size();
     for (int64_t s = 0; s < n_swaps; s++) {
-        int64_t swap_i = ++O->swap_counter;
+        int64_t rng_ctr = ++O->swap_counter;
+        int64_t swap_i = rng_ctr - O->swap_base;
         for (auto& rep: O->reps) {
             rep->simulate(interval);
         }
         Rng& xr = (O->rng_kind == 1) ? O->reps[0]->rng : O->exchange_rng;
         if (O->rng_kind != 1) {
-            xr.begin_step(swap_i);
+            xr.begin_step(rng_ctr);
         }
         // each run of consecutive slots at one temperature is its own window
         // ladder (one reference PTMWUS run), paired from its first slot
@@ -427,7 +429,8 @@ int oracle_run(void* o, int64_t n_swaps, int64_t interval) {
         int R = (int)O->reps.size();
         std::vector<int> slot_of(R);
         for (int64_t s = 0; s < n_swaps; s++) {
-            int64_t swap_i = ++O->swap_counter;
+            int64_t rng_ctr = ++O->swap_counter;
+            int64_t swap_i = rng_ctr - O->swap_base;
             for (int q = 0; q < R; q++) {
                 slot_of[O->q_to_rep[q]] = q;
             }
@@ -445,7 +448,7 @@ int oracle_run(void* o, int64_t n_swaps, int64_t interval) {
             }
             Rng& xr = (O->rng_kind == 1) ? O->reps[0]->rng : O->exchange_rng;
             if (O->rng_kind != 1) {
-                xr.begin_step(swap_i);
+                xr.begin_step(rng_ctr);
             }
             int swap_set = (int)(swap_i % 2);
             for (int i = swap_set; i < R - 1; i += 2) {
@@ -559,6 +562,10 @@ int oracle_set_config(void* o, int32_t r, const ldo_config* cfg) {
 
 int oracle_record_tape(void* o, int32_t r, int32_t on) {
     ORACLE_TRY(o, { O->reps.at(r)->rng.set_record(on != 0); })
+}
+
+int oracle_restart_swaps(void* o) {
+    ORACLE_TRY(o, { O->swap_base = O->swap_counter; })
 }
 
 int oracle_set_tape(void* o, int32_t r, const uint64_t* values, int64_t n) {

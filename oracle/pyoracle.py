@@ -213,6 +213,11 @@ class Oracle:
         cfg = config_to_abi(chains, keep)
         _check(self.L, self.h, self.L.oracle_set_config(self.h, r, C.byref(cfg)))
 
+    def restart_swaps(self):
+        """The next exchange round pairs as the first of a run."""
+        self.L.oracle_restart_swaps.argtypes = [C.c_void_p]
+        _check(self.L, self.h, self.L.oracle_restart_swaps(self.h))
+
     def set_tape(self, r, values):
         """Replica r replays these draws (doubles as bit patterns)."""
         buf = (C.c_uint64 * max(1, len(values)))(*values)

@@ -29,6 +29,7 @@ struct HostCtx {
     Params P {};
     int R = 0;
     int64_t swap_counter = 0;
+    int64_t swap_base = 0;
     std::string err;
     char* blocks = nullptr;
     std::vector<double> tab, init, xdbl;
@@ -316,6 +317,7 @@ int eh_learned_set_network(void* ctx, int32_t n_ident, const float* params, doub
             !(fallback >= 0.0 && fallback <= 1.0) || c->P.rng_mode != 0) {
             throw ldo::AbiError {LDO_E_ARG, "bad learned-move arguments"};
         }
+        ldo::check_learned_types(c->plan);
         ldo_mlp_layout L;
         ldo::mlp_layout(n_ident, L);
         c->net.assign(params, params + L.total);
@@ -347,6 +349,7 @@ int eh_learned_samples(void* ctx, int32_t cap, uint8_t* channels, int32_t* ident
                        uint8_t* legal, int32_t* target, int32_t* n_out) {
     HostCtx* c = static_cast<HostCtx*>(ctx);
     return guarded(c, [&]() {
+        ldo::check_learned_types(c->plan);
         int32_t k = 0;
         for (int r = 0; r < c->R; r++) {
             Eng e;
@@ -399,19 +402,25 @@ int eh_run(void* ctx, int64_t n_swaps, int64_t interval) {
         X.reduced_u = c->xdbl.data() + 3 * c->R;
         X.counts = c->xcnt.data();
         for (int64_t s = 0; s < n_swaps; s++) {
-            int64_t swap_i = ++c->swap_counter;
+            int64_t rng_ctr = ++c->swap_counter;
+            int64_t swap_i = rng_ctr - c->swap_base;
             for (int r = 0; r < c->R; r++) {
                 sweep_replica(c->P, r, interval, X.rep_slot);
             }
             if (c->P.exchange_mode == LDO_EXCHANGE_WINDOWS) {
-                exchange_windows_round(c->P, X, swap_i);
+                exchange_windows_round(c->P, X, swap_i, rng_ctr);
             }
             else {
-                exchange_round(c->P, X, swap_i);
+                exchange_round(c->P, X, swap_i, rng_ctr);
             }
         }
         check_statuses(*c);
     });
+}
+
+int eh_restart_swaps(void* ctx) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() { c->swap_base = c->swap_counter; });
 }
 
 int eh_get_observables(void* ctx, int32_t replica, ldo_obs* out) {

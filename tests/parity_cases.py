@@ -605,3 +605,19 @@ def case_batch_full_workload(batch_cls, oracle_mod, steps=1000):
         ref = oracle_mod.Oracle(ss[k], temps=[ts[k]])
         ref.sweep(n_steps[k])
         _assert_chain(obs[k], ref, b.config(k), ref.config(0), k, "batch4096")
+
+
+def case_learned_rejects_single_domain_staples(sampler_cls, oracle_mod):
+    """The learned moves place a staple end next to its chain neighbour; a
+    staple type of one domain has none, so both learned entry points refuse
+    such a system instead of reading another chain's slot."""
+    say("case: learned moves refuse single-domain staple types")
+    s = systems.four_domain_system()
+    s.identities = [[-1, -2, -3, -4], [1], [2, 3]]
+    gpu = sampler_cls(s, temps=[330.0])
+    with pytest.raises(abi.LdoError) as e:
+        gpu.learned_set_network(4, random_network(4, 1))
+    assert e.value.code == abi.LDO_E_NOTIMPL
+    with pytest.raises(abi.LdoError) as e:
+        gpu.learned_samples(16)
+    assert e.value.code == abi.LDO_E_NOTIMPL

@@ -104,7 +104,13 @@ def test_driver_rejects_unsupported_inputs(tmp_path):
         pytest.skip("driver not built")
     s = systems.four_domain_system()
     for extra, msg in (({"domain_type": "Halfturn"}, "domain_type Halfturn"),
-                       ({"simulation_type": "annealing"}, "simulation_type annealing")):
+                       ({"simulation_type": "annealing"}, "simulation_type annealing"),
+                       # biases outside the umbrella-sampling drivers: refused,
+                       # not silently dropped
+                       ({"bias_functions_file": "biases.json"}, "bias_functions_file"),
+                       ({"simulation_type": "ut_parallel_tempering", "num_reps": "2",
+                         "temps": "330 340", "bias_functions_file": "biases.json"},
+                        "bias_functions_file")):
         inp = write_inputs(str(tmp_path), s, {"simulation_type": "constant_temp", **extra})
         p = subprocess.run([DRIVER, inp], capture_output=True, text=True, timeout=60)
         assert p.returncode == 1 and msg in p.stderr, p.stderr
@@ -321,11 +327,13 @@ def test_driver_mwus_iterations_match_oracle(tmp_path, oracle_mod):
 @pytest.mark.gpu
 def test_driver_ptmwus_iterations_match_oracle(tmp_path, oracle_mod):
     """Three numstaples windows exchanging configurations every 500 steps
-    (ptmw_umbrella_sampling): equilibration, then two iterations of 4 swaps.
+    (ptmw_umbrella_sampling): equilibration, then two iterations of 3 swaps.
     Each window's energies come from the configuration it held before the
     round's swap, its bias column from its own stored value; the swap file
     lists each window's configuration; the weights divide the visits by the
-    2000 steps of the iteration (run_swaps counts no extra step)."""
+    1500 steps of the iteration (run_swaps counts no extra step).  The odd
+    swap count checks that every iteration pairs from swap_i = 1 again
+    (us_simulation.cpp:670) while the exchange's random stream continues."""
     import numpy as np
     sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                                     "oracle"))
@@ -334,7 +342,7 @@ def test_driver_ptmwus_iterations_match_oracle(tmp_path, oracle_mod):
     d = str(tmp_path)
     windows = [(0, 2), (0, 3), (1, 4)]
     run_driver(write_mwus_inputs(d, s, windows, {
-        "simulation_type": "ptmw_umbrella_sampling", "equil_steps": 1000, "iter_swaps": 4,
+        "simulation_type": "ptmw_umbrella_sampling", "equil_steps": 1000, "iter_swaps": 3,
         "exchange_interval": 500, "max_num_iters": 2, "max_D_bias": 1.0,
         "energies_output_freq": 500, "configs_output_freq": 1000}))
     W = len(windows)
@@ -350,9 +358,11 @@ def test_driver_ptmwus_iterations_match_oracle(tmp_path, oracle_mod):
     us = [us_ref.WindowState() for _ in range(W)]
     summaries = [""] * W
     swapped = False
+    K = 3
     for n in range(2):
+        ref.restart_swaps()
         swp = [" ".join(str(q) for q in ref.permutation()) + " "]
-        for k in range(1, 5):
+        for k in range(1, K + 1):
             before = ref.permutation()
             ref.run(1, 500)
             after = ref.permutation()
@@ -361,14 +371,15 @@ def test_driver_ptmwus_iterations_match_oracle(tmp_path, oracle_mod):
                 e, _, _, _ = expect_lines(ref, before[w], s, 500 * k)
                 e = e.rsplit(" ", 2)[0] + " " + g10(ref.observables(after[w])["bias"]) + " "
                 assert read(d, f"{post[w]}_iter-{n}.ene")[k] == e, (n, k, w)
-            if k % 2 == 0:
+            if (500 * k) % 1000 == 0:
                 swp.append(" ".join(str(q) for q in before) + " ")
-        swp.append(" ".join(str(q) for q in ref.permutation()) + " ")
+        if (500 * K) % 1000 == 0:
+            swp.append(" ".join(str(q) for q in ref.permutation()) + " ")
         assert read(d, f"out_iter-{n}.swp")[:-1] == swp, n
         h = ref.histogram(reset=True)
         for w in range(W):
             us_ref.process_iteration(us[w], [int(x) for x in h[w, :glen]], us[w].grid(glen),
-                                     2000, 1.0)
+                                     500 * K, 1.0)
             summaries[w] += us_ref.summary(us[w], n)
             with open(os.path.join(d, f"{post[w]}_iter-{n}.biases")) as f:
                 assert f.read() == us_ref.weights_json(us[w]), (n, w)

@@ -20,7 +20,7 @@ HOST_DIR = os.path.join(HERE, "engine_host")
 HOST_LIB = os.path.join(HOST_DIR, "_build", "libldo_engine_host.so")
 NAMES = ["create", "destroy", "last_error", "set_config", "get_config", "set_ladder", "set_tape",
          "set_windows", "sweep", "run", "get_observables", "get_counters", "get_permutation",
-         "set_window_grid", "set_histogram", "get_histogram"]
+         "set_window_grid", "set_histogram", "get_histogram", "restart_swaps"]
 
 
 class HostLib:
@@ -34,6 +34,8 @@ class HostLib:
                                    C.POINTER(C.c_void_p)]
         cdll.eh_set_tape.restype = C.c_int
         cdll.eh_set_tape.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64), C.c_int64]
+        cdll.eh_restart_swaps.restype = C.c_int
+        cdll.eh_restart_swaps.argtypes = [C.c_void_p]
         abi.declare_learned_abi(cdll, "eh_")
         for n in NAMES + ["learned_set_network", "learned_sweep", "learned_samples"]:
             setattr(self, "ldo_" + n, getattr(cdll, "eh_" + n))
@@ -131,3 +133,7 @@ def test_fixed_configuration_exchange_chains(sampler_cls, oracle_mod):
 
 def test_met_known_answer_tapes(sampler_cls, oracle_mod):
     pc.case_met_known_answer_tapes(sampler_cls, oracle_mod)
+
+
+def test_learned_rejects_single_domain_staples(sampler_cls, oracle_mod):
+    pc.case_learned_rejects_single_domain_staples(sampler_cls, oracle_mod)

@@ -90,3 +90,7 @@ def test_batch_lengths_8_to_128(oracle_mod):
 def test_batch_full_workload(oracle_mod):
     from latticednaorigami_amd.sampler import BatchSampler
     pc.case_batch_full_workload(BatchSampler, oracle_mod)
+
+
+def test_learned_rejects_single_domain_staples(sampler_cls, oracle_mod):
+    pc.case_learned_rejects_single_domain_staples(sampler_cls, oracle_mod)
