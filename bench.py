@@ -31,10 +31,37 @@ HBM_PEAK_GBS = 8000.0
 BYTES_EMPTY, BYTES_BIND, BYTES_COMMIT, BYTES_MOVE = 24, 216, 32, 16
 
 
-# HBM bytes per 1000-step sweep launch of the raster workload, measured with
-# rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a copy of known size
-# (profiles/ldo_sweep_lds_kernel_raster64_ptmc32_pmc.md)
-RASTER_TRAFFIC_PER_1000 = 7.78e6
+# LDS bandwidth: 128 B/clk/CU at 2.4 GHz; the chip's 256 CUs, or only the CUs
+# a launch occupies (one workgroup per replica on a CU of its own)
+LDS_PEAK_GBS_PER_CU = 128 * 2.4
+N_CU = 256
+# the rocprofv3 FETCH_SIZE / WRITE_SIZE passes of the raster sweep, scaled by
+# a copy of known size; bench.py reads the measured HBM bytes per 1000-step
+# launch from the summary's "traffic_bytes_per_launch" line
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "ldo_sweep_lds_kernel_raster64_ptmc32_pmc.md")
+
+
+def profiled_traffic(path=TRAFFIC_PROFILE):
+    """HBM bytes per launch from a committed counter summary, or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1].split()[0])
+    except OSError:
+        return None
+    return None
+
+
+def host_cores():
+    """CPUs this process may run on (its affinity set), capped by the job
+    limit the host sets for build and worker pools where it sets one."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    for var in ("MAX_JOBS", "OMP_NUM_THREADS"):
+        v = os.environ.get(var, "")
+        if v.isdigit() and int(v) > 0:
+            n = min(n, int(v))
+    return max(1, n)
 
 
 def algorithmic_bytes(c0, c1):
@@ -47,33 +74,45 @@ def algorithmic_bytes(c0, c1):
 
 
 def _cpu_worker(args):
-    """One process of the CPU baseline: the oracle on the same workload."""
-    swaps, interval = args
+    """One process of the CPU baseline: the oracle on its share of the
+    (replica, seed) items of the raster workload, each replica at its ladder
+    temperature (as one MPI rank of the reference runs one replica)."""
+    items, steps, untimed = args
     sys.path.insert(0, ROOT)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import dataclasses
     import pyoracle
     from latticednaorigami_amd import systems
-    system = systems.raster_system()
-    ref = pyoracle.Oracle(system, temps=system.temps)
-    # one untimed interval: every ladder slot builds its energy tables on its
-    # first visit, a one-off cost of about a second
-    ref.run(1, interval)
-    c0 = ref.counters()
+    base = systems.raster_system()
+    refs = []
+    for r, seed in items:
+        s = dataclasses.replace(base, seed=base.seed + 1000 * seed)
+        refs.append(pyoracle.Oracle(s, temps=[base.temps[r]]))
+    for ref in refs:
+        ref.sweep(untimed)
+    c0 = [ref.counters() for ref in refs]
     t0 = time.perf_counter()
-    ref.run(swaps, interval)
+    for ref in refs:
+        ref.sweep(steps)
     dt = time.perf_counter() - t0
-    c1 = ref.counters()
-    return sum(c1.accepts) - sum(c0.accepts), sum(c1.attempts) - sum(c0.attempts), dt
+    acc = sum(sum(ref.counters().accepts) - sum(c.accepts) for ref, c in zip(refs, c0))
+    att = sum(sum(ref.counters().attempts) - sum(c.attempts) for ref, c in zip(refs, c0))
+    return acc, att, dt
 
 
-def cpu_baseline(workers, swaps, interval):
+def cpu_baseline(workers, steps, untimed, repeats, n_rep=32):
+    """The oracle on `workers` concurrent processes over the same window of
+    each replica's chain as the GPU line (`untimed` steps, then `steps`
+    timed), for the 32 replicas x `repeats` seeds."""
     from concurrent.futures import ProcessPoolExecutor
     import multiprocessing as mp
     ctx = mp.get_context("spawn")
+    items = [(r, k) for k in range(repeats) for r in range(n_rep)]
+    shares = [items[w::workers] for w in range(workers)]
     with ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as ex:
-        # every worker builds its system outside its own timed region; the
+        # every worker builds its systems outside its own timed region; the
         # aggregate is the sum of the concurrent per-process rates
-        res = list(ex.map(_cpu_worker, [(swaps, interval)] * workers))
+        res = list(ex.map(_cpu_worker, [(s, steps, untimed) for s in shares]))
     accepted = sum(r[0] for r in res)
     rate = sum(r[0] / r[2] for r in res)
     longest = max(r[2] for r in res)
@@ -81,11 +120,14 @@ def cpu_baseline(workers, swaps, interval):
         "value": rate,
         "unit": "moves/s",
         "cores": workers,
+        "host_cpus": os.cpu_count(),
         "kind": "port",
-        "sample": (f"{workers} concurrent processes, each the 32-replica PTMC raster workload "
-                   f"for {swaps} exchange interval(s) of {interval} steps per replica after one "
-                   f"untimed interval from the initial configuration ({accepted} accepted moves, "
-                   f"longest timed region {longest:.1f} s)"),
+        "sample": (f"{workers} concurrent processes over the 32 replicas of the raster workload "
+                   f"x {repeats} seeds (each replica at its ladder temperature, as one MPI rank "
+                   f"of the reference runs one replica; exchanges, one per 1000 steps, left "
+                   f"out): steps {untimed}..{untimed + steps} of every chain, the GPU line's "
+                   f"timed window ({accepted} accepted moves, longest timed region "
+                   f"{longest:.1f} s)"),
     }
 
 
@@ -125,6 +167,7 @@ def cpu_baseline_batch(workers, chains_per_worker, steps):
         "value": rate,
         "unit": "moves/s",
         "cores": workers,
+        "host_cpus": os.cpu_count(),
         "kind": "port",
         "sample": (f"{workers} concurrent processes, each {chains_per_worker} chains of the "
                    f"4096-chain batch (every {stride}-th chain, all lengths represented) for "
@@ -216,6 +259,7 @@ def cpu_baseline_mwus3d(workers, reps_per_worker, swaps, interval):
         "value": rate,
         "unit": "moves/s",
         "cores": workers,
+        "host_cpus": os.cpu_count(),
         "kind": "port",
         "sample": (f"{workers} concurrent processes, each {reps_per_worker} consecutive windows "
                    f"of the 256 (64 windows x 4 T) for {swaps} exchange interval(s) of "
@@ -391,17 +435,30 @@ def gpu_leg(args, system):
     nbytes = algorithmic_bytes(c0, c1)
     kernel_s = prof["sweep_ms"] / 1e3
     achieved = nbytes / kernel_s / 1e9 if kernel_s > 0 else None
+    traffic = profiled_traffic() if args.interval == 1000 else None
+    n_cu = min(N_CU, len(system.temps))
     roofline = {
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS if achieved is not None else None,
-        "traffic": RASTER_TRAFFIC_PER_1000 if args.interval == 1000 else None,
+        "traffic": traffic,
+        "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
         "kernel": smp.lib.ldo_kernel_name(smp.h).decode(),
         "launches": prof["sweep_launches"],
         "avg_launch_ms": prof["sweep_ms"] / max(1, prof["sweep_launches"]),
         "algorithmic_bytes_per_launch": nbytes / max(1, prof["sweep_launches"]),
+        # the replica state is LDS-resident: the same bytes priced against LDS
+        # bandwidth, of the chip and of the CUs the launch occupies
+        "lds": {
+            "achieved": achieved,
+            "peak_chip": LDS_PEAK_GBS_PER_CU * N_CU,
+            "frac_chip": achieved / (LDS_PEAK_GBS_PER_CU * N_CU) if achieved else None,
+            "peak_occupied_cus": LDS_PEAK_GBS_PER_CU * n_cu,
+            "frac_occupied_cus": achieved / (LDS_PEAK_GBS_PER_CU * n_cu) if achieved else None,
+            "occupied_cus": n_cu,
+        },
     }
     extra = {
         "attempted_per_s": attempted / wall,
@@ -419,10 +476,12 @@ def main():
     ap.add_argument("--steps", type=int, default=10, help="timed exchange intervals")
     ap.add_argument("--warmup", type=int, default=2, help="untimed exchange intervals")
     ap.add_argument("--interval", type=int, default=1000, help="MC steps per replica per step")
-    ap.add_argument("--cpu-steps", type=int, default=20000,
-                    help="MC steps per replica in each CPU-baseline process")
+    ap.add_argument("--cpu-repeats", type=int, default=16,
+                    help="raster: seeds per replica in the CPU baseline (each runs the GPU "
+                         "line's warm-up and timed steps)")
     ap.add_argument("--cpu-workers", type=int, default=0,
-                    help="CPU-baseline processes (0: min(8, host cores))")
+                    help="CPU-baseline processes (0: every available host core, at most one "
+                         "per replica)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="raster64_ptmc32",
                     choices=["raster64_ptmc32", "batch4096", "mwus3d", "learned", "hbm_copy"],
@@ -508,8 +567,9 @@ def main():
         line["error"] = f"{type(e).__name__}: {e}"
         status = 1
     if not args.no_cpu_baseline:
-        workers = args.cpu_workers or min(8, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(workers, 1, args.cpu_steps)
+        workers = args.cpu_workers or min(32, host_cores())
+        line["cpu_baseline"] = cpu_baseline(workers, args.steps * args.interval,
+                                            args.warmup * args.interval, args.cpu_repeats)
     print(json.dumps(line), flush=True)
     sys.exit(status)
 
@@ -550,7 +610,7 @@ def main_batch(args):
         line["error"] = f"{type(e).__name__}: {e}"
         status = 1
     if not args.no_cpu_baseline:
-        workers = args.cpu_workers or min(8, os.cpu_count() or 1)
+        workers = args.cpu_workers or min(32, host_cores())
         line["cpu_baseline"] = cpu_baseline_batch(workers, args.cpu_chains, args.cpu_batch_steps)
     print(json.dumps(line), flush=True)
     sys.exit(status)
@@ -598,7 +658,7 @@ def main_mwus3d(args):
         line["error"] = f"{type(e).__name__}: {e}"
         status = 1
     if not args.no_cpu_baseline:
-        workers = args.cpu_workers or min(8, os.cpu_count() or 1)
+        workers = args.cpu_workers or min(32, host_cores())
         line["cpu_baseline"] = cpu_baseline_mwus3d(workers, 8, args.cpu_swaps, args.interval)
     print(json.dumps(line), flush=True)
     sys.exit(status)
