@@ -186,6 +186,8 @@ typedef struct ldo_obs {
     int32_t num_unassigned_domains;
     int32_t status; /* 0, or LDO_E_CONSTRAINT after a failed constraint check */
     double bias;    /* the replica's move-level bias (kT) under its window */
+    int32_t last_move;     /* moveset slot of the replica's last MC step */
+    int32_t last_accepted; /* whether that move was accepted */
 } ldo_obs;
 
 #define LDO_MAX_MOVETYPES 8
@@ -332,6 +334,21 @@ int ldo_run(void* ctx, int64_t n_swaps, int64_t exchange_interval);
  * iteration (us_simulation.cpp:662-706).  The exchange's random stream
  * continues. */
 int ldo_restart_swaps(void* ctx);
+
+/* Per-move tracking counts of one replica, as the movetypes'
+ * write_log_summary report them in the .moves file (MCMovetype::add_tracker,
+ * movetypes.cpp:41-51; met_movetypes.cpp:145-190; cb_movetypes.cpp:264-290,
+ * 626-678).  With T = number of chain identities (scaffold included, index =
+ * identity), ns scaffold domains and S = max_total_staples, out holds
+ *   MetStapleExchange insertion attempts[T], accepts[T],
+ *                     deletion attempts[T], accepts[T],
+ *   CBStapleRegrowth attempts[T], accepts[T] (moves with staples present),
+ *   CTCBScaffoldRegrowth attempts[ns+1], accepts[ns+1] by segment length,
+ *                        attempts[S+1], accepts[S+1] by staples regrown;
+ * 6T + 2(ns+1) + 2(S+1) values in all, written to *n_out (cap must hold
+ * them).  Counts are per move kind, summed over moveset slots of that kind. */
+int ldo_get_move_tracking(void* ctx, int32_t replica, int64_t* out, int64_t cap,
+                          int64_t* n_out);
 
 int ldo_get_observables(void* ctx, int32_t replica, ldo_obs* out);
 int ldo_get_counters(void* ctx, ldo_counters* out);

@@ -118,6 +118,8 @@ class Obs(C.Structure):
         ("num_unassigned_domains", C.c_int32),
         ("status", C.c_int32),
         ("bias", C.c_double),
+        ("last_move", C.c_int32),
+        ("last_accepted", C.c_int32),
     ]
 
     def as_dict(self):
@@ -158,6 +160,22 @@ class LdoError(RuntimeError):
         self.code = code
 
 
+def split_tracking(values, n_types, n_scaffold, max_staples):
+    """The ldo_get_move_tracking array as named per-index lists."""
+    T, n1, s1 = n_types, n_scaffold + 1, max_staples + 1
+    v = list(values)
+    out, o = {}, 0
+    for name, n in (("met_insert_attempts", T), ("met_insert_accepts", T),
+                    ("met_delete_attempts", T), ("met_delete_accepts", T),
+                    ("cb_attempts", T), ("cb_accepts", T),
+                    ("ctcb_length_attempts", n1), ("ctcb_length_accepts", n1),
+                    ("ctcb_staples_attempts", s1), ("ctcb_staples_accepts", s1)):
+        out[name] = v[o:o + n]
+        o += n
+    assert o == len(v), (o, len(v))
+    return out
+
+
 def declare_abi(lib, prefix):
     """Attach argtypes/restype for the ldo_* (or oracle_*) entry points."""
     v = C.c_void_p
@@ -176,6 +194,8 @@ def declare_abi(lib, prefix):
         "set_window_grid": (C.c_int, [v, _f64p]),
         "set_histogram": (C.c_int, [v, C.c_int32]),
         "get_histogram": (C.c_int, [v, C.POINTER(C.c_int64), C.c_int32]),
+        "get_move_tracking": (C.c_int, [v, C.c_int32, C.POINTER(C.c_int64), C.c_int64,
+                                        C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, prefix + name)

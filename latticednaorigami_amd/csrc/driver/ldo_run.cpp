@@ -277,6 +277,8 @@ struct Chain {
 struct Run {
     Params p;
     std::vector<std::vector<int>> identities;
+    std::vector<std::string> move_labels; // the moveset's "label" fields
+    std::vector<int32_t> move_kinds;
     int max_staple_size = 2, max_total_staples = 999, n_scaffold = 0;
     void* ctx = nullptr;
     int R = 1;
@@ -417,6 +419,145 @@ bool due(long long freq, long long step) { return freq != 0 && step % freq == 0;
 // Outputs of replica r at `step`; bias_rep >= 0 takes the bias column from
 // that replica instead (the replica that now holds the window whose stored
 // bias the record must show, after a window exchange)
+// GCMCSimulation::write_log_entry (simulation.cpp:667-704) for replica r
+// after its step `step`; the helix-triplet and junction-quadruplet counters
+// of the reference are never updated (origami_system.hpp:194-195) and print 0,
+// as does the domain-update bias (no domain-level biases here).
+void write_log_entry(Run& run, std::ostream& log, int r, long long step) {
+    ldo_obs o {};
+    run.check(ldo_get_observables(run.ctx, r, &o));
+    std::vector<Chain> chains = run.config(r);
+    std::vector<int> counts(run.identities.size() - 1, 0);
+    for (size_t c = 1; c < chains.size(); c++) {
+        counts[chains[c].identity - 1]++;
+    }
+    int unique = 0;
+    for (int n: counts) {
+        unique += n > 0 ? 1 : 0;
+    }
+    log << "Step: " << step << "\n";
+    log << "Temperature: " << o.temperature << "\n";
+    log << "Bound staples: " << o.num_staples << "\n";
+    log << "Unique bound staples: " << unique << "\n";
+    log << "Fully bound domain pairs: " << o.num_fully_bound_domain_pairs << "\n";
+    log << "Misbound domain pairs: " << o.num_misbound_domain_pairs << "\n";
+    log << "Stacked domain pairs: " << o.num_stacked_domain_pairs << "\n";
+    log << "Linear helix triplets: " << 0 << "\n";
+    log << "Stacked junction quadruplets: " << 0 << "\n";
+    log << "Staple counts: ";
+    for (int n: counts) {
+        log << n << " ";
+    }
+    log << "\n";
+    log << "System energy: " << o.energy << "\n";
+    log << "Total external bias: " << o.bias << "\n";
+    log << "Domain update external bias: " << 0.0 << "\n";
+    log << "Move update external bias: " << o.bias << "\n";
+    log << "Movetype: " << run.move_labels.at(o.last_move) << "\n";
+    log << "Accepted: " << std::boolalpha << (o.last_accepted != 0) << std::noboolalpha << "\n";
+    log << "\n";
+}
+
+// GCMCSimulation::write_log_summary (simulation.cpp:706-718) of a one-replica
+// run: each movetype's header (movetypes.cpp:88-96) to the log, and the
+// movetype summaries (met_movetypes.cpp:145-190, cb_movetypes.cpp:264-290,
+// 626-678; none for orientation rotation) to the .moves file.
+void write_log_summary(Run& run, std::ostream& log, const std::string& moves_path) {
+    ldo_counters c {};
+    run.check(ldo_get_counters(run.ctx, &c));
+    int T = (int)run.identities.size();
+    int ns = run.n_scaffold;
+    int S1 = run.max_total_staples + 1;
+    std::vector<int64_t> tr((size_t)(6 * T + 2 * (ns + 1) + 2 * S1));
+    int64_t n_tr = 0;
+    run.check(ldo_get_move_tracking(run.ctx, 0, tr.data(), (int64_t)tr.size(), &n_tr));
+    auto at = [&](int part, int i) { return tr[(size_t)part * T + i]; };
+    std::ofstream moves;
+    Outputs::open(moves, moves_path);
+    auto header = [&](std::ostream& os, size_t m) {
+        int att = (int)c.attempts[m], acc = (int)c.accepts[m];
+        os << "Movetype: " << run.move_labels[m] << "\n";
+        os << "    Attempts: " << att << "\n";
+        os << "    Accepts: " << acc << "\n";
+        os << "    Frequency: " << static_cast<double>(acc) / att << "\n";
+    };
+    log << "Run summary" << "\n\n";
+    for (size_t m = 0; m < run.move_kinds.size(); m++) {
+        header(log, m);
+        int kind = run.move_kinds[m];
+        if (kind == LDO_MOVE_MET_STAPLE_EXCHANGE) {
+            header(moves, m);
+            std::vector<int> types;
+            for (int t = 1; t < T; t++) {
+                if (at(0, t) + at(2, t) > 0) {
+                    types.push_back(t);
+                }
+            }
+            moves << "       Exchange multipliers\n        ";
+            for (size_t i = 0; i < types.size(); i++) {
+                moves << 1 << ", ";
+            }
+            moves << "\n";
+            for (int t: types) {
+                int iats = (int)at(0, t), iacs = (int)at(1, t);
+                int dats = (int)at(2, t), dacs = (int)at(3, t);
+                moves << "    Staple type: " << t << "\n";
+                moves << "        Insertion attempts: " << iats << "\n";
+                moves << "        Insertion accepts: " << iacs << "\n";
+                moves << "        Insertion frequency: " << static_cast<float>(iacs) / iats
+                      << "\n";
+                moves << "        Deletion attempts: " << dats << "\n";
+                moves << "        Deletion accepts: " << dacs << "\n";
+                moves << "        Deletion frequency: " << static_cast<float>(dacs) / dats
+                      << "\n";
+            }
+        }
+        else if (kind == LDO_MOVE_CB_STAPLE_REGROWTH) {
+            header(moves, m);
+            moves << "\n";
+            for (int t = 1; t < T; t++) {
+                int ats = (int)at(4, t), acs = (int)at(5, t);
+                if (ats == 0) {
+                    continue;
+                }
+                moves << "    Staple type: " << t << "\n";
+                moves << "        Attempts: " << ats << "\n";
+                moves << "        Accepts: " << acs << "\n";
+                moves << "        Frequency: " << static_cast<double>(acs) / ats << "\n";
+            }
+        }
+        else if (kind == LDO_MOVE_CTCB_SCAFFOLD_REGROWTH) {
+            header(moves, m);
+            const int64_t* len_att = tr.data() + 6 * T;
+            const int64_t* len_acc = len_att + (ns + 1);
+            const int64_t* st_att = len_acc + (ns + 1);
+            const int64_t* st_acc = st_att + S1;
+            for (int l = 0; l <= ns; l++) {
+                if (len_att[l] == 0) {
+                    continue;
+                }
+                moves << "    Number of scaffold domains: " << l << "\n";
+                moves << "        Attempts: " << (int)len_att[l] << "\n";
+                moves << "        Accepts: " << (int)len_acc[l] << "\n";
+                moves << "        Frequency: " << static_cast<double>(len_acc[l]) / len_att[l]
+                      << "\n";
+            }
+            moves << "\n";
+            for (int s = 0; s < S1; s++) {
+                if (st_att[s] == 0) {
+                    continue;
+                }
+                moves << "    Number of staples: " << s << "\n";
+                moves << "        Attempts: " << (int)st_att[s] << "\n";
+                moves << "        Accepts: " << (int)st_acc[s] << "\n";
+                moves << "        Frequency: " << static_cast<double>(st_acc[s]) / st_att[s]
+                      << "\n";
+            }
+        }
+        log << "\n";
+    }
+}
+
 void write_outputs(Run& run, Outputs& o, int r, long long step, double seconds,
                    int bias_rep = -1) {
     bool need_cfg = due(o.f_cfg, step) || due(o.f_vtf, step) || due(o.f_counts, step);
@@ -1035,12 +1176,14 @@ int main_run(int argc, char** argv) {
                                 {"OrientationRotation", "MetStapleExchange", "CBStapleRegrowth",
                                  "CTCBScaffoldRegrowth"},
                                 "movetype"));
+        run.move_labels.push_back(m.has("label") ? m.at("label").str : m.at("type").str);
         freqs.push_back(fraction(m.at("freq")));
         maxreg.push_back(m.has("max_regrowth") ? (int32_t)m.at("max_regrowth").num : 0);
     }
     run.max_total_staples = (int)p.integer("max_total_staples", 999);
     run.max_staple_size = (int)p.integer("max_staple_size", 2);
     ldo_run_params rp {};
+    run.move_kinds = kinds;
     rp.n_movetypes = (int32_t)kinds.size();
     rp.movetype_kind = kinds.data();
     rp.movetype_freq = freqs.data();
@@ -1162,8 +1305,9 @@ int main_run(int argc, char** argv) {
         outs.push_back(setup_outputs(run, pt ? base + "-" + std::to_string(r) : base));
     }
     const Outputs& o0 = *outs[0];
+    long long log_freq = p.integer("logging_freq", 0);
     long long chunk = 0;
-    for (long long f: {o0.f_cfg, o0.f_vtf, o0.f_counts, o0.f_times, o0.f_ene, o0.f_ops}) {
+    for (long long f: {o0.f_cfg, o0.f_vtf, o0.f_counts, o0.f_times, o0.f_ene, o0.f_ops, log_freq}) {
         if (f > 0) {
             chunk = gcd_of(f, chunk);
         }
@@ -1182,8 +1326,16 @@ int main_run(int argc, char** argv) {
             long long n = std::min(chunk - step % chunk, total - step);
             run.check(ldo_sweep(run.ctx, n));
             step += n;
+            // the constant-T driver logs to standard output
+            // (constant_temp_simulation.cpp:19)
+            if (log_freq != 0 && step % log_freq == 0) {
+                write_log_entry(run, std::cout, 0, step);
+            }
             write_outputs(run, *outs[0], 0, step, elapsed());
         }
+        // simulate(ct_steps) ends with the run summary and the .moves file
+        write_log_summary(run, std::cout, base + ".moves");
+        std::cout.flush();
     }
     else {
         long long swaps = p.integer("swaps", 0);
@@ -1201,10 +1353,20 @@ int main_run(int argc, char** argv) {
         swp << "\n";
         long long cfg_freq = o0.f_cfg;
         std::vector<int32_t> q_to_rep(run.R);
+        // each rank logs to its own .out file (ptmc_simulation.cpp:48-56);
+        // simulate runs without a summary between exchanges
+        std::vector<std::unique_ptr<std::ofstream>> logs;
+        for (int r = 0; r < run.R; r++) {
+            logs.push_back(std::make_unique<std::ofstream>());
+            Outputs::open(*logs.back(), base + "-" + std::to_string(r) + ".out");
+        }
         for (long long s = 1; s <= swaps; s++) {
             run.check(ldo_run(run.ctx, 1, interval));
             step += interval;
             for (int r = 0; r < run.R; r++) {
+                if (log_freq != 0 && step % log_freq == 0) {
+                    write_log_entry(run, *logs[r], r, step);
+                }
                 write_outputs(run, *outs[r], r, step, elapsed());
             }
             if (cfg_freq != 0 && step % cfg_freq == 0) {

@@ -148,6 +148,7 @@ struct Eng {
     NetFrame* sframes;
     GrowFrame* gstack;
     int16_t* gpl;
+    int64_t* track;
     const double* t_hyb;
     const double* t_stack;
     double init_e;
@@ -272,6 +273,7 @@ struct Eng {
         sframes = (NetFrame*)(blk + L.off_sframes);
         gstack = (GrowFrame*)(blk + L.off_gstack);
         gpl = (int16_t*)(blk + L.off_gpl);
+        track = (int64_t*)(blk + L.off_track);
         ns = L.ns;
         lshift = L.lshift;
         lmask = (1 << L.lshift) - 1;
@@ -648,6 +650,7 @@ struct Eng {
     DEVNI bool ctcb_scaffold_move(int max_regrowth);
     DEVIN bool attempt(int m);
     DEVNI bool attempt_kind(int kind, int max_regrowth);
+    DEVIN void track_move(int m, bool acc);
     // learned domain regrowth (lane 0 parts)
     DEVNI bool chain_connected(int cs);
     DEVNI int learned_begin(LearnedMove& m);

@@ -243,6 +243,7 @@ DEVNI bool Eng::orientation_move() {
 DEVNI bool Eng::insert_staple() {
     int ntypes = P->L.ntypes;
     int ident = uniform_int(1, ntypes - 1);
+    S->trk_type = ident;
     if (S->num_staples == P->max_total_staples) {
         return false;
     }
@@ -289,6 +290,7 @@ DEVNI bool Eng::insert_staple() {
 DEVNI bool Eng::delete_staple() {
     int ntypes = P->L.ntypes;
     int ident = uniform_int(1, ntypes - 1);
+    S->trk_type = ident;
     int n = tcount[ident];
     if (n == 0) {
         S->rejected = 1;
@@ -515,10 +517,13 @@ DEVIN void Eng::cb_set_old_growth_point(int gnew, int gold) {
 // CBStapleRegrowthMCMovetype::internal_attempt_move (cb_movetypes.cpp:292-348)
 DEVNI bool Eng::cb_staple_move() {
     if (S->num_staples == 0) {
+        S->trk_nost = 1;
         return false;
     }
+    S->trk_nost = 0;
     int w = uniform_int(1, S->num_staples);
     int cs = order[w];
+    S->trk_type = ch[cs].ident;
     if (staple_is_connector(cs)) {
         return false;
     }
@@ -1040,6 +1045,7 @@ DEVNI bool Eng::ctcb_scaffold_move(int max_regrowth) {
     seg_first = first;
     seg_count = count;
     seg_dir = dir;
+    S->trk_len = count;
     cp_find(first, count, dir);
     bool whole_ring = P->cyclic && count == ns;
     if (!whole_ring) {
@@ -1059,6 +1065,7 @@ DEVNI bool Eng::ctcb_scaffold_move(int max_regrowth) {
             sl[k] = (int16_t)c2;
         }
     }
+    S->trk_nst = nst;
     for (int phase = 0; phase < 2; phase++) {
         if (phase == 1) {
             cb_setup_regrow_old();
@@ -1119,10 +1126,8 @@ DEVNI bool Eng::attempt_kind(int kind, int max_regrowth) {
         return orientation_move();
     case 1:
         S->kind = LDO_MOVE_KIND_MET;
-        if (uniform_real() < 0.5) {
-            return insert_staple();
-        }
-        return delete_staple();
+        S->trk_ins = uniform_real() < 0.5 ? 1 : 0;
+        return S->trk_ins ? insert_staple() : delete_staple();
     case 2:
         S->kind = LDO_MOVE_KIND_CB;
         return cb_staple_move();
@@ -1132,6 +1137,38 @@ DEVNI bool Eng::attempt_kind(int kind, int max_regrowth) {
     }
     fail(ST_MISUSE);
     return false;
+}
+
+// MCMovetype::add_tracker (movetypes.cpp:41-51) for the move kinds whose
+// summaries list tracked counts: MetStapleExchange by staple type and
+// direction (met_movetypes.cpp:145-190), CBStapleRegrowth by staple type
+// unless there were no staples (cb_movetypes.cpp:264-290),
+// CTCBScaffoldRegrowth by segment length and staples regrown
+// (cb_movetypes.cpp:626-678)
+DEVIN void Eng::track_move(int m, bool acc) {
+    S->last_move = m;
+    S->last_acc = acc ? 1 : 0;
+    int T = P->L.ntypes;
+    int a = acc ? 1 : 0;
+    int kind = P->move_kind[m];
+    if (kind == 1) {
+        int b = S->trk_ins ? 0 : 2 * T;
+        track[b + S->trk_type] += 1;
+        track[b + T + S->trk_type] += a;
+    }
+    else if (kind == 2) {
+        if (!S->trk_nost) {
+            track[4 * T + S->trk_type] += 1;
+            track[5 * T + S->trk_type] += a;
+        }
+    }
+    else if (kind == 3) {
+        int o = 6 * T, n1 = ns + 1, s1 = P->L.smax + 1;
+        track[o + S->trk_len] += 1;
+        track[o + n1 + S->trk_len] += a;
+        track[o + 2 * n1 + S->trk_nst] += 1;
+        track[o + 2 * n1 + s1 + S->trk_nst] += a;
+    }
 }
 
 // GCMCSimulation::simulate (simulation.cpp:568-653) and select_movetype
@@ -1154,6 +1191,7 @@ DEVNI void Eng::simulate(int64_t steps) {
         S->attempts[m]++;
         bool acc = attempt(m);
         S->accepts[m] += acc ? 1 : 0;
+        track_move(m, acc);
         if (!acc) {
             reset_origami();
             calc_move_bias(0);

@@ -117,7 +117,16 @@ struct RepScalars {
     int32_t rng_idx;
     int32_t added[4];
     int32_t rng_sub; // 0: MC step stream; 1: learned-move stream
-    int32_t pad[1];
+    // the move in flight's tracking key (the movetypes' m_tracker): staple
+    // type, insertion or deletion, no staples to regrow, CTCB segment length
+    // and staples regrown
+    int32_t trk_type;
+    int32_t trk_ins;
+    int32_t trk_nost;
+    int32_t trk_len;
+    int32_t trk_nst;
+    int32_t last_move; // move slot of the last step and whether it was accepted
+    int32_t last_acc;
 };
 
 // Byte offsets of the sections of a replica block, and sizes.
@@ -133,13 +142,19 @@ struct Layout {
     int32_t hcap;      // hash capacity (power of two)
     int32_t hbits;
     int32_t ecap;      // endpoint list capacity
-    int32_t pad_layout;
+    int32_t track_len; // per-move tracking counters (see off_track)
     int64_t off_dom, off_dident, off_chain, off_order, off_free, off_tlist, off_tcount;
     int64_t off_hkey, off_hval, off_modified, off_assigned, off_prev, off_old;
     int64_t off_seg, off_gp, off_stem, off_inact, off_netseg, off_netds_flag;
     int64_t off_checked, off_netcs, off_regrow, off_dir0, off_dir1;
     int64_t off_active, off_active0, off_gps, off_iaes, off_netds, off_frames;
     int64_t off_part, off_sframes, off_gstack, off_gpl;
+    // int64 tracking counters by staple type T = ntypes (index = identity)
+    // and scaffold length / staple count: Met insertion attempts[T],
+    // accepts[T], deletion attempts[T], accepts[T]; CB attempts[T],
+    // accepts[T]; CTCB by segment length attempts[ns+1], accepts[ns+1]; by
+    // staples regrown attempts[smax+1], accepts[smax+1]
+    int64_t off_track;
     int64_t bytes; // block size (multiple of 256)
 };
 

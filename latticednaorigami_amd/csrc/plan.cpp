@@ -102,6 +102,8 @@ Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap) {
     take(L.off_sframes, C * (int64_t)sizeof(NetFrame));
     take(L.off_gstack, LDO_GROW_DEPTH * (int64_t)sizeof(GrowFrame));
     take(L.off_gpl, 2 * LDO_MAX_STAPLE_LEN * 2);
+    L.track_len = 6 * ntypes + 2 * (ns + 1) + 2 * (smax + 1);
+    take(L.off_track, (int64_t)L.track_len * 8);
     L.bytes = round_up(off, 256);
     return L;
 }
@@ -388,6 +390,8 @@ void fill_observables(const Plan& p, const RepScalars& S, ldo_obs* out) {
     out->num_stacked_domain_pairs = S.num_stacked;
     out->num_unassigned_domains = S.num_unassigned;
     out->status = S.status;
+    out->last_move = S.last_move;
+    out->last_accepted = S.last_acc;
 }
 
 void add_counters(const RepScalars& S, ldo_counters* out) {

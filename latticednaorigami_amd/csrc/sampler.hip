@@ -1097,6 +1097,23 @@ int ldo_get_observables(void* ctx, int32_t replica, ldo_obs* out) {
     });
 }
 
+int ldo_get_move_tracking(void* ctx, int32_t replica, int64_t* out, int64_t cap,
+                          int64_t* n_out) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        if (replica < 0 || replica >= c->R || !out || !n_out) {
+            throw AbiError {LDO_E_ARG, "replica out of range or null buffer"};
+        }
+        int64_t n = c->P.L.track_len;
+        if (cap < n) {
+            throw AbiError {LDO_E_ARG, "tracking capacity too small"};
+        }
+        download(*c, out, c->d_blocks + (size_t)replica * c->P.L.bytes + c->P.L.off_track,
+                 (size_t)n * sizeof(int64_t), "copy tracking");
+        *n_out = n;
+    });
+}
+
 int ldo_get_counters(void* ctx, ldo_counters* out) {
     Ctx* c = static_cast<Ctx*>(ctx);
     return guarded(c, [&]() {

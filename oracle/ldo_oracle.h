@@ -59,6 +59,8 @@ int oracle_set_config(void* o, int32_t replica, const ldo_config* cfg);
 
 /* Tape recording (mt19937_64 mode): every draw of replica r is appended to
  * its tape; the tape is what ldo_set_tape replays on the GPU. */
+// per-move tracking counts (layout of ldo_get_move_tracking)
+int oracle_get_move_tracking(void* o, int32_t replica, int64_t* out, int64_t cap, int64_t* n_out);
 // the next exchange round pairs as the first of a run (ldo_restart_swaps)
 int oracle_restart_swaps(void* o);
 // replica `replica` replays these draws (as ldo_set_tape does)

@@ -501,6 +501,24 @@ int oracle_get_observables(void* o, int32_t r, ldo_obs* out) {
         out->num_unassigned_domains = rep.sys.num_unassigned;
         out->status = rep.status;
         out->bias = rep.win ? (rep.win_state->restraint + rep.win_state->grid) * rep.win->mult : 0;
+        out->last_move = rep.last_move;
+        out->last_accepted = rep.last_accepted;
+    })
+}
+
+int oracle_get_move_tracking(void* o, int32_t r, int64_t* out, int64_t cap, int64_t* n_out) {
+    ORACLE_TRY(o, {
+        Replica& rep = *O->reps.at(r);
+        int64_t T = (int64_t)rep.sys.identities.size();
+        int64_t ns = (int64_t)rep.sys.identities[0].size();
+        int64_t n = 6 * T + 2 * (ns + 1) + 2 * ((int64_t)O->lim.max_total_staples + 1);
+        if (cap < n) {
+            throw OrigamiError(LDO_E_ARG, "tracking capacity too small");
+        }
+        for (int64_t k = 0; k < n; k++) {
+            out[k] = rep.track.empty() ? 0 : rep.track[k];
+        }
+        *n_out = n;
     })
 }
 

@@ -390,6 +390,19 @@ class Replica {
     int64_t step {0};
     int status {0};
     std::vector<int64_t> attempts, accepts;
+    // the movetypes' m_tracker fields of the move in flight, the tracked
+    // counts in the layout of ldo_get_move_tracking, and the last step's
+    // move slot and outcome
+    struct Tracker {
+        int staple_type {0};
+        bool insertion {false};
+        bool no_staples {false};
+        int num_scaffold_domains {0};
+        int num_staples {0};
+    } tracker;
+    std::vector<int64_t> track;
+    int last_move {0};
+    int last_accepted {0};
     // the umbrella window this replica samples in, and that window's stored
     // bias values (both null without biases)
     const WindowBias* win {nullptr};
@@ -399,6 +412,7 @@ class Replica {
     int64_t* hist {nullptr};
 
     void simulate(int64_t steps); // simulation.cpp:568-653
+    void add_tracker(int slot, bool accepted);
     int window_op(int staple_offset) const;
     double calc_move_bias(int staple_offset);
 

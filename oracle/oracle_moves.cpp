@@ -283,6 +283,43 @@ Replica::Replica(const ldo_system_desc& sd, const std::vector<std::vector<int>>&
 }
 
 // GCMCSimulation::simulate (simulation.cpp:568-653) without file output
+// MCMovetype::add_tracker (movetypes.cpp:41-51) summed as the summaries of
+// met_movetypes.cpp:145-190, cb_movetypes.cpp:264-290 and 626-678 read them
+void Replica::add_tracker(int slot, bool accepted) {
+    last_move = slot;
+    last_accepted = accepted ? 1 : 0;
+    int T = (int)sys.identities.size();
+    int ns = (int)sys.identities[0].size();
+    int a = accepted ? 1 : 0;
+    if (track.empty()) {
+        track.assign(6 * T + 2 * (ns + 1) + 2 * (m_lim.max_total_staples + 1), 0);
+    }
+    switch (m_moves[slot].kind) {
+    case LDO_MOVE_MET_STAPLE_EXCHANGE: {
+        int b = tracker.insertion ? 0 : 2 * T;
+        track[b + tracker.staple_type] += 1;
+        track[b + T + tracker.staple_type] += a;
+        break;
+    }
+    case LDO_MOVE_CB_STAPLE_REGROWTH:
+        if (!tracker.no_staples) {
+            track[4 * T + tracker.staple_type] += 1;
+            track[5 * T + tracker.staple_type] += a;
+        }
+        break;
+    case LDO_MOVE_CTCB_SCAFFOLD_REGROWTH: {
+        int o = 6 * T, n1 = ns + 1, s1 = m_lim.max_total_staples + 1;
+        track[o + tracker.num_scaffold_domains] += 1;
+        track[o + n1 + tracker.num_scaffold_domains] += a;
+        track[o + 2 * n1 + tracker.num_staples] += 1;
+        track[o + 2 * n1 + s1 + tracker.num_staples] += a;
+        break;
+    }
+    default:
+        break;
+    }
+}
+
 void Replica::simulate(int64_t steps) {
     int64_t end = step + steps;
     while (step < end) {
@@ -303,6 +340,7 @@ void Replica::simulate(int64_t steps) {
         attempts[i]++;
         bool accepted = attempt(m_moves[i].kind, m_moves[i].max_regrowth);
         accepts[i] += accepted;
+        add_tracker((int)i, accepted);
         if (!accepted) {
             reset_origami();
             calc_move_bias(0);
@@ -552,7 +590,8 @@ bool Replica::orientation_move() {
 // ------------------------------------------------------ staple exchange move
 // met_movetypes.cpp:192-206
 bool Replica::staple_exchange_move() {
-    if (rng.uniform_real() < 0.5) {
+    tracker.insertion = rng.uniform_real() < 0.5;
+    if (tracker.insertion) {
         return insert_staple();
     }
     return delete_staple();
@@ -589,6 +628,7 @@ void Replica::met_unassign_domains(const std::vector<Dom*>& ds) {
 // move-level external bias is 0 without bias functions)
 bool Replica::insert_staple() {
     int ident = rng.uniform_int(1, (int)sys.identities.size() - 1);
+    tracker.staple_type = ident;
     if (sys.num_staples == m_lim.max_total_staples) {
         return false;
     }
@@ -628,6 +668,7 @@ bool Replica::insert_staple() {
 // met_movetypes.cpp:361-403, 255-301
 bool Replica::delete_staple() {
     int ident = rng.uniform_int(1, (int)sys.identities.size() - 1);
+    tracker.staple_type = ident;
     int n = (int)sys.ident_to_index[ident].size();
     if (n == 0) {
         m_rejected = true;
@@ -825,10 +866,13 @@ bool Replica::cb_test_acceptance() {
 // cb_movetypes.cpp:292-348, 389-404
 bool Replica::cb_staple_move() {
     if (sys.num_staples == 0) {
+        tracker.no_staples = true;
         return false;
     }
+    tracker.no_staples = false;
     int w = rng.uniform_int(1, sys.num_staples);
     std::vector<Dom*> chain = sys.chains[w];
+    tracker.staple_type = chain[0]->c_ident;
     if (staple_is_connector(chain)) {
         return false;
     }
@@ -939,12 +983,14 @@ bool Replica::ctcb_scaffold_move(int max_regrowth) {
     m_regrow_old = false;
     std::vector<Dom*> scaffold = sys.chains[0];
     std::vector<Dom*> sds = select_indices(scaffold, 2, (unsigned)max_regrowth);
+    tracker.num_scaffold_domains = (int)sds.size();
     m_cp.calculate(sds, m_dir);
     bool whole_ring = sys.cyclic && sds.size() == scaffold.size();
     if (!whole_ring) {
         m_cp.remove_active_endpoint(sds[0]);
     }
     std::set<int> staples = m_cp.staples_to_be_regrown();
+    tracker.num_staples = (int)staples.size();
     std::vector<Dom*> to_unassign(sds.begin() + 1, sds.end());
     cb_unassign_domains(to_unassign);
     for (int c_i: staples) {

@@ -180,6 +180,15 @@ class Oracle:
         _check(self.L, self.h, self.L.oracle_get_counters(self.h, C.byref(c)))
         return c
 
+    def move_tracking(self, r=0):
+        cap = 6 * len(self.system.identities) + 2 * (len(self.system.identities[0]) + 1) \
+            + 2 * (self.system.max_total_staples + 1)
+        buf = (C.c_int64 * cap)()
+        n = C.c_int64()
+        _check(self.L, self.h, self.L.oracle_get_move_tracking(self.h, r, buf, cap, C.byref(n)))
+        return abi.split_tracking(buf[:n.value], len(self.system.identities),
+                                  len(self.system.identities[0]), self.system.max_total_staples)
+
     def permutation(self):
         p = (C.c_int32 * self.n)()
         _check(self.L, self.h, self.L.oracle_get_permutation(self.h, p))

@@ -438,6 +438,19 @@ int eh_get_observables(void* ctx, int32_t replica, ldo_obs* out) {
     });
 }
 
+int eh_get_move_tracking(void* ctx, int32_t replica, int64_t* out, int64_t cap, int64_t* n_out) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        int64_t n = c->P.L.track_len;
+        if (replica < 0 || replica >= c->R || cap < n) {
+            throw ldo::AbiError {LDO_E_ARG, "bad tracking arguments"};
+        }
+        std::memcpy(out, c->blocks + (size_t)replica * c->P.L.bytes + c->P.L.off_track,
+                    (size_t)n * sizeof(int64_t));
+        *n_out = n;
+    });
+}
+
 int eh_get_counters(void* ctx, ldo_counters* out) {
     HostCtx* c = static_cast<HostCtx*>(ctx);
     return guarded(c, [&]() {

@@ -17,7 +17,8 @@ from latticednaorigami_amd import abi, systems
 ENERGY_RTOL = 1e-12
 EXACT_KEYS = ["num_staples", "num_domains", "num_bound_domain_pairs",
               "num_fully_bound_domain_pairs", "num_misbound_domain_pairs",
-              "num_stacked_domain_pairs", "num_unassigned_domains", "step", "status"]
+              "num_stacked_domain_pairs", "num_unassigned_domains", "step", "status",
+              "last_move", "last_accepted"]
 
 
 def say(msg):
@@ -51,6 +52,13 @@ def case_four_domain_constant_t_bit_exact(sampler_cls, oracle_mod):
         ref.sweep(10000)
         assert_same(gpu, ref, 0)
     assert_counters(gpu, ref)
+    # the per-move tracking behind the .moves summary
+    tr = gpu.move_tracking(0)
+    assert tr == ref.move_tracking(0)
+    assert sum(tr["met_insert_attempts"]) + sum(tr["met_delete_attempts"]) == \
+        gpu.counters().attempts[1]
+    assert sum(tr["ctcb_length_attempts"]) == sum(tr["ctcb_staples_attempts"]) == \
+        gpu.counters().attempts[3]
 
 
 def case_lane_mappings_agree(sampler_cls, oracle_mod):
@@ -79,6 +87,8 @@ def case_raster_nearest_neighbour_bit_exact(sampler_cls, oracle_mod):
         for r in range(2):
             assert_same(gpu, ref, r)
     assert_counters(gpu, ref)
+    for r in range(2):
+        assert gpu.move_tracking(r) == ref.move_tracking(r)
 
 
 def case_sweep_kernels_agree(sampler_cls, oracle_mod):
@@ -403,7 +413,8 @@ def case_config_roundtrip_and_errors(sampler_cls, oracle_mod):
     assert ref.config(1) == chains
     a, b = gpu.observables(1), ref.observables(1)
     for k in EXACT_KEYS:
-        if k != "step":  # the oracle's sweep also stepped its replica 1
+        if k not in ("step", "last_move", "last_accepted"):
+            # the oracle's sweep also stepped its replica 1
             assert a[k] == b[k], (k, a[k], b[k])
     for k in ("energy", "enthalpy", "entropy", "stacking_energy"):
         assert a[k] == pytest.approx(b[k], rel=ENERGY_RTOL, abs=1e-12), (k, a[k], b[k])

@@ -84,6 +84,129 @@ def expect_lines(ref, r, s, step):
     return ene, ops, counts, trj
 
 
+def expect_config_records(s, cfg, step):
+    """The .vcf frame, .states and .ores lines and the .staples and
+    .staplestates lines of a configuration (files.cpp:529-691).  Every domain
+    of a saved configuration is assigned; two domains on one site are a bound
+    pair (complementary identities) or a misbound one."""
+    max_dom = s.max_staple_size * s.max_total_staples + len(s.identities[0])
+    site = {}
+    for ci, c in enumerate(cfg):
+        for d, p in enumerate(c.positions):
+            site.setdefault(tuple(p), []).append((ci, d, s.identities[c.identity][d]))
+    state = {}
+    for occ in site.values():
+        if len(occ) == 1:
+            state[occ[0][:2]] = 1
+        else:
+            a, b = occ
+            state[a[:2]] = state[b[:2]] = 2 if a[2] == -b[2] else 3
+    vcf, states, ores, n = ["timestep"], "", "", 0
+    for ci, c in enumerate(cfg):
+        k = 0
+        for d, p in enumerate(c.positions):
+            vcf.append("".join(f"{v} " for v in p))
+            states += f"{state[(ci, d)]} "
+            ores += "".join(f"{v} " for v in c.orientations[d])
+            k += 1
+        for _ in range(k, s.max_staple_size):
+            vcf.append("0 0 0 ")
+            states += "-1 "
+            ores += "0 0 0 "
+        n += max(k, s.max_staple_size)
+    for _ in range(n, max_dom):
+        vcf.append("0 0 0 ")
+        states += "-1 "
+        ores += "0 0 0 "
+    counts = [0] * (len(s.identities) - 1)
+    full = [0] * (len(s.identities) - 1)
+    for ci, c in enumerate(cfg[1:], start=1):
+        counts[c.identity - 1] += 1
+        if all(state[(ci, d)] == 2 for d in range(len(c.positions))):
+            full[c.identity - 1] = 1
+    staples = f"{step} " + "".join(f"{x} " for x in counts)
+    staplestates = f"{step} " + "".join(f"{x} " for x in full)
+    return vcf + [""], states, ores, staples, staplestates
+
+
+def gdef(x):
+    """A double as std::ostream writes it by default (6 significant digits)."""
+    return "%g" % x
+
+
+def ratio(a, b, single=False):
+    """a / b as the movetype summaries print it: a float or double quotient,
+    and 0 / 0 as glibc prints the x86 default NaN."""
+    import numpy as np
+    if b == 0:
+        return "-nan"
+    return gdef(float(np.float32(a) / np.float32(b)) if single else a / b)
+
+
+def expect_log_entry(ref, r, s, step):
+    """GCMCSimulation::write_log_entry (simulation.cpp:667-704)."""
+    ob = ref.observables(r)
+    cfg = ref.config(r)
+    counts = [0] * (len(s.identities) - 1)
+    for c in cfg[1:]:
+        counts[c.identity - 1] += 1
+    label = MOVE_TYPES[s.moves[ob["last_move"]].kind]
+    return (f"Step: {step}\nTemperature: {gdef(ob['temperature'])}\n"
+            f"Bound staples: {ob['num_staples']}\n"
+            f"Unique bound staples: {sum(1 for n in counts if n)}\n"
+            f"Fully bound domain pairs: {ob['num_fully_bound_domain_pairs']}\n"
+            f"Misbound domain pairs: {ob['num_misbound_domain_pairs']}\n"
+            f"Stacked domain pairs: {ob['num_stacked_domain_pairs']}\n"
+            f"Linear helix triplets: 0\nStacked junction quadruplets: 0\n"
+            f"Staple counts: {''.join(f'{n} ' for n in counts)}\n"
+            f"System energy: {gdef(ob['energy'])}\nTotal external bias: {gdef(ob['bias'])}\n"
+            f"Domain update external bias: 0\nMove update external bias: {gdef(ob['bias'])}\n"
+            f"Movetype: {label}\nAccepted: {'true' if ob['last_accepted'] else 'false'}\n\n")
+
+
+def expect_summary(ref, s):
+    """The run summary written to the log and the .moves file
+    (simulation.cpp:706-718; movetypes.cpp:88-96; met_movetypes.cpp:145-190;
+    cb_movetypes.cpp:264-290, 626-678)."""
+    c = ref.counters()
+    tr = ref.move_tracking(0)
+    log, moves = ["Run summary\n\n"], []
+    for m, mv in enumerate(s.moves):
+        label = MOVE_TYPES[mv.kind]
+        head = (f"Movetype: {label}\n    Attempts: {c.attempts[m]}\n    Accepts: {c.accepts[m]}\n"
+                f"    Frequency: {ratio(c.accepts[m], c.attempts[m])}\n")
+        log.append(head + "\n")
+        if mv.kind == abi.MOVE_MET_STAPLE_EXCHANGE:
+            types = [t for t in range(1, len(s.identities))
+                     if tr["met_insert_attempts"][t] + tr["met_delete_attempts"][t] > 0]
+            moves.append(head + "       Exchange multipliers\n        " + "1, " * len(types) + "\n")
+            for t in types:
+                ia, ic = tr["met_insert_attempts"][t], tr["met_insert_accepts"][t]
+                da, dc = tr["met_delete_attempts"][t], tr["met_delete_accepts"][t]
+                moves.append(f"    Staple type: {t}\n        Insertion attempts: {ia}\n"
+                             f"        Insertion accepts: {ic}\n"
+                             f"        Insertion frequency: {ratio(ic, ia, True)}\n"
+                             f"        Deletion attempts: {da}\n        Deletion accepts: {dc}\n"
+                             f"        Deletion frequency: {ratio(dc, da, True)}\n")
+        elif mv.kind == abi.MOVE_CB_STAPLE_REGROWTH:
+            moves.append(head + "\n")
+            for t in range(1, len(s.identities)):
+                a, k = tr["cb_attempts"][t], tr["cb_accepts"][t]
+                if a:
+                    moves.append(f"    Staple type: {t}\n        Attempts: {a}\n        Accepts: {k}\n"
+                                 f"        Frequency: {ratio(k, a)}\n")
+        elif mv.kind == abi.MOVE_CTCB_SCAFFOLD_REGROWTH:
+            moves.append(head)
+            for name, sep in (("length", "Number of scaffold domains"), ("staples", "Number of staples")):
+                for n, (a, k) in enumerate(zip(tr[f"ctcb_{name}_attempts"], tr[f"ctcb_{name}_accepts"])):
+                    if a:
+                        moves.append(f"    {sep}: {n}\n        Attempts: {a}\n        Accepts: {k}\n"
+                                     f"        Frequency: {ratio(k, a)}\n")
+                if name == "length":
+                    moves.append("\n")
+    return "".join(log), "".join(moves)
+
+
 def read(d, name):
     with open(os.path.join(d, name)) as f:
         return f.read().split("\n")
@@ -93,6 +216,7 @@ def run_driver(inp):
     print("case: ldo_run", inp, flush=True)
     p = subprocess.run([DRIVER, inp], capture_output=True, text=True, timeout=180)
     assert p.returncode == 0, p.stderr
+    return p
     return p
 
 
@@ -132,26 +256,44 @@ def test_driver_constant_temp_matches_oracle(tmp_path, oracle_mod):
                               "ct_steps": 20000, "configs_output_freq": 4000,
                               "vtf_output_freq": 4000, "counts_output_freq": 2000,
                               "times_output_freq": 5000, "energies_output_freq": 2000,
-                              "order_params_output_freq": 2000})
-    run_driver(inp)
+                              "order_params_output_freq": 2000, "logging_freq": 4000})
+    out = run_driver(inp).stdout
     ref = oracle_mod.Oracle(s, temps=[330.0])
     ene, ops, counts, trj = read(d, "out.ene"), read(d, "out.ops"), read(d, "out.counts"), \
         read(d, "out.trj")
     assert ene[0] == "step tenergy henthalpy hentropy stacking bias"
     assert ops[0] == "numfulldomains, nummisdomains, numstackedpairs, numstaples, "
-    want_trj = []
+    want_trj, want_vcf, want_states, want_ores = [], [], [], []
+    want_log = ""
+    staples, staplestates = read(d, "out.staples"), read(d, "out.staplestates")
     for k in range(1, 11):
         ref.sweep(2000)
         e, o, c, t = expect_lines(ref, 0, s, 2000 * k)
         assert ene[k] == e and ops[k] == o and counts[k - 1] == c, k
+        vcf, st, orr, stp, sts = expect_config_records(s, ref.config(0), 2000 * k)
+        assert staples[k - 1] == stp and staplestates[k - 1] == sts, k
         if k % 2 == 0:
             want_trj += t
+            want_vcf += vcf
+            want_states.append(st)
+            want_ores.append(orr)
+        if (2000 * k) % 4000 == 0:
+            want_log += expect_log_entry(ref, 0, s, 2000 * k)
     assert trj[:len(want_trj)] == want_trj
+    assert read(d, "out.vcf")[:len(want_vcf)] == want_vcf
+    assert read(d, "out.states")[:5] == want_states
+    assert read(d, "out.ores")[:5] == want_ores
+    times = read(d, "out.times")
+    assert times[0] == "step time" and [x.split()[0] for x in times[1:5]] == \
+        ["5000", "10000", "15000", "20000"]
+    # the per-step log and the run summary on standard output, the movetype
+    # summaries in the .moves file
+    log, moves = expect_summary(ref, s)
+    assert out == want_log + log
+    with open(os.path.join(d, "out.moves")) as f:
+        assert f.read() == moves
     assert read(d, "out.vsf") == ["atom 0:4 radius 0.25 type scaffold",
                                   "atom 4:11 radius 0.25 type staple"]
-    states = read(d, "out.states")
-    assert len(states) == 6 and all(len(x.split()) == 12 for x in states[:5])
-    assert len(read(d, "out.times")) == 6
 
 
 @pytest.mark.gpu
@@ -164,11 +306,13 @@ def test_driver_ptmc_matches_oracle(tmp_path, oracle_mod):
                               "chem_pot_mults": "1 1 1 1", "bias_mults": "1 1 1 1",
                               "stacking_mults": "1 1 1 1", "swaps": 8,
                               "exchange_interval": 500, "configs_output_freq": 1000,
-                              "energies_output_freq": 1000, "order_params_output_freq": 1000})
+                              "energies_output_freq": 1000, "order_params_output_freq": 1000,
+                              "logging_freq": 1000})
     run_driver(inp)
     ref = oracle_mod.Oracle(s, temps=temps, exchange_mode=abi.EXCHANGE_TEMPERATURE)
     swp = read(d, "out.swp")
     assert swp[0] == "".join(f"{g10(t)}/1/ " for t in temps)
+    logs = [""] * 4
     for k in range(1, 5):
         ref.run(2, 500)
         assert swp[k] == "".join(f"{q} " for q in ref.permutation())
@@ -176,6 +320,11 @@ def test_driver_ptmc_matches_oracle(tmp_path, oracle_mod):
             e, o, _, _ = expect_lines(ref, r, s, 1000 * k)
             assert read(d, f"out-{r}.ene")[k] == e
             assert read(d, f"out-{r}.ops")[k] == o
+            logs[r] += expect_log_entry(ref, r, s, 1000 * k)
+    # each rank's log (ptmc_simulation.cpp:48-56), no run summary
+    for r in range(4):
+        with open(os.path.join(d, f"out-{r}.out")) as f:
+            assert f.read() == logs[r], r
 
 
 @pytest.mark.gpu

@@ -20,7 +20,8 @@ HOST_DIR = os.path.join(HERE, "engine_host")
 HOST_LIB = os.path.join(HOST_DIR, "_build", "libldo_engine_host.so")
 NAMES = ["create", "destroy", "last_error", "set_config", "get_config", "set_ladder", "set_tape",
          "set_windows", "sweep", "run", "get_observables", "get_counters", "get_permutation",
-         "set_window_grid", "set_histogram", "get_histogram", "restart_swaps"]
+         "set_window_grid", "set_histogram", "get_histogram", "restart_swaps",
+         "get_move_tracking"]
 
 
 class HostLib:
