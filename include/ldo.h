@@ -54,7 +54,8 @@ enum ldo_movetype {
     LDO_MOVE_ORIENTATION_ROTATION = 0,  /* orientation_movetype.cpp:30-65 */
     LDO_MOVE_MET_STAPLE_EXCHANGE = 1,   /* met_movetypes.cpp:192-403 */
     LDO_MOVE_CB_STAPLE_REGROWTH = 2,    /* cb_movetypes.cpp:292-348 */
-    LDO_MOVE_CTCB_SCAFFOLD_REGROWTH = 3 /* cb_movetypes.cpp:680-747 */
+    LDO_MOVE_CTCB_SCAFFOLD_REGROWTH = 3, /* cb_movetypes.cpp:680-747 */
+    LDO_MOVE_CTRG_SCAFFOLD_REGROWTH = 4  /* rg_movetypes.cpp:177-533, 629-689 */
 };
 
 /* Random streams.  PHILOX: a counter-based Philox4x32-10 stream keyed by
@@ -137,6 +138,11 @@ typedef struct ldo_run_params {
     int32_t exchange_mode; /* ldo_exchange_mode */
     int32_t lanes_per_replica; /* 0: pick automatically; 1 or 64 */
     int32_t flags;             /* ldo_run_flags, 0 = the reference's behaviour */
+    /* CTRG recoil growth (simulation.cpp:493-495): max_num_recoils (>= 0) and
+     * max_c_attempts (1..36) per movetype; read for CTRG entries only, and
+     * both must be given when the moveset holds one */
+    const int32_t* movetype_max_num_recoils;
+    const int32_t* movetype_max_c_attempts;
 } ldo_run_params;
 
 /* Opt-in deviations from the reference (ldo_run_params.flags).
@@ -344,8 +350,10 @@ int ldo_restart_swaps(void* ctx);
  *                     deletion attempts[T], accepts[T],
  *   CBStapleRegrowth attempts[T], accepts[T] (moves with staples present),
  *   CTCBScaffoldRegrowth attempts[ns+1], accepts[ns+1] by segment length,
- *                        attempts[S+1], accepts[S+1] by staples regrown;
- * 6T + 2(ns+1) + 2(S+1) values in all, written to *n_out (cap must hold
+ *                        attempts[S+1], accepts[S+1] by staples regrown,
+ *   CTRGScaffoldRegrowth attempts[ns+1], accepts[ns+1] by segment length
+ *                        (rg_movetypes.cpp:583-627);
+ * 6T + 4(ns+1) + 2(S+1) values in all, written to *n_out (cap must hold
  * them).  Counts are per move kind, summed over moveset slots of that kind. */
 int ldo_get_move_tracking(void* ctx, int32_t replica, int64_t* out, int64_t cap,
                           int64_t* n_out);

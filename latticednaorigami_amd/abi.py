@@ -26,6 +26,7 @@ MOVE_ORIENTATION_ROTATION = 0
 MOVE_MET_STAPLE_EXCHANGE = 1
 MOVE_CB_STAPLE_REGROWTH = 2
 MOVE_CTCB_SCAFFOLD_REGROWTH = 3
+MOVE_CTRG_SCAFFOLD_REGROWTH = 4
 RNG_PHILOX, RNG_TAPE = 0, 1
 FLAG_CTCB_SCAFFOLD_ENDPOINTS = 1
 EXCHANGE_NONE, EXCHANGE_TEMPERATURE, EXCHANGE_WINDOWS = 0, 1, 2
@@ -37,6 +38,7 @@ MOVETYPE_NAMES = {
     "MetStapleExchange": MOVE_MET_STAPLE_EXCHANGE,
     "CBStapleRegrowth": MOVE_CB_STAPLE_REGROWTH,
     "CTCBScaffoldRegrowth": MOVE_CTCB_SCAFFOLD_REGROWTH,
+    "CTRGScaffoldRegrowth": MOVE_CTRG_SCAFFOLD_REGROWTH,
 }
 
 _i32p = C.POINTER(C.c_int32)
@@ -86,6 +88,8 @@ class RunParams(C.Structure):
         ("exchange_mode", C.c_int32),
         ("lanes_per_replica", C.c_int32),
         ("flags", C.c_int32),
+        ("movetype_max_num_recoils", _i32p),
+        ("movetype_max_c_attempts", _i32p),
     ]
 
 
@@ -160,6 +164,11 @@ class LdoError(RuntimeError):
         self.code = code
 
 
+def tracking_len(n_types, n_scaffold, max_staples):
+    """Length of the ldo_get_move_tracking array."""
+    return 6 * n_types + 4 * (n_scaffold + 1) + 2 * (max_staples + 1)
+
+
 def split_tracking(values, n_types, n_scaffold, max_staples):
     """The ldo_get_move_tracking array as named per-index lists."""
     T, n1, s1 = n_types, n_scaffold + 1, max_staples + 1
@@ -169,7 +178,8 @@ def split_tracking(values, n_types, n_scaffold, max_staples):
                     ("met_delete_attempts", T), ("met_delete_accepts", T),
                     ("cb_attempts", T), ("cb_accepts", T),
                     ("ctcb_length_attempts", n1), ("ctcb_length_accepts", n1),
-                    ("ctcb_staples_attempts", s1), ("ctcb_staples_accepts", s1)):
+                    ("ctcb_staples_attempts", s1), ("ctcb_staples_accepts", s1),
+                    ("ctrg_length_attempts", n1), ("ctrg_length_accepts", n1)):
         out[name] = v[o:o + n]
         o += n
     assert o == len(v), (o, len(v))

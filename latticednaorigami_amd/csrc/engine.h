@@ -110,6 +110,18 @@ struct Delta {
     int st;
 };
 
+// The CTRG level being grown (CTRGRegrowthMCMovetype's m_di, m_d, m_dir,
+// m_ref_d, m_c_attempts, m_d_max_c_attempts, m_stemd and m_avail_cis, the
+// configurations left as bits of the 36 (site, orientation) pairs)
+struct RgCur {
+    int di, d, dir, ref, att, maxatt;
+    bool stemd;
+    uint64_t avail;
+};
+#define LDO_RG_ALL ((1ull << 36) - 1)
+// how a CTRG assignment is recorded (Eng::rg_set_config)
+enum { RG_NONE = 0, RG_LIST = 1, RG_DUP = 2 };
+
 struct Eng {
     const Params* P;
     int rep;
@@ -148,6 +160,14 @@ struct Eng {
     NetFrame* sframes;
     GrowFrame* gstack;
     int16_t* gpl;
+    int16_t* rgs;
+    int8_t* rgatt;
+    int8_t* rgwatt;
+    uint64_t* rgavail;
+    uint64_t* rgwavail;
+    double* rgopen;
+    Endpoint* erstk;
+    int16_t* ercnt;
     int64_t* track;
     const double* t_hyb;
     const double* t_stack;
@@ -160,6 +180,10 @@ struct Eng {
     bool pot_viol;
     int n_gps, n_iaes, n_netds;
     bool net_external;
+    // CTRG: the move's recoil and attempt limits, and the assignments the
+    // reference records beyond the recoil growth's (see rg_set_config)
+    int rg_max_recoils, rg_max_att;
+    int64_t rg_dup;
     // par = 1: the 64 lanes of a wavefront run this engine together on
     // identical values (every lane computes and stores the same thing), and
     // the lane-parallel sections split independent reads over the lanes and
@@ -273,6 +297,14 @@ struct Eng {
         sframes = (NetFrame*)(blk + L.off_sframes);
         gstack = (GrowFrame*)(blk + L.off_gstack);
         gpl = (int16_t*)(blk + L.off_gpl);
+        rgs = (int16_t*)(blk + L.off_rgs);
+        rgatt = (int8_t*)(blk + L.off_rgatt);
+        rgwatt = (int8_t*)(blk + L.off_rgwatt);
+        rgavail = (uint64_t*)(blk + L.off_rgavail);
+        rgwavail = (uint64_t*)(blk + L.off_rgwavail);
+        rgopen = (double*)(blk + L.off_rgopen);
+        erstk = (Endpoint*)(blk + L.off_erstk);
+        ercnt = (int16_t*)(blk + L.off_ercnt);
         track = (int64_t*)(blk + L.off_track);
         ns = L.ns;
         lshift = L.lshift;
@@ -639,17 +671,34 @@ struct Eng {
     // constraint points
     DEVNI void cp_reset();
     DEVIN void cp_add_active(int s, I3 p, int sg);
-    DEVIN void cp_remove_active(int s);
+    DEVIN void cp_remove_active(int s, bool rec = false);
+    DEVIN void cp_remove_activated(int s);
     DEVIN void cp_update_endpoints(int s);
     DEVIN bool cp_endpoint_reached(int s, I3 p);
     DEVIN bool cp_walks_remain(int s, I3 p);
     DEVNI void cp_scan_network(int d);
     DEVNI void cp_find(int first, int count, int dir);
+    DEVIN bool ct_select_indices(int max_regrowth, int& first, int& count, int& dir);
     DEVNI void ctcb_grow_chain_scaffold(int first, int count, int dir);
     DEVNI void ctcb_grow_staple_from(int gold);
     DEVNI bool ctcb_scaffold_move(int max_regrowth);
+    DEVIN int rg_get_dir(int s);
+    DEVNI double rg_unassign_and_save(bool save);
+    DEVNI double rg_recoil_regrow();
+    DEVNI double rg_set_config(int d, I3 p, int o, int rec);
+    DEVIN void rg_prepare_for_growth(RgCur& c);
+    DEVNI double rg_prepare_for_regrowth(RgCur& c, int rec);
+    DEVNI void rg_restore_endpoints(int d);
+    DEVIN void rg_select(RgCur& c, I3& p, int& o);
+    DEVNI double rg_p_open(const RgCur& c, I3 p, int o);
+    DEVIN bool rg_test_open(double p);
+    DEVNI double rg_calc_weights(const PosOre* actual);
+    DEVNI bool rg_test_avail(RgCur& c);
+    DEVNI void rg_calc_old_c_opens();
+    DEVIN void rg_reset_active(bool remove_first);
+    DEVNI bool ctrg_scaffold_move(int max_regrowth);
     DEVIN bool attempt(int m);
-    DEVNI bool attempt_kind(int kind, int max_regrowth);
+    DEVNI bool attempt_kind(int kind, int max_regrowth, int m = -1);
     DEVIN void track_move(int m, bool acc);
     // learned domain regrowth (lane 0 parts)
     DEVNI bool chain_connected(int cs);

@@ -605,29 +605,77 @@ DEVIN void Eng::cp_add_active(int s, I3 p, int sg) {
     e.z = p.z;
 }
 
-// Constraintpoints::remove_active_endpoint (:299-320)
-DEVIN void Eng::cp_remove_active(int s) {
+// Constraintpoints::remove_active_endpoint (:299-320).  With rec, the
+// removed endpoints (get_erased_endpoints) go on top of the erased-endpoint
+// stack as one level.
+DEVIN void Eng::cp_remove_active(int s, bool rec) {
     int sg = seg[s];
     int cs = cslot_of(s), dd = d_of(s);
     int n = S->n_active;
-    int k = 0;
+    int k = 0, r = 0;
+    int top = rec ? S->er_top : 0;
     // lanes: order-preserving compaction, 64 entries at a time (an entry
     // moves to an index at or below its own, after every lane has read)
     for (int base = 0; base < n; base += pwidth()) {
         int i = base + plane();
-        bool keep = false;
+        bool keep = false, drop = false;
         Endpoint e;
         if (i < n) {
             e = active[i];
             keep = !(e.cslot == cs && e.seg == sg && e.d == dd);
+            drop = !keep;
         }
+        uint64_t below = (1ull << plane()) - 1;
         uint64_t m = pballot(keep);
+        uint64_t md = pballot(drop);
         if (keep) {
-            active[k + __builtin_popcountll(m & ((1ull << plane()) - 1))] = e;
+            active[k + __builtin_popcountll(m & below)] = e;
+        }
+        if (rec && drop) {
+            int j = top + r + __builtin_popcountll(md & below);
+            if (j < P->L.ecap) {
+                erstk[j] = e;
+            }
         }
         k += __builtin_popcountll(m);
+        r += __builtin_popcountll(md);
     }
     S->n_active = k;
+    if (rec) {
+        if (top + r > P->L.ecap || S->er_depth >= nslots) {
+            fail(ST_CAPACITY);
+            return;
+        }
+        ercnt[S->er_depth++] = (int16_t)r;
+        S->er_top = top + r;
+    }
+}
+
+// Constraintpoints::remove_activated_endpoint (:322-338): the first active
+// endpoint of the inactive-endpoint domain paired with s
+DEVIN void Eng::cp_remove_activated(int s) {
+    int e = inact[s];
+    if (e < 0) {
+        return;
+    }
+    int sg = seg[e];
+    int cs = cslot_of(e), dd = d_of(e);
+    int n = S->n_active;
+    int j = -1;
+    for (int i = 0; i < n; i++) {
+        const Endpoint& a = active[i];
+        if (a.cslot == cs && a.seg == sg && a.d == dd) {
+            j = i;
+            break;
+        }
+    }
+    if (j < 0) {
+        return;
+    }
+    for (int i = j; i + 1 < n; i++) {
+        active[i] = active[i + 1];
+    }
+    S->n_active = n - 1;
 }
 
 // Constraintpoints::update_endpoints (:340-349)
@@ -815,11 +863,18 @@ DEVNI void Eng::cp_scan_network(int d0) {
 // (top_constraint_points.cpp:207-221, 454-569) for the scaffold segment
 // first, first+dir, ... (count domains)
 DEVNI void Eng::cp_find(int first, int count, int dir) {
+    // CTRG grows Constraintpoints::m_d_stack (domains_to_be_regrown): each
+    // segment domain, then the domain stack of the network it roots
+    bool rec = S->kind == LDO_MOVE_KIND_CTRG;
+    S->n_rgs = 0;
     dir0[0] = (int8_t)dir;
     int d = first;
     for (int i = 0; i < count; i++) {
         if (i > 0) {
             d = walk(d, dir);
+        }
+        if (rec) {
+            rgs[S->n_rgs++] = (int16_t)d;
         }
         seg[d] = 0;
         int b = dom[d].bound;
@@ -854,6 +909,15 @@ DEVNI void Eng::cp_find(int first, int count, int dir) {
             }
             for (int a = 0; a < n_iaes; a++) {
                 inact[iaes[2 * a]] = iaes[2 * a + 1];
+            }
+            if (rec) {
+                if (S->n_rgs + n_netds > nslots) {
+                    fail(ST_CAPACITY);
+                    return;
+                }
+                for (int q = 0; q < n_netds; q++) {
+                    rgs[S->n_rgs++] = netds[q];
+                }
             }
             for (int q = 0; q < n_netds; q++) {
                 int s = netds[q];
@@ -998,12 +1062,10 @@ DEVNI void Eng::ctcb_grow_chain_scaffold(int first, int count, int dir) {
     }
 }
 
-// CTCBScaffoldRegrowthMCMovetype::internal_attempt_move (cb_movetypes.cpp:680-747)
-// with CTRegrowthMCMovetype::select_indices (movetypes.cpp:469-510)
-DEVNI bool Eng::ctcb_scaffold_move(int max_regrowth) {
-    S->regrow_old = 0;
-    cp_reset();
-    int first, count, dir;
+// CTRegrowthMCMovetype::select_indices (movetypes.cpp:469-510): the
+// segment first, first + dir, ... (count domains), and the active endpoint
+// of the domain past its end
+DEVIN bool Eng::ct_select_indices(int max_regrowth, int& first, int& count, int& dir) {
     int guard = 0;
     while (true) {
         int max_len = ns < max_regrowth ? ns : max_regrowth;
@@ -1040,7 +1102,17 @@ DEVNI bool Eng::ctcb_scaffold_move(int max_regrowth) {
         if (cur_end >= 0) {
             cp_add_active(cur_end, pos(cur_end), 0);
         }
-        break;
+        return true;
+    }
+}
+
+// CTCBScaffoldRegrowthMCMovetype::internal_attempt_move (cb_movetypes.cpp:680-747)
+DEVNI bool Eng::ctcb_scaffold_move(int max_regrowth) {
+    S->regrow_old = 0;
+    cp_reset();
+    int first, count, dir;
+    if (!ct_select_indices(max_regrowth, first, count, dir)) {
+        return false;
     }
     seg_first = first;
     seg_count = count;
@@ -1102,13 +1174,455 @@ DEVNI bool Eng::ctcb_scaffold_move(int max_regrowth) {
     return cb_test_acceptance();
 }
 
+// ------------------------------------------------- CTRG scaffold regrowth
+// CTRGRegrowthMCMovetype (rg_movetypes.cpp) with
+// CTRGScaffoldRegrowthMCMovetype::internal_attempt_move (:629-689).  The
+// moveset's excluded staples are always none (simulation.cpp:450-452), so
+// nothing is drawn for them and the excluded-staple test passes.  A trial
+// configuration is one of the 36 (site vector, orientation vector) pairs of
+// m_all_configs (:58, utility.hpp:168-177), index 6 * site + orientation;
+// the configurations a level has left are a bit mask, whose set bits in
+// ascending order are the reference's list after its erasures.
+
+// Constraintpoints::get_dir (top_constraint_points.cpp:179-185)
+DEVIN int Eng::rg_get_dir(int s) {
+    int cs = cslot_of(s);
+    return seg[s] == 1 ? dir1[cs] : dir0[cs];
+}
+
+// unassign_and_save_domains (:87-104) with save and modify,
+// unassign_domains (:122-129) with neither, and the third pass of the move
+// (whose saved positions are the ones the old configuration keeps in `old`)
+// with modify only
+DEVNI double Eng::rg_unassign_and_save(bool save) {
+    double de = 0;
+    int n = S->n_rgs;
+    for (int i = 0; i < n; i++) {
+        int d = rgs[i];
+        if (save) {
+            prev[d].x = dom[d].x;
+            prev[d].y = dom[d].y;
+            prev[d].z = dom[d].z;
+            prev[d].ore = dom[d].ore;
+        }
+        if (i > 0) {
+            push_modified(d);
+            de += unassign_domain(d);
+        }
+    }
+    S->er_top = 0;
+    S->er_depth = 0;
+    return de;
+}
+
+// set_config (:233-241) and, with rec = RG_NONE, the
+// set_checked_domain_config + update_endpoints of calc_weights (:388-392,
+// 412-418).  RG_LIST (recoil growth) records the assignment for
+// reset_origami; RG_DUP (feelers, calc_old_c_opens) only counts it: those
+// records repeat domains the recoil growth recorded first, and their second
+// unassignment in an accepting reset_origami changes nothing but the
+// unassignment count.
+DEVNI double Eng::rg_set_config(int d, I3 p, int o, int rec) {
+    double de = set_checked_domain_config(d, p, o);
+    if (rec == RG_LIST) {
+        push_assigned(d);
+    }
+    else if (rec == RG_DUP) {
+        rg_dup++;
+    }
+    cp_remove_active(d, true);
+    int e = inact[d];
+    if (e >= 0) {
+        cp_add_active(e, pos(d), seg[e]);
+    }
+    return de;
+}
+
+// prepare_for_growth (:243-260)
+DEVIN void Eng::rg_prepare_for_growth(RgCur& c) {
+    c.di++;
+    c.d = rgs[c.di];
+    c.stemd = stem[c.d] >= 0;
+    c.dir = rg_get_dir(c.d);
+    c.att = 0;
+    if (c.stemd) {
+        c.maxatt = 1;
+        c.avail = 0;
+        c.ref = stem[c.d];
+    }
+    else {
+        c.maxatt = rg_max_att;
+        c.avail = LDO_RG_ALL;
+        c.ref = walk(c.d, -c.dir);
+    }
+}
+
+// prepare_for_regrowth (:262-286)
+DEVNI double Eng::rg_prepare_for_regrowth(RgCur& c, int rec) {
+    c.di--;
+    c.d = rgs[c.di];
+    c.dir = rg_get_dir(c.d);
+    double de = unassign_domain(c.d);
+    if (rec == RG_LIST) {
+        S->n_assigned--;
+    }
+    else if (rec == RG_DUP) {
+        rg_dup--;
+    }
+    rg_restore_endpoints(c.d);
+    c.stemd = stem[c.d] >= 0;
+    if (c.stemd) {
+        c.maxatt = 1;
+        c.att = 1;
+        c.avail = 0;
+        c.ref = stem[c.d];
+    }
+    else {
+        c.maxatt = rg_max_att;
+        c.att = rgatt[c.di];
+        c.avail = rgavail[c.di];
+        c.ref = walk(c.d, -c.dir);
+    }
+    return de;
+}
+
+// restore_endpoints (:288-296)
+DEVNI void Eng::rg_restore_endpoints(int d) {
+    cp_remove_activated(d);
+    if (S->er_depth <= 0) {
+        fail(ST_MISUSE);
+        return;
+    }
+    int r = ercnt[--S->er_depth];
+    int top = S->er_top - r;
+    for (int k = 0; k < r; k++) {
+        const Endpoint& e = erstk[top + k];
+        cp_add_active(d, i3(e.x, e.y, e.z), seg[d]);
+    }
+    S->er_top = top;
+}
+
+// select_trial_config (:298-313)
+DEVIN void Eng::rg_select(RgCur& c, I3& p, int& o) {
+    if (c.stemd) {
+        p = pos(c.ref);
+        o = dom[c.ref].ore ^ 1;
+        return;
+    }
+    int k = uniform_int(0, __builtin_popcountll(c.avail) - 1);
+    int i = 0;
+    for (uint64_t m = c.avail;; m &= m - 1) {
+        if (k-- == 0) {
+            i = __builtin_ctzll(m);
+            break;
+        }
+    }
+    c.avail &= ~(1ull << i);
+    p = add3(pos(c.ref), unit3(i / 6));
+    o = i % 6;
+}
+
+// calc_p_config_open (:315-343)
+DEVNI double Eng::rg_p_open(const RgCur& c, I3 p, int o) {
+    double de = check_domain_constraints(c.d, p, o);
+    if (S->violated) {
+        S->violated = 0;
+        return 0;
+    }
+    if (!cp_walks_remain(c.d, p)) {
+        return 0;
+    }
+    if (occupancy(p) == OCC_UNBOUND) {
+        int ub = unbound_at(p);
+        bool same_chain = cslot_of(ub) == cslot_of(c.d);
+        bool endpoint = cp_endpoint_reached(c.d, p);
+        if (!(same_chain || endpoint || c.stemd)) {
+            return 0;
+        }
+    }
+    return fmin(1.0, exp(-de));
+}
+
+// test_config_open (:345-361)
+DEVIN bool Eng::rg_test_open(double p) {
+    p = fmin(1.0, p);
+    if (p == 1) {
+        return true;
+    }
+    return p > uniform_real();
+}
+
+// recoil_regrow (:177-231)
+DEVNI double Eng::rg_recoil_regrow() {
+    double de = 0;
+    int n = S->n_rgs;
+    RgCur c;
+    c.di = 0;
+    c.d = rgs[0];
+    c.dir = rg_get_dir(c.d);
+    rgopen[0] = 1;
+    rg_prepare_for_growth(c);
+    int recoils = 0;
+    while (S->status == ST_OK) {
+        I3 p = i3(0, 0, 0);
+        int o = 0;
+        double pc = 0;
+        bool open = false;
+        while (!open && c.att != c.maxatt) {
+            c.att++;
+            rg_select(c, p, o);
+            pc = rg_p_open(c, p, o);
+            open = rg_test_open(pc);
+        }
+        if (open) {
+            if (recoils != 0) {
+                recoils--;
+            }
+            de += rg_set_config(c.d, p, o, RG_LIST);
+            rgatt[c.di] = (int8_t)c.att;
+            rgavail[c.di] = c.avail;
+            rgopen[c.di] = pc;
+            if (c.di == n - 1) {
+                break;
+            }
+            rg_prepare_for_growth(c);
+        }
+        else {
+            if (recoils == rg_max_recoils || c.di == 1) {
+                S->rejected = 1;
+                break;
+            }
+            recoils++;
+            de += rg_prepare_for_regrowth(c, RG_LIST);
+        }
+    }
+    return de;
+}
+
+// test_config_avail (:426-480): feeler growth past level c.di
+DEVNI bool Eng::rg_test_avail(RgCur& c) {
+    int n = S->n_rgs;
+    int feels = 0;
+    if (feels == rg_max_recoils || c.di == n - 1) {
+        return true;
+    }
+    rg_prepare_for_growth(c);
+    bool avail = false;
+    while (S->status == ST_OK) {
+        I3 p = i3(0, 0, 0);
+        int o = 0;
+        bool open = false;
+        while (!open && c.att != c.maxatt) {
+            c.att++;
+            rg_select(c, p, o);
+            double pc = rg_p_open(c, p, o);
+            open = rg_test_open(pc);
+        }
+        if (open) {
+            feels++;
+            if (feels == rg_max_recoils || c.di == n - 1) {
+                feels--;
+                avail = true;
+                break;
+            }
+            rg_set_config(c.d, p, o, RG_DUP);
+            rgatt[c.di] = (int8_t)c.att;
+            rgavail[c.di] = c.avail;
+            rg_prepare_for_growth(c);
+        }
+        else {
+            if (feels == 0) {
+                avail = false;
+                break;
+            }
+            feels--;
+            rg_prepare_for_regrowth(c, RG_DUP);
+        }
+    }
+    while (feels != 0) {
+        c.di--;
+        feels--;
+        c.d = rgs[c.di];
+        unassign_domain(c.d);
+        rg_restore_endpoints(c.d);
+    }
+    c.di--;
+    c.d = rgs[c.di];
+    return avail;
+}
+
+// calc_weights (:363-421): the Rosenbluth-like weight of the configuration
+// in `actual`, built level by level from the attempts and configurations
+// the growth left (rgwatt, rgwavail) and the open probabilities (rgopen)
+DEVNI double Eng::rg_calc_weights(const PosOre* actual) {
+    int n = S->n_rgs;
+    double weight = 1;
+    RgCur c;
+    c.di = 0;
+    c.d = rgs[0];
+    while (c.di != n - 1 && S->status == ST_OK) {
+        c.di++;
+        c.d = rgs[c.di];
+        c.stemd = stem[c.d] >= 0;
+        c.dir = rg_get_dir(c.d);
+        int avail_cs = 1;
+        if (!c.stemd) {
+            c.ref = walk(c.d, -c.dir);
+            int att = rgwatt[c.di];
+            c.avail = rgwavail[c.di];
+            while (att != rg_max_att) {
+                att++;
+                I3 p;
+                int o;
+                rg_select(c, p, o);
+                double pc = rg_p_open(c, p, o);
+                if (rg_test_open(pc)) {
+                    rg_set_config(c.d, p, o, RG_NONE);
+                    RgCur keep = c;
+                    avail_cs += rg_test_avail(c) ? 1 : 0;
+                    c.avail = keep.avail;
+                    c.stemd = keep.stemd;
+                    c.ref = keep.ref;
+                    c.dir = keep.dir;
+                    unassign_domain(c.d);
+                    rg_restore_endpoints(c.d);
+                }
+            }
+        }
+        weight *= avail_cs / rgopen[c.di - 1];
+        const PosOre& a = actual[c.d];
+        rg_set_config(c.d, i3(a.x, a.y, a.z), a.ore, RG_NONE);
+    }
+    weight /= rgopen[c.di];
+    return weight;
+}
+
+// calc_old_c_opens (:482-512): open probabilities of the old configuration
+// (in `old`) and the configurations its growth would have left
+DEVNI void Eng::rg_calc_old_c_opens() {
+    int n = S->n_rgs;
+    RgCur c;
+    c.di = 0;
+    rgopen[0] = 1;
+    while (c.di != n - 1 && S->status == ST_OK) {
+        c.di++;
+        c.d = rgs[c.di];
+        rgatt[c.di] = 1;
+        I3 p = i3(old[c.d].x, old[c.d].y, old[c.d].z);
+        int o = old[c.d].ore;
+        c.stemd = stem[c.d] >= 0;
+        rgopen[c.di] = rg_p_open(c, p, o);
+        rg_set_config(c.d, p, o, RG_DUP);
+        if (c.stemd) {
+            rgavail[c.di] = 0;
+        }
+        else {
+            c.dir = rg_get_dir(c.d);
+            c.ref = walk(c.d, -c.dir);
+            I3 v = sub3(p, i3(old[c.ref].x, old[c.ref].y, old[c.ref].z));
+            int l1 = (v.x < 0 ? -v.x : v.x) + (v.y < 0 ? -v.y : v.y) + (v.z < 0 ? -v.z : v.z);
+            if (l1 != 1) {
+                fail(ST_CONSTRAINT);
+                return;
+            }
+            rgavail[c.di] = LDO_RG_ALL & ~(1ull << (6 * unit_index(v) + o));
+        }
+    }
+}
+
+// Constraintpoints::reset_active_endpoints (:295-297), then
+// remove_active_endpoint of the segment's first domain when asked
+DEVIN void Eng::rg_reset_active(bool remove_first) {
+    for (int i = plane(), n = S->n_active0; i < n; i += pwidth()) {
+        active[i] = active0[i];
+    }
+    S->n_active = S->n_active0;
+    if (remove_first) {
+        cp_remove_active(rgs[0]);
+    }
+}
+
+// CTRGScaffoldRegrowthMCMovetype::internal_attempt_move (:629-689) with
+// setup_constraints (:131-142), setup_for_calc_new_weights (:144-150),
+// setup_for_calc_old_weights (:152-160) and test_rg_acceptance (:514-533)
+DEVNI bool Eng::ctrg_scaffold_move(int max_regrowth) {
+    cp_reset();
+    int first, count, dir;
+    if (!ct_select_indices(max_regrowth, first, count, dir)) {
+        return false;
+    }
+    seg_first = first;
+    seg_count = count;
+    seg_dir = dir;
+    S->trk_len = count;
+    cp_find(first, count, dir);
+    if (S->status != ST_OK) {
+        return false;
+    }
+    if (!(P->cyclic && count == ns)) {
+        cp_remove_active(first);
+    }
+    int n = S->n_rgs;
+    bool whole = P->cyclic && n == ns;
+    rg_dup = 0;
+    S->delta_e += rg_unassign_and_save(true);
+    S->delta_e += rg_recoil_regrow();
+    if (S->rejected || S->status != ST_OK) {
+        return false;
+    }
+    S->delta_e += calc_move_bias(0); // add_external_bias (:82-85)
+
+    // weights of the new configuration (kept in prev)
+    for (int i = 0; i < n; i++) {
+        int d = rgs[i];
+        rgwatt[i] = rgatt[i];
+        rgwavail[i] = rgavail[i];
+        old[d] = prev[d];
+    }
+    S->n_modified = 0;
+    rg_unassign_and_save(true);
+    rg_reset_active(!whole);
+    double w_new = rg_calc_weights(prev);
+
+    // weights of the old configuration (kept in old); the reference's test
+    // before calc_old_c_opens (:665-668) differs from the other three
+    for (int i = 1; i < n; i++) {
+        unassign_domain(rgs[i]);
+    }
+    S->er_top = 0;
+    S->er_depth = 0;
+    rg_reset_active(!P->cyclic && n == ns);
+    rg_calc_old_c_opens();
+    for (int i = 0; i < n; i++) {
+        rgwatt[i] = rgatt[i];
+        rgwavail[i] = rgavail[i];
+    }
+    S->n_modified = 0;
+    rg_unassign_and_save(false);
+    rg_reset_active(!whole);
+    double w_old = rg_calc_weights(old);
+    if (S->status != ST_OK) {
+        return false;
+    }
+    double ratio = w_new / w_old * exp(-S->delta_e);
+    if (test_acceptance(ratio)) {
+        // reset_origami with the new configuration as the one to restore
+        reset_origami();
+        S->commits += rg_dup;
+        return true;
+    }
+    S->n_modified = 0;
+    S->n_assigned = 0;
+    return false;
+}
+
 // MCMovetype::attempt_move and the reset_internal chain (movetypes.cpp:41-51,
 // 242-250; met_movetypes.cpp:45-48, 139-143; cb_movetypes.cpp:44-50, 457-461)
 DEVIN bool Eng::attempt(int m) {
-    return attempt_kind(P->move_kind[m], P->move_maxreg[m]);
+    return attempt_kind(P->move_kind[m], P->move_maxreg[m], m);
 }
 
-DEVNI bool Eng::attempt_kind(int kind, int max_regrowth) {
+DEVNI bool Eng::attempt_kind(int kind, int max_regrowth, int m) {
     S->n_modified = 0;
     S->n_assigned = 0;
     S->n_added = 0;
@@ -1134,6 +1648,11 @@ DEVNI bool Eng::attempt_kind(int kind, int max_regrowth) {
     case 3:
         S->kind = LDO_MOVE_KIND_CTCB;
         return ctcb_scaffold_move(max_regrowth);
+    case 4:
+        S->kind = LDO_MOVE_KIND_CTRG;
+        rg_max_recoils = m >= 0 ? P->move_recoils[m] : 0;
+        rg_max_att = m >= 0 ? P->move_cattempts[m] : 36;
+        return ctrg_scaffold_move(max_regrowth);
     }
     fail(ST_MISUSE);
     return false;
@@ -1168,6 +1687,11 @@ DEVIN void Eng::track_move(int m, bool acc) {
         track[o + n1 + S->trk_len] += a;
         track[o + 2 * n1 + S->trk_nst] += 1;
         track[o + 2 * n1 + s1 + S->trk_nst] += a;
+    }
+    else if (kind == 4) {
+        int o = 6 * T + 2 * (ns + 1) + 2 * (P->L.smax + 1);
+        track[o + S->trk_len] += 1;
+        track[o + ns + 1 + S->trk_len] += a;
     }
 }
 

@@ -127,6 +127,12 @@ struct RepScalars {
     int32_t trk_nst;
     int32_t last_move; // move slot of the last step and whether it was accepted
     int32_t last_acc;
+    // CTRG: regrowth stack length, erased-endpoint stack height (positions)
+    // and depth (levels)
+    int32_t n_rgs;
+    int32_t er_top;
+    int32_t er_depth;
+    int32_t pad_rg;
 };
 
 // Byte offsets of the sections of a replica block, and sizes.
@@ -149,11 +155,17 @@ struct Layout {
     int64_t off_checked, off_netcs, off_regrow, off_dir0, off_dir1;
     int64_t off_active, off_active0, off_gps, off_iaes, off_netds, off_frames;
     int64_t off_part, off_sframes, off_gstack, off_gpl;
+    // CTRG recoil growth (empty without a CTRG move): regrowth stack, per
+    // level attempts (growth, weights), configurations left as a 36-bit mask
+    // (growth, weights) and open probability, erased-endpoint stack and counts
+    int64_t off_rgs, off_rgatt, off_rgwatt, off_rgavail, off_rgwavail, off_rgopen;
+    int64_t off_erstk, off_ercnt;
     // int64 tracking counters by staple type T = ntypes (index = identity)
     // and scaffold length / staple count: Met insertion attempts[T],
     // accepts[T], deletion attempts[T], accepts[T]; CB attempts[T],
     // accepts[T]; CTCB by segment length attempts[ns+1], accepts[ns+1]; by
-    // staples regrown attempts[smax+1], accepts[smax+1]
+    // staples regrown attempts[smax+1], accepts[smax+1]; CTRG by segment
+    // length attempts[ns+1], accepts[ns+1]
     int64_t off_track;
     int64_t bytes; // block size (multiple of 256)
 };
@@ -169,6 +181,8 @@ struct Params {
     int32_t move_kind[LDO_MAX_MOVES];
     double move_cum[LDO_MAX_MOVES];
     int32_t move_maxreg[LDO_MAX_MOVES];
+    int32_t move_recoils[LDO_MAX_MOVES];   // CTRG max_num_recoils
+    int32_t move_cattempts[LDO_MAX_MOVES]; // CTRG max_c_attempts
     int32_t max_total_staples;
     int32_t max_type_staples;
     int32_t centering_freq;
@@ -238,6 +252,7 @@ struct LearnedMove {
 #define LDO_MOVE_KIND_MET 1
 #define LDO_MOVE_KIND_CB 2
 #define LDO_MOVE_KIND_CTCB 3
+#define LDO_MOVE_KIND_CTRG 4
 
 // status codes written into RepScalars::status
 #define ST_OK 0

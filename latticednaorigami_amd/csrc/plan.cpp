@@ -33,7 +33,7 @@ void mlp_layout(int n_ident, ldo_mlp_layout& L) {
     L.total = o;
 }
 
-Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap) {
+Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap, bool rg) {
     Layout L;
     std::memset(&L, 0, sizeof L);
     L.ns = ns;
@@ -102,7 +102,20 @@ Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap) {
     take(L.off_sframes, C * (int64_t)sizeof(NetFrame));
     take(L.off_gstack, LDO_GROW_DEPTH * (int64_t)sizeof(GrowFrame));
     take(L.off_gpl, 2 * LDO_MAX_STAPLE_LEN * 2);
-    L.track_len = 6 * ntypes + 2 * (ns + 1) + 2 * (smax + 1);
+    // CTRG recoil growth, only when the moveset holds it: the regrowth
+    // stack, per level the attempts made and configurations left (growth and
+    // weight copies) and the open probability, and the erased endpoints of
+    // the assigned levels (a stack of positions and one count per level)
+    int64_t R = rg ? S : 0;
+    take(L.off_rgs, R * 2);
+    take(L.off_rgatt, R);
+    take(L.off_rgwatt, R);
+    take(L.off_rgavail, R * 8);
+    take(L.off_rgwavail, R * 8);
+    take(L.off_rgopen, R * 8);
+    take(L.off_erstk, (rg ? (int64_t)L.ecap : 0) * (int64_t)sizeof(Endpoint));
+    take(L.off_ercnt, R * 2);
+    L.track_len = 6 * ntypes + 4 * (ns + 1) + 2 * (smax + 1);
     take(L.off_track, (int64_t)L.track_len * 8);
     L.bytes = round_up(off, 256);
     return L;
@@ -191,7 +204,11 @@ void make_plan(const ldo_system_desc* sys, const ldo_run_params* run, int32_t n_
     int tcap = std::min(std::max(run->max_type_staples, 1), smax);
     Params& P = out.P;
     std::memset(&P, 0, sizeof P);
-    P.L = make_layout(ns, lmax, smax, sys->n_chain_types, tcap);
+    bool rg = false;
+    for (int m = 0; m < run->n_movetypes; m++) {
+        rg = rg || run->movetype_kind[m] == LDO_MOVE_CTRG_SCAFFOLD_REGROWTH;
+    }
+    P.L = make_layout(ns, lmax, smax, sys->n_chain_types, tcap, rg);
     P.n_replicas = n_replicas;
     P.domain_type = sys->domain_type;
     P.misbinding_pot = sys->misbinding_pot;
@@ -200,15 +217,30 @@ void make_plan(const ldo_system_desc* sys, const ldo_run_params* run, int32_t n_
     double cum = 0;
     for (int m = 0; m < run->n_movetypes; m++) {
         int kind = run->movetype_kind[m];
-        if (kind < 0 || kind > 3) {
+        if (kind < 0 || kind > 4) {
             throw AbiError {LDO_E_NOTIMPL, "movetype not supported"};
         }
         P.move_kind[m] = kind;
         cum += run->movetype_freq[m];
         P.move_cum[m] = cum;
         P.move_maxreg[m] = run->movetype_max_regrowth ? run->movetype_max_regrowth[m] : 0;
-        if (kind == 3 && P.move_maxreg[m] < 2) {
-            throw AbiError {LDO_E_ARG, "CTCB max_regrowth must be at least 2"};
+        P.move_recoils[m] = 0;
+        P.move_cattempts[m] = 0;
+        if ((kind == 3 || kind == 4) && P.move_maxreg[m] < 2) {
+            throw AbiError {LDO_E_ARG, "scaffold regrowth max_regrowth must be at least 2"};
+        }
+        if (kind == 4) {
+            if (!run->movetype_max_num_recoils || !run->movetype_max_c_attempts) {
+                throw AbiError {LDO_E_ARG, "CTRG needs max_num_recoils and max_c_attempts"};
+            }
+            P.move_recoils[m] = run->movetype_max_num_recoils[m];
+            P.move_cattempts[m] = run->movetype_max_c_attempts[m];
+            // the trial configurations of a domain are the 36 (site,
+            // orientation) pairs, drawn without replacement
+            if (P.move_recoils[m] < 0 || P.move_cattempts[m] < 1 || P.move_cattempts[m] > 36) {
+                throw AbiError {LDO_E_ARG,
+                                "CTRG needs max_num_recoils >= 0 and 1 <= max_c_attempts <= 36"};
+            }
         }
     }
     P.max_total_staples = smax;

@@ -138,8 +138,8 @@ class Sampler:
 
     def move_tracking(self, r=0):
         """Per-move tracking counts of replica r (ldo_get_move_tracking)."""
-        cap = 6 * len(self.system.identities) + 2 * (len(self.system.identities[0]) + 1) \
-            + 2 * (self.system.max_total_staples + 1)
+        cap = abi.tracking_len(len(self.system.identities), len(self.system.identities[0]),
+                               self.system.max_total_staples)
         buf = (C.c_int64 * cap)()
         n = C.c_int64()
         self._check(self.lib.ldo_get_move_tracking(self.h, r, buf, cap, C.byref(n)))

@@ -28,6 +28,9 @@ class Move:
     kind: int
     freq: float
     max_regrowth: int = 0
+    # CTRG recoil growth: recoils allowed and trial configurations per domain
+    max_num_recoils: int = 0
+    max_c_attempts: int = 0
 
 
 @dataclass
@@ -121,6 +124,8 @@ class System:
         rp.movetype_kind = arr(C.c_int32, [m.kind for m in self.moves])
         rp.movetype_freq = arr(C.c_double, [m.freq for m in self.moves])
         rp.movetype_max_regrowth = arr(C.c_int32, [m.max_regrowth for m in self.moves])
+        rp.movetype_max_num_recoils = arr(C.c_int32, [m.max_num_recoils for m in self.moves])
+        rp.movetype_max_c_attempts = arr(C.c_int32, [m.max_c_attempts for m in self.moves])
         rp.max_total_staples = self.max_total_staples
         rp.max_type_staples = self.max_type_staples
         rp.max_staple_size = self.max_staple_size
@@ -217,7 +222,8 @@ def read_moveset_json(path_or_dict):
         if m["type"] not in abi.MOVETYPE_NAMES:
             raise NotImplementedError(f"{m['type']}: movetype not supported")
         moves.append(Move(abi.MOVETYPE_NAMES[m["type"]], parse_fraction(m["freq"]),
-                          int(m.get("max_regrowth", 0))))
+                          int(m.get("max_regrowth", 0)), int(m.get("max_num_recoils", 0)),
+                          int(m.get("max_c_attempts", 0))))
     return moves
 
 
@@ -228,6 +234,17 @@ def standard_moveset(max_regrowth):
             Move(abi.MOVE_MET_STAPLE_EXCHANGE, 4 / 10),
             Move(abi.MOVE_CB_STAPLE_REGROWTH, 3 / 10),
             Move(abi.MOVE_CTCB_SCAFFOLD_REGROWTH, 2 / 10, max_regrowth)]
+
+
+def ctrg_moveset(max_regrowth, max_num_recoils=1, max_c_attempts=36):
+    """The reference's examples/moveset_standard.json with its contiguous
+    CTRG scaffold regrowth taking the non-contiguous one's share: orientation
+    2/6, staple exchange 1/6, CB staple 1/6, CTRG 2/6."""
+    return [Move(abi.MOVE_ORIENTATION_ROTATION, 2 / 6),
+            Move(abi.MOVE_MET_STAPLE_EXCHANGE, 1 / 6),
+            Move(abi.MOVE_CB_STAPLE_REGROWTH, 1 / 6),
+            Move(abi.MOVE_CTRG_SCAFFOLD_REGROWTH, 2 / 6, max_regrowth, max_num_recoils,
+                 max_c_attempts)]
 
 
 def four_domain_system(seed=20261202, max_total_staples=4):

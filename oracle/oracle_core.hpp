@@ -296,6 +296,8 @@ struct MovesetEntry {
     int kind;
     double freq;
     int max_regrowth;
+    int max_num_recoils {0}; // CTRG (simulation.cpp:493-495)
+    int max_c_attempts {0};
 };
 
 struct Limits {
@@ -342,7 +344,11 @@ class Constraintpoints {
     void add_active_endpoint(Dom* d, V3 pos, int seg);
     void reset_active_endpoints() { m_active = m_initial_active; }
     void remove_active_endpoint(Dom* d);
+    void remove_activated_endpoint(Dom* d);
     void update_endpoints(Dom* d);
+    const std::vector<V3>& get_erased_endpoints() const { return m_erased; }
+    const std::vector<Dom*>& domains_to_be_regrown() const { return m_d_stack; }
+    Dom* get_growthpoint(Dom* d) { return m_stemdomains[d]; }
     Dom* domain_to_grow(Dom* d) { return m_growthpoints[d]; }
     bool endpoint_reached(Dom* d, V3 pos);
     bool walks_remain(Dom* d, V3 pos);
@@ -374,6 +380,7 @@ class Constraintpoints {
     std::map<std::pair<int, int>, std::vector<std::pair<int, V3>>> m_active;
     std::map<std::pair<int, int>, std::vector<std::pair<int, V3>>> m_initial_active;
     std::unordered_map<Dom*, Dom*> m_inactive;
+    std::vector<V3> m_erased; // positions the last remove_active_endpoint took
 };
 
 // One replica: system + moves + RNG (the per-rank GCMCSimulation state)
@@ -489,6 +496,41 @@ class Replica {
                                      unsigned max_regrowth);
     void ctcb_grow_chain(const std::vector<Dom*>& ds);
     void ctcb_grow_staple_and_update_endpoints(Dom* gold);
+
+    // CTRGRegrowthMCMovetype (rg_movetypes.hpp): the regrowth stack, per
+    // level the trial attempts made and configuration indices left (growth
+    // and weight copies), the open probability and the erased endpoints, and
+    // the level being grown
+    using Cfg = std::pair<V3, V3>;
+    std::vector<Dom*> m_regrow_ds;
+    std::vector<int> m_c_attempts_q, m_c_attempts_wq;
+    std::vector<std::vector<int>> m_avail_cis_q, m_avail_cis_wq;
+    std::vector<double> m_c_opens;
+    std::vector<std::vector<V3>> m_erased_q;
+    std::map<std::pair<int, int>, V3> m_new_pos, m_new_ore;
+    size_t m_di {0};
+    Dom* m_d {nullptr};
+    Dom* m_ref_d {nullptr};
+    bool m_stemd {false};
+    int m_c_attempts {0}, m_d_max_c_attempts {0};
+    std::vector<int> m_avail_cis;
+    double m_weight {1}, m_weight_new {1};
+    int m_max_recoils {0}, m_max_c_attempts {0};
+    bool ctrg_scaffold_move(int max_regrowth);
+    double rg_unassign_and_save();
+    void rg_unassign_domains();
+    double rg_recoil_regrow();
+    double rg_set_config(Dom* d, Cfg c);
+    void rg_prepare_for_growth();
+    double rg_prepare_for_regrowth();
+    void rg_restore_endpoints();
+    Cfg rg_select_trial_config();
+    double rg_calc_p_config_open(Cfg c);
+    bool rg_test_config_open(double p);
+    void rg_calc_weights();
+    bool rg_test_config_avail();
+    void rg_calc_old_c_opens();
+    bool rg_test_acceptance();
 };
 
 // ------------------------------------------------------------ enumeration

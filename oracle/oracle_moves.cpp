@@ -27,6 +27,7 @@ void Constraintpoints::reset() {
     m_active.clear();
     m_initial_active.clear();
     m_inactive.clear();
+    m_erased.clear();
     m_net_scaffold_ds.clear();
     m_net_dirs.clear();
 }
@@ -56,6 +57,7 @@ void Constraintpoints::add_active_endpoint(Dom* d, V3 pos, int seg) {
 
 // top_constraint_points.cpp:299-320
 void Constraintpoints::remove_active_endpoint(Dom* d) {
+    m_erased.clear();
     int seg = m_segs.at(d);
     auto& lst = m_active[{d->c, seg}];
     std::vector<std::pair<int, V3>> kept;
@@ -63,8 +65,28 @@ void Constraintpoints::remove_active_endpoint(Dom* d) {
         if (e.first != d->d) {
             kept.push_back(e);
         }
+        else {
+            m_erased.push_back(e.second);
+        }
     }
     lst = kept;
+}
+
+// top_constraint_points.cpp:322-338: the first endpoint of the inactive
+// endpoint domain paired with d goes
+void Constraintpoints::remove_activated_endpoint(Dom* d) {
+    auto it = m_inactive.find(d);
+    if (it == m_inactive.end()) {
+        return;
+    }
+    Dom* ed = it->second;
+    auto& lst = m_active[{ed->c, m_segs.at(ed)}];
+    for (size_t j = 0; j < lst.size(); j++) {
+        if (lst[j].first == ed->d) {
+            lst.erase(lst.begin() + (long)j);
+            break;
+        }
+    }
 }
 
 // top_constraint_points.cpp:340-349
@@ -292,7 +314,7 @@ void Replica::add_tracker(int slot, bool accepted) {
     int ns = (int)sys.identities[0].size();
     int a = accepted ? 1 : 0;
     if (track.empty()) {
-        track.assign(6 * T + 2 * (ns + 1) + 2 * (m_lim.max_total_staples + 1), 0);
+        track.assign(6 * T + 4 * (ns + 1) + 2 * (m_lim.max_total_staples + 1), 0);
     }
     switch (m_moves[slot].kind) {
     case LDO_MOVE_MET_STAPLE_EXCHANGE: {
@@ -313,6 +335,13 @@ void Replica::add_tracker(int slot, bool accepted) {
         track[o + n1 + tracker.num_scaffold_domains] += a;
         track[o + 2 * n1 + tracker.num_staples] += 1;
         track[o + 2 * n1 + s1 + tracker.num_staples] += a;
+        break;
+    }
+    case LDO_MOVE_CTRG_SCAFFOLD_REGROWTH: {
+        // rg_movetypes.cpp:583-627 reads num_scaffold_domains only
+        int o = 6 * T + 2 * (ns + 1) + 2 * (m_lim.max_total_staples + 1);
+        track[o + tracker.num_scaffold_domains] += 1;
+        track[o + ns + 1 + tracker.num_scaffold_domains] += a;
         break;
     }
     default:
@@ -338,6 +367,8 @@ void Replica::simulate(int64_t steps) {
         }
         m_kind = m_moves[i].kind;
         attempts[i]++;
+        m_max_recoils = m_moves[i].max_num_recoils;
+        m_max_c_attempts = m_moves[i].max_c_attempts;
         bool accepted = attempt(m_moves[i].kind, m_moves[i].max_regrowth);
         accepts[i] += accepted;
         add_tracker((int)i, accepted);
@@ -392,6 +423,19 @@ bool Replica::attempt(int kind, int max_regrowth) {
     case LDO_MOVE_CTCB_SCAFFOLD_REGROWTH:
         m_cp.reset();
         return ctcb_scaffold_move(max_regrowth);
+    case LDO_MOVE_CTRG_SCAFFOLD_REGROWTH:
+        // CTRGRegrowthMCMovetype::reset_internal (rg_movetypes.cpp:66-80)
+        m_cp.reset();
+        m_regrow_ds.clear();
+        m_c_attempts_q.clear();
+        m_c_attempts_wq.clear();
+        m_avail_cis_q.clear();
+        m_avail_cis_wq.clear();
+        m_c_opens.clear();
+        m_erased_q.clear();
+        m_weight = 1;
+        m_weight_new = 1;
+        return ctrg_scaffold_move(max_regrowth);
     }
     throw OrigamiError(LDO_E_NOTIMPL, "movetype not supported");
 }
@@ -1022,6 +1066,403 @@ bool Replica::ctcb_scaffold_move(int max_regrowth) {
     ctcb_grow_chain(sds);
     m_modifier = 1;
     return cb_test_acceptance();
+}
+
+// ------------------------------------------ CTRG scaffold regrowth move
+// CTRGRegrowthMCMovetype (rg_movetypes.cpp) with
+// CTRGScaffoldRegrowthMCMovetype::internal_attempt_move (:629-689).  Excluded
+// staples are always empty (simulation.cpp:450-452), so sel_excluded_staples
+// draws nothing and excluded_staples_bound is false.
+
+// m_all_configs = all_pairs(utility::vectors) (:58, utility.hpp:168-177):
+// index 6 * (site vector) + (orientation vector)
+static std::pair<V3, V3> rg_config(int i) { return {kUnit[i / 6], kUnit[i % 6]}; }
+
+static int rg_vector_index(V3 v) {
+    for (int k = 0; k < 6; k++) {
+        if (kUnit[k] == v) {
+            return k;
+        }
+    }
+    throw OrigamiError(LDO_E_CONSTRAINT, "CTRG: domain not adjacent to its reference");
+}
+
+// :87-104
+double Replica::rg_unassign_and_save() {
+    double de = 0;
+    Dom* d = m_regrow_ds[0];
+    m_prev_pos[key_of(d)] = d->pos;
+    m_prev_ore[key_of(d)] = d->ore;
+    for (size_t i = 1; i < m_regrow_ds.size(); i++) {
+        d = m_regrow_ds[i];
+        m_prev_pos[key_of(d)] = d->pos;
+        m_prev_ore[key_of(d)] = d->ore;
+        m_modified.push_back(key_of(d));
+        de += sys.unassign_domain(*d);
+    }
+    m_erased_q.clear();
+    return de;
+}
+
+// :122-129
+void Replica::rg_unassign_domains() {
+    for (size_t i = 1; i < m_regrow_ds.size(); i++) {
+        sys.unassign_domain(*m_regrow_ds[i]);
+    }
+    m_erased_q.clear();
+}
+
+// :177-231
+double Replica::rg_recoil_regrow() {
+    double de = 0;
+    m_di = 0;
+    m_d = m_regrow_ds[0];
+    m_dir = m_cp.get_dir(m_d);
+    size_t n = m_regrow_ds.size();
+    m_c_attempts_q.resize(n);
+    m_avail_cis_q.resize(n);
+    m_c_opens.resize(n);
+    m_c_opens[0] = 1;
+    rg_prepare_for_growth();
+    int recoils = 0;
+    while (true) {
+        Cfg c;
+        double p = 0;
+        bool open = false;
+        while (!open && m_c_attempts != m_d_max_c_attempts) {
+            m_c_attempts++;
+            c = rg_select_trial_config();
+            p = rg_calc_p_config_open(c);
+            open = rg_test_config_open(p);
+        }
+        if (open) {
+            if (recoils != 0) {
+                recoils--;
+            }
+            de += rg_set_config(m_d, c);
+            m_c_attempts_q[m_di] = m_c_attempts;
+            m_avail_cis_q[m_di] = m_avail_cis;
+            m_c_opens[m_di] = p;
+            if (m_di == n - 1) {
+                break;
+            }
+            rg_prepare_for_growth();
+        }
+        else {
+            if (recoils == m_max_recoils || m_di == 1) {
+                m_rejected = true;
+                break;
+            }
+            recoils++;
+            de += rg_prepare_for_regrowth();
+        }
+    }
+    return de;
+}
+
+// :233-241
+double Replica::rg_set_config(Dom* d, Cfg c) {
+    double de = sys.set_checked_domain_config(*d, c.first, c.second);
+    m_assigned.push_back(key_of(d));
+    m_cp.update_endpoints(m_d);
+    m_erased_q.push_back(m_cp.get_erased_endpoints());
+    return de;
+}
+
+// :243-260
+void Replica::rg_prepare_for_growth() {
+    m_di++;
+    m_d = m_regrow_ds[m_di];
+    m_stemd = m_cp.is_stemdomain(m_d);
+    m_dir = m_cp.get_dir(m_d);
+    m_c_attempts = 0;
+    if (m_stemd) {
+        m_d_max_c_attempts = 1;
+        m_avail_cis.clear();
+        m_ref_d = m_cp.get_growthpoint(m_d);
+    }
+    else {
+        m_d_max_c_attempts = m_max_c_attempts;
+        m_avail_cis.resize(36);
+        for (int i = 0; i < 36; i++) {
+            m_avail_cis[i] = i;
+        }
+        m_ref_d = walk(m_d, -m_dir);
+    }
+}
+
+// :262-286
+double Replica::rg_prepare_for_regrowth() {
+    m_di--;
+    m_d = m_regrow_ds[m_di];
+    m_dir = m_cp.get_dir(m_d);
+    double de = sys.unassign_domain(*m_d);
+    m_assigned.pop_back();
+    rg_restore_endpoints();
+    m_stemd = m_cp.is_stemdomain(m_d);
+    if (m_stemd) {
+        m_d_max_c_attempts = 1;
+        m_c_attempts = 1;
+        m_avail_cis.clear();
+        m_ref_d = m_cp.get_growthpoint(m_d);
+    }
+    else {
+        m_d_max_c_attempts = m_max_c_attempts;
+        m_c_attempts = m_c_attempts_q[m_di];
+        m_avail_cis = m_avail_cis_q[m_di];
+        m_ref_d = walk(m_d, -m_dir);
+    }
+    return de;
+}
+
+// :288-296
+void Replica::rg_restore_endpoints() {
+    m_cp.remove_activated_endpoint(m_d);
+    std::vector<V3> erased = m_erased_q.back();
+    m_erased_q.pop_back();
+    for (V3 p: erased) {
+        m_cp.add_active_endpoint(m_d, p);
+    }
+}
+
+// :298-313
+Replica::Cfg Replica::rg_select_trial_config() {
+    if (m_stemd) {
+        return {m_ref_d->pos, -m_ref_d->ore};
+    }
+    int ci = rng.uniform_int(0, (int)m_avail_cis.size() - 1);
+    int i = m_avail_cis[ci];
+    m_avail_cis.erase(m_avail_cis.begin() + ci);
+    Cfg c = rg_config(i);
+    c.first = c.first + m_ref_d->pos;
+    return c;
+}
+
+// :315-343
+double Replica::rg_calc_p_config_open(Cfg c) {
+    double de = sys.check_domain_constraints(*m_d, c.first, c.second);
+    if (sys.violated) {
+        sys.violated = false;
+        return 0;
+    }
+    if (!m_cp.walks_remain(m_d, c.first)) {
+        return 0;
+    }
+    if (sys.occupancy(c.first) == Occ::unbound) {
+        Dom* occ = sys.unbound_at(c.first);
+        bool same_chain = occ->c == m_d->c;
+        bool endpoint = m_cp.endpoint_reached(m_d, c.first);
+        if (!(same_chain || endpoint || m_stemd)) {
+            return 0;
+        }
+    }
+    return std::fmin(1, std::exp(-de));
+}
+
+// :345-361
+bool Replica::rg_test_config_open(double p) {
+    p = std::fmin(1, p);
+    if (p == 1) {
+        return true;
+    }
+    return p > rng.uniform_real();
+}
+
+// :363-421
+void Replica::rg_calc_weights() {
+    m_di = 0;
+    m_d = m_regrow_ds[0];
+    while (m_di != m_regrow_ds.size() - 1) {
+        m_di++;
+        m_d = m_regrow_ds[m_di];
+        m_stemd = m_cp.is_stemdomain(m_d);
+        m_dir = m_cp.get_dir(m_d);
+        int avail_cs = 1;
+        if (!m_stemd) {
+            m_ref_d = walk(m_d, -m_dir);
+            int c_attempts = m_c_attempts_wq[m_di];
+            m_avail_cis = m_avail_cis_wq[m_di];
+            while (c_attempts != m_max_c_attempts) {
+                c_attempts++;
+                Cfg c = rg_select_trial_config();
+                double p = rg_calc_p_config_open(c);
+                if (rg_test_config_open(p)) {
+                    sys.set_checked_domain_config(*m_d, c.first, c.second);
+                    m_cp.update_endpoints(m_d);
+                    m_erased_q.push_back(m_cp.get_erased_endpoints());
+                    int dir = m_dir;
+                    Dom* ref_d = m_ref_d;
+                    bool stemd = m_stemd;
+                    std::vector<int> avail = m_avail_cis;
+                    avail_cs += rg_test_config_avail();
+                    m_avail_cis = avail;
+                    m_stemd = stemd;
+                    m_ref_d = ref_d;
+                    m_dir = dir;
+                    sys.unassign_domain(*m_d);
+                    rg_restore_endpoints();
+                }
+            }
+        }
+        m_weight *= avail_cs / m_c_opens[m_di - 1];
+        auto key = key_of(m_d);
+        sys.set_checked_domain_config(*m_d, m_prev_pos[key], m_prev_ore[key]);
+        m_cp.update_endpoints(m_d);
+        m_erased_q.push_back(m_cp.get_erased_endpoints());
+    }
+    m_weight /= m_c_opens[m_di];
+}
+
+// :426-480
+bool Replica::rg_test_config_avail() {
+    int feels = 0;
+    if (feels == m_max_recoils || m_di == m_regrow_ds.size() - 1) {
+        return true;
+    }
+    rg_prepare_for_growth();
+    bool avail;
+    while (true) {
+        Cfg c;
+        bool open = false;
+        while (!open && m_c_attempts != m_d_max_c_attempts) {
+            m_c_attempts++;
+            c = rg_select_trial_config();
+            double p = rg_calc_p_config_open(c);
+            open = rg_test_config_open(p);
+        }
+        if (open) {
+            feels++;
+            if (feels == m_max_recoils || m_di == m_regrow_ds.size() - 1) {
+                feels--;
+                avail = true;
+                break;
+            }
+            rg_set_config(m_d, c);
+            m_c_attempts_q[m_di] = m_c_attempts;
+            m_avail_cis_q[m_di] = m_avail_cis;
+            rg_prepare_for_growth();
+        }
+        else {
+            if (feels == 0) {
+                avail = false;
+                break;
+            }
+            feels--;
+            rg_prepare_for_regrowth();
+        }
+    }
+    while (feels != 0) {
+        m_di--;
+        feels--;
+        m_d = m_regrow_ds[m_di];
+        sys.unassign_domain(*m_d);
+        rg_restore_endpoints();
+    }
+    m_di--;
+    m_d = m_regrow_ds[m_di];
+    return avail;
+}
+
+// :482-512
+void Replica::rg_calc_old_c_opens() {
+    m_di = 0;
+    m_c_opens[0] = 1;
+    while (m_di != m_regrow_ds.size() - 1) {
+        m_di++;
+        m_d = m_regrow_ds[m_di];
+        m_c_attempts_q[m_di] = 1;
+        auto key = key_of(m_d);
+        V3 p = m_old_pos[key];
+        Cfg c {p, m_old_ore[key]};
+        m_stemd = m_cp.is_stemdomain(m_d);
+        m_c_opens[m_di] = rg_calc_p_config_open(c);
+        rg_set_config(m_d, c);
+        if (m_stemd) {
+            m_avail_cis_q[m_di].clear();
+        }
+        else {
+            std::vector<int> all(36);
+            for (int i = 0; i < 36; i++) {
+                all[i] = i;
+            }
+            m_dir = m_cp.get_dir(m_d);
+            m_ref_d = walk(m_d, -m_dir);
+            V3 pref = m_old_pos[key_of(m_ref_d)];
+            int ci = 6 * rg_vector_index(p - pref) + rg_vector_index(c.second);
+            all.erase(std::remove(all.begin(), all.end(), ci), all.end());
+            m_avail_cis_q[m_di] = all;
+        }
+    }
+}
+
+// :514-533
+bool Replica::rg_test_acceptance() {
+    long double ratio = m_weight_new / m_weight * std::exp(-m_delta_e);
+    if (test_acceptance(ratio)) {
+        m_prev_pos = m_new_pos;
+        m_prev_ore = m_new_ore;
+        reset_origami();
+        return true;
+    }
+    m_modified.clear();
+    m_assigned.clear();
+    return false;
+}
+
+// :629-689 with setup_constraints (:131-142), setup_for_calc_new_weights
+// (:144-150) and setup_for_calc_old_weights (:152-160)
+bool Replica::ctrg_scaffold_move(int max_regrowth) {
+    std::vector<Dom*> scaffold = sys.chains[0];
+    std::vector<Dom*> sds = select_indices(scaffold, 2, (unsigned)max_regrowth);
+    tracker.num_scaffold_domains = (int)sds.size();
+    m_cp.calculate(sds, m_dir);
+    if (!(sys.cyclic && sds.size() == scaffold.size())) {
+        m_cp.remove_active_endpoint(sds[0]);
+    }
+    m_regrow_ds = m_cp.domains_to_be_regrown();
+    bool whole = sys.cyclic && m_regrow_ds.size() == scaffold.size();
+    m_delta_e += rg_unassign_and_save();
+    m_delta_e += rg_recoil_regrow();
+    if (m_rejected) {
+        return false;
+    }
+    m_delta_e += calc_move_bias(0); // add_external_bias (:82-85)
+
+    // weights of the new configuration
+    m_c_attempts_wq = m_c_attempts_q;
+    m_avail_cis_wq = m_avail_cis_q;
+    m_old_pos = m_prev_pos;
+    m_old_ore = m_prev_ore;
+    m_modified.clear();
+    rg_unassign_and_save();
+    m_cp.reset_active_endpoints();
+    if (!whole) {
+        m_cp.remove_active_endpoint(m_regrow_ds[0]);
+    }
+    rg_calc_weights();
+
+    // weights of the old configuration; the reference's test before
+    // calc_old_c_opens (:665-668) differs from the other three
+    rg_unassign_domains();
+    m_cp.reset_active_endpoints();
+    if (!sys.cyclic && m_regrow_ds.size() == scaffold.size()) {
+        m_cp.remove_active_endpoint(m_regrow_ds[0]);
+    }
+    rg_calc_old_c_opens();
+    m_weight_new = m_weight;
+    m_weight = 1;
+    m_c_attempts_wq = m_c_attempts_q;
+    m_avail_cis_wq = m_avail_cis_q;
+    m_new_pos = m_prev_pos;
+    m_new_ore = m_prev_ore;
+    m_modified.clear();
+    rg_unassign_and_save();
+    m_cp.reset_active_endpoints();
+    if (!whole) {
+        m_cp.remove_active_endpoint(m_regrow_ds[0]);
+    }
+    rg_calc_weights();
+    return rg_test_acceptance();
 }
 
 // ------------------------------------------------------ move-level biases

@@ -632,3 +632,75 @@ def case_learned_rejects_single_domain_staples(sampler_cls, oracle_mod):
     with pytest.raises(abi.LdoError) as e:
         gpu.learned_samples(16)
     assert e.value.code == abi.LDO_E_NOTIMPL
+
+
+def _ctrg_chain(sampler_cls, oracle_mod, s, temps, rounds, steps):
+    gpu = sampler_cls(s, temps=temps)
+    ref = oracle_mod.Oracle(s, temps=temps)
+    for _ in range(rounds):
+        gpu.sweep(steps)
+        ref.sweep(steps)
+        for r in range(len(temps)):
+            assert_same(gpu, ref, r)
+    assert_counters(gpu, ref)
+    for r in range(len(temps)):
+        tr = gpu.move_tracking(r)
+        assert tr == ref.move_tracking(r)
+        slot = [m.kind for m in s.moves].index(abi.MOVE_CTRG_SCAFFOLD_REGROWTH)
+        assert sum(tr["ctrg_length_attempts"]) <= gpu.counters().attempts[slot]
+    return gpu
+
+
+def case_ctrg_scaffold_regrowth_bit_exact(sampler_cls, oracle_mod):
+    """CTRG recoil regrowth (rg_movetypes.cpp) with the reference's example
+    limits (max_num_recoils 1, max_c_attempts 36): the four-domain system,
+    whose segments reach both scaffold ends, and the 64-domain raster, where
+    staples bound to the segment are regrown with it (stem domains, inactive
+    endpoints) and networks bound outside it pin their scaffold domains."""
+    say("case: CTRG scaffold regrowth, four-domain at 330/345 K (3 x 5000 steps) and "
+        "64-domain raster at 300/360 K (3 x 5000 steps)")
+    four = systems.four_domain_system()
+    four.moves = systems.ctrg_moveset(4)
+    _ctrg_chain(sampler_cls, oracle_mod, four, [330.0, 345.0], 3, 5000)
+    raster = systems.raster_system(replicas=2)
+    raster.moves = systems.ctrg_moveset(12)
+    g = _ctrg_chain(sampler_cls, oracle_mod, raster, raster.temps, 3, 5000)
+    assert g.observables(0)["num_staples"] >= 3  # staples bound at 300 K
+
+
+def case_ctrg_recoil_and_attempt_limits(sampler_cls, oracle_mod):
+    """Recoil growth without recoils and with one trial configuration per
+    domain, with two recoils (feeler growth two levels deep), and with three
+    recoils and six trial configurations, on the raster at 300 K with staples
+    bound."""
+    say("case: CTRG (recoils, attempts) = (0, 1), (2, 36), (3, 6) on the 64-domain raster "
+        "at 300 K, 2 x 8000 steps")
+    for rec, att in ((0, 1), (2, 36), (3, 6)):
+        s = systems.raster_system(replicas=1)
+        s.moves = systems.ctrg_moveset(12, rec, att)
+        _ctrg_chain(sampler_cls, oracle_mod, s, [300.0], 2, 8000)
+
+
+def case_ctrg_three_domain_staples(sampler_cls, oracle_mod):
+    """CTRG through networks of three-domain staples (growth from the middle
+    domain, both endpoint rules) and ThreeQuarterTurn domains."""
+    say("case: CTRG on the three-domain staple system (both endpoint rules, HalfTurn and "
+        "ThreeQuarterTurn, 4 replicas at 345-348 K, 3 x 4000 steps)")
+    for flags in (0, abi.FLAG_CTCB_SCAFFOLD_ENDPOINTS):
+        for dt in (abi.HALFTURN, abi.THREEQUARTERTURN):
+            s = systems.three_domain_system(domain_type=dt, max_total_staples=2, flags=flags)
+            s.moves = systems.ctrg_moveset(3, 2, 12)
+            _ctrg_chain(sampler_cls, oracle_mod, s, [345.0 + i for i in range(4)], 3, 4000)
+
+
+def case_ctrg_rejects_bad_limits(sampler_cls, oracle_mod):
+    """max_c_attempts outside 1..36 (the trial configurations are the 36
+    (site, orientation) pairs, drawn without replacement), negative
+    max_num_recoils and max_regrowth below 2 are refused at creation."""
+    say("case: CTRG limits outside their ranges are refused")
+    for maxreg, rec, att in ((12, 1, 37), (12, 1, 0), (12, -1, 36), (1, 1, 36)):
+        s = systems.raster_system(replicas=1)
+        s.moves = systems.ctrg_moveset(maxreg, rec, att)
+        with pytest.raises(abi.LdoError) as e:
+            sampler_cls(s, temps=[300.0])
+        assert e.value.code == abi.LDO_E_ARG

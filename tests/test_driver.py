@@ -15,7 +15,8 @@ from latticednaorigami_amd.build import DRIVER
 MOVE_TYPES = {abi.MOVE_ORIENTATION_ROTATION: "OrientationRotation",
               abi.MOVE_MET_STAPLE_EXCHANGE: "MetStapleExchange",
               abi.MOVE_CB_STAPLE_REGROWTH: "CBStapleRegrowth",
-              abi.MOVE_CTCB_SCAFFOLD_REGROWTH: "CTCBScaffoldRegrowth"}
+              abi.MOVE_CTCB_SCAFFOLD_REGROWTH: "CTCBScaffoldRegrowth",
+              abi.MOVE_CTRG_SCAFFOLD_REGROWTH: "CTRGScaffoldRegrowth"}
 HYB = {abi.HYB_NEARESTNEIGHBOUR: "NearestNeighbour", abi.HYB_UNIFORM: "Uniform"}
 
 
@@ -31,9 +32,14 @@ def write_inputs(d, s, extra):
         sysfile["origami"]["sequences"] = s.sequences
     with open(os.path.join(d, "system.json"), "w") as f:
         json.dump(sysfile, f)
-    moves = {"origami": {"movetypes": [
-        {"label": MOVE_TYPES[m.kind], "type": MOVE_TYPES[m.kind], "freq": m.freq,
-         "max_regrowth": m.max_regrowth} for m in s.moves]}}
+    def entry(m):
+        e = {"label": MOVE_TYPES[m.kind], "type": MOVE_TYPES[m.kind], "freq": m.freq,
+             "max_regrowth": m.max_regrowth}
+        if m.kind == abi.MOVE_CTRG_SCAFFOLD_REGROWTH:
+            e["max_num_recoils"] = m.max_num_recoils
+            e["max_c_attempts"] = m.max_c_attempts
+        return e
+    moves = {"origami": {"movetypes": [entry(m) for m in s.moves]}}
     with open(os.path.join(d, "moves.json"), "w") as f:
         json.dump(moves, f)
     lines = {
@@ -167,7 +173,7 @@ def expect_log_entry(ref, r, s, step):
 def expect_summary(ref, s):
     """The run summary written to the log and the .moves file
     (simulation.cpp:706-718; movetypes.cpp:88-96; met_movetypes.cpp:145-190;
-    cb_movetypes.cpp:264-290, 626-678)."""
+    cb_movetypes.cpp:264-290, 626-678; rg_movetypes.cpp:583-627)."""
     c = ref.counters()
     tr = ref.move_tracking(0)
     log, moves = ["Run summary\n\n"], []
@@ -204,6 +210,14 @@ def expect_summary(ref, s):
                                      f"        Frequency: {ratio(k, a)}\n")
                 if name == "length":
                     moves.append("\n")
+        elif mv.kind == abi.MOVE_CTRG_SCAFFOLD_REGROWTH:
+            # rg_movetypes.cpp:583-627
+            moves.append(head)
+            for n, (a, k) in enumerate(zip(tr["ctrg_length_attempts"], tr["ctrg_length_accepts"])):
+                if a:
+                    moves.append(f"    Number of scaffold domains: {n}\n        Attempts: {a}\n"
+                                 f"        Accepts: {k}\n        Frequency: {ratio(k, a)}\n")
+            moves.append("\n")
     return "".join(log), "".join(moves)
 
 
@@ -241,16 +255,30 @@ def test_driver_rejects_unsupported_inputs(tmp_path):
     s.moves = [systems.Move(abi.MOVE_ORIENTATION_ROTATION, 1.0)]
     inp = write_inputs(str(tmp_path), s, {"simulation_type": "constant_temp"})
     with open(os.path.join(str(tmp_path), "moves.json")) as f:
-        text = f.read().replace("OrientationRotation", "CTRGScaffoldRegrowth")
+        text = f.read().replace("OrientationRotation", "CTRGJumpScaffoldRegrowth")
     with open(os.path.join(str(tmp_path), "moves.json"), "w") as f:
         f.write(text)
     p = subprocess.run([DRIVER, inp], capture_output=True, text=True, timeout=60)
-    assert p.returncode == 1 and "movetype CTRGScaffoldRegrowth" in p.stderr, p.stderr
+    assert p.returncode == 1 and "movetype CTRGJumpScaffoldRegrowth" in p.stderr, p.stderr
+    # a CTRG entry without its recoil-growth limits (the reference's
+    # get_int_option throws for them too)
+    with open(os.path.join(str(tmp_path), "moves.json"), "w") as f:
+        f.write(text.replace("CTRGJumpScaffoldRegrowth", "CTRGScaffoldRegrowth")
+                .replace('"max_regrowth": 0', '"max_regrowth": 4'))
+    p = subprocess.run([DRIVER, inp], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 1 and "max_c_attempts" in p.stderr, p.stderr
 
 
 @pytest.mark.gpu
-def test_driver_constant_temp_matches_oracle(tmp_path, oracle_mod):
+@pytest.mark.parametrize("moveset", ["ctcb", "ctrg"])
+def test_driver_constant_temp_matches_oracle(tmp_path, oracle_mod, moveset):
     s = systems.four_domain_system()
+    if moveset == "ctrg":
+        # the reference's examples/moveset_standard.json, its non-contiguous
+        # CTRG entry's share split between contiguous CTRG and CTCB
+        s.moves = systems.ctrg_moveset(4)[:3] + [
+            systems.Move(abi.MOVE_CTRG_SCAFFOLD_REGROWTH, 1 / 6, 4, 1, 36),
+            systems.Move(abi.MOVE_CTCB_SCAFFOLD_REGROWTH, 1 / 6, 4)]
     d = str(tmp_path)
     inp = write_inputs(d, s, {"simulation_type": "constant_temp", "temp": "330",
                               "ct_steps": 20000, "configs_output_freq": 4000,

@@ -94,3 +94,19 @@ def test_batch_full_workload(oracle_mod):
 
 def test_learned_rejects_single_domain_staples(sampler_cls, oracle_mod):
     pc.case_learned_rejects_single_domain_staples(sampler_cls, oracle_mod)
+
+
+def test_ctrg_scaffold_regrowth_bit_exact(sampler_cls, oracle_mod):
+    pc.case_ctrg_scaffold_regrowth_bit_exact(sampler_cls, oracle_mod)
+
+
+def test_ctrg_recoil_and_attempt_limits(sampler_cls, oracle_mod):
+    pc.case_ctrg_recoil_and_attempt_limits(sampler_cls, oracle_mod)
+
+
+def test_ctrg_three_domain_staples(sampler_cls, oracle_mod):
+    pc.case_ctrg_three_domain_staples(sampler_cls, oracle_mod)
+
+
+def test_ctrg_rejects_bad_limits(sampler_cls, oracle_mod):
+    pc.case_ctrg_rejects_bad_limits(sampler_cls, oracle_mod)
