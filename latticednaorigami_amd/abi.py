@@ -27,6 +27,7 @@ MOVE_MET_STAPLE_EXCHANGE = 1
 MOVE_CB_STAPLE_REGROWTH = 2
 MOVE_CTCB_SCAFFOLD_REGROWTH = 3
 MOVE_CTRG_SCAFFOLD_REGROWTH = 4
+MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH = 5
 RNG_PHILOX, RNG_TAPE = 0, 1
 FLAG_CTCB_SCAFFOLD_ENDPOINTS = 1
 EXCHANGE_NONE, EXCHANGE_TEMPERATURE, EXCHANGE_WINDOWS = 0, 1, 2
@@ -39,6 +40,7 @@ MOVETYPE_NAMES = {
     "CBStapleRegrowth": MOVE_CB_STAPLE_REGROWTH,
     "CTCBScaffoldRegrowth": MOVE_CTCB_SCAFFOLD_REGROWTH,
     "CTRGScaffoldRegrowth": MOVE_CTRG_SCAFFOLD_REGROWTH,
+    "CTRGJumpScaffoldRegrowth": MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH,
 }
 
 _i32p = C.POINTER(C.c_int32)
@@ -90,6 +92,7 @@ class RunParams(C.Structure):
         ("flags", C.c_int32),
         ("movetype_max_num_recoils", _i32p),
         ("movetype_max_c_attempts", _i32p),
+        ("movetype_max_seg_regrowth", _i32p),
     ]
 
 
@@ -166,7 +169,7 @@ class LdoError(RuntimeError):
 
 def tracking_len(n_types, n_scaffold, max_staples):
     """Length of the ldo_get_move_tracking array."""
-    return 6 * n_types + 4 * (n_scaffold + 1) + 2 * (max_staples + 1)
+    return 6 * n_types + 4 * (n_scaffold + 1) + 2 * (max_staples + 1) + 2
 
 
 def split_tracking(values, n_types, n_scaffold, max_staples):
@@ -179,7 +182,8 @@ def split_tracking(values, n_types, n_scaffold, max_staples):
                     ("cb_attempts", T), ("cb_accepts", T),
                     ("ctcb_length_attempts", n1), ("ctcb_length_accepts", n1),
                     ("ctcb_staples_attempts", s1), ("ctcb_staples_accepts", s1),
-                    ("ctrg_length_attempts", n1), ("ctrg_length_accepts", n1)):
+                    ("ctrg_length_attempts", n1), ("ctrg_length_accepts", n1),
+                    ("ctrg_jump_attempts", 1), ("ctrg_jump_accepts", 1)):
         out[name] = v[o:o + n]
         o += n
     assert o == len(v), (o, len(v))

@@ -12,8 +12,9 @@
 // per-iteration files of us_simulation.cpp:103-248: _iter-<n>.* outputs,
 // -inp.biases / .biases weights and the .out iteration summary) with the
 // movetypes OrientationRotation,
-// MetStapleExchange, CBStapleRegrowth, CTCBScaffoldRegrowth and
-// CTRGScaffoldRegrowth.
+// MetStapleExchange, CBStapleRegrowth, CTCBScaffoldRegrowth,
+// CTRGScaffoldRegrowth and CTRGJumpScaffoldRegrowth (the movetypes of the
+// reference's examples/moveset_standard.json).
 //
 //   ldo_run <parameter file> [key=value ...]
 //
@@ -462,15 +463,15 @@ void write_log_entry(Run& run, std::ostream& log, int r, long long step) {
 // GCMCSimulation::write_log_summary (simulation.cpp:706-718) of a one-replica
 // run: each movetype's header (movetypes.cpp:88-96) to the log, and the
 // movetype summaries (met_movetypes.cpp:145-190, cb_movetypes.cpp:264-290,
-// 626-678, rg_movetypes.cpp:583-627; none for orientation rotation) to the
-// .moves file.
+// 626-678, rg_movetypes.cpp:583-627, 745-789; none for orientation
+// rotation) to the .moves file.
 void write_log_summary(Run& run, std::ostream& log, const std::string& moves_path) {
     ldo_counters c {};
     run.check(ldo_get_counters(run.ctx, &c));
     int T = (int)run.identities.size();
     int ns = run.n_scaffold;
     int S1 = run.max_total_staples + 1;
-    std::vector<int64_t> tr((size_t)(6 * T + 4 * (ns + 1) + 2 * S1));
+    std::vector<int64_t> tr((size_t)(6 * T + 4 * (ns + 1) + 2 * S1 + 2));
     int64_t n_tr = 0;
     run.check(ldo_get_move_tracking(run.ctx, 0, tr.data(), (int64_t)tr.size(), &n_tr));
     auto at = [&](int part, int i) { return tr[(size_t)part * T + i]; };
@@ -570,6 +571,19 @@ void write_log_summary(Run& run, std::ostream& log, const std::string& moves_pat
                 moves << "        Accepts: " << (int)len_acc[l] << "\n";
                 moves << "        Frequency: " << static_cast<double>(len_acc[l]) / len_att[l]
                       << "\n";
+            }
+            moves << "\n";
+        }
+        else if (kind == LDO_MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH) {
+            // CTRGJumpScaffoldRegrowthMCMovetype::write_log_summary
+            // (rg_movetypes.cpp:745-789): every move is tracked under 0 domains
+            header(moves, m);
+            const int64_t* jt = tr.data() + 6 * T + 4 * (ns + 1) + 2 * S1;
+            if (jt[0] != 0) {
+                moves << "    Number of scaffold domains: " << 0 << "\n";
+                moves << "        Attempts: " << (int)jt[0] << "\n";
+                moves << "        Accepts: " << (int)jt[1] << "\n";
+                moves << "        Frequency: " << static_cast<double>(jt[1]) / jt[0] << "\n";
             }
             moves << "\n";
         }
@@ -1188,12 +1202,13 @@ int main_run(int argc, char** argv) {
     sd.staple_M = p.num("staple_M", 1);
     // moveset JSON (setup_movetypes, files.cpp:296-306)
     const Json mv = read_json(p.str("movetype_file", "")).at("origami").at("movetypes");
-    std::vector<int32_t> kinds, maxreg, recoils, c_attempts;
+    std::vector<int32_t> kinds, maxreg, recoils, c_attempts, seg_regrowth;
     std::vector<double> freqs;
     for (auto& m: mv.arr) {
         kinds.push_back(enum_of(m.at("type").str,
                                 {"OrientationRotation", "MetStapleExchange", "CBStapleRegrowth",
-                                 "CTCBScaffoldRegrowth", "CTRGScaffoldRegrowth"},
+                                 "CTCBScaffoldRegrowth", "CTRGScaffoldRegrowth",
+                                 "CTRGJumpScaffoldRegrowth"},
                                 "movetype"));
         run.move_labels.push_back(m.has("label") ? m.at("label").str : m.at("type").str);
         freqs.push_back(fraction(m.at("freq")));
@@ -1201,6 +1216,8 @@ int main_run(int argc, char** argv) {
         // setup_rg_movetype (simulation.cpp:493-495)
         recoils.push_back(m.has("max_num_recoils") ? (int32_t)m.at("max_num_recoils").num : 0);
         c_attempts.push_back(m.has("max_c_attempts") ? (int32_t)m.at("max_c_attempts").num : 0);
+        seg_regrowth.push_back(m.has("max_seg_regrowth") ? (int32_t)m.at("max_seg_regrowth").num
+                                                         : 0);
     }
     run.max_total_staples = (int)p.integer("max_total_staples", 999);
     run.max_staple_size = (int)p.integer("max_staple_size", 2);
@@ -1212,6 +1229,7 @@ int main_run(int argc, char** argv) {
     rp.movetype_max_regrowth = maxreg.data();
     rp.movetype_max_num_recoils = recoils.data();
     rp.movetype_max_c_attempts = c_attempts.data();
+    rp.movetype_max_seg_regrowth = seg_regrowth.data();
     rp.max_total_staples = run.max_total_staples;
     rp.max_type_staples = (int32_t)p.integer("max_type_staples", 999);
     rp.max_staple_size = run.max_staple_size;

@@ -168,6 +168,12 @@ struct Eng {
     double* rgopen;
     Endpoint* erstk;
     int16_t* ercnt;
+    int8_t* jdom;
+    int16_t* jps;
+    JumpSeg* jseg;
+    int16_t* jstem;
+    int8_t* stemseg;
+    int8_t* sdir;
     int64_t* track;
     const double* t_hyb;
     const double* t_stack;
@@ -182,7 +188,7 @@ struct Eng {
     bool net_external;
     // CTRG: the move's recoil and attempt limits, and the assignments the
     // reference records beyond the recoil growth's (see rg_set_config)
-    int rg_max_recoils, rg_max_att;
+    int rg_max_recoils, rg_max_att, rg_max_segreg;
     int64_t rg_dup;
     // par = 1: the 64 lanes of a wavefront run this engine together on
     // identical values (every lane computes and stores the same thing), and
@@ -305,6 +311,12 @@ struct Eng {
         rgopen = (double*)(blk + L.off_rgopen);
         erstk = (Endpoint*)(blk + L.off_erstk);
         ercnt = (int16_t*)(blk + L.off_ercnt);
+        jdom = (int8_t*)(blk + L.off_jdom);
+        jps = (int16_t*)(blk + L.off_jps);
+        jseg = (JumpSeg*)(blk + L.off_jseg);
+        jstem = (int16_t*)(blk + L.off_jstem);
+        stemseg = (int8_t*)(blk + L.off_stemseg);
+        sdir = (int8_t*)(blk + L.off_sdir);
         track = (int64_t*)(blk + L.off_track);
         ns = L.ns;
         lshift = L.lshift;
@@ -676,8 +688,12 @@ struct Eng {
     DEVIN void cp_update_endpoints(int s);
     DEVIN bool cp_endpoint_reached(int s, I3 p);
     DEVIN bool cp_walks_remain(int s, I3 p);
+    DEVIN bool cp_walks_seg(int s, I3 p, int sg);
+    DEVIN int cp_seg_dir(int cs, int sg);
+    DEVIN bool cp_selected(int b);
+    DEVNI void cp_save_initial();
     DEVNI void cp_scan_network(int d);
-    DEVNI void cp_find(int first, int count, int dir);
+    DEVNI void cp_find(int first, int count, int dir, int sg);
     DEVIN bool ct_select_indices(int max_regrowth, int& first, int& count, int& dir);
     DEVNI void ctcb_grow_chain_scaffold(int first, int count, int dir);
     DEVNI void ctcb_grow_staple_from(int gold);
@@ -696,7 +712,13 @@ struct Eng {
     DEVNI bool rg_test_avail(RgCur& c);
     DEVNI void rg_calc_old_c_opens();
     DEVIN void rg_reset_active(bool remove_first);
+    DEVNI bool rg_regrow_and_accept(bool rm_new, bool rm_opens, bool rm_old);
     DEVNI bool ctrg_scaffold_move(int max_regrowth);
+    DEVIN void jump_check_stems(int cur, int& tail);
+    DEVNI bool jump_fill(int start, int have, int max_length, int seg_max, int dir, int& tail,
+                         int& last);
+    DEVNI bool jump_select(int max_regrowth);
+    DEVNI bool ctrg_jump_move(int max_regrowth);
     DEVIN bool attempt(int m);
     DEVNI bool attempt_kind(int kind, int max_regrowth, int m = -1);
     DEVIN void track_move(int m, bool acc);

@@ -587,6 +587,9 @@ DEVNI void Eng::cp_reset() {
     fill_section(dir1, nchain, 0u);
     S->n_active = 0;
     S->n_active0 = 0;
+    S->n_rgs = 0;
+    S->n_jsegs = 0;
+    S->n_jstems = 0;
     n_netds = 0;
 }
 
@@ -703,18 +706,35 @@ DEVIN bool Eng::cp_endpoint_reached(int s, I3 p) {
     return pany(hit);
 }
 
+// m_domain_to_dir of (chain, segment): the scaffold segments of a
+// non-contiguous selection have their own (0 for an empty one, which the
+// reference never sets)
+DEVIN int Eng::cp_seg_dir(int cs, int sg) {
+    if (cs == 0 && S->n_jsegs > 0) {
+        return sdir[sg];
+    }
+    return sg == 0 ? dir0[cs] : dir1[cs];
+}
+
 // Constraintpoints::walks_remain (:397-442, 571-601).  The ideal-walk count
 // is only tested against zero, which holds exactly when the Manhattan
 // distance exceeds the remaining steps or differs from them in parity
 // (ideal_random_walk.cpp:35-41).  Stem domains look up their segment pair in
-// a map the contiguous move never fills, so they have no walk constraint.
+// m_stemd_to_segs, which only holds the scaffold stems of a non-contiguous
+// selection; other stems have no walk constraint.
 DEVIN bool Eng::cp_walks_remain(int s, I3 p) {
     if (stem[s] >= 0) {
-        return true;
+        if (S->n_jsegs == 0 || cslot_of(s) != 0 || stemseg[s] < 0) {
+            return true;
+        }
+        return cp_walks_seg(s, p, stemseg[s]) && cp_walks_seg(s, p, stemseg[s] + 1);
     }
-    int sg = seg[s];
+    return cp_walks_seg(s, p, seg[s]);
+}
+
+DEVIN bool Eng::cp_walks_seg(int s, I3 p, int sg) {
     int cs = cslot_of(s), dd = d_of(s);
-    int dir = sg == 0 ? dir0[cs] : dir1[cs];
+    int dir = cp_seg_dir(cs, sg);
     int n = S->n_active;
     bool bad = false;
     for (int i = plane(); i < n && !(bad && !par); i += pwidth()) {
@@ -810,11 +830,7 @@ DEVNI void Eng::cp_scan_network(int d0) {
         if (netcs[bci]) {
             bool external = false;
             if (bci == 0) {
-                int rel = (d_of(b) - d_of(seg_first)) * seg_dir;
-                if (P->cyclic) {
-                    rel = ((rel % ns) + ns) % ns;
-                }
-                external = !(rel >= 0 && rel < seg_count);
+                external = !cp_selected(b);
             }
             if (!external) {
                 if (netds_flag[b]) {
@@ -859,24 +875,40 @@ DEVNI void Eng::cp_scan_network(int d0) {
     }
 }
 
-// Constraintpoints::calculate_constraintpoints + find_growthpoints_endpoints
-// (top_constraint_points.cpp:207-221, 454-569) for the scaffold segment
-// first, first+dir, ... (count domains)
-DEVNI void Eng::cp_find(int first, int count, int dir) {
+// whether scaffold domain b is among the selected (m_scaffold_domains)
+DEVIN bool Eng::cp_selected(int b) {
+    if (S->n_jsegs > 0) {
+        return jdom[b] != 0;
+    }
+    int rel = (d_of(b) - d_of(seg_first)) * seg_dir;
+    if (P->cyclic) {
+        rel = ((rel % ns) + ns) % ns;
+    }
+    return rel >= 0 && rel < seg_count;
+}
+
+// Constraintpoints::find_growthpoints_endpoints (top_constraint_points.cpp:
+// 454-569) for scaffold segment sg: first, first+dir, ... (count domains)
+DEVNI void Eng::cp_find(int first, int count, int dir, int sg) {
     // CTRG grows Constraintpoints::m_d_stack (domains_to_be_regrown): each
     // segment domain, then the domain stack of the network it roots
-    bool rec = S->kind == LDO_MOVE_KIND_CTRG;
-    S->n_rgs = 0;
-    dir0[0] = (int8_t)dir;
+    bool rec = S->kind == LDO_MOVE_KIND_CTRG || S->kind == LDO_MOVE_KIND_CTRG_JUMP;
+    if (S->n_jsegs == 0) {
+        dir0[0] = (int8_t)dir;
+    }
     int d = first;
     for (int i = 0; i < count; i++) {
         if (i > 0) {
             d = walk(d, dir);
         }
         if (rec) {
+            if (S->n_rgs >= nslots) {
+                fail(ST_CAPACITY);
+                return;
+            }
             rgs[S->n_rgs++] = (int16_t)d;
         }
-        seg[d] = 0;
+        seg[d] = (int8_t)sg;
         int b = dom[d].bound;
         if (b < 0 || cslot_of(b) == 0 || checked[cslot_of(b)]) {
             continue;
@@ -889,7 +921,7 @@ DEVNI void Eng::cp_find(int first, int count, int dir) {
             for (int g = 0; g < n_gps; g++) {
                 int g1 = gps[2 * g];
                 if (cslot_of(g1) == 0) {
-                    cp_add_active(g1, pos(g1), 0);
+                    cp_add_active(g1, pos(g1), sg);
                 }
             }
             // the reference keys these endpoints by the staple domain (a1),
@@ -898,7 +930,7 @@ DEVNI void Eng::cp_find(int first, int count, int dir) {
             for (int a = 0; a < n_iaes; a++) {
                 int a1 = iaes[2 * a], a2 = iaes[2 * a + 1];
                 if (cslot_of(a2) == 0) {
-                    cp_add_active(on_scaffold ? a2 : a1, pos(a2), 0);
+                    cp_add_active(on_scaffold ? a2 : a1, pos(a2), sg);
                 }
             }
         }
@@ -937,6 +969,11 @@ DEVNI void Eng::cp_find(int first, int count, int dir) {
         }
         checked[0] = 1;
     }
+}
+
+// the end of calculate_constraintpoints (top_constraint_points.cpp:220,
+// 247): m_initial_active_endpoints = m_active_endpoints
+DEVNI void Eng::cp_save_initial() {
     for (int i = plane(), n = S->n_active; i < n; i += pwidth()) {
         active0[i] = active[i];
     }
@@ -1118,7 +1155,8 @@ DEVNI bool Eng::ctcb_scaffold_move(int max_regrowth) {
     seg_count = count;
     seg_dir = dir;
     S->trk_len = count;
-    cp_find(first, count, dir);
+    cp_find(first, count, dir, 0);
+    cp_save_initial();
     bool whole_ring = P->cyclic && count == ns;
     if (!whole_ring) {
         cp_remove_active(first);
@@ -1187,7 +1225,11 @@ DEVNI bool Eng::ctcb_scaffold_move(int max_regrowth) {
 // Constraintpoints::get_dir (top_constraint_points.cpp:179-185)
 DEVIN int Eng::rg_get_dir(int s) {
     int cs = cslot_of(s);
-    return seg[s] == 1 ? dir1[cs] : dir0[cs];
+    int sg = seg[s];
+    if (cs == 0 && S->n_jsegs > 0) {
+        return sdir[sg < 0 ? 0 : sg];
+    }
+    return sg == 1 ? dir1[cs] : dir0[cs];
 }
 
 // unassign_and_save_domains (:87-104) with save and modify,
@@ -1542,28 +1584,16 @@ DEVIN void Eng::rg_reset_active(bool remove_first) {
     }
 }
 
-// CTRGScaffoldRegrowthMCMovetype::internal_attempt_move (:629-689) with
-// setup_constraints (:131-142), setup_for_calc_new_weights (:144-150),
-// setup_for_calc_old_weights (:152-160) and test_rg_acceptance (:514-533)
-DEVNI bool Eng::ctrg_scaffold_move(int max_regrowth) {
-    cp_reset();
-    int first, count, dir;
-    if (!ct_select_indices(max_regrowth, first, count, dir)) {
-        return false;
-    }
-    seg_first = first;
-    seg_count = count;
-    seg_dir = dir;
-    S->trk_len = count;
-    cp_find(first, count, dir);
-    if (S->status != ST_OK) {
-        return false;
-    }
-    if (!(P->cyclic && count == ns)) {
-        cp_remove_active(first);
-    }
+// The common part of CTRGScaffoldRegrowthMCMovetype::internal_attempt_move
+// (:640-686) and CTRGJumpScaffoldRegrowthMCMovetype's (:811-850) once the
+// regrowth stack is set up: recoil growth, the weights of the new and the
+// old configuration, and test_rg_acceptance (:514-533), with
+// setup_for_calc_new_weights (:144-150) and setup_for_calc_old_weights
+// (:152-160).  rm_new, rm_opens and rm_old say whether the stack's first
+// domain loses its active endpoints before the new weights, the old open
+// probabilities and the old weights.
+DEVNI bool Eng::rg_regrow_and_accept(bool rm_new, bool rm_opens, bool rm_old) {
     int n = S->n_rgs;
-    bool whole = P->cyclic && n == ns;
     rg_dup = 0;
     S->delta_e += rg_unassign_and_save(true);
     S->delta_e += rg_recoil_regrow();
@@ -1581,17 +1611,16 @@ DEVNI bool Eng::ctrg_scaffold_move(int max_regrowth) {
     }
     S->n_modified = 0;
     rg_unassign_and_save(true);
-    rg_reset_active(!whole);
+    rg_reset_active(rm_new);
     double w_new = rg_calc_weights(prev);
 
-    // weights of the old configuration (kept in old); the reference's test
-    // before calc_old_c_opens (:665-668) differs from the other three
+    // weights of the old configuration (kept in old)
     for (int i = 1; i < n; i++) {
         unassign_domain(rgs[i]);
     }
     S->er_top = 0;
     S->er_depth = 0;
-    rg_reset_active(!P->cyclic && n == ns);
+    rg_reset_active(rm_opens);
     rg_calc_old_c_opens();
     for (int i = 0; i < n; i++) {
         rgwatt[i] = rgatt[i];
@@ -1599,7 +1628,7 @@ DEVNI bool Eng::ctrg_scaffold_move(int max_regrowth) {
     }
     S->n_modified = 0;
     rg_unassign_and_save(false);
-    rg_reset_active(!whole);
+    rg_reset_active(rm_old);
     double w_old = rg_calc_weights(old);
     if (S->status != ST_OK) {
         return false;
@@ -1614,6 +1643,237 @@ DEVNI bool Eng::ctrg_scaffold_move(int max_regrowth) {
     S->n_modified = 0;
     S->n_assigned = 0;
     return false;
+}
+
+// CTRGScaffoldRegrowthMCMovetype::internal_attempt_move (:629-689) with
+// setup_constraints (:131-142)
+DEVNI bool Eng::ctrg_scaffold_move(int max_regrowth) {
+    cp_reset();
+    int first, count, dir;
+    if (!ct_select_indices(max_regrowth, first, count, dir)) {
+        return false;
+    }
+    seg_first = first;
+    seg_count = count;
+    seg_dir = dir;
+    S->trk_len = count;
+    cp_find(first, count, dir, 0);
+    cp_save_initial();
+    if (S->status != ST_OK) {
+        return false;
+    }
+    if (!(P->cyclic && count == ns)) {
+        cp_remove_active(first);
+    }
+    int n = S->n_rgs;
+    bool whole = P->cyclic && n == ns;
+    // the reference's test before calc_old_c_opens (:665-668) differs from
+    // the other two
+    return rg_regrow_and_accept(!whole, !P->cyclic && n == ns, !whole);
+}
+
+// ------------------------------------ non-contiguous scaffold selection
+// CTRegrowthMCMovetype::check_for_stemds (movetypes.cpp:704-720): queue the
+// scaffold domains bound to the chain neighbours of the staple domain bound
+// to cur
+DEVIN void Eng::jump_check_stems(int cur, int& tail) {
+    if (dom[cur].state != OCC_BOUND) {
+        return;
+    }
+    int b = dom[cur].bound;
+    for (int sd = -1; sd <= 1; sd += 2) {
+        int nd = walk(b, sd);
+        if (nd >= 0 && dom[nd].state == OCC_BOUND && cslot_of(dom[nd].bound) == 0) {
+            if (tail >= LDO_JUMP_QUEUE) {
+                fail(ST_CAPACITY);
+                return;
+            }
+            jps[tail++] = dom[nd].bound;
+        }
+    }
+}
+
+// CTRegrowthMCMovetype::fill_seg (movetypes.cpp:666-702) for a segment that
+// already holds `have` domains; last is the last domain added (unchanged
+// when none is)
+DEVNI bool Eng::jump_fill(int start, int have, int max_length, int seg_max, int dir, int& tail,
+                          int& last) {
+    int cur = start;
+    jump_check_stems(cur, tail);
+    int next = start;
+    int size = have;
+    while (size != seg_max && next >= 0) {
+        next = walk(cur, dir);
+        if (next < 0) {
+            break;
+        }
+        int nn = walk(next, dir);
+        if (nn >= 0 && jdom[nn]) {
+            break;
+        }
+        size++;
+        jdom[next] = 1;
+        S->n_jdom++;
+        last = next;
+        if (S->n_jdom == max_length) {
+            return true;
+        }
+        cur = next;
+        jump_check_stems(cur, tail);
+    }
+    return false;
+}
+
+// CTRegrowthMCMovetype::select_noncontig_segs (movetypes.cpp:512-648): the
+// segments (jseg), their directions (sdir), the scaffold stems (jstem) with
+// their growthpoints and segment pairs, and the active endpoints past each
+// segment's end
+DEVNI bool Eng::jump_select(int max_regrowth) {
+    fill_section(jdom, ns, 0u);
+    fill_section(stemseg, ns, 0xffffffffu);
+    fill_section(sdir, LDO_JUMP_MAX_SEGS, 0u);
+    S->n_jdom = 0;
+    int max_length = uniform_int(2, max_regrowth);
+    int seg_max = uniform_int(2, rg_max_segreg + 1);
+    int start = slot(0, uniform_int(0, ns - 1));
+    int dir = uniform_int(0, 1);
+    if (dir == 0) {
+        dir = -1;
+    }
+    if (walk(start, dir) < 0) {
+        dir = -dir;
+    }
+    jdom[start] = 1;
+    S->n_jdom = 1;
+    int head = 0, tail = 0;
+    int last = start;
+    bool reached = jump_fill(start, 1, max_length, seg_max, dir, tail, last);
+    JumpSeg s0;
+    s0.first = (int16_t)start;
+    s0.count = (int16_t)S->n_jdom; // start and the domains filled after it
+    s0.last = (int16_t)last;
+    s0.dir = (int8_t)dir;
+    s0.pad = 0;
+    jseg[0] = s0;
+    S->n_jsegs = 1;
+    S->n_jstems = 0;
+    while (!reached && head < tail && S->status == ST_OK) {
+        int stemd = jps[head++];
+        if (jdom[stemd]) {
+            continue;
+        }
+        int a = walk(stemd, -1), b = walk(stemd, 1);
+        if ((a >= 0 && jdom[a]) || (b >= 0 && jdom[b])) {
+            continue;
+        }
+        jdom[stemd] = 1;
+        S->n_jdom++;
+        if (S->n_jstems >= LDO_JUMP_MAX_SEGS / 2 || S->n_jsegs + 2 > LDO_JUMP_MAX_SEGS) {
+            fail(ST_CAPACITY);
+            return false;
+        }
+        jstem[S->n_jstems++] = (int16_t)stemd;
+        int k = S->n_jsegs;
+        S->n_jsegs += 2;
+        JumpSeg p0, p1;
+        p0.first = (int16_t)stemd;
+        p0.count = 1;
+        p0.last = (int16_t)stemd;
+        p0.pad = 0;
+        p1.first = LDO_NONE;
+        p1.count = 0;
+        p1.last = (int16_t)stemd;
+        p1.pad = 0;
+        if (S->n_jdom == max_length) {
+            reached = true;
+            p0.dir = 1;
+            p1.dir = -1;
+            jseg[k] = p0;
+            jseg[k + 1] = p1;
+            break;
+        }
+        int dir1 = uniform_int(0, 1);
+        if (dir1 == 0) {
+            dir1 = -1;
+        }
+        p0.dir = (int8_t)dir1;
+        p1.dir = (int8_t)-dir1;
+        for (int i = 0; i < 2; i++) {
+            int d = i == 0 ? dir1 : -dir1;
+            int sm = uniform_int(0, rg_max_segreg);
+            if (i == 0) {
+                max_length++;
+            }
+            int lst = stemd;
+            int before = S->n_jdom;
+            reached = jump_fill(stemd, 0, max_length, sm, d, tail, lst);
+            int added = S->n_jdom - before;
+            if (i == 0) {
+                p0.count = (int16_t)(1 + added);
+                p0.last = (int16_t)lst;
+            }
+            else {
+                p1.first = added > 0 ? (int16_t)walk(stemd, d) : LDO_NONE;
+                p1.count = (int16_t)added;
+                p1.last = (int16_t)lst;
+            }
+            if (reached) {
+                break;
+            }
+        }
+        jseg[k] = p0;
+        jseg[k + 1] = p1;
+    }
+    if (S->status != ST_OK) {
+        return false;
+    }
+    int nx = walk(jseg[0].last, jseg[0].dir);
+    if (nx >= 0) {
+        cp_add_active(nx, pos(nx), 0);
+    }
+    for (int i = 0; i < S->n_jsegs; i++) {
+        sdir[i] = jseg[i].count > 0 ? jseg[i].dir : 0;
+    }
+    for (int t = 0; t < S->n_jstems; t++) {
+        int stemd = jstem[t];
+        int k = 1 + 2 * t;
+        gp[dom[stemd].bound] = (int16_t)stemd;
+        stem[stemd] = dom[stemd].bound;
+        stemseg[stemd] = (int8_t)k;
+        for (int j = 0; j < 2; j++) {
+            int x = walk(jseg[k + j].last, jseg[k + j].dir);
+            if (x >= 0) {
+                cp_add_active(x, pos(x), k + j);
+            }
+        }
+    }
+    return true;
+}
+
+// CTRGJumpScaffoldRegrowthMCMovetype::internal_attempt_move (:791-853)
+DEVNI bool Eng::ctrg_jump_move(int max_regrowth) {
+    cp_reset();
+    if (!jump_select(max_regrowth)) {
+        return false;
+    }
+    S->trk_len = 0;
+    for (int i = 0; i < S->n_jsegs; i++) {
+        if (jseg[i].count > 0) {
+            cp_find(jseg[i].first, jseg[i].count, jseg[i].dir, i);
+        }
+    }
+    cp_save_initial();
+    if (S->status != ST_OK) {
+        return false;
+    }
+    int first = jseg[0].first;
+    cp_remove_active(first);
+    for (int t = 0; t < S->n_jstems; t++) {
+        int stemd = jstem[t];
+        gp[dom[stemd].bound] = (int16_t)stemd;
+        stem[stemd] = dom[stemd].bound;
+    }
+    return rg_regrow_and_accept(true, true, true);
 }
 
 // MCMovetype::attempt_move and the reset_internal chain (movetypes.cpp:41-51,
@@ -1653,6 +1913,12 @@ DEVNI bool Eng::attempt_kind(int kind, int max_regrowth, int m) {
         rg_max_recoils = m >= 0 ? P->move_recoils[m] : 0;
         rg_max_att = m >= 0 ? P->move_cattempts[m] : 36;
         return ctrg_scaffold_move(max_regrowth);
+    case 5:
+        S->kind = LDO_MOVE_KIND_CTRG_JUMP;
+        rg_max_recoils = m >= 0 ? P->move_recoils[m] : 0;
+        rg_max_att = m >= 0 ? P->move_cattempts[m] : 36;
+        rg_max_segreg = m >= 0 ? P->move_segreg[m] : 0;
+        return ctrg_jump_move(max_regrowth);
     }
     fail(ST_MISUSE);
     return false;
@@ -1692,6 +1958,11 @@ DEVIN void Eng::track_move(int m, bool acc) {
         int o = 6 * T + 2 * (ns + 1) + 2 * (P->L.smax + 1);
         track[o + S->trk_len] += 1;
         track[o + ns + 1 + S->trk_len] += a;
+    }
+    else if (kind == 5) {
+        int o = 6 * T + 4 * (ns + 1) + 2 * (P->L.smax + 1);
+        track[o] += 1;
+        track[o + 1] += a;
     }
 }
 

@@ -71,6 +71,21 @@ struct GrowFrame {
 
 #define LDO_MAX_MOVES 8
 
+// non-contiguous scaffold selection: at most LDO_JUMP_MAX_SEGS segments
+// (1 + 2 per stem), LDO_JUMP_MAX_SEGS / 2 stems; max_regrowth is capped so
+// that these and the possible-stem queue (LDO_JUMP_QUEUE) always suffice
+#define LDO_JUMP_MAX_SEGS 128
+#define LDO_JUMP_QUEUE 512
+#define LDO_JUMP_MAX_REGROWTH 60
+
+// one segment of a non-contiguous selection: count domains from first in
+// direction dir; last is the domain its end endpoint lies past (the stem for
+// an empty fill)
+struct JumpSeg {
+    int16_t first, count, last;
+    int8_t dir, pad;
+};
+
 // Per-replica scalars at the head of a replica block (kept 8-byte aligned).
 struct RepScalars {
     int64_t step;
@@ -132,6 +147,11 @@ struct RepScalars {
     int32_t n_rgs;
     int32_t er_top;
     int32_t er_depth;
+    // non-contiguous selection: segments (0 for a contiguous move), stems,
+    // selected scaffold domains
+    int32_t n_jsegs;
+    int32_t n_jstems;
+    int32_t n_jdom;
     int32_t pad_rg;
 };
 
@@ -160,12 +180,16 @@ struct Layout {
     // (growth, weights) and open probability, erased-endpoint stack and counts
     int64_t off_rgs, off_rgatt, off_rgwatt, off_rgavail, off_rgwavail, off_rgopen;
     int64_t off_erstk, off_ercnt;
+    // non-contiguous selection (empty without a jump move): selected
+    // scaffold domains, possible-stem queue, segments, stems, the first
+    // segment of each scaffold stem's pair, and segment directions
+    int64_t off_jdom, off_jps, off_jseg, off_jstem, off_stemseg, off_sdir;
     // int64 tracking counters by staple type T = ntypes (index = identity)
     // and scaffold length / staple count: Met insertion attempts[T],
     // accepts[T], deletion attempts[T], accepts[T]; CB attempts[T],
     // accepts[T]; CTCB by segment length attempts[ns+1], accepts[ns+1]; by
     // staples regrown attempts[smax+1], accepts[smax+1]; CTRG by segment
-    // length attempts[ns+1], accepts[ns+1]
+    // length attempts[ns+1], accepts[ns+1]; CTRG jump attempts, accepts
     int64_t off_track;
     int64_t bytes; // block size (multiple of 256)
 };
@@ -183,6 +207,7 @@ struct Params {
     int32_t move_maxreg[LDO_MAX_MOVES];
     int32_t move_recoils[LDO_MAX_MOVES];   // CTRG max_num_recoils
     int32_t move_cattempts[LDO_MAX_MOVES]; // CTRG max_c_attempts
+    int32_t move_segreg[LDO_MAX_MOVES];    // CTRG jump max_seg_regrowth
     int32_t max_total_staples;
     int32_t max_type_staples;
     int32_t centering_freq;
@@ -253,6 +278,7 @@ struct LearnedMove {
 #define LDO_MOVE_KIND_CB 2
 #define LDO_MOVE_KIND_CTCB 3
 #define LDO_MOVE_KIND_CTRG 4
+#define LDO_MOVE_KIND_CTRG_JUMP 5
 
 // status codes written into RepScalars::status
 #define ST_OK 0

@@ -33,7 +33,7 @@ void mlp_layout(int n_ident, ldo_mlp_layout& L) {
     L.total = o;
 }
 
-Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap, bool rg) {
+Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap, bool rg, bool jump) {
     Layout L;
     std::memset(&L, 0, sizeof L);
     L.ns = ns;
@@ -115,7 +115,14 @@ Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap, bool rg) {
     take(L.off_rgopen, R * 8);
     take(L.off_erstk, (rg ? (int64_t)L.ecap : 0) * (int64_t)sizeof(Endpoint));
     take(L.off_ercnt, R * 2);
-    L.track_len = 6 * ntypes + 4 * (ns + 1) + 2 * (smax + 1);
+    int64_t J = jump ? 1 : 0;
+    take(L.off_jdom, J * ns);
+    take(L.off_jps, J * LDO_JUMP_QUEUE * 2);
+    take(L.off_jseg, J * LDO_JUMP_MAX_SEGS * (int64_t)sizeof(JumpSeg));
+    take(L.off_jstem, J * LDO_JUMP_MAX_SEGS);
+    take(L.off_stemseg, J * ns);
+    take(L.off_sdir, J * LDO_JUMP_MAX_SEGS);
+    L.track_len = 6 * ntypes + 4 * (ns + 1) + 2 * (smax + 1) + 2;
     take(L.off_track, (int64_t)L.track_len * 8);
     L.bytes = round_up(off, 256);
     return L;
@@ -204,11 +211,13 @@ void make_plan(const ldo_system_desc* sys, const ldo_run_params* run, int32_t n_
     int tcap = std::min(std::max(run->max_type_staples, 1), smax);
     Params& P = out.P;
     std::memset(&P, 0, sizeof P);
-    bool rg = false;
+    bool rg = false, jump = false;
     for (int m = 0; m < run->n_movetypes; m++) {
-        rg = rg || run->movetype_kind[m] == LDO_MOVE_CTRG_SCAFFOLD_REGROWTH;
+        int k = run->movetype_kind[m];
+        jump = jump || k == LDO_MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH;
+        rg = rg || k == LDO_MOVE_CTRG_SCAFFOLD_REGROWTH || k == LDO_MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH;
     }
-    P.L = make_layout(ns, lmax, smax, sys->n_chain_types, tcap, rg);
+    P.L = make_layout(ns, lmax, smax, sys->n_chain_types, tcap, rg, jump);
     P.n_replicas = n_replicas;
     P.domain_type = sys->domain_type;
     P.misbinding_pot = sys->misbinding_pot;
@@ -217,7 +226,7 @@ void make_plan(const ldo_system_desc* sys, const ldo_run_params* run, int32_t n_
     double cum = 0;
     for (int m = 0; m < run->n_movetypes; m++) {
         int kind = run->movetype_kind[m];
-        if (kind < 0 || kind > 4) {
+        if (kind < 0 || kind > 5) {
             throw AbiError {LDO_E_NOTIMPL, "movetype not supported"};
         }
         P.move_kind[m] = kind;
@@ -226,10 +235,22 @@ void make_plan(const ldo_system_desc* sys, const ldo_run_params* run, int32_t n_
         P.move_maxreg[m] = run->movetype_max_regrowth ? run->movetype_max_regrowth[m] : 0;
         P.move_recoils[m] = 0;
         P.move_cattempts[m] = 0;
-        if ((kind == 3 || kind == 4) && P.move_maxreg[m] < 2) {
+        P.move_segreg[m] = 0;
+        if (kind >= 3 && P.move_maxreg[m] < 2) {
             throw AbiError {LDO_E_ARG, "scaffold regrowth max_regrowth must be at least 2"};
         }
-        if (kind == 4) {
+        if (kind == 5) {
+            // the first segment's length is drawn from [2, max_seg_regrowth + 1]
+            // (movetypes.cpp:522-523)
+            if (!run->movetype_max_seg_regrowth || run->movetype_max_seg_regrowth[m] < 1) {
+                throw AbiError {LDO_E_ARG, "CTRG jump needs max_seg_regrowth >= 1"};
+            }
+            if (P.move_maxreg[m] > LDO_JUMP_MAX_REGROWTH) {
+                throw AbiError {LDO_E_ARG, "CTRG jump max_regrowth must be at most 60"};
+            }
+            P.move_segreg[m] = run->movetype_max_seg_regrowth[m];
+        }
+        if (kind == 4 || kind == 5) {
             if (!run->movetype_max_num_recoils || !run->movetype_max_c_attempts) {
                 throw AbiError {LDO_E_ARG, "CTRG needs max_num_recoils and max_c_attempts"};
             }

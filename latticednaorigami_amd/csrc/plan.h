@@ -25,7 +25,7 @@ struct AbiError {
     std::string msg;
 };
 
-Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap, bool rg);
+Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap, bool rg, bool jump);
 
 // element offsets of the learned proposal's flat parameter vector (ldo_mlp.h)
 void mlp_layout(int n_ident, ldo_mlp_layout& L);

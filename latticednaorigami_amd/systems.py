@@ -31,6 +31,8 @@ class Move:
     # CTRG recoil growth: recoils allowed and trial configurations per domain
     max_num_recoils: int = 0
     max_c_attempts: int = 0
+    max_seg_regrowth: int = 0  # CTRG jump
+    label: str = None          # the moveset file's label (None: the type name)
 
 
 @dataclass
@@ -126,6 +128,7 @@ class System:
         rp.movetype_max_regrowth = arr(C.c_int32, [m.max_regrowth for m in self.moves])
         rp.movetype_max_num_recoils = arr(C.c_int32, [m.max_num_recoils for m in self.moves])
         rp.movetype_max_c_attempts = arr(C.c_int32, [m.max_c_attempts for m in self.moves])
+        rp.movetype_max_seg_regrowth = arr(C.c_int32, [m.max_seg_regrowth for m in self.moves])
         rp.max_total_staples = self.max_total_staples
         rp.max_type_staples = self.max_type_staples
         rp.max_staple_size = self.max_staple_size
@@ -223,7 +226,8 @@ def read_moveset_json(path_or_dict):
             raise NotImplementedError(f"{m['type']}: movetype not supported")
         moves.append(Move(abi.MOVETYPE_NAMES[m["type"]], parse_fraction(m["freq"]),
                           int(m.get("max_regrowth", 0)), int(m.get("max_num_recoils", 0)),
-                          int(m.get("max_c_attempts", 0))))
+                          int(m.get("max_c_attempts", 0)), int(m.get("max_seg_regrowth", 0)),
+                          m.get("label")))
     return moves
 
 
@@ -245,6 +249,19 @@ def ctrg_moveset(max_regrowth, max_num_recoils=1, max_c_attempts=36):
             Move(abi.MOVE_CB_STAPLE_REGROWTH, 1 / 6),
             Move(abi.MOVE_CTRG_SCAFFOLD_REGROWTH, 2 / 6, max_regrowth, max_num_recoils,
                  max_c_attempts)]
+
+
+def jump_moveset(max_regrowth, max_num_recoils=1, max_c_attempts=36, max_seg_regrowth=2):
+    """The reference's examples/moveset_standard.json: orientation 2/6,
+    staple exchange 1/6, CB staple 1/6, contiguous and non-contiguous CTRG
+    1/6 each."""
+    return [Move(abi.MOVE_ORIENTATION_ROTATION, 2 / 6),
+            Move(abi.MOVE_MET_STAPLE_EXCHANGE, 1 / 6),
+            Move(abi.MOVE_CB_STAPLE_REGROWTH, 1 / 6),
+            Move(abi.MOVE_CTRG_SCAFFOLD_REGROWTH, 1 / 6, max_regrowth, max_num_recoils,
+                 max_c_attempts),
+            Move(abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH, 1 / 6, max_regrowth, max_num_recoils,
+                 max_c_attempts, max_seg_regrowth)]
 
 
 def four_domain_system(seed=20261202, max_total_staples=4):
@@ -274,6 +291,22 @@ def four_domain_system(seed=20261202, max_total_staples=4):
         name="four-domain HalfTurn scaffold, 2 two-domain staple types, Uniform, 330 K",
         temps=[330.0],
     )
+
+
+def hairpin_system(n_domains=4, seed=20261202, max_total_staples=None):
+    """A scaffold of n_domains (even) on a line with crossover staples
+    [k+1, n-k] that fold it into a hairpin: each staple binds two scaffold
+    domains far apart along the chain, so the non-contiguous scaffold
+    selection finds stem domains.  Energies and concentrations as the
+    four-domain workload; at most one staple of each type."""
+    n = n_domains
+    staples = [[k + 1, n - k] for k in range(n // 2)]
+    s = four_domain_system(seed=seed, max_total_staples=max_total_staples or n // 2)
+    s.identities = [[-(i + 1) for i in range(n)]] + staples
+    s.chains = [Chain(0, 0, [(i, 0, 0) for i in range(n)], [(0, 0, 1)] * n)]
+    s.max_type_staples = 1
+    s.name = f"{n}-domain hairpin scaffold, {len(staples)} crossover staple types, Uniform"
+    return s
 
 
 def three_domain_system(domain_type=abi.HALFTURN, misbinding_pot=abi.MISBIND_OPPOSING,

@@ -103,7 +103,8 @@ void* oracle_create(const ldo_system_desc* sd, const ldo_run_params* run,
         for (int m = 0; m < run->n_movetypes; m++) {
             MovesetEntry e {run->movetype_kind[m], run->movetype_freq[m],
                             run->movetype_max_regrowth ? run->movetype_max_regrowth[m] : 0};
-            if (e.kind == LDO_MOVE_CTRG_SCAFFOLD_REGROWTH) {
+            if (e.kind == LDO_MOVE_CTRG_SCAFFOLD_REGROWTH ||
+                e.kind == LDO_MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH) {
                 // simulation.cpp:493-495; the trial configurations of a
                 // domain are the 36 pairs of utility::vectors
                 if (!run->movetype_max_num_recoils || !run->movetype_max_c_attempts) {
@@ -114,6 +115,12 @@ void* oracle_create(const ldo_system_desc* sd, const ldo_run_params* run,
                 if (e.max_regrowth < 2 || e.max_num_recoils < 0 || e.max_c_attempts < 1 ||
                     e.max_c_attempts > 36) {
                     throw OrigamiError(LDO_E_ARG, "CTRG parameters out of range");
+                }
+                if (e.kind == LDO_MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH) {
+                    if (!run->movetype_max_seg_regrowth || run->movetype_max_seg_regrowth[m] < 1) {
+                        throw OrigamiError(LDO_E_ARG, "CTRG jump needs max_seg_regrowth >= 1");
+                    }
+                    e.max_seg_regrowth = run->movetype_max_seg_regrowth[m];
                 }
             }
             moves.push_back(e);
@@ -525,7 +532,7 @@ int oracle_get_move_tracking(void* o, int32_t r, int64_t* out, int64_t cap, int6
         Replica& rep = *O->reps.at(r);
         int64_t T = (int64_t)rep.sys.identities.size();
         int64_t ns = (int64_t)rep.sys.identities[0].size();
-        int64_t n = 6 * T + 4 * (ns + 1) + 2 * ((int64_t)O->lim.max_total_staples + 1);
+        int64_t n = 6 * T + 4 * (ns + 1) + 2 * ((int64_t)O->lim.max_total_staples + 1) + 2;
         if (cap < n) {
             throw OrigamiError(LDO_E_ARG, "tracking capacity too small");
         }

@@ -11,6 +11,7 @@
 
 #include <cstdint>
 #include <array>
+#include <deque>
 #include <map>
 #include <random>
 #include <set>
@@ -298,6 +299,7 @@ struct MovesetEntry {
     int max_regrowth;
     int max_num_recoils {0}; // CTRG (simulation.cpp:493-495)
     int max_c_attempts {0};
+    int max_seg_regrowth {0}; // CTRGJump
 };
 
 struct Limits {
@@ -337,6 +339,12 @@ class Constraintpoints {
     bool scaffold_endpoints {false};
     void reset();
     void calculate(const std::vector<Dom*>& scaffold_domains, int dir);
+    void calculate(const std::vector<std::vector<Dom*>>& segments, const std::vector<int>& dirs);
+    void add_growthpoint(Dom* growthpoint, Dom* stemd) {
+        m_growthpoints[growthpoint] = stemd;
+        m_stemdomains[stemd] = growthpoint;
+    }
+    void add_stem_seg_pair(Dom* stemd, std::vector<int> segs) { m_stemd_to_segs[stemd] = segs; }
     std::set<int> staples_to_be_regrown() const { return m_regrowth_staples; }
     bool is_growthpoint(Dom* d) const { return m_growthpoints.count(d) > 0; }
     bool is_stemdomain(Dom* d) const { return m_stemdomains.count(d) > 0; }
@@ -352,6 +360,7 @@ class Constraintpoints {
     Dom* domain_to_grow(Dom* d) { return m_growthpoints[d]; }
     bool endpoint_reached(Dom* d, V3 pos);
     bool walks_remain(Dom* d, V3 pos);
+    bool walks_remain_seg(Dom* d, V3 pos, int seg);
     int get_dir(Dom* d);
 
   private:
@@ -381,6 +390,7 @@ class Constraintpoints {
     std::map<std::pair<int, int>, std::vector<std::pair<int, V3>>> m_initial_active;
     std::unordered_map<Dom*, Dom*> m_inactive;
     std::vector<V3> m_erased; // positions the last remove_active_endpoint took
+    std::unordered_map<Dom*, std::vector<int>> m_stemd_to_segs;
 };
 
 // One replica: system + moves + RNG (the per-rank GCMCSimulation state)
@@ -517,6 +527,17 @@ class Replica {
     double m_weight {1}, m_weight_new {1};
     int m_max_recoils {0}, m_max_c_attempts {0};
     bool ctrg_scaffold_move(int max_regrowth);
+    // the non-contiguous selection (movetypes.cpp:512-648, 666-721)
+    int m_max_seg_regrowth {0};
+    void select_noncontig_segs(std::vector<std::vector<Dom*>>& segs,
+                               std::vector<std::vector<std::vector<Dom*>>>& paired_segs,
+                               std::vector<Dom*>& seg_stems, std::vector<int>& dirs,
+                               int max_regrowth);
+    bool fill_seg(Dom* start_d, size_t max_length, size_t seg_max_length, int dir,
+                  std::set<Dom*>& domains, std::deque<Dom*>& possible_stems,
+                  std::vector<Dom*>& seg);
+    void check_for_stemds(Dom* cur_d, std::deque<Dom*>& possible_stems);
+    bool ctrg_jump_move(int max_regrowth);
     double rg_unassign_and_save();
     void rg_unassign_domains();
     double rg_recoil_regrow();

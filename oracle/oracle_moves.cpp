@@ -4,6 +4,7 @@
 // TEST INFRASTRUCTURE ONLY (see ldo_oracle.h).
 #include <algorithm>
 #include <cmath>
+#include <deque>
 #include <functional>
 #include <set>
 #include <unordered_map>
@@ -28,6 +29,7 @@ void Constraintpoints::reset() {
     m_initial_active.clear();
     m_inactive.clear();
     m_erased.clear();
+    m_stemd_to_segs.clear();
     m_net_scaffold_ds.clear();
     m_net_dirs.clear();
 }
@@ -43,6 +45,23 @@ void Constraintpoints::calculate(const std::vector<Dom*>& sds, int dir) {
     m_net_scaffold_ds = sds;
     m_dirs[{sds[0]->c, 0}] = dir;
     find_growthpoints_endpoints(sds, 0);
+    m_initial_active = m_active;
+}
+
+// top_constraint_points.cpp:223-248: segment i is segment index i, empty
+// segments included (their direction stays unset)
+void Constraintpoints::calculate(const std::vector<std::vector<Dom*>>& segs,
+                                 const std::vector<int>& dirs) {
+    m_net_scaffold_ds.clear();
+    for (auto& sg: segs) {
+        m_net_scaffold_ds.insert(m_net_scaffold_ds.end(), sg.begin(), sg.end());
+    }
+    for (size_t i = 0; i < segs.size(); i++) {
+        if (!segs[i].empty()) {
+            m_dirs[{segs[i][0]->c, (int)i}] = dirs[i];
+            find_growthpoints_endpoints(segs[i], (int)i);
+        }
+    }
     m_initial_active = m_active;
 }
 
@@ -122,10 +141,22 @@ static bool walks_exist(V3 a, V3 b, int steps) {
 // segments up in m_stemd_to_segs, which only the non-contiguous move fills,
 // so for the contiguous move a stem domain has no walk constraint.
 bool Constraintpoints::walks_remain(Dom* d, V3 pos) {
+    std::vector<int> segs;
     if (is_stemdomain(d)) {
-        return true;
+        segs = m_stemd_to_segs[d];
     }
-    int seg = m_segs.at(d);
+    else {
+        segs = {m_segs.at(d)};
+    }
+    for (int seg: segs) {
+        if (!walks_remain_seg(d, pos, seg)) {
+            return false;
+        }
+    }
+    return true;
+}
+
+bool Constraintpoints::walks_remain_seg(Dom* d, V3 pos, int seg) {
     std::pair<int, int> key {d->c, seg};
     int dir = m_dirs[key];
     for (auto& e: m_active[key]) {
@@ -314,7 +345,7 @@ void Replica::add_tracker(int slot, bool accepted) {
     int ns = (int)sys.identities[0].size();
     int a = accepted ? 1 : 0;
     if (track.empty()) {
-        track.assign(6 * T + 4 * (ns + 1) + 2 * (m_lim.max_total_staples + 1), 0);
+        track.assign(6 * T + 4 * (ns + 1) + 2 * (m_lim.max_total_staples + 1) + 2, 0);
     }
     switch (m_moves[slot].kind) {
     case LDO_MOVE_MET_STAPLE_EXCHANGE: {
@@ -344,6 +375,12 @@ void Replica::add_tracker(int slot, bool accepted) {
         track[o + ns + 1 + tracker.num_scaffold_domains] += a;
         break;
     }
+    case LDO_MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH: {
+        int o = 6 * T + 4 * (ns + 1) + 2 * (m_lim.max_total_staples + 1);
+        track[o] += 1;
+        track[o + 1] += a;
+        break;
+    }
     default:
         break;
     }
@@ -369,6 +406,7 @@ void Replica::simulate(int64_t steps) {
         attempts[i]++;
         m_max_recoils = m_moves[i].max_num_recoils;
         m_max_c_attempts = m_moves[i].max_c_attempts;
+        m_max_seg_regrowth = m_moves[i].max_seg_regrowth;
         bool accepted = attempt(m_moves[i].kind, m_moves[i].max_regrowth);
         accepts[i] += accepted;
         add_tracker((int)i, accepted);
@@ -436,6 +474,18 @@ bool Replica::attempt(int kind, int max_regrowth) {
         m_weight = 1;
         m_weight_new = 1;
         return ctrg_scaffold_move(max_regrowth);
+    case LDO_MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH:
+        m_cp.reset();
+        m_regrow_ds.clear();
+        m_c_attempts_q.clear();
+        m_c_attempts_wq.clear();
+        m_avail_cis_q.clear();
+        m_avail_cis_wq.clear();
+        m_c_opens.clear();
+        m_erased_q.clear();
+        m_weight = 1;
+        m_weight_new = 1;
+        return ctrg_jump_move(max_regrowth);
     }
     throw OrigamiError(LDO_E_NOTIMPL, "movetype not supported");
 }
@@ -1461,6 +1511,208 @@ bool Replica::ctrg_scaffold_move(int max_regrowth) {
     if (!whole) {
         m_cp.remove_active_endpoint(m_regrow_ds[0]);
     }
+    rg_calc_weights();
+    return rg_test_acceptance();
+}
+
+// ------------------------------------ non-contiguous scaffold selection
+// CTRegrowthMCMovetype::check_for_stemds (movetypes.cpp:704-720): the
+// scaffold domains bound to the chain neighbours of the staple domain bound
+// to cur_d
+void Replica::check_for_stemds(Dom* cur_d, std::deque<Dom*>& possible_stems) {
+    if (cur_d->st != Occ::bound) {
+        return;
+    }
+    Dom* bound_d = cur_d->bd;
+    for (int staple_dir: {-1, 1}) {
+        Dom* nd = walk(bound_d, staple_dir);
+        if (nd != nullptr && nd->st == Occ::bound && nd->bd->c == 0) {
+            possible_stems.push_back(nd->bd);
+        }
+    }
+}
+
+// CTRegrowthMCMovetype::fill_seg (movetypes.cpp:666-702)
+bool Replica::fill_seg(Dom* start_d, size_t max_length, size_t seg_max_length, int dir,
+                       std::set<Dom*>& domains, std::deque<Dom*>& possible_stems,
+                       std::vector<Dom*>& seg) {
+    Dom* cur_d = start_d;
+    check_for_stemds(cur_d, possible_stems);
+    Dom* next_d = start_d;
+    while (seg.size() != seg_max_length && next_d != nullptr) {
+        next_d = walk(cur_d, dir);
+        if (next_d == nullptr) {
+            break;
+        }
+        Dom* next_next_d = walk(next_d, dir);
+        if (next_next_d != nullptr && domains.count(next_next_d) != 0) {
+            break;
+        }
+        seg.push_back(next_d);
+        domains.insert(next_d);
+        if (domains.size() == max_length) {
+            return true;
+        }
+        cur_d = next_d;
+        check_for_stemds(cur_d, possible_stems);
+    }
+    return false;
+}
+
+// CTRegrowthMCMovetype::select_noncontig_segs (movetypes.cpp:512-648)
+void Replica::select_noncontig_segs(std::vector<std::vector<Dom*>>& segs,
+                                    std::vector<std::vector<std::vector<Dom*>>>& paired_segs,
+                                    std::vector<Dom*>& seg_stems, std::vector<int>& dirs,
+                                    int max_regrowth) {
+    std::vector<Dom*>& given = sys.chains[0];
+    std::set<Dom*> domains;
+    size_t max_length = (size_t)rng.uniform_int(2, max_regrowth);
+    size_t seg_max = (size_t)rng.uniform_int(2, m_max_seg_regrowth + 1);
+    int start_i = rng.uniform_int(0, (int)given.size() - 1);
+    Dom* seg_start_d = given[start_i];
+    int dir = rng.uniform_int(0, 1);
+    if (dir == 0) {
+        dir = -1;
+    }
+    if (walk(seg_start_d, dir) == nullptr) {
+        dir *= -1;
+    }
+    std::vector<Dom*> cur_seg {seg_start_d};
+    domains.insert(seg_start_d);
+    std::deque<Dom*> possible_stems;
+    dirs.push_back(dir);
+    bool reached = fill_seg(seg_start_d, max_length, seg_max, dir, domains, possible_stems,
+                            cur_seg);
+    segs.push_back(cur_seg);
+    while (!reached && !possible_stems.empty()) {
+        cur_seg.clear();
+        Dom* stemd = possible_stems.front();
+        possible_stems.pop_front();
+        if (domains.count(stemd) != 0) {
+            continue;
+        }
+        bool adjacent = false;
+        for (int test_dir: {-1, 1}) {
+            Dom* nd = walk(stemd, test_dir);
+            if (nd != nullptr && domains.count(nd) != 0) {
+                adjacent = true;
+                break;
+            }
+        }
+        if (adjacent) {
+            continue;
+        }
+        cur_seg.push_back(stemd);
+        domains.insert(stemd);
+        seg_stems.push_back(stemd);
+        if (domains.size() == max_length) {
+            reached = true;
+            segs.push_back(cur_seg);
+            segs.push_back({});
+            paired_segs.push_back({{}, {}});
+            dirs.push_back(1);
+            dirs.push_back(-1);
+            break;
+        }
+        int dir1 = rng.uniform_int(0, 1);
+        if (dir1 == 0) {
+            dir1 = -1;
+        }
+        int dir_pair[2] = {dir1, -dir1};
+        dirs.push_back(dir_pair[0]);
+        dirs.push_back(dir_pair[1]);
+        std::vector<std::vector<Dom*>> seg_pair {{}, {}};
+        segs.push_back({});
+        segs.push_back({});
+        for (size_t i: {0, 1}) {
+            dir = dir_pair[i];
+            seg_max = (size_t)rng.uniform_int(0, m_max_seg_regrowth);
+            if (i == 0) {
+                max_length++;
+            }
+            std::vector<Dom*> seg;
+            reached = fill_seg(stemd, max_length, seg_max, dir, domains, possible_stems, seg);
+            seg_pair[i] = seg;
+            if (!cur_seg.empty()) {
+                seg.insert(seg.begin(), cur_seg[0]);
+            }
+            segs[segs.size() - 2 + i] = seg;
+            if (reached) {
+                break;
+            }
+            cur_seg.clear();
+        }
+        paired_segs.push_back(seg_pair);
+    }
+
+    Dom* last_d = segs[0].back();
+    Dom* next_d = walk(last_d, dirs[0]);
+    if (next_d != nullptr) {
+        m_cp.add_active_endpoint(next_d, next_d->pos, 0);
+    }
+    int seg_i = 1;
+    for (size_t i = 0; i < paired_segs.size(); i++) {
+        Dom* stem_d = seg_stems[i];
+        m_cp.add_growthpoint(stem_d->bd, stem_d);
+        m_cp.add_stem_seg_pair(stem_d, {seg_i, seg_i + 1});
+        for (auto& sg: paired_segs[i]) {
+            int d = dirs[seg_i];
+            last_d = sg.empty() ? stem_d : sg.back();
+            Dom* nd = walk(last_d, d);
+            if (nd != nullptr) {
+                m_cp.add_active_endpoint(nd, nd->pos, seg_i);
+            }
+            seg_i++;
+        }
+    }
+}
+
+// CTRGJumpScaffoldRegrowthMCMovetype::internal_attempt_move
+// (rg_movetypes.cpp:791-853)
+bool Replica::ctrg_jump_move(int max_regrowth) {
+    std::vector<std::vector<Dom*>> segs;
+    std::vector<std::vector<std::vector<Dom*>>> paired_segs;
+    std::vector<Dom*> seg_stems;
+    std::vector<int> dirs;
+    select_noncontig_segs(segs, paired_segs, seg_stems, dirs, max_regrowth);
+    tracker.num_scaffold_domains = 0;
+    m_cp.calculate(segs, dirs);
+    m_cp.remove_active_endpoint(segs[0][0]);
+    for (Dom* stem_d: seg_stems) {
+        m_cp.add_growthpoint(stem_d->bd, stem_d);
+    }
+    m_regrow_ds = m_cp.domains_to_be_regrown();
+    m_delta_e += rg_unassign_and_save();
+    m_delta_e += rg_recoil_regrow();
+    if (m_rejected) {
+        return false;
+    }
+    m_delta_e += calc_move_bias(0); // add_external_bias (:82-85)
+
+    m_c_attempts_wq = m_c_attempts_q;
+    m_avail_cis_wq = m_avail_cis_q;
+    m_old_pos = m_prev_pos;
+    m_old_ore = m_prev_ore;
+    m_modified.clear();
+    rg_unassign_and_save();
+    m_cp.reset_active_endpoints();
+    m_cp.remove_active_endpoint(segs[0][0]);
+    rg_calc_weights();
+
+    rg_unassign_domains();
+    m_cp.reset_active_endpoints();
+    m_cp.remove_active_endpoint(segs[0][0]);
+    rg_calc_old_c_opens();
+    m_weight_new = m_weight;
+    m_weight = 1;
+    m_c_attempts_wq = m_c_attempts_q;
+    m_avail_cis_wq = m_avail_cis_q;
+    m_new_pos = m_prev_pos;
+    m_new_ore = m_prev_ore;
+    m_modified.clear();
+    rg_unassign_and_save();
+    m_cp.reset_active_endpoints();
+    m_cp.remove_active_endpoint(segs[0][0]);
     rg_calc_weights();
     return rg_test_acceptance();
 }

@@ -646,8 +646,10 @@ def _ctrg_chain(sampler_cls, oracle_mod, s, temps, rounds, steps):
     for r in range(len(temps)):
         tr = gpu.move_tracking(r)
         assert tr == ref.move_tracking(r)
-        slot = [m.kind for m in s.moves].index(abi.MOVE_CTRG_SCAFFOLD_REGROWTH)
-        assert sum(tr["ctrg_length_attempts"]) <= gpu.counters().attempts[slot]
+        kinds = [m.kind for m in s.moves]
+        if abi.MOVE_CTRG_SCAFFOLD_REGROWTH in kinds:
+            slot = kinds.index(abi.MOVE_CTRG_SCAFFOLD_REGROWTH)
+            assert sum(tr["ctrg_length_attempts"]) <= gpu.counters().attempts[slot]
     return gpu
 
 
@@ -696,11 +698,47 @@ def case_ctrg_three_domain_staples(sampler_cls, oracle_mod):
 def case_ctrg_rejects_bad_limits(sampler_cls, oracle_mod):
     """max_c_attempts outside 1..36 (the trial configurations are the 36
     (site, orientation) pairs, drawn without replacement), negative
-    max_num_recoils and max_regrowth below 2 are refused at creation."""
+    max_num_recoils, max_regrowth below 2, and for the non-contiguous move
+    max_seg_regrowth below 1 (its first segment's length is drawn from
+    [2, max_seg_regrowth + 1]) or max_regrowth above 60 are refused at
+    creation."""
     say("case: CTRG limits outside their ranges are refused")
-    for maxreg, rec, att in ((12, 1, 37), (12, 1, 0), (12, -1, 36), (1, 1, 36)):
+    bad = [systems.ctrg_moveset(maxreg, rec, att)
+           for maxreg, rec, att in ((12, 1, 37), (12, 1, 0), (12, -1, 36), (1, 1, 36))]
+    bad += [systems.jump_moveset(12, 1, 36, 0), systems.jump_moveset(61, 1, 36, 2)]
+    for moves in bad:
         s = systems.raster_system(replicas=1)
-        s.moves = systems.ctrg_moveset(maxreg, rec, att)
+        s.moves = moves
         with pytest.raises(abi.LdoError) as e:
             sampler_cls(s, temps=[300.0])
         assert e.value.code == abi.LDO_E_ARG
+
+
+def case_ctrg_jump_bit_exact(sampler_cls, oracle_mod):
+    """Non-contiguous CTRG (select_noncontig_segs, movetypes.cpp:512-721;
+    rg_movetypes.cpp:791-853) in the reference's example moveset: hairpin
+    scaffolds of 4 and 6 domains whose crossover staples give stem domains
+    (segment pairs grown from a staple domain, walk checks over both), and
+    the 64-domain raster, whose selections are mostly single segments."""
+    say("case: CTRG jump, hairpins of 4 and 6 domains at 330/345 K (3 x 5000 steps), "
+        "64-domain raster at 300 K (3 x 5000 steps)")
+    for n in (4, 6):
+        s = systems.hairpin_system(n)
+        s.moves = systems.jump_moveset(n)
+        g = _ctrg_chain(sampler_cls, oracle_mod, s, [330.0, 345.0], 3, 5000)
+        assert g.move_tracking(0)["ctrg_jump_attempts"][0] > 1000
+    s = systems.raster_system(replicas=1)
+    s.moves = systems.jump_moveset(12)
+    _ctrg_chain(sampler_cls, oracle_mod, s, [300.0], 3, 5000)
+
+
+def case_ctrg_jump_limits(sampler_cls, oracle_mod):
+    """Non-contiguous CTRG with other limits: (recoils, attempts,
+    max_seg_regrowth) = (0, 36, 1), (2, 6, 3), (3, 12, 4) on the six-domain
+    hairpin at 330 K."""
+    say("case: CTRG jump limits (0, 36, 1), (2, 6, 3), (3, 12, 4) on the 6-domain hairpin, "
+        "2 x 6000 steps")
+    for rec, att, sreg in ((0, 36, 1), (2, 6, 3), (3, 12, 4)):
+        s = systems.hairpin_system(6)
+        s.moves = systems.jump_moveset(6, rec, att, sreg)
+        _ctrg_chain(sampler_cls, oracle_mod, s, [330.0, 340.0], 2, 6000)

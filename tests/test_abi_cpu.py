@@ -48,3 +48,22 @@ int main(void) {
             C.sizeof(abi.Obs), C.sizeof(abi.Counters), abi.SystemDesc.binding_h.offset,
             abi.RunParams.seed.offset]
     assert got == want
+
+
+def test_reference_example_moveset_parses():
+    """The reference's examples/moveset_standard.json (a fixture copy): every
+    movetype it names is supported, with its labels, fractions and limits."""
+    import os
+    from latticednaorigami_amd import abi, systems
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "moveset_standard.json")
+    moves = systems.read_moveset_json(path)
+    assert [m.kind for m in moves] == [abi.MOVE_ORIENTATION_ROTATION, abi.MOVE_MET_STAPLE_EXCHANGE,
+                                       abi.MOVE_CB_STAPLE_REGROWTH,
+                                       abi.MOVE_CTRG_SCAFFOLD_REGROWTH,
+                                       abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH]
+    assert [m.freq for m in moves] == [2 / 6, 1 / 6, 1 / 6, 1 / 6, 1 / 6]
+    assert (moves[3].max_num_recoils, moves[3].max_c_attempts, moves[3].max_regrowth) == (1, 36, 12)
+    assert (moves[4].max_num_recoils, moves[4].max_c_attempts, moves[4].max_regrowth,
+            moves[4].max_seg_regrowth) == (1, 36, 12, 2)
+    assert moves[3].label == "Contiguous CTRG scaffold regrowth"

@@ -5,6 +5,7 @@ ptmc_simulation.cpp:91-104, 315-322)."""
 import json
 import sys
 import os
+import shutil
 import subprocess
 
 import pytest
@@ -16,8 +17,16 @@ MOVE_TYPES = {abi.MOVE_ORIENTATION_ROTATION: "OrientationRotation",
               abi.MOVE_MET_STAPLE_EXCHANGE: "MetStapleExchange",
               abi.MOVE_CB_STAPLE_REGROWTH: "CBStapleRegrowth",
               abi.MOVE_CTCB_SCAFFOLD_REGROWTH: "CTCBScaffoldRegrowth",
-              abi.MOVE_CTRG_SCAFFOLD_REGROWTH: "CTRGScaffoldRegrowth"}
+              abi.MOVE_CTRG_SCAFFOLD_REGROWTH: "CTRGScaffoldRegrowth",
+              abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH: "CTRGJumpScaffoldRegrowth"}
 HYB = {abi.HYB_NEARESTNEIGHBOUR: "NearestNeighbour", abi.HYB_UNIFORM: "Uniform"}
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+# the reference's examples/moveset_standard.json
+MOVESET_STANDARD = os.path.join(GOLDEN, "moveset_standard.json")
+
+
+def label_of(m):
+    return m.label or MOVE_TYPES[m.kind]
 
 
 def write_inputs(d, s, extra):
@@ -33,11 +42,13 @@ def write_inputs(d, s, extra):
     with open(os.path.join(d, "system.json"), "w") as f:
         json.dump(sysfile, f)
     def entry(m):
-        e = {"label": MOVE_TYPES[m.kind], "type": MOVE_TYPES[m.kind], "freq": m.freq,
+        e = {"label": label_of(m), "type": MOVE_TYPES[m.kind], "freq": m.freq,
              "max_regrowth": m.max_regrowth}
-        if m.kind == abi.MOVE_CTRG_SCAFFOLD_REGROWTH:
+        if m.kind in (abi.MOVE_CTRG_SCAFFOLD_REGROWTH, abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH):
             e["max_num_recoils"] = m.max_num_recoils
             e["max_c_attempts"] = m.max_c_attempts
+        if m.kind == abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH:
+            e["max_seg_regrowth"] = m.max_seg_regrowth
         return e
     moves = {"origami": {"movetypes": [entry(m) for m in s.moves]}}
     with open(os.path.join(d, "moves.json"), "w") as f:
@@ -156,7 +167,7 @@ def expect_log_entry(ref, r, s, step):
     counts = [0] * (len(s.identities) - 1)
     for c in cfg[1:]:
         counts[c.identity - 1] += 1
-    label = MOVE_TYPES[s.moves[ob["last_move"]].kind]
+    label = label_of(s.moves[ob["last_move"]])
     return (f"Step: {step}\nTemperature: {gdef(ob['temperature'])}\n"
             f"Bound staples: {ob['num_staples']}\n"
             f"Unique bound staples: {sum(1 for n in counts if n)}\n"
@@ -178,7 +189,7 @@ def expect_summary(ref, s):
     tr = ref.move_tracking(0)
     log, moves = ["Run summary\n\n"], []
     for m, mv in enumerate(s.moves):
-        label = MOVE_TYPES[mv.kind]
+        label = label_of(mv)
         head = (f"Movetype: {label}\n    Attempts: {c.attempts[m]}\n    Accepts: {c.accepts[m]}\n"
                 f"    Frequency: {ratio(c.accepts[m], c.attempts[m])}\n")
         log.append(head + "\n")
@@ -218,6 +229,14 @@ def expect_summary(ref, s):
                     moves.append(f"    Number of scaffold domains: {n}\n        Attempts: {a}\n"
                                  f"        Accepts: {k}\n        Frequency: {ratio(k, a)}\n")
             moves.append("\n")
+        elif mv.kind == abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH:
+            # rg_movetypes.cpp:745-789, every move under 0 scaffold domains
+            moves.append(head)
+            a, k = tr["ctrg_jump_attempts"][0], tr["ctrg_jump_accepts"][0]
+            if a:
+                moves.append(f"    Number of scaffold domains: 0\n        Attempts: {a}\n"
+                             f"        Accepts: {k}\n        Frequency: {ratio(k, a)}\n")
+            moves.append("\n")
     return "".join(log), "".join(moves)
 
 
@@ -255,36 +274,37 @@ def test_driver_rejects_unsupported_inputs(tmp_path):
     s.moves = [systems.Move(abi.MOVE_ORIENTATION_ROTATION, 1.0)]
     inp = write_inputs(str(tmp_path), s, {"simulation_type": "constant_temp"})
     with open(os.path.join(str(tmp_path), "moves.json")) as f:
-        text = f.read().replace("OrientationRotation", "CTRGJumpScaffoldRegrowth")
+        text = f.read().replace("OrientationRotation", "CTRGLinkerRegrowth")
     with open(os.path.join(str(tmp_path), "moves.json"), "w") as f:
         f.write(text)
     p = subprocess.run([DRIVER, inp], capture_output=True, text=True, timeout=60)
-    assert p.returncode == 1 and "movetype CTRGJumpScaffoldRegrowth" in p.stderr, p.stderr
+    assert p.returncode == 1 and "movetype CTRGLinkerRegrowth" in p.stderr, p.stderr
     # a CTRG entry without its recoil-growth limits (the reference's
     # get_int_option throws for them too)
     with open(os.path.join(str(tmp_path), "moves.json"), "w") as f:
-        f.write(text.replace("CTRGJumpScaffoldRegrowth", "CTRGScaffoldRegrowth")
+        f.write(text.replace("CTRGLinkerRegrowth", "CTRGScaffoldRegrowth")
                 .replace('"max_regrowth": 0', '"max_regrowth": 4'))
     p = subprocess.run([DRIVER, inp], capture_output=True, text=True, timeout=60)
     assert p.returncode == 1 and "max_c_attempts" in p.stderr, p.stderr
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("moveset", ["ctcb", "ctrg"])
+@pytest.mark.parametrize("moveset", ["ctcb", "standard_json"])
 def test_driver_constant_temp_matches_oracle(tmp_path, oracle_mod, moveset):
     s = systems.four_domain_system()
-    if moveset == "ctrg":
-        # the reference's examples/moveset_standard.json, its non-contiguous
-        # CTRG entry's share split between contiguous CTRG and CTCB
-        s.moves = systems.ctrg_moveset(4)[:3] + [
-            systems.Move(abi.MOVE_CTRG_SCAFFOLD_REGROWTH, 1 / 6, 4, 1, 36),
-            systems.Move(abi.MOVE_CTCB_SCAFFOLD_REGROWTH, 1 / 6, 4)]
+    if moveset == "standard_json":
+        # the reference's examples/moveset_standard.json as it is, labels and
+        # "a/b" frequencies included (max_regrowth 12 > 4 domains: a
+        # contiguous selection takes at most the whole scaffold)
+        s.moves = systems.read_moveset_json(MOVESET_STANDARD)
     d = str(tmp_path)
     inp = write_inputs(d, s, {"simulation_type": "constant_temp", "temp": "330",
                               "ct_steps": 20000, "configs_output_freq": 4000,
                               "vtf_output_freq": 4000, "counts_output_freq": 2000,
                               "times_output_freq": 5000, "energies_output_freq": 2000,
                               "order_params_output_freq": 2000, "logging_freq": 4000})
+    if moveset == "standard_json":
+        shutil.copyfile(MOVESET_STANDARD, os.path.join(d, "moves.json"))
     out = run_driver(inp).stdout
     ref = oracle_mod.Oracle(s, temps=[330.0])
     ene, ops, counts, trj = read(d, "out.ene"), read(d, "out.ops"), read(d, "out.counts"), \

@@ -154,3 +154,11 @@ def test_ctrg_three_domain_staples(sampler_cls, oracle_mod):
 
 def test_ctrg_rejects_bad_limits(sampler_cls, oracle_mod):
     pc.case_ctrg_rejects_bad_limits(sampler_cls, oracle_mod)
+
+
+def test_ctrg_jump_bit_exact(sampler_cls, oracle_mod):
+    pc.case_ctrg_jump_bit_exact(sampler_cls, oracle_mod)
+
+
+def test_ctrg_jump_limits(sampler_cls, oracle_mod):
+    pc.case_ctrg_jump_limits(sampler_cls, oracle_mod)

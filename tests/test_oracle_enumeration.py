@@ -290,6 +290,87 @@ def test_reference_ctcb_rule_unbinds_three_domain_staples(oracle_mod):
     assert exact[(0, 0, 0, 0)] < 0.4 and m_bare > 0.7
 
 
+def _scaffold_only(n_domains, misbinding_h=0.0):
+    """A bare HalfTurn scaffold of n_domains on a line (no staples allowed):
+    its conformations are the self-avoiding-or-misbound walks the
+    enumerator counts, so only the scaffold moves shape the distribution."""
+    s = systems.four_domain_system(seed=7, max_total_staples=0)
+    s.identities = [[-(i + 1) for i in range(n_domains)], [1, 2]]
+    s.chains = [systems.Chain(0, 0, [(i, 0, 0) for i in range(n_domains)],
+                              [(0, 0, 1)] * n_domains)]
+    s.max_type_staples = 0
+    s.misbinding_h = misbinding_h
+    return s
+
+
+@pytest.mark.parametrize("misbinding_h,attempts,max_regrowth,n_samples", [
+    # the reference's limits (one recoil, all 36 trial configurations);
+    # inverting the acceptance ratio's weights (a planted mistake) gives
+    # p(0,0,2,0) = 0.110 against 0.132, seven standard errors out
+    (0.0, 36, 6, 1500),
+    # a misbound pair costs +1.24 kT, so open probabilities fall below 1;
+    # dropping them from the weights (planted) gives 0.979 / 0.021 for no /
+    # one misbound pair against 0.962 / 0.037, twelve standard errors out
+    (1000.0, 6, 6, 1500),
+    # two trial configurations per domain: dead ends are common and the
+    # feeler growth (test_config_avail) decides which trials count.  Counting
+    # every open trial instead (planted) moves p(0,0,0,0) by 0.010, about
+    # three standard errors: this case exercises the feelers but pins their
+    # count only loosely.
+    (0.0, 2, 4, 8000),
+])
+def test_ctrg_scaffold_regrowth_matches_enumeration(oracle_mod, misbinding_h, attempts,
+                                                     max_regrowth, n_samples):
+    """CTRG recoil regrowth (rg_movetypes.cpp) with orientation rotations
+    samples the enumerated conformations of a bare six-domain scaffold at
+    330 K: the Rosenbluth-like weights (attempts left, open probabilities,
+    feelers) and the Metropolis ratio of new to old weight."""
+    from latticednaorigami_amd import abi
+    s = _scaffold_only(6, misbinding_h)
+    s.moves = [systems.Move(abi.MOVE_ORIENTATION_ROTATION, 0.5),
+               systems.Move(abi.MOVE_CTRG_SCAFFOLD_REGROWTH, 0.5, max_regrowth, 1, attempts)]
+    o = oracle_mod.Oracle(s, temps=[330.0] * 8)
+    exact, _, _ = o.enumerate()
+    counts = _mc_counts(o, 8, n_samples, stride=20, burn=5000)
+    assert o.counters().accepts[1] > 0.3 * o.counters().attempts[1]
+    for key, p in exact.items():
+        if p < 0.01:
+            continue
+        m, se = _mean_se(counts, key, n_samples)
+        assert abs(m - p) <= 4 * se + 0.002, (key, m, p, se)
+
+
+def test_ctrg_jump_matches_enumeration_with_hairpin_staples(oracle_mod):
+    """Non-contiguous CTRG (segment pairs grown from the stems that the
+    crossover staples of a four-domain hairpin offer) with CB staple regrowth
+    and orientation rotations samples the enumerated ensemble restricted to
+    two bound staples at 345 K.  Without staple exchange the staple count
+    stays 2, so the target is the exact distribution conditioned on it."""
+    from latticednaorigami_amd import abi
+    s = systems.hairpin_system(4, seed=7)
+    s.moves = [systems.Move(abi.MOVE_ORIENTATION_ROTATION, 0.2),
+               systems.Move(abi.MOVE_MET_STAPLE_EXCHANGE, 0.6),
+               systems.Move(abi.MOVE_CB_STAPLE_REGROWTH, 0.2)]
+    grow = oracle_mod.Oracle(s, temps=[345.0])
+    while grow.observables(0)["num_staples"] != 2:
+        grow.sweep(1000)
+    s.chains = grow.config(0)
+    s.moves = [systems.Move(abi.MOVE_ORIENTATION_ROTATION, 0.3),
+               systems.Move(abi.MOVE_CB_STAPLE_REGROWTH, 0.2),
+               systems.Move(abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH, 0.5, 4, 1, 36, 2)]
+    o = oracle_mod.Oracle(s, temps=[345.0] * 8)
+    exact, _, _ = o.enumerate()
+    two = {k: v for k, v in exact.items() if k[0] == 2}
+    z = sum(two.values())
+    counts = _mc_counts(o, 8, 1000, stride=20, burn=2000)
+    assert o.counters().accepts[2] > 0.3 * o.counters().attempts[2]
+    for key, p in two.items():
+        if p / z < 0.01:
+            continue
+        m, se = _mean_se(counts, key, 1000)
+        assert abs(m - p / z) <= 4 * se + 0.01, (key, m, p / z, se)
+
+
 def _learned_system():
     """A staple [1, 5] whose domain 5 has no complement (misbinding
     disallowed): bound through domain 1, its end domain 5 always dangles, so
