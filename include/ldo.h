@@ -56,8 +56,9 @@ enum ldo_movetype {
     LDO_MOVE_CB_STAPLE_REGROWTH = 2,    /* cb_movetypes.cpp:292-348 */
     LDO_MOVE_CTCB_SCAFFOLD_REGROWTH = 3, /* cb_movetypes.cpp:680-747 */
     LDO_MOVE_CTRG_SCAFFOLD_REGROWTH = 4, /* rg_movetypes.cpp:177-533, 629-689 */
-    LDO_MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH = 5 /* rg_movetypes.cpp:791-853,
-                                              movetypes.cpp:512-721 */
+    LDO_MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH = 5, /* rg_movetypes.cpp:791-853,
+                                               movetypes.cpp:512-721 */
+    LDO_MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH = 6  /* cb_movetypes.cpp:872-983 */
 };
 
 /* Random streams.  PHILOX: a counter-based Philox4x32-10 stream keyed by
@@ -145,8 +146,8 @@ typedef struct ldo_run_params {
      * both must be given when the moveset holds one */
     const int32_t* movetype_max_num_recoils;
     const int32_t* movetype_max_c_attempts;
-    /* CTRGJump max_seg_regrowth (>= 1; jump max_regrowth <= 60), read for
-     * those entries only */
+    /* max_seg_regrowth of the non-contiguous moves (CTRGJump, CTCBJump;
+     * >= 1, with their max_regrowth <= 60), read for those entries only */
     const int32_t* movetype_max_seg_regrowth;
 } ldo_run_params;
 
@@ -359,8 +360,10 @@ int ldo_restart_swaps(void* ctx);
  *   CTRGScaffoldRegrowth attempts[ns+1], accepts[ns+1] by segment length
  *                        (rg_movetypes.cpp:583-627),
  *   CTRGJumpScaffoldRegrowth attempts, accepts (the reference tracks them
- *                        under 0 scaffold domains, rg_movetypes.cpp:745-789);
- * 6T + 4(ns+1) + 2(S+1) + 2 values in all, written to *n_out (cap must hold
+ *                        under 0 scaffold domains, rg_movetypes.cpp:745-789),
+ *   CTCBJumpScaffoldRegrowth attempts[S+1], accepts[S+1] by staples regrown
+ *                        (all under 0 scaffold domains, cb_movetypes.cpp:816-870);
+ * 6T + 4(ns+1) + 4(S+1) + 2 values in all, written to *n_out (cap must hold
  * them).  Counts are per move kind, summed over moveset slots of that kind. */
 int ldo_get_move_tracking(void* ctx, int32_t replica, int64_t* out, int64_t cap,
                           int64_t* n_out);

@@ -28,6 +28,7 @@ MOVE_CB_STAPLE_REGROWTH = 2
 MOVE_CTCB_SCAFFOLD_REGROWTH = 3
 MOVE_CTRG_SCAFFOLD_REGROWTH = 4
 MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH = 5
+MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH = 6
 RNG_PHILOX, RNG_TAPE = 0, 1
 FLAG_CTCB_SCAFFOLD_ENDPOINTS = 1
 EXCHANGE_NONE, EXCHANGE_TEMPERATURE, EXCHANGE_WINDOWS = 0, 1, 2
@@ -41,6 +42,7 @@ MOVETYPE_NAMES = {
     "CTCBScaffoldRegrowth": MOVE_CTCB_SCAFFOLD_REGROWTH,
     "CTRGScaffoldRegrowth": MOVE_CTRG_SCAFFOLD_REGROWTH,
     "CTRGJumpScaffoldRegrowth": MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH,
+    "CTCBJumpScaffoldRegrowth": MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH,
 }
 
 _i32p = C.POINTER(C.c_int32)
@@ -169,7 +171,7 @@ class LdoError(RuntimeError):
 
 def tracking_len(n_types, n_scaffold, max_staples):
     """Length of the ldo_get_move_tracking array."""
-    return 6 * n_types + 4 * (n_scaffold + 1) + 2 * (max_staples + 1) + 2
+    return 6 * n_types + 4 * (n_scaffold + 1) + 4 * (max_staples + 1) + 2
 
 
 def split_tracking(values, n_types, n_scaffold, max_staples):
@@ -183,7 +185,8 @@ def split_tracking(values, n_types, n_scaffold, max_staples):
                     ("ctcb_length_attempts", n1), ("ctcb_length_accepts", n1),
                     ("ctcb_staples_attempts", s1), ("ctcb_staples_accepts", s1),
                     ("ctrg_length_attempts", n1), ("ctrg_length_accepts", n1),
-                    ("ctrg_jump_attempts", 1), ("ctrg_jump_accepts", 1)):
+                    ("ctrg_jump_attempts", 1), ("ctrg_jump_accepts", 1),
+                    ("ctcb_jump_staples_attempts", s1), ("ctcb_jump_staples_accepts", s1)):
         out[name] = v[o:o + n]
         o += n
     assert o == len(v), (o, len(v))

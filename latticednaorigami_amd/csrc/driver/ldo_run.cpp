@@ -13,8 +13,8 @@
 // -inp.biases / .biases weights and the .out iteration summary) with the
 // movetypes OrientationRotation,
 // MetStapleExchange, CBStapleRegrowth, CTCBScaffoldRegrowth,
-// CTRGScaffoldRegrowth and CTRGJumpScaffoldRegrowth (the movetypes of the
-// reference's examples/moveset_standard.json).
+// CTRGScaffoldRegrowth, CTRGJumpScaffoldRegrowth (the movetypes of the
+// reference's examples/moveset_standard.json) and CTCBJumpScaffoldRegrowth.
 //
 //   ldo_run <parameter file> [key=value ...]
 //
@@ -463,7 +463,7 @@ void write_log_entry(Run& run, std::ostream& log, int r, long long step) {
 // GCMCSimulation::write_log_summary (simulation.cpp:706-718) of a one-replica
 // run: each movetype's header (movetypes.cpp:88-96) to the log, and the
 // movetype summaries (met_movetypes.cpp:145-190, cb_movetypes.cpp:264-290,
-// 626-678, rg_movetypes.cpp:583-627, 745-789; none for orientation
+// 626-678, 816-870, rg_movetypes.cpp:583-627, 745-789; none for orientation
 // rotation) to the .moves file.
 void write_log_summary(Run& run, std::ostream& log, const std::string& moves_path) {
     ldo_counters c {};
@@ -471,7 +471,7 @@ void write_log_summary(Run& run, std::ostream& log, const std::string& moves_pat
     int T = (int)run.identities.size();
     int ns = run.n_scaffold;
     int S1 = run.max_total_staples + 1;
-    std::vector<int64_t> tr((size_t)(6 * T + 4 * (ns + 1) + 2 * S1 + 2));
+    std::vector<int64_t> tr((size_t)(6 * T + 4 * (ns + 1) + 4 * S1 + 2));
     int64_t n_tr = 0;
     run.check(ldo_get_move_tracking(run.ctx, 0, tr.data(), (int64_t)tr.size(), &n_tr));
     auto at = [&](int part, int i) { return tr[(size_t)part * T + i]; };
@@ -573,6 +573,35 @@ void write_log_summary(Run& run, std::ostream& log, const std::string& moves_pat
                       << "\n";
             }
             moves << "\n";
+        }
+        else if (kind == LDO_MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH) {
+            // CTCBJumpScaffoldRegrowthMCMovetype::write_log_summary
+            // (cb_movetypes.cpp:816-870): every move under 0 scaffold domains
+            header(moves, m);
+            const int64_t* st_att = tr.data() + 6 * T + 4 * (ns + 1) + 2 * S1 + 2;
+            const int64_t* st_acc = st_att + S1;
+            int64_t att = 0, acc = 0;
+            for (int s = 0; s < S1; s++) {
+                att += st_att[s];
+                acc += st_acc[s];
+            }
+            if (att != 0) {
+                moves << "    Number of scaffold domains: " << 0 << "\n";
+                moves << "        Attempts: " << (int)att << "\n";
+                moves << "        Accepts: " << (int)acc << "\n";
+                moves << "        Frequency: " << static_cast<double>(acc) / att << "\n";
+            }
+            moves << "\n";
+            for (int s = 0; s < S1; s++) {
+                if (st_att[s] == 0) {
+                    continue;
+                }
+                moves << "    Number of staples: " << s << "\n";
+                moves << "        Attempts: " << (int)st_att[s] << "\n";
+                moves << "        Accepts: " << (int)st_acc[s] << "\n";
+                moves << "        Frequency: " << static_cast<double>(st_acc[s]) / st_att[s]
+                      << "\n";
+            }
         }
         else if (kind == LDO_MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH) {
             // CTRGJumpScaffoldRegrowthMCMovetype::write_log_summary
@@ -1208,7 +1237,7 @@ int main_run(int argc, char** argv) {
         kinds.push_back(enum_of(m.at("type").str,
                                 {"OrientationRotation", "MetStapleExchange", "CBStapleRegrowth",
                                  "CTCBScaffoldRegrowth", "CTRGScaffoldRegrowth",
-                                 "CTRGJumpScaffoldRegrowth"},
+                                 "CTRGJumpScaffoldRegrowth", "CTCBJumpScaffoldRegrowth"},
                                 "movetype"));
         run.move_labels.push_back(m.has("label") ? m.at("label").str : m.at("type").str);
         freqs.push_back(fraction(m.at("freq")));

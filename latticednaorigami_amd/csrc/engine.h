@@ -695,7 +695,7 @@ struct Eng {
     DEVNI void cp_scan_network(int d);
     DEVNI void cp_find(int first, int count, int dir, int sg);
     DEVIN bool ct_select_indices(int max_regrowth, int& first, int& count, int& dir);
-    DEVNI void ctcb_grow_chain_scaffold(int first, int count, int dir);
+    DEVNI void ctcb_grow_chain_scaffold(int first, int count, int dir, bool first_gp = true);
     DEVNI void ctcb_grow_staple_from(int gold);
     DEVNI bool ctcb_scaffold_move(int max_regrowth);
     DEVIN int rg_get_dir(int s);
@@ -719,6 +719,9 @@ struct Eng {
                          int& last);
     DEVNI bool jump_select(int max_regrowth);
     DEVNI bool ctrg_jump_move(int max_regrowth);
+    DEVIN void ctcb_set_first_seg_domain(int stemd);
+    DEVNI void ctcb_jump_grow();
+    DEVNI bool ctcb_jump_move(int max_regrowth);
     DEVIN bool attempt(int m);
     DEVNI bool attempt_kind(int kind, int max_regrowth, int m = -1);
     DEVIN void track_move(int m, bool acc);

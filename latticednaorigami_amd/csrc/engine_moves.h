@@ -892,7 +892,8 @@ DEVIN bool Eng::cp_selected(int b) {
 DEVNI void Eng::cp_find(int first, int count, int dir, int sg) {
     // CTRG grows Constraintpoints::m_d_stack (domains_to_be_regrown): each
     // segment domain, then the domain stack of the network it roots
-    bool rec = S->kind == LDO_MOVE_KIND_CTRG || S->kind == LDO_MOVE_KIND_CTRG_JUMP;
+    bool rec = S->kind == LDO_MOVE_KIND_CTRG || S->kind == LDO_MOVE_KIND_CTRG_JUMP ||
+               S->n_jsegs > 0;
     if (S->n_jsegs == 0) {
         dir0[0] = (int8_t)dir;
     }
@@ -984,7 +985,7 @@ DEVNI void Eng::cp_save_initial() {
 // grow_staple_and_update_endpoints (:541-562), interleaved through an
 // explicit stack.  Growing starts with the scaffold segment and descends
 // into staples at growthpoints; any rejection unwinds the whole growth.
-DEVNI void Eng::ctcb_grow_chain_scaffold(int first, int count, int dir) {
+DEVNI void Eng::ctcb_grow_chain_scaffold(int first, int count, int dir, bool first_gp) {
     GrowFrame* stk = gstack;
     int sp = 0;
     if (count > 1) {
@@ -998,7 +999,7 @@ DEVNI void Eng::ctcb_grow_chain_scaffold(int first, int count, int dir) {
         f.phase = 0;
         stk[sp++] = f;
     }
-    if (gp[first] >= 0) {
+    if (first_gp && gp[first] >= 0) {
         GrowFrame f;
         f.gold = (int16_t)first;
         f.kind = 1;
@@ -1876,6 +1877,97 @@ DEVNI bool Eng::ctrg_jump_move(int max_regrowth) {
     return rg_regrow_and_accept(true, true, true);
 }
 
+// CTCBJumpScaffoldRegrowthMCMovetype::set_first_seg_domain
+// (cb_movetypes.cpp:973-983): a stem binds its growthpoint as in growth
+// from a staple, in the new and the old growth alike
+DEVIN void Eng::ctcb_set_first_seg_domain(int stemd) {
+    int d_old = stem[stemd];
+    if (!cp_walks_remain(stemd, pos(d_old))) {
+        S->rejected = 1;
+        return;
+    }
+    double de = set_growth_point(stemd, d_old);
+    S->bias *= exp(-de);
+}
+
+// the growth of both passes of CTCBJumpScaffoldRegrowthMCMovetype
+// (cb_movetypes.cpp:896-921, 935-960): the first segment with the staple
+// grown from its first domain, then per stem the stem and each of its two
+// segments grown out of it
+DEVNI void Eng::ctcb_jump_grow() {
+    ctcb_grow_chain_scaffold(jseg[0].first, jseg[0].count, jseg[0].dir);
+    for (int t = 0; t < S->n_jstems && !S->rejected && S->status == ST_OK; t++) {
+        int stemd = jstem[t];
+        int k = 1 + 2 * t;
+        ctcb_set_first_seg_domain(stemd);
+        for (int j = 0; j < 2 && !S->rejected && S->status == ST_OK; j++) {
+            int count = j == 0 ? jseg[k].count : 1 + jseg[k + 1].count;
+            ctcb_grow_chain_scaffold(stemd, count, jseg[k + j].dir, false);
+        }
+    }
+}
+
+// CTCBJumpScaffoldRegrowthMCMovetype::internal_attempt_move
+// (cb_movetypes.cpp:872-967); unlike the contiguous move, a dead end in the
+// growth of the old configuration rejects too
+DEVNI bool Eng::ctcb_jump_move(int max_regrowth) {
+    S->regrow_old = 0;
+    cp_reset();
+    if (!jump_select(max_regrowth)) {
+        return false;
+    }
+    S->trk_len = 0;
+    for (int i = 0; i < S->n_jsegs; i++) {
+        if (jseg[i].count > 0) {
+            cp_find(jseg[i].first, jseg[i].count, jseg[i].dir, i);
+        }
+    }
+    cp_save_initial();
+    if (S->status != ST_OK) {
+        return false;
+    }
+    int first = jseg[0].first;
+    cp_remove_active(first);
+    int nst = 0;
+    for (int c = 1; c < nchain; c++) {
+        if (regrow[c] && ch[c].ident >= 0) {
+            nst++;
+        }
+    }
+    S->trk_nst = nst;
+    int n = S->n_rgs;
+    for (int phase = 0; phase < 2; phase++) {
+        if (phase == 1) {
+            cb_setup_regrow_old();
+            for (int i = plane(), m = S->n_active0; i < m; i += pwidth()) {
+                active[i] = active0[i];
+            }
+            S->n_active = S->n_active0;
+            cp_remove_active(first);
+        }
+        // CBMCMovetype::unassign_domains (cb_movetypes.cpp:200-213) over the
+        // regrowth stack past its first domain
+        for (int i = 1; i < n; i++) {
+            int d = rgs[i];
+            prev[d].x = dom[d].x;
+            prev[d].y = dom[d].y;
+            prev[d].z = dom[d].z;
+            prev[d].ore = dom[d].ore;
+            push_modified(d);
+            unassign_domain(d);
+        }
+        ctcb_jump_grow();
+        if (S->rejected || S->status != ST_OK) {
+            return false;
+        }
+        if (phase == 0) {
+            S->bias *= exp(-calc_move_bias(0)); // add_external_bias (cb_movetypes.cpp:928)
+        }
+    }
+    S->modifier = 1;
+    return cb_test_acceptance();
+}
+
 // MCMovetype::attempt_move and the reset_internal chain (movetypes.cpp:41-51,
 // 242-250; met_movetypes.cpp:45-48, 139-143; cb_movetypes.cpp:44-50, 457-461)
 DEVIN bool Eng::attempt(int m) {
@@ -1919,6 +2011,11 @@ DEVNI bool Eng::attempt_kind(int kind, int max_regrowth, int m) {
         rg_max_att = m >= 0 ? P->move_cattempts[m] : 36;
         rg_max_segreg = m >= 0 ? P->move_segreg[m] : 0;
         return ctrg_jump_move(max_regrowth);
+    case 6:
+        // a CTCB move throughout (grow_chain, calc_bias), on a jump selection
+        S->kind = LDO_MOVE_KIND_CTCB;
+        rg_max_segreg = m >= 0 ? P->move_segreg[m] : 0;
+        return ctcb_jump_move(max_regrowth);
     }
     fail(ST_MISUSE);
     return false;
@@ -1963,6 +2060,12 @@ DEVIN void Eng::track_move(int m, bool acc) {
         int o = 6 * T + 4 * (ns + 1) + 2 * (P->L.smax + 1);
         track[o] += 1;
         track[o + 1] += a;
+    }
+    else if (kind == 6) {
+        int s1 = P->L.smax + 1;
+        int o = 6 * T + 4 * (ns + 1) + 2 * s1 + 2;
+        track[o + S->trk_nst] += 1;
+        track[o + s1 + S->trk_nst] += a;
     }
 }
 

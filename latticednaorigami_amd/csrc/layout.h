@@ -189,7 +189,8 @@ struct Layout {
     // accepts[T], deletion attempts[T], accepts[T]; CB attempts[T],
     // accepts[T]; CTCB by segment length attempts[ns+1], accepts[ns+1]; by
     // staples regrown attempts[smax+1], accepts[smax+1]; CTRG by segment
-    // length attempts[ns+1], accepts[ns+1]; CTRG jump attempts, accepts
+    // length attempts[ns+1], accepts[ns+1]; CTRG jump attempts, accepts;
+    // CTCB jump by staples regrown attempts[smax+1], accepts[smax+1]
     int64_t off_track;
     int64_t bytes; // block size (multiple of 256)
 };

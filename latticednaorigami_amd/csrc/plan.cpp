@@ -122,7 +122,7 @@ Layout make_layout(int ns, int lmax, int smax, int ntypes, int tcap, bool rg, bo
     take(L.off_jstem, J * LDO_JUMP_MAX_SEGS);
     take(L.off_stemseg, J * ns);
     take(L.off_sdir, J * LDO_JUMP_MAX_SEGS);
-    L.track_len = 6 * ntypes + 4 * (ns + 1) + 2 * (smax + 1) + 2;
+    L.track_len = 6 * ntypes + 4 * (ns + 1) + 4 * (smax + 1) + 2;
     take(L.off_track, (int64_t)L.track_len * 8);
     L.bytes = round_up(off, 256);
     return L;
@@ -214,8 +214,10 @@ void make_plan(const ldo_system_desc* sys, const ldo_run_params* run, int32_t n_
     bool rg = false, jump = false;
     for (int m = 0; m < run->n_movetypes; m++) {
         int k = run->movetype_kind[m];
-        jump = jump || k == LDO_MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH;
-        rg = rg || k == LDO_MOVE_CTRG_SCAFFOLD_REGROWTH || k == LDO_MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH;
+        bool j = k == LDO_MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH || k == LDO_MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH;
+        jump = jump || j;
+        // the jump moves keep their regrowth stack in the CTRG sections
+        rg = rg || j || k == LDO_MOVE_CTRG_SCAFFOLD_REGROWTH;
     }
     P.L = make_layout(ns, lmax, smax, sys->n_chain_types, tcap, rg, jump);
     P.n_replicas = n_replicas;
@@ -226,7 +228,7 @@ void make_plan(const ldo_system_desc* sys, const ldo_run_params* run, int32_t n_
     double cum = 0;
     for (int m = 0; m < run->n_movetypes; m++) {
         int kind = run->movetype_kind[m];
-        if (kind < 0 || kind > 5) {
+        if (kind < 0 || kind > 6) {
             throw AbiError {LDO_E_NOTIMPL, "movetype not supported"};
         }
         P.move_kind[m] = kind;
@@ -239,14 +241,14 @@ void make_plan(const ldo_system_desc* sys, const ldo_run_params* run, int32_t n_
         if (kind >= 3 && P.move_maxreg[m] < 2) {
             throw AbiError {LDO_E_ARG, "scaffold regrowth max_regrowth must be at least 2"};
         }
-        if (kind == 5) {
+        if (kind == 5 || kind == 6) {
             // the first segment's length is drawn from [2, max_seg_regrowth + 1]
             // (movetypes.cpp:522-523)
             if (!run->movetype_max_seg_regrowth || run->movetype_max_seg_regrowth[m] < 1) {
-                throw AbiError {LDO_E_ARG, "CTRG jump needs max_seg_regrowth >= 1"};
+                throw AbiError {LDO_E_ARG, "a jump move needs max_seg_regrowth >= 1"};
             }
             if (P.move_maxreg[m] > LDO_JUMP_MAX_REGROWTH) {
-                throw AbiError {LDO_E_ARG, "CTRG jump max_regrowth must be at most 60"};
+                throw AbiError {LDO_E_ARG, "a jump move's max_regrowth must be at most 60"};
             }
             P.move_segreg[m] = run->movetype_max_seg_regrowth[m];
         }

@@ -123,6 +123,14 @@ void* oracle_create(const ldo_system_desc* sd, const ldo_run_params* run,
                     e.max_seg_regrowth = run->movetype_max_seg_regrowth[m];
                 }
             }
+            if (e.kind == LDO_MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH) {
+                // setup_scaffold_regrowth_movetype (simulation.cpp:403-438)
+                if (!run->movetype_max_seg_regrowth || run->movetype_max_seg_regrowth[m] < 1 ||
+                    e.max_regrowth < 2) {
+                    throw OrigamiError(LDO_E_ARG, "CTCB jump parameters out of range");
+                }
+                e.max_seg_regrowth = run->movetype_max_seg_regrowth[m];
+            }
             moves.push_back(e);
         }
         if (run->flags & ~LDO_FLAG_CTCB_SCAFFOLD_ENDPOINTS) {
@@ -532,7 +540,7 @@ int oracle_get_move_tracking(void* o, int32_t r, int64_t* out, int64_t cap, int6
         Replica& rep = *O->reps.at(r);
         int64_t T = (int64_t)rep.sys.identities.size();
         int64_t ns = (int64_t)rep.sys.identities[0].size();
-        int64_t n = 6 * T + 4 * (ns + 1) + 2 * ((int64_t)O->lim.max_total_staples + 1) + 2;
+        int64_t n = 6 * T + 4 * (ns + 1) + 4 * ((int64_t)O->lim.max_total_staples + 1) + 2;
         if (cap < n) {
             throw OrigamiError(LDO_E_ARG, "tracking capacity too small");
         }

@@ -538,6 +538,11 @@ class Replica {
                   std::vector<Dom*>& seg);
     void check_for_stemds(Dom* cur_d, std::deque<Dom*>& possible_stems);
     bool ctrg_jump_move(int max_regrowth);
+    void ctcb_set_first_seg_domain(Dom* d_new);
+    void ctcb_jump_grow(const std::vector<std::vector<Dom*>>& segs,
+                        const std::vector<std::vector<std::vector<Dom*>>>& paired_segs,
+                        const std::vector<Dom*>& seg_stems);
+    bool ctcb_jump_move(int max_regrowth);
     double rg_unassign_and_save();
     void rg_unassign_domains();
     double rg_recoil_regrow();

@@ -345,7 +345,7 @@ void Replica::add_tracker(int slot, bool accepted) {
     int ns = (int)sys.identities[0].size();
     int a = accepted ? 1 : 0;
     if (track.empty()) {
-        track.assign(6 * T + 4 * (ns + 1) + 2 * (m_lim.max_total_staples + 1) + 2, 0);
+        track.assign(6 * T + 4 * (ns + 1) + 4 * (m_lim.max_total_staples + 1) + 2, 0);
     }
     switch (m_moves[slot].kind) {
     case LDO_MOVE_MET_STAPLE_EXCHANGE: {
@@ -379,6 +379,14 @@ void Replica::add_tracker(int slot, bool accepted) {
         int o = 6 * T + 4 * (ns + 1) + 2 * (m_lim.max_total_staples + 1);
         track[o] += 1;
         track[o + 1] += a;
+        break;
+    }
+    case LDO_MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH: {
+        // cb_movetypes.cpp:816-870: all under 0 scaffold domains, by staples
+        int s1 = m_lim.max_total_staples + 1;
+        int o = 6 * T + 4 * (ns + 1) + 2 * s1 + 2;
+        track[o + tracker.num_staples] += 1;
+        track[o + s1 + tracker.num_staples] += a;
         break;
     }
     default:
@@ -486,6 +494,9 @@ bool Replica::attempt(int kind, int max_regrowth) {
         m_weight = 1;
         m_weight_new = 1;
         return ctrg_jump_move(max_regrowth);
+    case LDO_MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH:
+        m_cp.reset();
+        return ctcb_jump_move(max_regrowth);
     }
     throw OrigamiError(LDO_E_NOTIMPL, "movetype not supported");
 }
@@ -646,6 +657,7 @@ void Replica::grow_chain(const std::vector<Dom*>& ds) {
         }
         break;
     case LDO_MOVE_CTCB_SCAFFOLD_REGROWTH:
+    case LDO_MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH:
         ctcb_grow_chain(ds);
         break;
     }
@@ -1715,6 +1727,83 @@ bool Replica::ctrg_jump_move(int max_regrowth) {
     m_cp.remove_active_endpoint(segs[0][0]);
     rg_calc_weights();
     return rg_test_acceptance();
+}
+
+// CTCBJumpScaffoldRegrowthMCMovetype::set_first_seg_domain
+// (cb_movetypes.cpp:973-983): a stem binds its growthpoint as in growth
+// from a staple, in the new and the old growth alike
+void Replica::ctcb_set_first_seg_domain(Dom* d_new) {
+    Dom* d_old = m_cp.get_growthpoint(d_new);
+    if (!m_cp.walks_remain(d_new, d_old->pos)) {
+        m_rejected = true;
+        return;
+    }
+    m_dir = m_cp.get_dir(d_new);
+    double de = set_growth_point(*d_new, *d_old);
+    m_bias *= std::exp(-de);
+}
+
+// the growth of both passes of CTCBJumpScaffoldRegrowthMCMovetype
+// (cb_movetypes.cpp:896-921, 935-960)
+void Replica::ctcb_jump_grow(const std::vector<std::vector<Dom*>>& segs,
+                             const std::vector<std::vector<std::vector<Dom*>>>& paired_segs,
+                             const std::vector<Dom*>& seg_stems) {
+    if (m_cp.is_growthpoint(segs[0][0])) {
+        ctcb_grow_staple_and_update_endpoints(segs[0][0]);
+        if (m_rejected) {
+            return;
+        }
+    }
+    ctcb_grow_chain(segs[0]);
+    if (m_rejected) {
+        return;
+    }
+    for (size_t i = 0; i < seg_stems.size(); i++) {
+        ctcb_set_first_seg_domain(seg_stems[i]);
+        if (m_rejected) {
+            return;
+        }
+        for (auto sg: paired_segs[i]) {
+            sg.insert(sg.begin(), seg_stems[i]);
+            ctcb_grow_chain(sg);
+            if (m_rejected) {
+                return;
+            }
+        }
+    }
+}
+
+// CTCBJumpScaffoldRegrowthMCMovetype::internal_attempt_move
+// (cb_movetypes.cpp:872-967)
+bool Replica::ctcb_jump_move(int max_regrowth) {
+    m_regrow_old = false;
+    std::vector<std::vector<Dom*>> segs;
+    std::vector<std::vector<std::vector<Dom*>>> paired_segs;
+    std::vector<Dom*> seg_stems;
+    std::vector<int> dirs;
+    select_noncontig_segs(segs, paired_segs, seg_stems, dirs, max_regrowth);
+    tracker.num_scaffold_domains = 0;
+    m_cp.calculate(segs, dirs);
+    m_cp.remove_active_endpoint(segs[0][0]);
+    tracker.num_staples = (int)m_cp.staples_to_be_regrown().size();
+    std::vector<Dom*> regrow = m_cp.domains_to_be_regrown();
+    std::vector<Dom*> to_unassign(regrow.begin() + 1, regrow.end());
+    cb_unassign_domains(to_unassign);
+    ctcb_jump_grow(segs, paired_segs, seg_stems);
+    if (m_rejected) {
+        return false;
+    }
+    cb_add_external_bias();
+    cb_setup_regrow_old();
+    m_cp.reset_active_endpoints();
+    m_cp.remove_active_endpoint(segs[0][0]);
+    cb_unassign_domains(to_unassign);
+    ctcb_jump_grow(segs, paired_segs, seg_stems);
+    if (m_rejected) {
+        return false;
+    }
+    m_modifier = 1;
+    return cb_test_acceptance();
 }
 
 // ------------------------------------------------------ move-level biases

@@ -742,3 +742,22 @@ def case_ctrg_jump_limits(sampler_cls, oracle_mod):
         s = systems.hairpin_system(6)
         s.moves = systems.jump_moveset(6, rec, att, sreg)
         _ctrg_chain(sampler_cls, oracle_mod, s, [330.0, 340.0], 2, 6000)
+
+
+def case_ctcb_jump_bit_exact(sampler_cls, oracle_mod):
+    """Non-contiguous CTCB (cb_movetypes.cpp:872-983): the jump selection
+    grown by configurational bias, stems bound by set_first_seg_domain, on
+    the 4- and 6-domain hairpins and the 64-domain raster, with the
+    reference's limits and with (max_seg_regrowth 1, 4)."""
+    say("case: CTCB jump, hairpins of 4 and 6 domains at 330/345 K and the 64-domain raster "
+        "at 300 K, 3 x 5000 steps")
+    for n, sreg in ((4, 2), (6, 1), (6, 4)):
+        s = systems.hairpin_system(n)
+        s.moves = systems.ctrg_moveset(n)[:3] + [
+            systems.Move(abi.MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH, 2 / 6, n, 0, 0, sreg)]
+        g = _ctrg_chain(sampler_cls, oracle_mod, s, [330.0, 345.0], 3, 5000)
+        assert sum(g.move_tracking(0)["ctcb_jump_staples_attempts"]) > 1000
+    s = systems.raster_system(replicas=1)
+    s.moves = systems.ctrg_moveset(12)[:3] + [
+        systems.Move(abi.MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH, 2 / 6, 12, 0, 0, 2)]
+    _ctrg_chain(sampler_cls, oracle_mod, s, [300.0], 3, 5000)

@@ -18,7 +18,8 @@ MOVE_TYPES = {abi.MOVE_ORIENTATION_ROTATION: "OrientationRotation",
               abi.MOVE_CB_STAPLE_REGROWTH: "CBStapleRegrowth",
               abi.MOVE_CTCB_SCAFFOLD_REGROWTH: "CTCBScaffoldRegrowth",
               abi.MOVE_CTRG_SCAFFOLD_REGROWTH: "CTRGScaffoldRegrowth",
-              abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH: "CTRGJumpScaffoldRegrowth"}
+              abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH: "CTRGJumpScaffoldRegrowth",
+              abi.MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH: "CTCBJumpScaffoldRegrowth"}
 HYB = {abi.HYB_NEARESTNEIGHBOUR: "NearestNeighbour", abi.HYB_UNIFORM: "Uniform"}
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # the reference's examples/moveset_standard.json
@@ -47,7 +48,7 @@ def write_inputs(d, s, extra):
         if m.kind in (abi.MOVE_CTRG_SCAFFOLD_REGROWTH, abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH):
             e["max_num_recoils"] = m.max_num_recoils
             e["max_c_attempts"] = m.max_c_attempts
-        if m.kind == abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH:
+        if m.kind in (abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH, abi.MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH):
             e["max_seg_regrowth"] = m.max_seg_regrowth
         return e
     moves = {"origami": {"movetypes": [entry(m) for m in s.moves]}}
@@ -229,6 +230,19 @@ def expect_summary(ref, s):
                     moves.append(f"    Number of scaffold domains: {n}\n        Attempts: {a}\n"
                                  f"        Accepts: {k}\n        Frequency: {ratio(k, a)}\n")
             moves.append("\n")
+        elif mv.kind == abi.MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH:
+            # cb_movetypes.cpp:816-870, every move under 0 scaffold domains
+            moves.append(head)
+            att, acc = tr["ctcb_jump_staples_attempts"], tr["ctcb_jump_staples_accepts"]
+            if sum(att):
+                moves.append(f"    Number of scaffold domains: 0\n        Attempts: {sum(att)}\n"
+                             f"        Accepts: {sum(acc)}\n"
+                             f"        Frequency: {ratio(sum(acc), sum(att))}\n")
+            moves.append("\n")
+            for n, (a, k) in enumerate(zip(att, acc)):
+                if a:
+                    moves.append(f"    Number of staples: {n}\n        Attempts: {a}\n"
+                                 f"        Accepts: {k}\n        Frequency: {ratio(k, a)}\n")
         elif mv.kind == abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH:
             # rg_movetypes.cpp:745-789, every move under 0 scaffold domains
             moves.append(head)

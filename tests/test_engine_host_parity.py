@@ -162,3 +162,7 @@ def test_ctrg_jump_bit_exact(sampler_cls, oracle_mod):
 
 def test_ctrg_jump_limits(sampler_cls, oracle_mod):
     pc.case_ctrg_jump_limits(sampler_cls, oracle_mod)
+
+
+def test_ctcb_jump_bit_exact(sampler_cls, oracle_mod):
+    pc.case_ctcb_jump_bit_exact(sampler_cls, oracle_mod)

@@ -371,6 +371,28 @@ def test_ctrg_jump_matches_enumeration_with_hairpin_staples(oracle_mod):
         assert abs(m - p / z) <= 4 * se + 0.01, (key, m, p / z, se)
 
 
+@pytest.mark.parametrize("kind", ["ctcb_jump", "ctrg_jump"])
+def test_jump_scaffold_regrowth_matches_enumeration(oracle_mod, kind):
+    """The non-contiguous selections (select_noncontig_segs) grown by CTCB
+    and by CTRG sample the enumerated conformations of the bare six-domain
+    scaffold at 330 K (no staples, so every selection is one segment)."""
+    from latticednaorigami_amd import abi
+    k = {"ctcb_jump": abi.MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH,
+         "ctrg_jump": abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH}[kind]
+    s = _scaffold_only(6)
+    s.moves = [systems.Move(abi.MOVE_ORIENTATION_ROTATION, 0.5),
+               systems.Move(k, 0.5, 6, 1, 36, 2)]
+    o = oracle_mod.Oracle(s, temps=[330.0] * 8)
+    exact, _, _ = o.enumerate()
+    counts = _mc_counts(o, 8, 3000, stride=20, burn=5000)
+    assert o.counters().accepts[1] > 0.3 * o.counters().attempts[1]
+    for key, p in exact.items():
+        if p < 0.01:
+            continue
+        m, se = _mean_se(counts, key, 3000)
+        assert abs(m - p) <= 4 * se + 0.002, (key, m, p, se)
+
+
 def _learned_system():
     """A staple [1, 5] whose domain 5 has no complement (misbinding
     disallowed): bound through domain 1, its end domain 5 always dangles, so
