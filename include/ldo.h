@@ -413,17 +413,22 @@ void ldo_batch_destroy(void* batch);
 const char* ldo_batch_last_error(void* batch);
 
 /* Sweep kernels a batch can use: one chain per workgroup with its block in
- * LDS, one chain per wavefront with its block in device memory, or one chain
- * per lane (64 chains per wavefront). */
+ * LDS, one chain per wavefront with its block in device memory (all 64 lanes
+ * run the chain, splitting its independent lookups), one chain per lane (64
+ * chains per wavefront), or one chain per wavefront run by its first lane
+ * alone (fewer memory-pipeline issues per step when thousands of wavefronts
+ * share the chip). */
 enum ldo_batch_kernel {
     LDO_BATCH_LDS = 0,
     LDO_BATCH_WAVE = 1,
-    LDO_BATCH_LANE = 2
+    LDO_BATCH_LANE = 2,
+    LDO_BATCH_WAVE_SOLO = 3
 };
 
 /* Times every available sweep kernel on `steps_per_trial` steps of every
  * chain (real steps: the chains advance), keeps the fastest and returns it
- * with the milliseconds of each trial (ms[k] < 0: kernel k unavailable). */
+ * with the milliseconds of each trial in ms[0 .. LDO_BATCH_WAVE_SOLO]
+ * (ms[k] < 0: kernel k unavailable; ms may be NULL). */
 int ldo_batch_tune(void* batch, int64_t steps_per_trial, int32_t* chosen, double* ms);
 int ldo_batch_set_kernel(void* batch, int32_t kernel);
 

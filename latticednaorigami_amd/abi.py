@@ -220,12 +220,14 @@ def declare_abi(lib, prefix):
         fn.argtypes = args
 
 
-BATCH_LDS, BATCH_WAVE, BATCH_LANE = 0, 1, 2
+BATCH_LDS, BATCH_WAVE, BATCH_LANE, BATCH_WAVE_SOLO = 0, 1, 2, 3
 # int32 values staged by ldo_set_config_noalloc (2 per chain + 6 per domain)
 NOALLOC_CONFIG_INTS = 65536
 BATCH_KERNEL_NAMES = {BATCH_LDS: "ldo_batch_sweep_lds_kernel",
                       BATCH_WAVE: "ldo_batch_sweep_kernel (one chain per wavefront)",
-                      BATCH_LANE: "ldo_batch_sweep_kernel (one chain per lane)"}
+                      BATCH_LANE: "ldo_batch_sweep_kernel (one chain per lane)",
+                      BATCH_WAVE_SOLO: "ldo_batch_sweep_kernel (one chain per wavefront, "
+                                       "first lane only)"}
 
 
 def declare_batch_abi(lib):

@@ -297,12 +297,12 @@ extern "C" __global__ void __launch_bounds__(64)
     }
 }
 
-// Blocks in device memory: one chain per wavefront (lane 0 works) or, with
-// per_lane, one chain per lane.
+// Blocks in device memory: mode 0, one chain per wavefront run by all its
+// lanes; 1, one chain per lane; 2, one chain per wavefront run by lane 0.
 extern "C" __global__ void ldo_batch_sweep_kernel(const Params* plist, int32_t n, int64_t steps,
-                                                  int32_t per_lane) {
+                                                  int32_t mode) {
     int k;
-    if (per_lane) {
+    if (mode == 1) {
         k = blockIdx.x * blockDim.x + threadIdx.x;
         if (k < n) {
             sweep_replica(plist[k], 0, steps, nullptr);
@@ -310,8 +310,14 @@ extern "C" __global__ void ldo_batch_sweep_kernel(const Params* plist, int32_t n
         return;
     }
     k = blockIdx.x;
-    if (k < n) {
+    if (k >= n) {
+        return;
+    }
+    if (mode == 0) {
         sweep_replica(plist[k], 0, steps, nullptr, 1);
+    }
+    else if (threadIdx.x == 0) {
+        sweep_replica(plist[k], 0, steps, nullptr, 0);
     }
 }
 
@@ -1311,9 +1317,9 @@ double batch_launch(BatchCtx& b, int kernel, int64_t steps) {
         hipLaunchKernelGGL(ldo_batch_sweep_lds_kernel, dim3(b.n), dim3(64), b.max_block, b.stream,
                            b.d_params, b.n, steps);
     }
-    else if (kernel == LDO_BATCH_WAVE) {
+    else if (kernel == LDO_BATCH_WAVE || kernel == LDO_BATCH_WAVE_SOLO) {
         hipLaunchKernelGGL(ldo_batch_sweep_kernel, dim3(b.n), dim3(64), 0, b.stream, b.d_params,
-                           b.n, steps, 0);
+                           b.n, steps, kernel == LDO_BATCH_WAVE ? 0 : 2);
     }
     else {
         hipLaunchKernelGGL(ldo_batch_sweep_kernel, dim3((b.n + 63) / 64), dim3(64), 0, b.stream,
@@ -1462,7 +1468,7 @@ const char* ldo_batch_last_error(void* batch) {
 int ldo_batch_set_kernel(void* batch, int32_t kernel) {
     BatchCtx* b = static_cast<BatchCtx*>(batch);
     return bguarded(b, [&]() {
-        if (kernel < LDO_BATCH_LDS || kernel > LDO_BATCH_LANE) {
+        if (kernel < LDO_BATCH_LDS || kernel > LDO_BATCH_WAVE_SOLO) {
             throw AbiError {LDO_E_ARG, "unknown batch kernel"};
         }
         if (kernel == LDO_BATCH_LDS && !b->lds_ok) {
@@ -1479,7 +1485,7 @@ int ldo_batch_tune(void* batch, int64_t steps_per_trial, int32_t* chosen, double
             throw AbiError {LDO_E_ARG, "steps_per_trial must be positive"};
         }
         double best = -1;
-        for (int k = LDO_BATCH_LDS; k <= LDO_BATCH_LANE; k++) {
+        for (int k = LDO_BATCH_LDS; k <= LDO_BATCH_WAVE_SOLO; k++) {
             double t = -1;
             if (k != LDO_BATCH_LDS || b->lds_ok) {
                 t = batch_launch(*b, k, steps_per_trial);

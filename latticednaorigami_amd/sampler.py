@@ -229,9 +229,10 @@ class BatchSampler:
         """Time each available sweep kernel on real steps; keep the fastest.
         Returns (chosen kernel, {kernel: ms or None})."""
         chosen = C.c_int32()
-        ms = (C.c_double * 3)()
+        n = len(abi.BATCH_KERNEL_NAMES)
+        ms = (C.c_double * n)()
         self._check(self.lib.ldo_batch_tune(self.h, steps_per_trial, C.byref(chosen), ms))
-        return chosen.value, {k: (ms[k] if ms[k] >= 0 else None) for k in range(3)}
+        return chosen.value, {k: (ms[k] if ms[k] >= 0 else None) for k in range(n)}
 
     def set_kernel(self, kernel):
         self._check(self.lib.ldo_batch_set_kernel(self.h, kernel))

@@ -362,11 +362,11 @@ def case_learned_moves_in_3d_umbrella_windows(sampler_cls, oracle_mod):
 
 def case_batch_matches_oracle(batch_cls, oracle_mod):
     say("case: batch of 12 chains (8-40 domains, 310-350 K), 3 x 500 steps, each segment on "
-        "a different sweep kernel (LDS, wavefront, lane)")
+        "a different sweep kernel (LDS, wavefront, lane, wavefront run by its first lane)")
     ss, ts = systems.batch_systems(12, len_lo=8, len_hi=40)
     b = batch_cls(ss, ts)
     refs = [oracle_mod.Oracle(s, temps=[t]) for s, t in zip(ss, ts)]
-    for kernel in (abi.BATCH_LDS, abi.BATCH_WAVE, abi.BATCH_LANE):
+    for kernel in (abi.BATCH_LDS, abi.BATCH_WAVE, abi.BATCH_LANE, abi.BATCH_WAVE_SOLO):
         b.set_kernel(kernel)
         b.sweep(500)
         obs = b.observables()
@@ -570,11 +570,11 @@ def case_batch_lengths_8_to_128(batch_cls, oracle_mod, steps=2000):
     LDS kernel then gives every workgroup the largest block of the
     batch), each sweep kernel for `steps` steps against per-chain oracles."""
     say(f"case: batch of {len(BATCH_LENGTHS)} chains, L = 8..128, {steps} steps on each of the "
-        "LDS, wavefront and lane kernels")
+        "LDS, wavefront, lane and first-lane kernels")
     ss, ts = _batch_systems(BATCH_LENGTHS)
     b = batch_cls(ss, ts)
     refs = [oracle_mod.Oracle(s, temps=[t]) for s, t in zip(ss, ts)]
-    for kernel in (abi.BATCH_LDS, abi.BATCH_WAVE, abi.BATCH_LANE):
+    for kernel in (abi.BATCH_LDS, abi.BATCH_WAVE, abi.BATCH_LANE, abi.BATCH_WAVE_SOLO):
         say(f"  kernel {kernel}")
         b.set_kernel(kernel)
         b.sweep(steps)
