@@ -39,6 +39,7 @@ N_CU = 256
 # a copy of known size; bench.py reads the measured HBM bytes per 1000-step
 # launch from the summary's "traffic_bytes_per_launch" line
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "ldo_sweep_lds_kernel_raster64_ptmc32_pmc.md")
+BATCH_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "ldo_batch_sweep_kernel_batch4096.md")
 
 
 def profiled_traffic(path=TRAFFIC_PROFILE):
@@ -199,13 +200,17 @@ def gpu_leg_batch(args):
     attempted = sum(c1.attempts) - sum(c0.attempts)
     nbytes = algorithmic_bytes(c0, c1)
     achieved = nbytes / (kernel_ms / 1e3) / 1e9 if kernel_ms > 0 else None
+    traffic = (profiled_traffic(BATCH_TRAFFIC_PROFILE)
+               if chosen == abi.BATCH_WAVE_SOLO and args.interval == 1000 else None)
     roofline = {
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS if achieved is not None else None,
-        "traffic": None,
+        # the counter profile's figure holds for the kernel it measured
+        "traffic": traffic,
+        "traffic_source": os.path.relpath(BATCH_TRAFFIC_PROFILE, ROOT) if traffic else None,
         "kernel": abi.BATCH_KERNEL_NAMES[chosen],
         "launches": args.steps,
         "avg_launch_ms": kernel_ms / max(1, args.steps),
