@@ -65,8 +65,11 @@ extern "C" __global__ void ldo_init_kernel(Params P, const int32_t* c_idx, const
 }
 
 // With a wavefront per replica, its 64 lanes run the replica's chain
-// together (lane-parallel sections spread independent reads over them).
+// together (lane-parallel sections spread independent reads over them); the
+// engine object sits in LDS, so its section pointers are not re-read from
+// the lanes' stacks in device memory.
 extern "C" __global__ void ldo_sweep_kernel(Params P, int64_t steps, const int32_t* rep_slot) {
+    __shared__ Eng lds_engine;
     if (P.lanes_per_replica == 1) {
         bool active;
         int r = replica_of_thread(P, active);
@@ -77,7 +80,7 @@ extern "C" __global__ void ldo_sweep_kernel(Params P, int64_t steps, const int32
     }
     int r = blockIdx.x;
     if (r < P.n_replicas) {
-        sweep_replica(P, r, steps, rep_slot, 1);
+        sweep_replica_on(lds_engine, P, r, P.blocks + (size_t)r * P.L.bytes, steps, rep_slot, 1);
     }
 }
 
@@ -299,8 +302,12 @@ extern "C" __global__ void __launch_bounds__(64)
 
 // Blocks in device memory: mode 0, one chain per wavefront run by all its
 // lanes; 1, one chain per lane; 2, one chain per wavefront run by lane 0.
+// With a chain per wavefront its parameters and engine object sit in LDS, so
+// their fields are not re-read from the lanes' stacks in device memory.
 extern "C" __global__ void ldo_batch_sweep_kernel(const Params* plist, int32_t n, int64_t steps,
                                                   int32_t mode) {
+    __shared__ Params lds_params;
+    __shared__ Eng lds_engine;
     int k;
     if (mode == 1) {
         k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -313,11 +320,15 @@ extern "C" __global__ void ldo_batch_sweep_kernel(const Params* plist, int32_t n
     if (k >= n) {
         return;
     }
+    if (threadIdx.x == 0) {
+        lds_params = plist[k];
+    }
+    __syncthreads();
     if (mode == 0) {
-        sweep_replica(plist[k], 0, steps, nullptr, 1);
+        sweep_replica_on(lds_engine, lds_params, 0, lds_params.blocks, steps, nullptr, 1);
     }
     else if (threadIdx.x == 0) {
-        sweep_replica(plist[k], 0, steps, nullptr, 0);
+        sweep_replica_on(lds_engine, lds_params, 0, lds_params.blocks, steps, nullptr, 0);
     }
 }
 
