@@ -179,6 +179,10 @@ struct Eng {
     const double* t_stack;
     double init_e;
     int ns, lshift, lmask, nslots, nchain, hmask, hbits, n_ident;
+    // launch parameters the hot paths read, copied out of Params so that a
+    // read is one access to the engine object rather than two dependent ones
+    int cyc, dtype, mispot, rngmode;
+    uint64_t seed;
     // FourBody accumulators (BindingPotential::m_delta_config)
     double dc_e;
     int dc_st;
@@ -326,6 +330,11 @@ struct Eng {
         hbits = L.hbits;
         hmask = L.hcap - 1;
         n_ident = p->n_ident;
+        cyc = p->cyclic;
+        dtype = p->domain_type;
+        mispot = p->misbinding_pot;
+        rngmode = p->rng_mode;
+        seed = p->seed;
         select_tables();
     }
 
@@ -366,7 +375,7 @@ struct Eng {
         uint64_t ctr = S->rng_sub ? (uint64_t)S->learned_step : (uint64_t)S->step;
         uint32_t c0 = (uint32_t)S->rng_idx, c1 = (uint32_t)S->rng_sub;
         uint32_t c2 = (uint32_t)ctr, c3 = (uint32_t)(ctr >> 32);
-        uint32_t k0 = (uint32_t)P->seed, k1 = (uint32_t)rep;
+        uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)rep;
 #pragma unroll
         for (int r = 0; r < 10; r++) {
             uint64_t p0 = (uint64_t)0xD2511F53u * c0;
@@ -398,7 +407,7 @@ struct Eng {
 
     // RandomGens::uniform_real (random_gens.cpp:22-24)
     DEV double uniform_real() {
-        if (P->rng_mode == 1) {
+        if (rngmode == 1) {
             return bits_to_double(tape_next());
         }
         return (double)(philox64() >> 11) * 0x1.0p-53;
@@ -406,7 +415,7 @@ struct Eng {
 
     // RandomGens::uniform_int (random_gens.cpp:26-43)
     DEV int uniform_int(int lo, int hi) {
-        if (P->rng_mode == 1) {
+        if (rngmode == 1) {
             return (int)(int64_t)tape_next();
         }
         uint64_t x = philox64();
@@ -421,27 +430,24 @@ struct Eng {
     DEV int len_of(int cs) const { return ch[cs].len; }
 
     // Domain::m_forward_domain / m_backward_domain (origami_system.cpp:427-433, 686-692)
+    // the scaffold occupies slots [0, ns) and is ns domains long
     DEV int fwd(int s) const {
         if (s < 0) {
             return LDO_NONE;
         }
-        int cs = cslot_of(s);
-        int d = d_of(s);
-        if (d + 1 < ch[cs].len) {
-            return s + 1;
+        if (s < ns) {
+            return s + 1 < ns ? s + 1 : (cyc ? 0 : LDO_NONE);
         }
-        return (cs == 0 && P->cyclic) ? 0 : LDO_NONE;
+        return d_of(s) + 1 < ch[cslot_of(s)].len ? s + 1 : LDO_NONE;
     }
     DEV int bwd(int s) const {
         if (s < 0) {
             return LDO_NONE;
         }
-        int cs = cslot_of(s);
-        int d = d_of(s);
-        if (d > 0) {
-            return s - 1;
+        if (s < ns) {
+            return s > 0 ? s - 1 : (cyc ? ns - 1 : LDO_NONE);
         }
-        return (cs == 0 && P->cyclic) ? ns - 1 : LDO_NONE;
+        return d_of(s) > 0 ? s - 1 : LDO_NONE;
     }
     // Domain::operator+ (domain.cpp:9-31)
     DEV int walk(int s, int incr) const {
@@ -550,7 +556,7 @@ struct Eng {
     // HalfTurnDomain / ThreeQuarterTurnDomain::check_twist_constraint
     // (domain.cpp:33-40, 69-78)
     DEV bool twist_ok(int a, I3 ndr, int b) const {
-        I3 r = (P->domain_type == 0) ? rotate_half3(ore(a), ndr)
+        I3 r = (dtype == 0) ? rotate_half3(ore(a), ndr)
                                      : rotate_minus_quarter3(ore(a), ndr);
         return eq3(r, ore(b));
     }
@@ -561,7 +567,7 @@ struct Eng {
             return false;
         }
         if (eq3(ndr, oa)) {
-            if (P->domain_type == 0) {
+            if (dtype == 0) {
                 return !eq3(ob, neg3(oa));
             }
             return !(eq3(oa, ob) || eq3(oa, neg3(ob)));
@@ -573,7 +579,7 @@ struct Eng {
     }
     // check_junction_constraint (domain.cpp:59-67, 99-118)
     DEV bool junction_ok(int j1, int j2, int k1, int k2) const {
-        if (P->domain_type == 0) {
+        if (dtype == 0) {
             return true;
         }
         I3 nk = sub3(pos(k2), pos(k1));

@@ -582,7 +582,7 @@ DEVIN Delta Eng::bind_domain(int di) {
         }
         pot_viol = fb_viol;
     }
-    else if (P->misbinding_pot == 0) {
+    else if (mispot == 0) {
         if (dom[di].ore != (dom[dj].ore ^ 1)) {
             pot_viol = true;
         }

@@ -744,7 +744,7 @@ DEVIN bool Eng::cp_walks_seg(int s, I3 p, int sg) {
         }
         int steps;
         int ed = e.d;
-        if (P->cyclic && cs == 0) {
+        if (cyc && cs == 0) {
             if (dir > 0 && ed < dd) {
                 steps = ns + ed - dd;
             }
