@@ -638,7 +638,7 @@ struct Eng {
     DEVIN void fb_central_single_junction(int a, int b);
     DEVIN void fb_regular_pair(int a, int b, int i);
     DEVIN void fb_helix_pair(int a, int b, int i, int j);
-    DEVNI void fb_around(int cd, int j);
+    DEVIN void fb_around(int cd, int j);
     DEVNI void fb_terms(int di, int dj);
     DEVIN Delta bind_domain(int di);
 

@@ -497,7 +497,7 @@ DEVIN void Eng::fb_helix_pair(int a, int b, int i, int j) {
 }
 
 // JunctionBindingPotential::check_constraints (:231-286)
-DEVNI void Eng::fb_around(int cd, int j) {
+DEVIN void Eng::fb_around(int cd, int j) {
     for (int i = -1; i <= 0; i++) {
         int a = walk(cd, i);
         int b = walk(cd, i + 1);
