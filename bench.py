@@ -275,10 +275,12 @@ def cpu_baseline_mwus3d(workers, reps_per_worker, swaps, interval):
 
 def gpu_leg_mwus3d(args):
     """The 3D umbrella workload: 256 replicas (64 windows x 4 T), window
-    exchange within each temperature every `interval` steps."""
+    exchange within each temperature and swaps of each window between
+    neighbouring temperatures every `interval` steps."""
     from latticednaorigami_amd import abi, systems
     from latticednaorigami_amd.sampler import Sampler
     s, wins, grid, glen = systems.mwus3d_system()
+    s.flags |= abi.FLAG_WINDOW_TEMPERATURE_SWAPS
     n_learned = args.mwus3d_learned
     train_info = None
     if n_learned > 0:
@@ -305,6 +307,7 @@ def gpu_leg_mwus3d(args):
     if args.warmup > 0:
         interval_round(args.warmup)
     c0 = smp.counters()
+    xa0, xc0 = smp.exchange_counts()
     smp.set_profiling(True)
     t0 = time.perf_counter()
     interval_round(args.steps)
@@ -312,6 +315,7 @@ def gpu_leg_mwus3d(args):
     prof = smp.profile()
     smp.set_profiling(False)
     c1 = smp.counters()
+    xa1, xc1 = smp.exchange_counts()
     accepted = sum(c1.accepts) - sum(c0.accepts)
     attempted = sum(c1.attempts) - sum(c0.attempts)
     nbytes = algorithmic_bytes(c0, c1)
@@ -336,6 +340,8 @@ def gpu_leg_mwus3d(args):
         "exchange_attempts": c1.exchange_attempts - c0.exchange_attempts,
         "exchange_accepts": c1.exchange_accepts - c0.exchange_accepts,
         "exchange_kernel_ms": prof["exchange_ms"],
+        "temperature_swap_attempts": int((xa1[1] - xa0[1]).sum()),
+        "temperature_swap_accepts": int((xc1[1] - xc0[1]).sum()),
     }
     if n_learned > 0:
         # counter slots 5, 6, 7: learned staple exchange, CB fallback, learned regrowth
@@ -644,6 +650,8 @@ def main_mwus3d(args):
             "windows": 64,
             "temps_K": [310.0, 315.0, 320.0, 325.0],
             "exchange_interval": args.interval,
+            "exchange": "windows within each temperature, then each window between "
+                        "neighbouring temperatures",
             "hybridization": "NearestNeighbour",
             "learned_proposal": args.mwus3d_learned > 0,
             "learned_attempts_per_interval": args.mwus3d_learned,

@@ -68,11 +68,29 @@ enum ldo_movetype {
  * would have produced them (random_gens.cpp:22-43). */
 enum ldo_rng_mode { LDO_RNG_PHILOX = 0, LDO_RNG_TAPE = 1 };
 
-/* Replica exchange (ptmc_simulation.cpp:360-412; us_simulation.cpp:770-864) */
+/* Replica exchange (ptmc_simulation.cpp:360-412, 495-560;
+ * us_simulation.cpp:770-864).  The parallel-tempering modes share one
+ * acceptance (calc_acceptance_p, ptmc_simulation.cpp:275-313) and differ in
+ * which control variables a replica takes from its ladder slot after each
+ * exchange (m_exchange_q_is, ptmc_simulation.cpp:602-650) and in what its
+ * sweep uses (update_control_qs, 595-680):
+ *   T   temperature; the sweep's stacking multiplier is 1
+ *   UT  temperature and staple chemical-potential multiplier; stacking 1
+ *   HUT as UT, plus the bias multiplier (which the system ignores:
+ *       OrigamiSystem::update_bias_mult is empty, origami_system.hpp:165)
+ *   ST  temperature and stacking multiplier; the sweep's stacking energies
+ *       are scaled by the slot's multiplier
+ *   2D  a grid of temperatures x stacking multipliers (ldo_set_ladder_2d),
+ *       swept as ST; exchanges alternate between the two directions.
+ * The chemical-potential multiplier only enters the exchange. */
 enum ldo_exchange_mode {
     LDO_EXCHANGE_NONE = 0,        /* independent constant-T chains */
     LDO_EXCHANGE_TEMPERATURE = 1, /* ut_parallel_tempering */
     LDO_EXCHANGE_WINDOWS = 2,     /* ptmwus: umbrella windows at one temperature */
+    LDO_EXCHANGE_T = 3,           /* t_parallel_tempering */
+    LDO_EXCHANGE_ST = 4,          /* st_parallel_tempering */
+    LDO_EXCHANGE_HUT = 5,         /* hut_parallel_tempering */
+    LDO_EXCHANGE_2D = 6           /* 2d_parallel_tempering */
 };
 
 /* System-wide order parameters a move-level bias can act on
@@ -165,9 +183,24 @@ typedef struct ldo_run_params {
  * binding trial, so the move does not satisfy detailed balance.  With this
  * flag the endpoint is placed on the scaffold domain at its own position, as
  * the growthpoint pairs of the same function are.  Two-domain staples that
- * bind only the scaffold never reach this code path. */
+ * bind only the scaffold never reach this code path.
+ *
+ * LDO_FLAG_WINDOW_TEMPERATURE_SWAPS (an extension: the reference's PTMWUS
+ * exchanges windows at one temperature, us_simulation.cpp:770-864): with
+ * LDO_EXCHANGE_WINDOWS and several window ladders (runs of consecutive slots
+ * at one temperature, all of W windows, slot g * W + w holding window w of
+ * ladder g, every ladder's windows equal to the first's), each exchange
+ * round ends with swaps between window w of ladders g and g + 1 for every w
+ * and every g = swap_i mod 2, 2 + swap_i mod 2, ...  A swap is accepted with
+ * min(1, exp(a + m [G_a(p1) - G_b(p1) + G_b(p2) - G_a(p2)])): a the PTMC
+ * log ratio of the two slots (calc_acceptance_p, ptmc_simulation.cpp:
+ * 275-313), G the slots' grid biases, p1 and p2 the replicas' order
+ * parameters and m the windows' bias multiplier; the restraints, equal in
+ * both windows, cancel.  A replica that changes temperature has its energy
+ * recomputed, as update_temp does. */
 enum ldo_run_flags {
-    LDO_FLAG_CTCB_SCAFFOLD_ENDPOINTS = 1
+    LDO_FLAG_CTCB_SCAFFOLD_ENDPOINTS = 1,
+    LDO_FLAG_WINDOW_TEMPERATURE_SWAPS = 2
 };
 
 /* One configuration: the Chain records of origami_system.hpp:46-62. */
@@ -256,10 +289,28 @@ int ldo_get_config(void* ctx, int32_t replica, ldo_config* cfg,
 
 /* Control variables of a temperature ladder slot (PTGCMCSimulation control
  * q's, ptmc_simulation.cpp:337-358): temperature, staple chemical potential
- * multiplier, bias multiplier and stacking multiplier. */
+ * multiplier, bias multiplier and stacking multiplier.  Every replica's own
+ * chemical-potential multiplier, the one its first exchange sees, is reset
+ * to bias_mults[replica]: OneDPTGCMCSimulation::initialize_control_qs stores
+ * the bias multiplier under index m_bias_i, which equals m_staple_u_mult_i
+ * (ptmc_simulation.cpp:349-356, ptmc_simulation.hpp:74, 80).  With
+ * LDO_EXCHANGE_T and LDO_EXCHANGE_ST a replica keeps that value for the whole
+ * run. */
 int ldo_set_ladder(void* ctx, int32_t n, const double* temps,
                    const double* staple_u_mults, const double* bias_mults,
                    const double* stacking_mults);
+
+/* The grid of LDO_EXCHANGE_2D (TwoDPTGCMCSimulation::initialize_control_qs,
+ * ptmc_simulation.cpp:452-493): slot i * n_stacking + j holds temps[i] and
+ * stacking_mults[j], every slot the chemical-potential multiplier
+ * staple_u_mult and bias multiplier 1 (so every replica's own multiplier
+ * starts at 1, as with ldo_set_ladder); n_temps * n_stacking must equal the
+ * replica count.  Exchange round swap_i pairs set swap_i % 4 of
+ * attempt_exchange (495-560): 0 and 2 neighbouring temperatures from even
+ * and odd rows, 1 and 3 neighbouring stacking multipliers from even and odd
+ * columns. */
+int ldo_set_ladder_2d(void* ctx, int32_t n_temps, const double* temps, int32_t n_stacking,
+                      const double* stacking_mults, double staple_u_mult);
 
 /* Umbrella windows, one per ladder slot (n = the replica count): window
  * parameters and grid biases grid[w * grid_len + p] for order-parameter
@@ -295,6 +346,14 @@ int ldo_get_histogram(void* ctx, int64_t* counts, int32_t reset);
  * is the k-th value the reference's RandomGens returned: a uniform_real as
  * its IEEE-754 bit pattern, a uniform_int as the integer. */
 int ldo_set_tape(void* ctx, int32_t replica, const uint64_t* tape, int64_t n);
+
+/* Set every replica's step counter: the next MC step is step + 1, as when
+ * GCMCSimulation::simulate starts from start_step (simulation.cpp:567-573).
+ * Centering, constraint checks and the Philox sub-streams follow this
+ * numbering.  AnnealingGCMCSimulation::run starts each temperature's
+ * simulate from the previous start plus that call's return value
+ * (annealing_simulation.cpp:38-50). */
+int ldo_set_step(void* ctx, int64_t step);
 
 /* Run `steps` MC steps on every replica with no exchange
  * (GCMCSimulation::simulate, simulation.cpp:568-653). */
@@ -370,6 +429,16 @@ int ldo_get_move_tracking(void* ctx, int32_t replica, int64_t* out, int64_t cap,
 
 int ldo_get_observables(void* ctx, int32_t replica, ldo_obs* out);
 int ldo_get_counters(void* ctx, ldo_counters* out);
+
+/* Exchange attempts and accepts per slot pair, for the acceptance
+ * frequencies the PTMC drivers print at the end of a run
+ * (write_acceptance_freqs, ptmc_simulation.cpp:414-426, 562-592):
+ * element d * R + q (n = 2R) counts the pair of slot q with its neighbour in
+ * direction d, where d = 0 is the ladder's own direction (slot q + 1; in the
+ * 2D grid the next temperature, slot q + n_stacking) and d = 1 the stacking
+ * direction of the 2D grid (slot q + 1).  Window exchanges count under d = 0
+ * too.  No reference counterpart as an entry point. */
+int ldo_get_exchange_counts(void* ctx, int32_t n, int64_t* attempts, int64_t* accepts);
 
 /* Ladder slot -> replica permutation (m_q_to_repi, written to .swp). */
 int ldo_get_permutation(void* ctx, int32_t* q_to_rep);

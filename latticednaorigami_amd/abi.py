@@ -31,7 +31,17 @@ MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH = 5
 MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH = 6
 RNG_PHILOX, RNG_TAPE = 0, 1
 FLAG_CTCB_SCAFFOLD_ENDPOINTS = 1
+# swaps between the same window of neighbouring temperature ladders
+FLAG_WINDOW_TEMPERATURE_SWAPS = 2
 EXCHANGE_NONE, EXCHANGE_TEMPERATURE, EXCHANGE_WINDOWS = 0, 1, 2
+# the other parallel-tempering drivers (ptmc_simulation.cpp:602-680);
+# EXCHANGE_TEMPERATURE is ut_parallel_tempering
+EXCHANGE_T, EXCHANGE_ST, EXCHANGE_HUT, EXCHANGE_2D = 3, 4, 5, 6
+PT_EXCHANGE_MODES = {"t_parallel_tempering": EXCHANGE_T,
+                     "ut_parallel_tempering": EXCHANGE_TEMPERATURE,
+                     "hut_parallel_tempering": EXCHANGE_HUT,
+                     "st_parallel_tempering": EXCHANGE_ST,
+                     "2d_parallel_tempering": EXCHANGE_2D}
 OP_NUMFULLDOMAINS, OP_NUMSTAPLES = 0, 1
 MAX_MOVETYPES = 8
 
@@ -200,6 +210,10 @@ def declare_abi(lib, prefix):
         "destroy": (None, [v]),
         "last_error": (C.c_char_p, [v]),
         "set_ladder": (C.c_int, [v, C.c_int32, _f64p, _f64p, _f64p, _f64p]),
+        "set_ladder_2d": (C.c_int, [v, C.c_int32, _f64p, C.c_int32, _f64p, C.c_double]),
+        "set_step": (C.c_int, [v, C.c_int64]),
+        "get_exchange_counts": (C.c_int, [v, C.c_int32, C.POINTER(C.c_int64),
+                                          C.POINTER(C.c_int64)]),
         "sweep": (C.c_int, [v, C.c_int64]),
         "run": (C.c_int, [v, C.c_int64, C.c_int64]),
         "get_observables": (C.c_int, [v, C.c_int32, C.POINTER(Obs)]),

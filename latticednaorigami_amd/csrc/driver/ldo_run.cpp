@@ -6,8 +6,9 @@
 // output files (files.cpp:493-791: .vsf .trj .vcf .states .ores .counts
 // .staples .staplestates .times .ene .ops, and .swp for parallel tempering,
 // ptmc_simulation.cpp:91-104, 315-322) while every replica runs on the GPU
-// through ldo.h.  Supported: simulation_type constant_temp,
-// ut_parallel_tempering, mw_umbrella_sampling and ptmw_umbrella_sampling
+// through ldo.h.  Supported: simulation_type constant_temp, annealing,
+// t_, ut_, hut_, st_ and 2d_parallel_tempering, mw_umbrella_sampling and
+// ptmw_umbrella_sampling
 // (with the per-window,
 // per-iteration files of us_simulation.cpp:103-248: _iter-<n>.* outputs,
 // -inp.biases / .biases weights and the .out iteration summary) with the
@@ -1301,41 +1302,101 @@ int main_run(int argc, char** argv) {
     if (sim == "mw_umbrella_sampling" || sim == "ptmw_umbrella_sampling") {
         return run_mwus(run, sd, rp, init, sim == "ptmw_umbrella_sampling");
     }
-    bool pt = sim == "ut_parallel_tempering";
-    if (!pt && sim != "constant_temp") {
+    // the PT drivers of apps/main.cpp and their exchange modes
+    static const std::map<std::string, int> pt_modes = {
+        {"t_parallel_tempering", LDO_EXCHANGE_T},
+        {"ut_parallel_tempering", LDO_EXCHANGE_TEMPERATURE},
+        {"hut_parallel_tempering", LDO_EXCHANGE_HUT},
+        {"st_parallel_tempering", LDO_EXCHANGE_ST},
+        {"2d_parallel_tempering", LDO_EXCHANGE_2D}};
+    auto ptm = pt_modes.find(sim);
+    bool pt = ptm != pt_modes.end();
+    bool anneal = sim == "annealing";
+    if (!pt && !anneal && sim != "constant_temp") {
         throw Fail(LDO_E_NOTIMPL, "simulation_type " + sim + " is not supported");
     }
     // The reference builds SystemBiases from the bias-function file for
     // every simulation type (bias_functions.cpp:304-305) and every move adds
     // them; here biases exist only as the umbrella windows of the MWUS
-    // drivers, so a biased constant-T or PTMC run is refused rather than run
-    // unbiased.
+    // drivers, so a biased constant-T, annealing or PTMC run is refused
+    // rather than run unbiased.
     if (!p.str("bias_functions_file", "").empty()) {
         throw Fail(LDO_E_NOTIMPL, "bias_functions_file is supported only with "
                                   "mw_umbrella_sampling and ptmw_umbrella_sampling");
     }
-    if (pt) {
+    // control quantities of every ladder slot: temperature, chemical
+    // potential, bias and stacking multipliers (m_control_qs)
+    std::vector<double> lad_t, lad_u, lad_b, lad_s;
+    std::vector<double> v1s, v2s; // the 2D grid's temperatures and stacking multipliers
+    int mode = pt ? ptm->second : LDO_EXCHANGE_NONE;
+    double max_temp = 0, min_temp = 0, temp_interval = 1;
+    long long steps_per_temp = 0;
+    if (mode == LDO_EXCHANGE_2D) {
+        // TwoDPTGCMCSimulation::initialize_control_qs (ptmc_simulation.cpp:452-493)
+        v1s = p.list("temps");
+        v2s = p.list("stacking_mults");
+        run.R = (int)p.integer("num_reps", (long long)(v1s.size() * v2s.size()));
+        if (v1s.empty() || v2s.empty() || (size_t)run.R != v1s.size() * v2s.size()) {
+            throw Fail(LDO_E_ARG, "num_reps must equal the temps x stacking_mults grid");
+        }
+        for (double t: v1s) {
+            for (double sm: v2s) {
+                lad_t.push_back(t);
+                lad_u.push_back(p.num("staple_u_mult", 1));
+                lad_b.push_back(1.0);
+                lad_s.push_back(sm);
+            }
+        }
+        run.temps = lad_t;
+    }
+    else if (pt) {
+        // OneDPTGCMCSimulation::initialize_control_qs (337-358); a missing
+        // multiplier list reads as ones (the reference indexes the empty list)
         run.temps = p.list("temps");
         run.R = (int)p.integer("num_reps", (long long)run.temps.size());
         if ((int)run.temps.size() != run.R) {
             throw Fail(LDO_E_ARG, "temps must list num_reps temperatures");
         }
-        for (const char* k: {"chem_pot_mults", "bias_mults", "stacking_mults"}) {
-            for (double v: p.list(k)) {
-                if (v != 1.0) {
-                    throw Fail(LDO_E_NOTIMPL, std::string(k) + " other than 1");
-                }
+        lad_t = run.temps;
+        std::vector<double>* mults[3] = {&lad_u, &lad_b, &lad_s};
+        const char* keys[3] = {"chem_pot_mults", "bias_mults", "stacking_mults"};
+        for (int k = 0; k < 3; k++) {
+            *mults[k] = p.list(keys[k]);
+            if (mults[k]->empty()) {
+                mults[k]->assign(run.R, 1.0);
+            }
+            if ((int)mults[k]->size() != run.R) {
+                throw Fail(LDO_E_ARG, std::string(keys[k]) + " must list num_reps values");
             }
         }
-        rp.exchange_mode = LDO_EXCHANGE_TEMPERATURE;
+    }
+    else if (anneal) {
+        // AnnealingGCMCSimulation (annealing_simulation.cpp:12-36)
+        max_temp = p.num("max_temp", 0);
+        min_temp = p.num("min_temp", 0);
+        temp_interval = p.num("temp_interval", 0);
+        steps_per_temp = p.integer("steps_per_temp", 0);
+        if (std::fmod(max_temp - min_temp, temp_interval) != 0) {
+            throw Fail(LDO_E_ARG, "Bad temperature interval");
+        }
+        run.temps = {max_temp};
     }
     else {
         run.temps = {p.num("temp", 300)};
     }
+    rp.exchange_mode = mode;
     rp.rng_mode = LDO_RNG_PHILOX;
     int rc = ldo_create(&sd, &rp, &init, run.R, run.temps.data(), &run.ctx);
     if (rc != LDO_OK) {
         throw Fail(rc, ldo_last_error(nullptr));
+    }
+    if (mode == LDO_EXCHANGE_2D) {
+        run.check(ldo_set_ladder_2d(run.ctx, (int32_t)v1s.size(), v1s.data(),
+                                    (int32_t)v2s.size(), v2s.data(), p.num("staple_u_mult", 1)));
+    }
+    else if (pt) {
+        run.check(ldo_set_ladder(run.ctx, run.R, lad_t.data(), lad_u.data(), lad_b.data(),
+                                 lad_s.data()));
     }
 
     // restarts of parallel-tempering runs (ptmc_simulation.cpp:38-84): each
@@ -1387,25 +1448,48 @@ int main_run(int argc, char** argv) {
     auto elapsed = [&]() {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     };
-    long long step = 0;
-    if (!pt) {
-        long long total = p.integer("ct_steps", 0);
-        if (chunk == 0) {
-            chunk = total;
-        }
-        while (step < total) {
-            long long n = std::min(chunk - step % chunk, total - step);
+    // steps start + 1 .. start + total of one replica run in chunks that end
+    // on every step an output or log entry falls on (simulation.cpp:568-653)
+    auto simulate = [&](long long start_step, long long total) {
+        long long step = start_step;
+        while (step < start_step + total) {
+            long long n = start_step + total - step;
+            if (chunk > 0) {
+                n = std::min(n, chunk - step % chunk);
+            }
             run.check(ldo_sweep(run.ctx, n));
             step += n;
-            // the constant-T driver logs to standard output
-            // (constant_temp_simulation.cpp:19)
+            // the constant-T and annealing drivers log to standard output
+            // (constant_temp_simulation.cpp:19, annealing_simulation.cpp:27)
             if (log_freq != 0 && step % log_freq == 0) {
                 write_log_entry(run, std::cout, 0, step);
             }
             write_outputs(run, *outs[0], 0, step, elapsed());
         }
+    };
+    if (!pt && !anneal) {
+        simulate(0, p.integer("ct_steps", 0));
         // simulate(ct_steps) ends with the run summary and the .moves file
         write_log_summary(run, std::cout, base + ".moves");
+        std::cout.flush();
+    }
+    else if (anneal) {
+        // AnnealingGCMCSimulation::run (annealing_simulation.cpp:38-50): every
+        // temperature from max_temp down to min_temp runs steps_per_temp
+        // steps and writes the run summary; simulate returns its last step
+        // index + 1, which run adds to its own count, so temperature k starts
+        // at step (2^k - 1)(steps_per_temp + 1)
+        long long step = 0;
+        for (double temp = max_temp; temp >= min_temp; temp -= temp_interval) {
+            double one = 1.0;
+            run.check(ldo_set_ladder(run.ctx, 1, &temp, &one, &one, &one));
+            // update_temp recomputes the energy (origami_system.cpp:618-622)
+            run.check(ldo_run(run.ctx, 1, 0));
+            run.check(ldo_set_step(run.ctx, step));
+            simulate(step, steps_per_temp);
+            write_log_summary(run, std::cout, base + ".moves");
+            step += step + steps_per_temp + 1;
+        }
         std::cout.flush();
     }
     else {
@@ -1416,14 +1500,38 @@ int main_run(int argc, char** argv) {
         if (interval < 1 || (chunk > 0 && chunk % interval != 0)) {
             throw Fail(LDO_E_NOTIMPL, "output frequencies must be multiples of exchange_interval");
         }
+        // PTGCMCSimulation::initialize_swap_file (ptmc_simulation.cpp:91-104):
+        // each slot's exchanged control quantities (m_exchange_q_is)
+        std::vector<int> q_is = {0};
+        if (mode == LDO_EXCHANGE_TEMPERATURE || mode == LDO_EXCHANGE_HUT ||
+            mode == LDO_EXCHANGE_2D) {
+            q_is.push_back(1);
+        }
+        if (mode == LDO_EXCHANGE_HUT) {
+            q_is.push_back(2);
+        }
+        if (mode == LDO_EXCHANGE_ST || mode == LDO_EXCHANGE_2D) {
+            q_is.push_back(3);
+        }
+        const std::vector<double>* qs[4] = {&lad_t, &lad_u, &lad_b, &lad_s};
         std::ofstream swp;
         Outputs::open(swp, base + ".swp");
         for (int r = 0; r < run.R; r++) {
-            swp << run.temps[r] << "/" << p.num("staple_u_mult", 1) << "/ ";
+            for (int i: q_is) {
+                swp << (*qs[i])[r] << "/";
+            }
+            swp << " ";
         }
         swp << "\n";
         long long cfg_freq = o0.f_cfg;
         std::vector<int32_t> q_to_rep(run.R);
+        auto swap_entry = [&]() {
+            for (int q = 0; q < run.R; q++) {
+                swp << q_to_rep[q] << " ";
+            }
+            swp << "\n";
+            swp.flush();
+        };
         // each rank logs to its own .out file (ptmc_simulation.cpp:48-56);
         // simulate runs without a summary between exchanges
         std::vector<std::unique_ptr<std::ofstream>> logs;
@@ -1431,7 +1539,15 @@ int main_run(int argc, char** argv) {
             logs.push_back(std::make_unique<std::ofstream>());
             Outputs::open(*logs.back(), base + "-" + std::to_string(r) + ".out");
         }
+        long long step = 0;
         for (long long s = 1; s <= swaps; s++) {
+            // write_swap_entry runs before the round's exchange
+            // (ptmc_simulation.cpp:140-147): it records the permutation the
+            // round's interval ran under
+            bool entry = cfg_freq != 0 && (step + interval) % cfg_freq == 0;
+            if (entry) {
+                run.check(ldo_get_permutation(run.ctx, q_to_rep.data()));
+            }
             run.check(ldo_run(run.ctx, 1, interval));
             step += interval;
             for (int r = 0; r < run.R; r++) {
@@ -1440,15 +1556,51 @@ int main_run(int argc, char** argv) {
                 }
                 write_outputs(run, *outs[r], r, step, elapsed());
             }
-            if (cfg_freq != 0 && step % cfg_freq == 0) {
-                run.check(ldo_get_permutation(run.ctx, q_to_rep.data()));
-                for (int q = 0; q < run.R; q++) {
-                    swp << q_to_rep[q] << " ";
-                }
-                swp << "\n";
-                swp.flush();
+            if (entry) {
+                swap_entry();
             }
         }
+        // the closing entry and the acceptance frequencies on standard
+        // output (ptmc_simulation.cpp:143-149, 414-426, 562-592)
+        if (cfg_freq != 0 && step % cfg_freq == 0) {
+            run.check(ldo_get_permutation(run.ctx, q_to_rep.data()));
+            swap_entry();
+        }
+        std::vector<int64_t> att(2 * run.R), acc(2 * run.R);
+        run.check(ldo_get_exchange_counts(run.ctx, 2 * run.R, att.data(), acc.data()));
+        auto freq_line = [&](double a, double b, double c, bool third, int64_t n_acc,
+                             int64_t n_att) {
+            std::cout << a << " " << b << " ";
+            if (third) {
+                std::cout << c << " ";
+            }
+            std::cout << n_acc << " " << n_att << " " << (static_cast<double>(n_acc) / n_att)
+                      << " \n";
+        };
+        if (mode == LDO_EXCHANGE_2D) {
+            int v1 = (int)v1s.size(), v2 = (int)v2s.size();
+            for (int i = 0; i + 1 < v1; i++) {
+                for (int j = 0; j < v2; j++) {
+                    int q = i * v2 + j;
+                    freq_line(v1s[i], v1s[i + 1], v2s[j], true, acc[q], att[q]);
+                }
+            }
+            std::cout << "\n";
+            for (int j = 0; j + 1 < v2; j++) {
+                for (int i = 0; i < v1; i++) {
+                    int q = run.R + i * v2 + j;
+                    freq_line(v2s[j], v2s[j + 1], v1s[i], true, acc[q], att[q]);
+                }
+            }
+            std::cout << "\n";
+        }
+        else {
+            for (int i = 0; i + 1 < run.R; i++) {
+                freq_line(lad_t[i], lad_t[i + 1], 0, false, acc[i], att[i]);
+            }
+            std::cout << "\n";
+        }
+        std::cout.flush();
     }
     ldo_destroy(run.ctx);
     return 0;

@@ -2115,17 +2115,22 @@ DEVNI void Eng::count_window_op() {
     P->hist[(size_t)S->temp_slot * (P->grid_len + 1) + b] += 1;
 }
 
-// TPT/UTPTGCMCSimulation::update_control_qs (ptmc_simulation.cpp:651-663):
-// switch to the tables of ladder slot `slot` and recompute the energy.  A
-// slot met for the first time rebuilds its tables in the reference, which
-// also refreshes the initiation terms (origami_potential.cpp:1019-1042,
-// 1057-1063); on later visits the initiation terms keep their last values.
+// update_control_qs of the PT drivers (ptmc_simulation.cpp:595-680): switch
+// to the tables of ladder slot `slot` and, with `recompute` (update_temp),
+// recompute the energy.  A (temperature, stacking multiplier) key that
+// update_temp meets for the first time rebuilds its tables in the reference,
+// which also refreshes the initiation terms (origami_potential.cpp:1019-1042,
+// 1057-1063); on later visits of the key the initiation terms keep their
+// last values.  slot_key numbers the keys.  Without `recompute` (a window
+// swap at one temperature) only the tables are selected.
 DEVNI void Eng::begin_interval(int slot_, bool recompute) {
     S->temp_slot = slot_;
-    uint64_t bit = 1ull << slot_;
-    if (!(S->visited & bit)) {
-        S->visited |= bit;
-        S->init_slot = slot_;
+    if (recompute) {
+        uint64_t bit = 1ull << (P->slot_key != nullptr ? P->slot_key[slot_] : slot_);
+        if (!(S->visited & bit)) {
+            S->visited |= bit;
+            S->init_slot = slot_;
+        }
     }
     select_tables();
     if (recompute) {

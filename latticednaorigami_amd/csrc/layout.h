@@ -237,6 +237,8 @@ struct Params {
     const int64_t* tape_off;  // [n_replicas]
     const int64_t* tape_len;  // [n_replicas]
     int32_t exchange_mode;    // ldo_exchange_mode
+    int32_t pt_dim2;          // stacking multipliers of a 2D ladder grid (else 1)
+    const int32_t* slot_key;  // [n_slots] table key class of each ladder slot
     int32_t grid_len;         // grid values per window
     const ldo_window* windows; // [n_slots] umbrella windows, or null
     const double* win_grid;    // [n_slots * grid_len]

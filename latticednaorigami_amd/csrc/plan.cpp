@@ -142,7 +142,11 @@ void make_plan(const ldo_system_desc* sys, const ldo_run_params* run, int32_t n_
     if (run->n_movetypes < 1 || run->n_movetypes > LDO_MAX_MOVES) {
         throw AbiError {LDO_E_ARG, "moveset must hold 1..8 movetypes"};
     }
-    if (run->exchange_mode == LDO_EXCHANGE_TEMPERATURE && n_replicas > 64) {
+    if (run->exchange_mode < LDO_EXCHANGE_NONE || run->exchange_mode > LDO_EXCHANGE_2D) {
+        throw AbiError {LDO_E_ARG, "unknown exchange_mode"};
+    }
+    if (run->exchange_mode != LDO_EXCHANGE_NONE && run->exchange_mode != LDO_EXCHANGE_WINDOWS &&
+        n_replicas > 64) {
         throw AbiError {LDO_E_ARG, "a temperature ladder holds at most 64 slots"};
     }
     out.R = n_replicas;
@@ -287,8 +291,12 @@ void make_plan(const ldo_system_desc* sys, const ldo_run_params* run, int32_t n_
         throw AbiError {LDO_E_ARG, "lanes_per_replica must be 0, 1 or 64"};
     }
     P.lanes_per_replica = lpr;
-    if (run->flags & ~LDO_FLAG_CTCB_SCAFFOLD_ENDPOINTS) {
+    if (run->flags & ~(LDO_FLAG_CTCB_SCAFFOLD_ENDPOINTS | LDO_FLAG_WINDOW_TEMPERATURE_SWAPS)) {
         throw AbiError {LDO_E_ARG, "unknown ldo_run_params.flags bits"};
+    }
+    if ((run->flags & LDO_FLAG_WINDOW_TEMPERATURE_SWAPS) &&
+        run->exchange_mode != LDO_EXCHANGE_WINDOWS) {
+        throw AbiError {LDO_E_ARG, "window temperature swaps need LDO_EXCHANGE_WINDOWS"};
     }
     P.flags = run->flags;
     P.n_slots = n_replicas;
@@ -328,7 +336,7 @@ void slot_tables(const Plan& p, std::vector<double>& tab, std::vector<double>& i
     tab.assign(4 * n2 * n, 0.0);
     init.assign(3 * (size_t)n, 0.0);
     for (int q = 0; q < n; q++) {
-        TableSet t = build_tables(p.spec, p.ladder_t[q], p.ladder_s[q]);
+        TableSet t = build_tables(p.spec, p.ladder_t[q], slot_stacking(p, q));
         std::copy(t.hyb_e.begin(), t.hyb_e.end(), tab.begin() + (0 * (size_t)n + q) * n2);
         std::copy(t.hyb_h.begin(), t.hyb_h.end(), tab.begin() + (1 * (size_t)n + q) * n2);
         std::copy(t.hyb_s.begin(), t.hyb_s.end(), tab.begin() + (2 * (size_t)n + q) * n2);
@@ -341,14 +349,90 @@ void slot_tables(const Plan& p, std::vector<double>& tab, std::vector<double>& i
 
 std::vector<double> ladder_values(const Plan& p) {
     int n = p.R;
-    std::vector<double> xd(3 * (size_t)n + p.reduced_u.size());
+    std::vector<double> xd(4 * (size_t)n + p.reduced_u.size());
     for (int q = 0; q < n; q++) {
         xd[q] = p.ladder_t[q];
         xd[n + q] = p.ladder_u[q];
         xd[2 * n + q] = p.ladder_s[q];
     }
     std::copy(p.reduced_u.begin(), p.reduced_u.end(), xd.begin() + 3 * n);
+    size_t o = 3 * (size_t)n + p.reduced_u.size();
+    for (int r = 0; r < n; r++) {
+        xd[o + r] = p.ladder_b[r];
+    }
     return xd;
+}
+
+double slot_stacking(const Plan& p, int q) {
+    int m = p.P.exchange_mode;
+    if (m == LDO_EXCHANGE_TEMPERATURE || m == LDO_EXCHANGE_T || m == LDO_EXCHANGE_HUT) {
+        return 1.0;
+    }
+    return p.ladder_s[q];
+}
+
+int slot_classes(const Plan& p, std::vector<int32_t>& key) {
+    key.assign(p.R, 0);
+    std::vector<std::pair<double, double>> seen;
+    for (int q = 0; q < p.R; q++) {
+        std::pair<double, double> k {p.ladder_t[q], slot_stacking(p, q)};
+        size_t c = 0;
+        while (c < seen.size() && seen[c] != k) {
+            c++;
+        }
+        if (c == seen.size()) {
+            seen.push_back(k);
+        }
+        key[q] = (int32_t)c;
+    }
+    return (int)seen.size();
+}
+
+int window_ladder_size(const Plan& p) {
+    int W = 1;
+    while (W < p.R && p.ladder_t[W] == p.ladder_t[0]) {
+        W++;
+    }
+    if (p.R % W != 0) {
+        return -1;
+    }
+    for (int q = 0; q < p.R; q++) {
+        if (p.ladder_t[q] != p.ladder_t[q - q % W]) {
+            return -1;
+        }
+        if (q % W == 0 && q > 0 && p.ladder_t[q] == p.ladder_t[q - 1]) {
+            return -1;
+        }
+    }
+    return W;
+}
+
+bool u_follows_slot(int m) {
+    return m == LDO_EXCHANGE_TEMPERATURE || m == LDO_EXCHANGE_HUT || m == LDO_EXCHANGE_2D;
+}
+
+void set_ladder_2d(Plan& p, int32_t n_temps, const double* temps, int32_t n_stacking,
+                   const double* stacking_mults, double staple_u_mult) {
+    if (p.P.exchange_mode != LDO_EXCHANGE_2D) {
+        throw AbiError {LDO_E_ARG, "a 2D ladder needs exchange_mode LDO_EXCHANGE_2D"};
+    }
+    if (!temps || !stacking_mults || n_temps < 1 || n_stacking < 1 ||
+        (int64_t)n_temps * n_stacking != p.R) {
+        throw AbiError {LDO_E_ARG, "the 2D grid must hold one slot per replica"};
+    }
+    p.ladder_t.clear();
+    p.ladder_u.clear();
+    p.ladder_b.clear();
+    p.ladder_s.clear();
+    for (int i = 0; i < n_temps; i++) {
+        for (int j = 0; j < n_stacking; j++) {
+            p.ladder_t.push_back(temps[i]);
+            p.ladder_u.push_back(staple_u_mult);
+            p.ladder_b.push_back(1.0);
+            p.ladder_s.push_back(stacking_mults[j]);
+        }
+    }
+    p.dim2 = n_stacking;
 }
 
 void check_learned_types(const Plan& p) {
@@ -490,12 +574,30 @@ void check_windows(const Plan& p, int32_t n, const ldo_window* windows, int32_t 
     if (n != p.R || !windows || grid_len < 0) {
         throw AbiError {LDO_E_ARG, "one window per replica (ladder slot) is required"};
     }
-    if (p.P.exchange_mode == LDO_EXCHANGE_TEMPERATURE) {
+    if (p.P.exchange_mode != LDO_EXCHANGE_NONE && p.P.exchange_mode != LDO_EXCHANGE_WINDOWS) {
         throw AbiError {LDO_E_NOTIMPL, "umbrella windows with a temperature ladder"};
     }
     for (int w = 0; w < n; w++) {
         if (windows[w].op != LDO_OP_NUMFULLDOMAINS && windows[w].op != LDO_OP_NUMSTAPLES) {
             throw AbiError {LDO_E_ARG, "unknown window order parameter"};
+        }
+    }
+    if (p.P.flags & LDO_FLAG_WINDOW_TEMPERATURE_SWAPS) {
+        // equal ladders of consecutive slots at one temperature, each holding
+        // the first ladder's windows in its order
+        int W = window_ladder_size(p);
+        if (W <= 0) {
+            throw AbiError {LDO_E_ARG, "window temperature swaps need window ladders of one size"};
+        }
+        for (int q = W; q < n; q++) {
+            const ldo_window& a = windows[q % W];
+            const ldo_window& b = windows[q];
+            if (a.op != b.op || a.min_op != b.min_op || a.max_op != b.max_op ||
+                a.well_bias != b.well_bias || a.min_bias != b.min_bias || a.slope != b.slope ||
+                a.off_grid_bias != b.off_grid_bias || a.bias_mult != b.bias_mult) {
+                throw AbiError {LDO_E_ARG, "window temperature swaps need every ladder to hold "
+                                           "the first ladder's windows"};
+            }
         }
     }
 }

@@ -39,6 +39,7 @@ struct Plan {
     std::vector<int16_t> type_ident; // [ntypes * P.type_ident_stride] table index
     std::vector<double> ladder_t, ladder_u, ladder_b, ladder_s;
     std::vector<double> reduced_u;   // [R * (ntypes - 1)]
+    int dim2 = 1;                    // stacking multipliers of a 2D ladder grid
 };
 
 // Validates the inputs of ldo_create and fills `out`; throws AbiError.
@@ -49,8 +50,38 @@ void make_plan(const ldo_system_desc* sys, const ldo_run_params* run, int32_t n_
 // stack, each [R][n_ident^2]; init = init_e | init_h | init_s, each [R].
 void slot_tables(const Plan& p, std::vector<double>& tab, std::vector<double>& init);
 
-// ladder_t | ladder_u | ladder_s | reduced_u, as the exchange reads them
+// ladder_t | ladder_u | ladder_s | reduced_u | rep_u, as the exchange reads
+// them; rep_u (each replica's own chemical-potential multiplier) starts at
+// the replica's bias multiplier, ladder_b[r]
 std::vector<double> ladder_values(const Plan& p);
+
+// The stacking multiplier slot q's sweep uses: its own with
+// LDO_EXCHANGE_ST and LDO_EXCHANGE_2D, 1 with the T/UT/HUT ladders, whose
+// update_control_qs calls update_temp(temp) (ptmc_simulation.cpp:652-670).
+// Modes without a ladder keep the slot's value.
+double slot_stacking(const Plan& p, int q);
+
+// Table key class of every slot: slots with the same (temperature, sweep
+// stacking multiplier) share a class, numbered from 0 in order of first
+// appearance.  A table-cache hit keeps the initiation terms of the last
+// build (OrigamiPotential::update_temp, origami_potential.cpp:1019-1042), so
+// a replica's initiation terms change only on its first visit of a class.
+// Returns the class count.
+int slot_classes(const Plan& p, std::vector<int32_t>& key);
+
+// The window count W of every ladder when the slots form equal ladders of
+// consecutive slots at one temperature (neighbouring ladders at different
+// temperatures), else -1.
+int window_ladder_size(const Plan& p);
+
+// Whether a replica takes its slot's chemical-potential multiplier after
+// every exchange (m_exchange_q_is holds m_staple_u_mult_i: UT, HUT, 2D).
+bool u_follows_slot(int exchange_mode);
+
+// Checks the 2D grid (n_temps x n_stacking = R) and lays it out in the
+// plan's ladder, slot i * n_stacking + j (ptmc_simulation.cpp:452-493).
+void set_ladder_2d(Plan& p, int32_t n_temps, const double* temps, int32_t n_stacking,
+                   const double* stacking_mults, double staple_u_mult);
 
 // The learned moves place a staple end next to its chain neighbour: every
 // staple type needs at least two domains (LDO_E_NOTIMPL otherwise).

@@ -101,7 +101,11 @@ DEV void sweep_replica_on(Eng& e, const Params& P, int r, char* blk, int64_t ste
         return;
     }
     if (rep_slot != nullptr) {
-        e.begin_interval(rep_slot[r], P.exchange_mode == LDO_EXCHANGE_TEMPERATURE);
+        // a window swap keeps the temperature; a temperature change
+        // recomputes the energy (update_temp)
+        int slot = rep_slot[r];
+        bool moved = P.slot_key != nullptr && P.slot_key[slot] != P.slot_key[e.S->temp_slot];
+        e.begin_interval(slot, P.exchange_mode != LDO_EXCHANGE_WINDOWS || moved);
     }
     e.simulate(steps);
     if (rep_slot != nullptr) {
@@ -134,7 +138,10 @@ struct XState {
     const double* ladder_u;
     const double* ladder_s;
     const double* reduced_u; // [n_replicas * (ntypes-1)]
-    int64_t* counts;         // attempts, accepts
+    double* rep_u;           // [n_replicas] each replica's chemical-potential multiplier
+    // attempts, accepts, then per slot pair attempts[2R] and accepts[2R]
+    // (element d * R + q: the pair of slot q with its neighbour in direction d)
+    int64_t* counts;
 };
 
 // The exchange stream: Philox4x32-10 keyed by (seed, 0xFFFFFFFF), counter
@@ -172,42 +179,82 @@ DEV double exchange_uniform(const Params& P, int64_t rng_ctr, uint32_t& idx) {
     return (double)(x >> 11) * 0x1.0p-53;
 }
 
-// OneDPTGCMCSimulation::attempt_exchange + calc_acceptance_p
-// (ptmc_simulation.cpp:255-313, 360-412).  The per-staple chemical
-// potential term runs over the staple types.  swap_i picks the even or odd
-// pairs; rng_ctr is the round's exchange-stream counter.
-DEV void exchange_round(const Params& P, const XState& X, int64_t swap_i, int64_t rng_ctr) {
+// calc_acceptance_p + test_acceptance (ptmc_simulation.cpp:255-313) for the
+// replicas in slots a and b, swapping them when accepted.  The per-staple
+// chemical potential of a replica is reduced_u * T * (its own multiplier),
+// as update_staple_us leaves m_staple_us (origami_system.cpp:624-628).
+// d: direction of the pair (per-pair counters).
+DEV void exchange_pair(const Params& P, const XState& X, int a, int b, int d, int64_t rng_ctr,
+                       uint32_t& idx) {
     int R = P.n_replicas;
     int nst = P.L.ntypes - 1;
+    X.counts[0]++;
+    X.counts[2 + (size_t)d * R + a]++;
+    int r1 = X.q_to_rep[a], r2 = X.q_to_rep[b];
+    RepScalars* S1 = (RepScalars*)(P.blocks + (size_t)r1 * P.L.bytes);
+    RepScalars* S2 = (RepScalars*)(P.blocks + (size_t)r2 * P.L.bytes);
+    const int16_t* tc1 = (const int16_t*)((char*)S1 + P.L.off_tcount);
+    const int16_t* tc2 = (const int16_t*)((char*)S2 + P.L.off_tcount);
+    double t1 = X.ladder_t[a], t2 = X.ladder_t[b];
+    double sm1 = X.ladder_s[a], sm2 = X.ladder_s[b];
+    double dbu_dn = 0;
+    for (int k = 0; k < nst; k++) {
+        double u1 = X.reduced_u[(size_t)r1 * nst + k] * t1 * X.rep_u[r1];
+        double u2 = X.reduced_u[(size_t)r2 * nst + k] * t2 * X.rep_u[r2];
+        dbu_dn += (u2 / t2 - u1 / t1) * ((double)tc2[k + 1] - (double)tc1[k + 1]);
+    }
+    double DB = 1 / t2 - 1 / t1;
+    double DH = S2->hyb_h * t2 - S1->hyb_h * t1;
+    double Dst = S2->stack_e * t2 - S1->stack_e * t1;
+    double DBM = sm2 / t2 - sm1 / t1;
+    double DBias = 0.0 * t2 - 0.0 * t1;
+    double p = fmin(1.0, exp(DB * (DH + DBias) + DBM * Dst - dbu_dn));
+    bool accept = (p == 1) ? true : (p > exchange_uniform(P, rng_ctr, idx));
+    if (accept) {
+        X.counts[1]++;
+        X.counts[2 + 2 * (size_t)R + (size_t)d * R + a]++;
+        X.q_to_rep[a] = r2;
+        X.q_to_rep[b] = r1;
+        X.rep_slot[r2] = a;
+        X.rep_slot[r1] = b;
+    }
+}
+
+// One exchange round of the PT drivers: OneDPTGCMCSimulation::
+// attempt_exchange (ptmc_simulation.cpp:360-412; swap_i picks the even or
+// odd neighbour pairs) or TwoDPTGCMCSimulation::attempt_exchange (495-560;
+// swap_i % 4 picks even or odd rows of temperature neighbours, or even or
+// odd columns of stacking neighbours, in the grid of pt_dim2 columns), then
+// master_send (212-226): with UT, HUT and 2D every replica takes its new
+// slot's chemical-potential multiplier.  rng_ctr is the round's
+// exchange-stream counter.
+DEV void exchange_round(const Params& P, const XState& X, int64_t swap_i, int64_t rng_ctr) {
+    int R = P.n_replicas;
     uint32_t idx = 0;
-    for (int i = (int)(swap_i % 2); i < R - 1; i += 2) {
-        X.counts[0]++;
-        int r1 = X.q_to_rep[i], r2 = X.q_to_rep[i + 1];
-        RepScalars* S1 = (RepScalars*)(P.blocks + (size_t)r1 * P.L.bytes);
-        RepScalars* S2 = (RepScalars*)(P.blocks + (size_t)r2 * P.L.bytes);
-        const int16_t* tc1 = (const int16_t*)((char*)S1 + P.L.off_tcount);
-        const int16_t* tc2 = (const int16_t*)((char*)S2 + P.L.off_tcount);
-        double t1 = X.ladder_t[i], t2 = X.ladder_t[i + 1];
-        double sm1 = X.ladder_s[i], sm2 = X.ladder_s[i + 1];
-        double dbu_dn = 0;
-        for (int k = 0; k < nst; k++) {
-            double u1 = X.reduced_u[(size_t)r1 * nst + k] * t1 * X.ladder_u[i];
-            double u2 = X.reduced_u[(size_t)r2 * nst + k] * t2 * X.ladder_u[i + 1];
-            dbu_dn += (u2 / t2 - u1 / t1) * ((double)tc2[k + 1] - (double)tc1[k + 1]);
+    if (P.exchange_mode == LDO_EXCHANGE_2D) {
+        int v2 = P.pt_dim2, v1 = R / v2;
+        int set = (int)(swap_i % 4);
+        bool temps = set % 2 == 0; // sets 0 and 2 pair temperatures
+        int i0 = set == 2 ? 1 : 0, j0 = set == 3 ? 1 : 0;
+        int di = temps ? 2 : 1, dj = temps ? 1 : 2;
+        int i1 = temps ? v1 - 1 : v1, j1 = temps ? v2 : v2 - 1;
+        int step = temps ? v2 : 1;
+        for (int i = i0; i < i1; i += di) {
+            for (int j = j0; j < j1; j += dj) {
+                int a = i * v2 + j;
+                exchange_pair(P, X, a, a + step, temps ? 0 : 1, rng_ctr, idx);
+            }
         }
-        double DB = 1 / t2 - 1 / t1;
-        double DH = S2->hyb_h * t2 - S1->hyb_h * t1;
-        double Dst = S2->stack_e * t2 - S1->stack_e * t1;
-        double DBM = sm2 / t2 - sm1 / t1;
-        double DBias = 0.0 * t2 - 0.0 * t1;
-        double p = fmin(1.0, exp(DB * (DH + DBias) + DBM * Dst - dbu_dn));
-        bool accept = (p == 1) ? true : (p > exchange_uniform(P, rng_ctr, idx));
-        if (accept) {
-            X.counts[1]++;
-            X.q_to_rep[i] = r2;
-            X.q_to_rep[i + 1] = r1;
-            X.rep_slot[r2] = i;
-            X.rep_slot[r1] = i + 1;
+    }
+    else {
+        for (int i = (int)(swap_i % 2); i < R - 1; i += 2) {
+            exchange_pair(P, X, i, i + 1, 0, rng_ctr, idx);
+        }
+    }
+    int m = P.exchange_mode;
+    if (m == LDO_EXCHANGE_TEMPERATURE || m == LDO_EXCHANGE_HUT || m == LDO_EXCHANGE_2D) {
+        for (int q = 0; q < R; q++) {
+            X.rep_u[X.q_to_rep[q]] = X.ladder_u[q];
         }
     }
 }
@@ -232,6 +279,7 @@ DEV void exchange_window_group(const Params& P, const XState& X, int64_t swap_i,
                                int64_t rng_ctr, int g0, int g1, uint32_t& idx) {
     for (int i = g0 + (int)(swap_i % 2); i < g1 - 1; i += 2) {
         X.counts[0]++;
+        X.counts[2 + i]++;
         const ldo_window& w1 = P.windows[i];
         const ldo_window& w2 = P.windows[i + 1];
         int r1 = X.q_to_rep[i], r2 = X.q_to_rep[i + 1];
@@ -249,6 +297,7 @@ DEV void exchange_window_group(const Params& P, const XState& X, int64_t swap_i,
         }
         if (accept) {
             X.counts[1]++;
+            X.counts[2 + 2 * (size_t)P.n_replicas + i]++;
             X.q_to_rep[i] = r2;
             X.q_to_rep[i + 1] = r1;
             X.rep_slot[r2] = i;
@@ -257,17 +306,71 @@ DEV void exchange_window_group(const Params& P, const XState& X, int64_t swap_i,
     }
 }
 
+// Swap of the replicas in window w of two ladders at neighbouring
+// temperatures (slots a < b, LDO_FLAG_WINDOW_TEMPERATURE_SWAPS): the PTMC log
+// ratio of the two slots plus the grid biases of both windows at both
+// replicas' order parameters; the windows' restraints are equal and cancel.
+DEV void exchange_window_temperatures(const Params& P, const XState& X, int a, int b,
+                                      int64_t rng_ctr, uint32_t& idx) {
+    int R = P.n_replicas;
+    int nst = P.L.ntypes - 1;
+    X.counts[0]++;
+    X.counts[2 + (size_t)R + a]++;
+    int r1 = X.q_to_rep[a], r2 = X.q_to_rep[b];
+    const RepScalars* S1 = (const RepScalars*)(P.blocks + (size_t)r1 * P.L.bytes);
+    const RepScalars* S2 = (const RepScalars*)(P.blocks + (size_t)r2 * P.L.bytes);
+    const int16_t* tc1 = (const int16_t*)((const char*)S1 + P.L.off_tcount);
+    const int16_t* tc2 = (const int16_t*)((const char*)S2 + P.L.off_tcount);
+    double t1 = X.ladder_t[a], t2 = X.ladder_t[b];
+    double sm1 = X.ladder_s[a], sm2 = X.ladder_s[b];
+    double dbu_dn = 0;
+    for (int k = 0; k < nst; k++) {
+        double u1 = X.reduced_u[(size_t)r1 * nst + k] * t1 * X.rep_u[r1];
+        double u2 = X.reduced_u[(size_t)r2 * nst + k] * t2 * X.rep_u[r2];
+        dbu_dn += (u2 / t2 - u1 / t1) * ((double)tc2[k + 1] - (double)tc1[k + 1]);
+    }
+    double DB = 1 / t2 - 1 / t1;
+    double DH = S2->hyb_h * t2 - S1->hyb_h * t1;
+    double Dst = S2->stack_e * t2 - S1->stack_e * t1;
+    double DBM = sm2 / t2 - sm1 / t1;
+    int p1 = replica_window_op(P, r1, P.windows[a]);
+    int p2 = replica_window_op(P, r2, P.windows[b]);
+    double dg = (exchange_grid(P, a, p1) - exchange_grid(P, b, p1)) +
+                (exchange_grid(P, b, p2) - exchange_grid(P, a, p2));
+    double p = fmin(1.0, exp(DB * DH + DBM * Dst - dbu_dn + P.windows[a].bias_mult * dg));
+    bool accept = (p == 1) ? true : (p > exchange_uniform(P, rng_ctr, idx));
+    if (accept) {
+        X.counts[1]++;
+        X.counts[2 + 3 * (size_t)R + a]++;
+        X.q_to_rep[a] = r2;
+        X.q_to_rep[b] = r1;
+        X.rep_slot[r2] = a;
+        X.rep_slot[r1] = b;
+    }
+}
+
 // One exchange round over every window ladder.
 DEV void exchange_windows_round(const Params& P, const XState& X, int64_t swap_i,
                                 int64_t rng_ctr) {
-    int W = P.n_replicas;
+    int R = P.n_replicas;
     uint32_t idx = 0;
     // each run of consecutive slots at one temperature is a window ladder of
     // its own (one PTMWUS run of the reference), paired from its first slot
-    for (int g0 = 0, g1; g0 < W; g0 = g1) {
-        for (g1 = g0 + 1; g1 < W && X.ladder_t[g1] == X.ladder_t[g0]; g1++) {
+    int W = R;
+    for (int g0 = 0, g1; g0 < R; g0 = g1) {
+        for (g1 = g0 + 1; g1 < R && X.ladder_t[g1] == X.ladder_t[g0]; g1++) {
+        }
+        if (g0 == 0) {
+            W = g1;
         }
         exchange_window_group(P, X, swap_i, rng_ctr, g0, g1, idx);
+    }
+    if (P.flags & LDO_FLAG_WINDOW_TEMPERATURE_SWAPS) {
+        for (int g = (int)(swap_i % 2); (g + 2) * W <= R; g += 2) {
+            for (int w = 0; w < W; w++) {
+                exchange_window_temperatures(P, X, g * W + w, (g + 1) * W + w, rng_ctr, idx);
+            }
+        }
     }
 }
 

@@ -385,9 +385,10 @@ struct Ctx {
     double* d_init = nullptr;       // init_e | init_h | init_s
     uint64_t* d_tapes = nullptr;
     int64_t* d_tape_meta = nullptr; // off | len
-    int32_t* d_xint = nullptr;      // q_to_rep | rep_slot | status
-    double* d_xdbl = nullptr;       // ladder_t | ladder_u | ladder_s | reduced_u
-    int64_t* d_xcnt = nullptr;      // exchange attempts, accepts
+    int32_t* d_xint = nullptr;      // q_to_rep | rep_slot | status | slot_key
+    double* d_xdbl = nullptr;       // ladder_t | ladder_u | ladder_s | reduced_u | rep_u
+    int64_t* d_xcnt = nullptr;      // exchange attempts, accepts, per-pair counts [4R]
+    int n_classes = 1;              // table key classes of the ladder
     int32_t* d_cfg = nullptr;
     size_t cfg_cap = 0;
     int32_t* scratch = nullptr;       // this device's `scratch` (ldo_set_config_noalloc)
@@ -511,6 +512,11 @@ void upload_tables(Ctx& c) {
         hip_check(hipMalloc(&c.d_xdbl, xd.size() * sizeof(double)), "hipMalloc ladder");
     }
     upload(c, c.d_xdbl, xd.data(), xd.size() * sizeof(double), "copy ladder");
+    std::vector<int32_t> key;
+    c.n_classes = ldo::slot_classes(c.plan, key);
+    upload(c, c.d_xint + 3 * (size_t)n, key.data(), key.size() * sizeof(int32_t), "copy keys");
+    c.P.slot_key = c.d_xint + 3 * (size_t)n;
+    c.P.pt_dim2 = c.plan.dim2;
 }
 
 dim3 grid_of(const Ctx& c) {
@@ -717,16 +723,17 @@ int ldo_create(const ldo_system_desc* sys, const ldo_run_params* run, const ldo_
         upload(*c, c->d_type_ident, c->plan.type_ident.data(),
                c->plan.type_ident.size() * sizeof(int16_t), "copy type_ident");
         P.type_ident = c->d_type_ident;
-        upload_tables(*c);
-        hip_check(hipMalloc(&c->d_xint, 3 * (size_t)n_replicas * sizeof(int32_t)), "hipMalloc x");
-        std::vector<int32_t> xi(3 * n_replicas, 0);
+        hip_check(hipMalloc(&c->d_xint, 4 * (size_t)n_replicas * sizeof(int32_t)), "hipMalloc x");
+        std::vector<int32_t> xi(4 * n_replicas, 0);
         for (int r = 0; r < n_replicas; r++) {
             xi[r] = r;
             xi[n_replicas + r] = r;
         }
         upload(*c, c->d_xint, xi.data(), xi.size() * sizeof(int32_t), "copy x");
-        hip_check(hipMalloc(&c->d_xcnt, 2 * sizeof(int64_t)), "hipMalloc xcnt");
-        zero(*c, c->d_xcnt, 2 * sizeof(int64_t), "clear xcnt");
+        upload_tables(*c);
+        size_t xcnt_bytes = (2 + 4 * (size_t)n_replicas) * sizeof(int64_t);
+        hip_check(hipMalloc(&c->d_xcnt, xcnt_bytes), "hipMalloc xcnt");
+        zero(*c, c->d_xcnt, xcnt_bytes, "clear xcnt");
         hip_check(hipMalloc(&c->d_tape_meta, 2 * (size_t)n_replicas * sizeof(int64_t)),
                   "hipMalloc tape meta");
         c->tapes.assign(n_replicas, {});
@@ -836,12 +843,50 @@ int ldo_set_ladder(void* ctx, int32_t n, const double* temps, const double* u, c
         if (n != c->R) {
             throw AbiError {LDO_E_ARG, "ladder size must equal the replica count"};
         }
+        if (!temps || !u || !b || !s) {
+            throw AbiError {LDO_E_ARG, "null ladder array"};
+        }
         c->plan.ladder_t.assign(temps, temps + n);
         c->plan.ladder_u.assign(u, u + n);
         c->plan.ladder_b.assign(b, b + n);
         c->plan.ladder_s.assign(s, s + n);
+        c->plan.dim2 = 1;
         hip_check(hipStreamSynchronize(c->stream), "sync");
         upload_tables(*c);
+    });
+}
+
+int ldo_set_ladder_2d(void* ctx, int32_t n_temps, const double* temps, int32_t n_stacking,
+                      const double* stacking_mults, double staple_u_mult) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        ldo::set_ladder_2d(c->plan, n_temps, temps, n_stacking, stacking_mults, staple_u_mult);
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        upload_tables(*c);
+    });
+}
+
+int ldo_set_step(void* ctx, int64_t step) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        if (step < 0) {
+            throw AbiError {LDO_E_ARG, "negative step"};
+        }
+        for (int r = 0; r < c->R; r++) {
+            upload(*c, c->d_blocks + (size_t)r * c->P.L.bytes + offsetof(RepScalars, step), &step,
+                   sizeof step, "copy step");
+        }
+    });
+}
+
+int ldo_get_exchange_counts(void* ctx, int32_t n, int64_t* attempts, int64_t* accepts) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        if (n != 2 * c->R || !attempts || !accepts) {
+            throw AbiError {LDO_E_ARG, "exchange counts hold 2 values per replica"};
+        }
+        download(*c, attempts, c->d_xcnt + 2, (size_t)n * sizeof(int64_t), "copy pair counts");
+        download(*c, accepts, c->d_xcnt + 2 + n, (size_t)n * sizeof(int64_t), "copy pair counts");
     });
 }
 
@@ -1063,6 +1108,9 @@ int ldo_run(void* ctx, int64_t n_swaps, int64_t interval) {
         if (n_swaps < 0 || interval < 0) {
             throw AbiError {LDO_E_ARG, "negative swap count or interval"};
         }
+        if (c->n_classes > 64) {
+            throw AbiError {LDO_E_ARG, "a ladder holds at most 64 (temperature, stacking) pairs"};
+        }
         XState X;
         X.q_to_rep = c->d_xint;
         X.rep_slot = c->d_xint + c->R;
@@ -1070,6 +1118,7 @@ int ldo_run(void* ctx, int64_t n_swaps, int64_t interval) {
         X.ladder_u = c->d_xdbl + c->R;
         X.ladder_s = c->d_xdbl + 2 * c->R;
         X.reduced_u = c->d_xdbl + 3 * c->R;
+        X.rep_u = c->d_xdbl + 3 * c->R + c->plan.reduced_u.size();
         X.counts = c->d_xcnt;
         for (int64_t s = 0; s < n_swaps; s++) {
             int64_t rng_ctr = ++c->swap_counter;

@@ -58,6 +58,30 @@ class Sampler:
         arrs = [(C.c_double * n)(*v) for v in (temps, u or one, b or one, s or one)]
         self._check(self.lib.ldo_set_ladder(self.h, n, *arrs))
 
+    def set_step(self, step):
+        """Every replica's next MC step is step + 1."""
+        self._check(self.lib.ldo_set_step(self.h, step))
+
+    def set_ladder_2d(self, temps, stacking_mults, staple_u_mult=1.0):
+        """The temperature x stacking-multiplier grid of 2d_parallel_tempering
+        (slot i * len(stacking_mults) + j)."""
+        t = (C.c_double * len(temps))(*temps)
+        s = (C.c_double * len(stacking_mults))(*stacking_mults)
+        self._check(self.lib.ldo_set_ladder_2d(self.h, len(temps), t, len(stacking_mults), s,
+                                               staple_u_mult))
+
+    def exchange_counts(self):
+        """Per slot pair exchange (attempts, accepts), each [2][n_replicas]:
+        [d][q] is the pair of slot q with its neighbour in direction d."""
+        import numpy as np
+        n = 2 * self.n
+        att = np.zeros(n, dtype=np.int64)
+        acc = np.zeros(n, dtype=np.int64)
+        self._check(self.lib.ldo_get_exchange_counts(
+            self.h, n, att.ctypes.data_as(C.POINTER(C.c_int64)),
+            acc.ctypes.data_as(C.POINTER(C.c_int64))))
+        return att.reshape(2, self.n), acc.reshape(2, self.n)
+
     def set_windows(self, windows, grid, grid_len):
         """Umbrella windows, one per replica slot (abi.Window records) and
         their grid biases grid[w * grid_len + p]."""

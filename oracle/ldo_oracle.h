@@ -40,6 +40,10 @@ int oracle_set_windows(void* o, int32_t n, const ldo_window* windows, const doub
                        int32_t grid_len);
 int oracle_sweep(void* o, int64_t steps);
 int oracle_set_permutation(void* o, const int32_t* q_to_rep);
+int oracle_set_ladder_2d(void* o, int32_t n_temps, const double* temps, int32_t n_stacking,
+                         const double* stacking_mults, double staple_u_mult);
+int oracle_get_exchange_counts(void* o, int32_t n, int64_t* attempts, int64_t* accepts);
+int oracle_set_step(void* o, int64_t step);
 /* ldo_set_window_grid, ldo_set_histogram, ldo_get_histogram */
 int oracle_set_window_grid(void* o, const double* grid);
 int oracle_set_histogram(void* o, int32_t on);

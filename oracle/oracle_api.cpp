@@ -2,6 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY (see ldo_oracle.h).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -21,6 +22,11 @@ struct Oracle {
     std::vector<std::vector<std::string>> seqs;
     std::vector<std::unique_ptr<Replica>> reps;
     std::vector<double> ladder_t, ladder_u, ladder_b, ladder_s;
+    int dim2 {1}; // columns (stacking multipliers) of a 2D grid
+    // each replica's m_replica_control_qs: temperature, staple chemical
+    // potential multiplier, bias multiplier, stacking multiplier
+    std::vector<std::array<double, 4>> ctrl;
+    std::vector<int64_t> pair_att, pair_acc; // [2R]: direction d, lower slot q
     std::vector<int> q_to_rep;
     int rng_kind {0};
     uint64_t seed {0};
@@ -133,8 +139,12 @@ void* oracle_create(const ldo_system_desc* sd, const ldo_run_params* run,
             }
             moves.push_back(e);
         }
-        if (run->flags & ~LDO_FLAG_CTCB_SCAFFOLD_ENDPOINTS) {
+        if (run->flags & ~(LDO_FLAG_CTCB_SCAFFOLD_ENDPOINTS | LDO_FLAG_WINDOW_TEMPERATURE_SWAPS)) {
             throw OrigamiError(LDO_E_ARG, "unknown ldo_run_params.flags bits");
+        }
+        if ((run->flags & LDO_FLAG_WINDOW_TEMPERATURE_SWAPS) &&
+            run->exchange_mode != LDO_EXCHANGE_WINDOWS) {
+            throw OrigamiError(LDO_E_ARG, "window temperature swaps need LDO_EXCHANGE_WINDOWS");
         }
         Limits lim {run->max_total_staples, run->max_type_staples, run->flags};
         o->lim = lim;
@@ -152,7 +162,10 @@ void* oracle_create(const ldo_system_desc* sd, const ldo_run_params* run,
             o->ladder_b.push_back(1.0);
             o->ladder_s.push_back(1.0);
             o->q_to_rep.push_back(r);
+            o->ctrl.push_back({temps[r], 1.0, 0.0, 1.0});
         }
+        o->pair_att.assign(2 * (size_t)n_replicas, 0);
+        o->pair_acc.assign(2 * (size_t)n_replicas, 0);
         o->exchange_rng.init(0, run->seed, 0xFFFFFFFFu);
         return o;
     } catch (std::exception& e) {
@@ -183,6 +196,40 @@ const char* oracle_last_error(void* o) {
     }                                                                                              \
     return LDO_OK;
 
+// OneDPTGCMCSimulation::initialize_control_qs (ptmc_simulation.cpp:337-358)
+// as a replica at slot q sees it: the bias multiplier lands in the
+// chemical-potential entry (m_bias_i == m_staple_u_mult_i,
+// ptmc_simulation.hpp:74, 80) and the bias-multiplier entry stays 0.
+// TwoDPTGCMCSimulation::initialize_control_qs (486-492) does the same with
+// its bias multipliers of 1.
+static void init_control_qs(Oracle* O) {
+    for (size_t q = 0; q < O->q_to_rep.size(); q++) {
+        int r = O->q_to_rep[q];
+        O->ctrl[r] = {O->ladder_t[q], O->ladder_b[r], 0.0, O->ladder_s[q]};
+    }
+}
+
+// control quantity i of ladder slot q (m_control_qs[i][q])
+static double control_q(const Oracle* O, int i, int q) {
+    switch (i) {
+    case 0: return O->ladder_t[q];
+    case 1: return O->ladder_u[q];
+    case 2: return O->ladder_b[q];
+    default: return O->ladder_s[q];
+    }
+}
+
+// m_exchange_q_is of each driver (ptmc_simulation.cpp:446-449, 602-650)
+static std::vector<int> exchange_q_is(int mode) {
+    switch (mode) {
+    case LDO_EXCHANGE_T: return {0};
+    case LDO_EXCHANGE_ST: return {0, 3};
+    case LDO_EXCHANGE_HUT: return {0, 1, 2};
+    case LDO_EXCHANGE_2D: return {0, 1, 3};
+    default: return {0, 1}; // UT
+    }
+}
+
 int oracle_set_ladder(void* o, int32_t n, const double* t, const double* u, const double* b,
                       const double* s) {
     ORACLE_TRY(o, {
@@ -193,6 +240,58 @@ int oracle_set_ladder(void* o, int32_t n, const double* t, const double* u, cons
         O->ladder_u.assign(u, u + n);
         O->ladder_b.assign(b, b + n);
         O->ladder_s.assign(s, s + n);
+        O->dim2 = 1;
+        init_control_qs(O);
+    })
+}
+
+// TwoDPTGCMCSimulation::initialize_control_qs (ptmc_simulation.cpp:452-493):
+// slot i * n_s + j holds temps[i] and stacking_mults[j]
+int oracle_set_ladder_2d(void* o, int32_t n_t, const double* temps, int32_t n_s,
+                         const double* stacking_mults, double staple_u_mult) {
+    ORACLE_TRY(o, {
+        if (O->exchange_mode != LDO_EXCHANGE_2D) {
+            throw OrigamiError(LDO_E_ARG, "a 2D ladder needs exchange_mode LDO_EXCHANGE_2D");
+        }
+        if (!temps || !stacking_mults || n_t < 1 || n_s < 1 ||
+            (int64_t)n_t * n_s != (int64_t)O->reps.size()) {
+            throw OrigamiError(LDO_E_ARG, "the 2D grid must hold one slot per replica");
+        }
+        O->ladder_t.clear();
+        O->ladder_u.clear();
+        O->ladder_b.clear();
+        O->ladder_s.clear();
+        for (int v1 = 0; v1 < n_t; v1++) {
+            for (int v2 = 0; v2 < n_s; v2++) {
+                O->ladder_t.push_back(temps[v1]);
+                O->ladder_u.push_back(staple_u_mult);
+                O->ladder_b.push_back(1.0);
+                O->ladder_s.push_back(stacking_mults[v2]);
+            }
+        }
+        O->dim2 = n_s;
+        init_control_qs(O);
+    })
+}
+
+int oracle_set_step(void* o, int64_t step) {
+    ORACLE_TRY(o, {
+        if (step < 0) {
+            throw OrigamiError(LDO_E_ARG, "negative step");
+        }
+        for (auto& r: O->reps) {
+            r->step = step;
+        }
+    })
+}
+
+int oracle_get_exchange_counts(void* o, int32_t n, int64_t* att, int64_t* acc) {
+    ORACLE_TRY(o, {
+        if (n != 2 * (int)O->reps.size() || !att || !acc) {
+            throw OrigamiError(LDO_E_ARG, "exchange counts hold 2 values per replica");
+        }
+        std::copy(O->pair_att.begin(), O->pair_att.end(), att);
+        std::copy(O->pair_acc.begin(), O->pair_acc.end(), acc);
     })
 }
 
@@ -298,6 +397,13 @@ int oracle_set_permutation(void* o, const int32_t* q_to_rep) {
             }
         }
         O->q_to_rep.assign(q_to_rep, q_to_rep + R);
+        // the replicas run at their new slots' temperature and stacking
+        // multiplier; their chemical-potential multipliers change at the
+        // next exchange
+        for (int q = 0; q < R; q++) {
+            O->ctrl[q_to_rep[q]][0] = O->ladder_t[q];
+            O->ctrl[q_to_rep[q]][3] = O->ladder_s[q];
+        }
     })
 }
 
@@ -320,11 +426,24 @@ int oracle_sweep(void* o, int64_t steps) {
 // reference keeps its own SystemBiases.
 static void run_windows(Oracle* O, int64_t n_swaps, int64_t interval) {
     int W = (int)O->reps.size();
+    bool t_swaps = (O->lim.flags & LDO_FLAG_WINDOW_TEMPERATURE_SWAPS) != 0;
     for (int64_t s = 0; s < n_swaps; s++) {
         int64_t rng_ctr = ++O->swap_counter;
         int64_t swap_i = rng_ctr - O->swap_base;
+        for (int q = 0; q < W; q++) {
+            Replica& rep = *O->reps[O->q_to_rep[q]];
+            // a replica moved to another temperature by the previous round
+            // switches to it (update_temp recomputes its energy)
+            if (rep.sys.temp != O->ladder_t[q]) {
+                rep.sys.update_temp(O->ladder_t[q], O->ladder_s[q]);
+                rep.sys.update_staple_us(O->ladder_t[q], 1.0);
+            }
+        }
         for (auto& rep: O->reps) {
             rep->simulate(interval);
+            if (t_swaps) {
+                rep->sys.update_enthalpy_and_entropy();
+            }
         }
         Rng& xr = (O->rng_kind == 1) ? O->reps[0]->rng : O->exchange_rng;
         if (O->rng_kind != 1) {
@@ -342,6 +461,7 @@ static void run_windows(Oracle* O, int64_t n_swaps, int64_t interval) {
         }
         for (int i: pairs) {
             O->counters.exchange_attempts++;
+            O->pair_att[i]++;
             const WindowBias& w1 = O->windows[i];
             const WindowBias& w2 = O->windows[i + 1];
             int r1 = O->q_to_rep[i], r2 = O->q_to_rep[i + 1];
@@ -364,8 +484,61 @@ static void run_windows(Oracle* O, int64_t n_swaps, int64_t interval) {
             }
             if (accepted) {
                 O->counters.exchange_accepts++;
+                O->pair_acc[i]++;
                 O->q_to_rep[i] = r2;
                 O->q_to_rep[i + 1] = r1;
+            }
+        }
+        if (t_swaps) {
+            // window w of ladders g and g + 1 (g = swap_i mod 2, + 2, ...):
+            // the PTMC acceptance of the two slots plus the windows' grid
+            // biases at both order parameters
+            int L = 1;
+            while (L < W && O->ladder_t[L] == O->ladder_t[0]) {
+                L++;
+            }
+            for (int g = (int)(swap_i % 2); (g + 2) * L <= W; g += 2) {
+                for (int w = 0; w < L; w++) {
+                    int a = g * L + w, b = (g + 1) * L + w;
+                    O->counters.exchange_attempts++;
+                    O->pair_att[(size_t)W + a]++;
+                    int r1 = O->q_to_rep[a], r2 = O->q_to_rep[b];
+                    System& s1 = O->reps[r1]->sys;
+                    System& s2 = O->reps[r2]->sys;
+                    double t1 = O->ladder_t[a], t2 = O->ladder_t[b];
+                    double sm1 = O->ladder_s[a], sm2 = O->ladder_s[b];
+                    std::vector<int> n1 = s1.staple_counts(), n2 = s2.staple_counts();
+                    double dbu_dn = 0;
+                    for (size_t k = 0; k < n1.size(); k++) {
+                        dbu_dn += (s2.staple_us[k] / t2 - s1.staple_us[k] / t1) *
+                                  ((double)n2[k] - (double)n1[k]);
+                    }
+                    double DB = 1 / t2 - 1 / t1;
+                    double DH = s2.hyb_enthalpy * t2 - s1.hyb_enthalpy * t1;
+                    double Dst = s2.stacking_energy * t2 - s1.stacking_energy * t1;
+                    double DBM = sm2 / t2 - sm1 / t1;
+                    const WindowBias& wa = O->windows[a];
+                    const WindowBias& wb = O->windows[b];
+                    // each replica's order parameter as its new slot's window reads it
+                    auto op_of = [](const System& sy, const WindowBias& x) {
+                        return x.op == LDO_OP_NUMSTAPLES ? sy.num_staples
+                                                         : sy.num_fully_bound_pairs;
+                    };
+                    int p1 = op_of(s1, wa);
+                    int p2 = op_of(s2, wb);
+                    auto g0 = [](const WindowBias& x, int p) {
+                        return (p >= 0 && p < (int)x.grid.size()) ? x.grid[p] : 0.0;
+                    };
+                    double dg = (g0(wa, p1) - g0(wb, p1)) + (g0(wb, p2) - g0(wa, p2));
+                    double p = std::min(1.0, std::exp(DB * DH + DBM * Dst - dbu_dn + wa.mult * dg));
+                    bool accepted = (p == 1) ? true : (p > xr.uniform_real());
+                    if (accepted) {
+                        O->counters.exchange_accepts++;
+                        O->pair_acc[(size_t)W + a]++;
+                        O->q_to_rep[a] = r2;
+                        O->q_to_rep[b] = r1;
+                    }
+                }
             }
         }
         assign_windows(O);
@@ -377,6 +550,28 @@ int oracle_set_windows(void* o, int32_t n, const ldo_window* windows, const doub
     ORACLE_TRY(o, {
         if (n != (int)O->reps.size() || grid_len < 0) {
             throw OrigamiError(LDO_E_ARG, "one window per replica is required");
+        }
+        if (O->lim.flags & LDO_FLAG_WINDOW_TEMPERATURE_SWAPS) {
+            // ladders of L consecutive slots at one temperature, neighbours at
+            // different ones, each holding the first ladder's windows
+            int L = 1;
+            while (L < n && O->ladder_t[L] == O->ladder_t[0]) {
+                L++;
+            }
+            bool ok = n % L == 0;
+            for (int q = 0; ok && q < n; q++) {
+                const ldo_window& a = windows[q % L];
+                const ldo_window& b = windows[q];
+                ok = O->ladder_t[q] == O->ladder_t[q - q % L] &&
+                     !(q % L == 0 && q > 0 && O->ladder_t[q] == O->ladder_t[q - 1]) &&
+                     a.op == b.op && a.min_op == b.min_op && a.max_op == b.max_op &&
+                     a.well_bias == b.well_bias && a.min_bias == b.min_bias &&
+                     a.slope == b.slope && a.off_grid_bias == b.off_grid_bias &&
+                     a.bias_mult == b.bias_mult;
+            }
+            if (!ok) {
+                throw OrigamiError(LDO_E_ARG, "window temperature swaps need equal window ladders");
+            }
         }
         O->windows.assign(n, WindowBias {});
         O->win_state.assign(n, WindowState {});
@@ -449,6 +644,10 @@ int oracle_get_histogram(void* o, int64_t* counts, int32_t reset) {
     })
 }
 
+// PTGCMCSimulation::run (ptmc_simulation.cpp:106-150) for the PT drivers:
+// update_control_qs, simulate(exchange_interval), update_dependent_qs, then
+// the exchange of OneDPTGCMCSimulation (360-412) or TwoDPTGCMCSimulation
+// (495-560) and master_send (212-226).
 int oracle_run(void* o, int64_t n_swaps, int64_t interval) {
     ORACLE_TRY(o, {
         if (O->exchange_mode == LDO_EXCHANGE_WINDOWS) {
@@ -456,20 +655,22 @@ int oracle_run(void* o, int64_t n_swaps, int64_t interval) {
             return LDO_OK;
         }
         int R = (int)O->reps.size();
-        std::vector<int> slot_of(R);
+        int mode = O->exchange_mode;
+        std::vector<int> q_is = exchange_q_is(mode);
         for (int64_t s = 0; s < n_swaps; s++) {
             int64_t rng_ctr = ++O->swap_counter;
             int64_t swap_i = rng_ctr - O->swap_base;
-            for (int q = 0; q < R; q++) {
-                slot_of[O->q_to_rep[q]] = q;
-            }
             std::vector<double> dh(R), dst(R);
             for (int r = 0; r < R; r++) {
                 Replica& rep = *O->reps[r];
-                int q = slot_of[r];
-                double temp = O->ladder_t[q];
-                rep.sys.update_temp(temp, O->ladder_s[q]);
-                rep.sys.update_staple_us(temp, O->ladder_u[q]);
+                const std::array<double, 4>& c = O->ctrl[r];
+                // update_control_qs (595-600, 651-680): T, UT and HUT call
+                // update_temp(temp), whose stacking multiplier defaults to 1
+                // (origami_system.hpp:163); HUT's update_bias_mult does
+                // nothing to the system (origami_system.hpp:165)
+                bool stack = mode == LDO_EXCHANGE_ST || mode == LDO_EXCHANGE_2D;
+                rep.sys.update_temp(c[0], stack ? c[3] : 1.0);
+                rep.sys.update_staple_us(c[0], c[1]);
                 rep.simulate(interval);
                 rep.sys.update_enthalpy_and_entropy();
                 dh[r] = rep.sys.hyb_enthalpy;
@@ -479,13 +680,15 @@ int oracle_run(void* o, int64_t n_swaps, int64_t interval) {
             if (O->rng_kind != 1) {
                 xr.begin_step(rng_ctr);
             }
-            int swap_set = (int)(swap_i % 2);
-            for (int i = swap_set; i < R - 1; i += 2) {
+            // calc_acceptance_p (275-313) and test_acceptance (255-271) for
+            // slots (q1, q2); d: direction of the pair for the counts
+            auto attempt = [&](int q1, int q2, int d) {
                 O->counters.exchange_attempts++;
-                int r1 = O->q_to_rep[i];
-                int r2 = O->q_to_rep[i + 1];
-                double t1 = O->ladder_t[i], t2 = O->ladder_t[i + 1];
-                double sm1 = O->ladder_s[i], sm2 = O->ladder_s[i + 1];
+                O->pair_att[(size_t)d * R + q1]++;
+                int r1 = O->q_to_rep[q1];
+                int r2 = O->q_to_rep[q2];
+                double t1 = control_q(O, 0, q1), t2 = control_q(O, 0, q2);
+                double sm1 = control_q(O, 3, q1), sm2 = control_q(O, 3, q2);
                 std::vector<int> n1 = O->reps[r1]->sys.staple_counts();
                 std::vector<int> n2 = O->reps[r2]->sys.staple_counts();
                 double dbu_dn = 0;
@@ -503,8 +706,39 @@ int oracle_run(void* o, int64_t n_swaps, int64_t interval) {
                 bool accept = (p == 1) ? true : (p > xr.uniform_real());
                 if (accept) {
                     O->counters.exchange_accepts++;
-                    O->q_to_rep[i] = r2;
-                    O->q_to_rep[i + 1] = r1;
+                    O->pair_acc[(size_t)d * R + q1]++;
+                    O->q_to_rep[q1] = r2;
+                    O->q_to_rep[q2] = r1;
+                }
+            };
+            if (mode == LDO_EXCHANGE_2D) {
+                // m_i_starts .. m_rep_incrs (ptmc_simulation.hpp:158-164)
+                int v1 = R / O->dim2, v2 = O->dim2;
+                const int i_starts[4] = {0, 0, 1, 0};
+                const int j_starts[4] = {0, 0, 0, 1};
+                const int i_incrs[4] = {2, 1, 2, 1};
+                const int j_incrs[4] = {1, 2, 1, 2};
+                const int i_ends[4] = {v1 - 1, v1, v1 - 1, v1};
+                const int j_ends[4] = {v2, v2 - 1, v2, v2 - 1};
+                const int rep_incrs[4] = {v2, 1, v2, 1};
+                int set = (int)(swap_i % 4);
+                int swap_v = (int)(swap_i % 2);
+                for (int i = i_starts[set]; i < i_ends[set]; i += i_incrs[set]) {
+                    for (int j = j_starts[set]; j < j_ends[set]; j += j_incrs[set]) {
+                        int rep_i = i * v2 + j;
+                        attempt(rep_i, rep_i + rep_incrs[set], swap_v);
+                    }
+                }
+            }
+            else {
+                for (int i = (int)(swap_i % 2); i < R - 1; i += 2) {
+                    attempt(i, i + 1, 0);
+                }
+            }
+            // master_send: each replica takes its slot's exchanged quantities
+            for (int q = 0; q < R; q++) {
+                for (int i: q_is) {
+                    O->ctrl[O->q_to_rep[q]][i] = control_q(O, i, q);
                 }
             }
         }

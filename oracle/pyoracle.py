@@ -170,6 +170,25 @@ class Oracle:
         arrs = [(C.c_double * n)(*v) for v in (temps, u or one, b or one, s or one)]
         _check(self.L, self.h, self.L.oracle_set_ladder(self.h, n, *arrs))
 
+    def set_step(self, step):
+        _check(self.L, self.h, self.L.oracle_set_step(self.h, step))
+
+    def set_ladder_2d(self, temps, stacking_mults, staple_u_mult=1.0):
+        t = (C.c_double * len(temps))(*temps)
+        s = (C.c_double * len(stacking_mults))(*stacking_mults)
+        _check(self.L, self.h, self.L.oracle_set_ladder_2d(
+            self.h, len(temps), t, len(stacking_mults), s, staple_u_mult))
+
+    def exchange_counts(self):
+        import numpy as np
+        n = 2 * self.n
+        att = np.zeros(n, dtype=np.int64)
+        acc = np.zeros(n, dtype=np.int64)
+        _check(self.L, self.h, self.L.oracle_get_exchange_counts(
+            self.h, n, att.ctypes.data_as(C.POINTER(C.c_int64)),
+            acc.ctypes.data_as(C.POINTER(C.c_int64))))
+        return att.reshape(2, self.n), acc.reshape(2, self.n)
+
     def observables(self, r=0):
         o = abi.Obs()
         _check(self.L, self.h, self.L.oracle_get_observables(self.h, r, C.byref(o)))

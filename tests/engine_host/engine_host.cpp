@@ -34,6 +34,8 @@ struct HostCtx {
     char* blocks = nullptr;
     std::vector<double> tab, init, xdbl;
     std::vector<int32_t> xint; // q_to_rep | rep_slot
+    std::vector<int32_t> keys; // table key class of each slot
+    int n_classes = 1;
     std::vector<int64_t> xcnt;
     std::vector<uint64_t> tapes_all;
     std::vector<int64_t> tape_meta;
@@ -80,6 +82,9 @@ void set_tables(HostCtx& c) {
     c.P.init_h = c.init.data() + n;
     c.P.init_s = c.init.data() + 2 * n;
     c.xdbl = ldo::ladder_values(c.plan);
+    c.n_classes = ldo::slot_classes(c.plan, c.keys);
+    c.P.slot_key = c.keys.data();
+    c.P.pt_dim2 = c.plan.dim2;
 }
 
 void set_tapes(HostCtx& c) {
@@ -156,7 +161,7 @@ int eh_create(const ldo_system_desc* sys, const ldo_run_params* run, const ldo_c
             c->xint[r] = r;
             c->xint[n_replicas + r] = r;
         }
-        c->xcnt.assign(2, 0);
+        c->xcnt.assign(2 + 4 * (size_t)n_replicas, 0);
         c->tapes.assign(n_replicas, {});
         set_tapes(*c);
         size_t total = (size_t)c->P.L.bytes * n_replicas;
@@ -215,7 +220,40 @@ int eh_set_ladder(void* ctx, int32_t n, const double* temps, const double* u, co
         c->plan.ladder_u.assign(u, u + n);
         c->plan.ladder_b.assign(b, b + n);
         c->plan.ladder_s.assign(s, s + n);
+        c->plan.dim2 = 1;
         set_tables(*c);
+    });
+}
+
+int eh_set_ladder_2d(void* ctx, int32_t n_temps, const double* temps, int32_t n_stacking,
+                     const double* stacking_mults, double staple_u_mult) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        ldo::set_ladder_2d(c->plan, n_temps, temps, n_stacking, stacking_mults, staple_u_mult);
+        set_tables(*c);
+    });
+}
+
+int eh_set_step(void* ctx, int64_t step) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        if (step < 0) {
+            throw ldo::AbiError {LDO_E_ARG, "negative step"};
+        }
+        for (int r = 0; r < c->R; r++) {
+            ((RepScalars*)(c->blocks + (size_t)r * c->P.L.bytes))->step = step;
+        }
+    });
+}
+
+int eh_get_exchange_counts(void* ctx, int32_t n, int64_t* attempts, int64_t* accepts) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        if (n != 2 * c->R || !attempts || !accepts) {
+            throw ldo::AbiError {LDO_E_ARG, "exchange counts hold 2 values per replica"};
+        }
+        std::memcpy(attempts, c->xcnt.data() + 2, (size_t)n * sizeof(int64_t));
+        std::memcpy(accepts, c->xcnt.data() + 2 + n, (size_t)n * sizeof(int64_t));
     });
 }
 
@@ -393,6 +431,10 @@ int eh_run(void* ctx, int64_t n_swaps, int64_t interval) {
         if (n_swaps < 0 || interval < 0) {
             throw ldo::AbiError {LDO_E_ARG, "negative swap count or interval"};
         }
+        if (c->n_classes > 64) {
+            throw ldo::AbiError {LDO_E_ARG,
+                                 "a ladder holds at most 64 (temperature, stacking) pairs"};
+        }
         XState X;
         X.q_to_rep = c->xint.data();
         X.rep_slot = c->xint.data() + c->R;
@@ -400,6 +442,7 @@ int eh_run(void* ctx, int64_t n_swaps, int64_t interval) {
         X.ladder_u = c->xdbl.data() + c->R;
         X.ladder_s = c->xdbl.data() + 2 * c->R;
         X.reduced_u = c->xdbl.data() + 3 * c->R;
+        X.rep_u = c->xdbl.data() + 3 * c->R + c->plan.reduced_u.size();
         X.counts = c->xcnt.data();
         for (int64_t s = 0; s < n_swaps; s++) {
             int64_t rng_ctr = ++c->swap_counter;

@@ -130,6 +130,64 @@ def case_ptmc_exchange_matches_oracle(sampler_cls, oracle_mod):
     assert cg.exchange_accepts == cr.exchange_accepts
 
 
+PT_LADDER = dict(temps=[320.0, 326.0, 326.0, 332.0, 338.0, 344.0],
+                 u=[1.0, 1.1, 0.9, 1.05, 1.0, 0.95],
+                 b=[1.0, 0.9, 1.1, 1.0, 1.2, 1.0],
+                 s=[1.0, 1.1, 0.9, 1.2, 1.0, 0.8])
+
+
+def _pt_pair(sampler_cls, oracle_mod, mode, system):
+    gpu = sampler_cls(system, temps=PT_LADDER["temps"], exchange_mode=mode)
+    ref = oracle_mod.Oracle(system, temps=PT_LADDER["temps"], exchange_mode=mode)
+    return gpu, ref
+
+
+def _pt_compare(gpu, ref, n, rounds, steps):
+    for _ in range(rounds):
+        gpu.run(1, steps)
+        ref.run(1, steps)
+        assert gpu.permutation() == ref.permutation()
+    for r in range(n):
+        assert_same(gpu, ref, r)
+    cg, cr = gpu.counters(), ref.counters()
+    assert (cg.exchange_attempts, cg.exchange_accepts) == (cr.exchange_attempts,
+                                                           cr.exchange_accepts)
+    ag, bg = gpu.exchange_counts()
+    ar, br = ref.exchange_counts()
+    assert ag.tolist() == ar.tolist() and bg.tolist() == br.tolist()
+    assert int(ag.sum()) == cg.exchange_attempts and int(bg.sum()) == cg.exchange_accepts
+    return cg
+
+
+def case_pt_variants_match_oracle(sampler_cls, oracle_mod):
+    """t/ut/hut/st_parallel_tempering on one ladder whose chemical-potential,
+    bias and stacking multipliers all differ from 1 (two slots share a
+    temperature), then 2d_parallel_tempering on a 2 x 3 grid."""
+    s = systems.four_domain_system()
+    L = PT_LADDER
+    for mode in (abi.EXCHANGE_T, abi.EXCHANGE_TEMPERATURE, abi.EXCHANGE_HUT, abi.EXCHANGE_ST):
+        say(f"case: four-domain system, 6-slot ladder, exchange mode {mode}, "
+            "12 swaps x 150 steps")
+        gpu, ref = _pt_pair(sampler_cls, oracle_mod, mode, s)
+        for x in (gpu, ref):
+            x.set_ladder(L["temps"], L["u"], L["b"], L["s"])
+        c = _pt_compare(gpu, ref, 6, 12, 150)
+        assert c.exchange_attempts == 6 * 2 + 6 * 3
+    say("case: four-domain system, 2D ladder 2 temperatures x 3 stacking multipliers, "
+        "12 swaps x 150 steps")
+    gpu, ref = _pt_pair(sampler_cls, oracle_mod, abi.EXCHANGE_2D, s)
+    for x in (gpu, ref):
+        x.set_ladder_2d([326.0, 334.0], [1.0, 1.15, 0.85], 1.0)
+    c = _pt_compare(gpu, ref, 6, 12, 150)
+    # sets 0/2 try 3 and 0 temperature pairs, sets 1/3 try 2 and 2 stacking pairs
+    assert c.exchange_attempts == 3 * (3 + 0 + 2 + 2)
+    with pytest.raises(abi.LdoError):
+        gpu.set_ladder_2d([320.0, 330.0], [1.0, 1.15], 1.0)
+    one = sampler_cls(s, temps=L["temps"], exchange_mode=abi.EXCHANGE_T)
+    with pytest.raises(abi.LdoError):
+        one.set_ladder_2d([320.0, 330.0], [1.0, 1.15, 0.85], 1.0)
+
+
 def _raster_windows(n):
     """One numstaples window per replica: nested LinearStepWell restraints
     [0, 2 + w] (the raster holds 0-4 staples over these runs, so every pair
@@ -280,6 +338,51 @@ def case_mwus3d_window_ladders_by_temperature(sampler_cls, oracle_mod):
     assert cg.exchange_attempts == cr.exchange_attempts == 3 * 3 + 3 * 2
     assert cg.exchange_accepts == cr.exchange_accepts > 0
     assert_counters(gpu, ref)
+
+
+def case_window_temperature_swaps(sampler_cls, oracle_mod):
+    """LDO_FLAG_WINDOW_TEMPERATURE_SWAPS: three ladders of the same three
+    windows at 318, 320 and 322 K, each slot with its own seeded grid bias;
+    every round swaps windows within the ladders, then window w between
+    neighbouring ladders."""
+    import random
+    say("case: 200-domain 3D scaffold, 3 windows x 3 temperatures with temperature-direction "
+        "swaps, 6 rounds x 500 steps")
+    s, wins, grid, glen = systems.mwus3d_system(windows=64, temps=(318.0,))
+    temps = [318.0] * 3 + [320.0] * 3 + [322.0] * 3
+    wins = [abi.Window(wins[i].op, 0, wins[i].max_op + 4 * i, 0, 0.0, 0.0, 1.0, 0.0, 1.0)
+            for i in (0, 1, 2)] * 3
+    rnd = random.Random(77)
+    grid = [rnd.uniform(-0.5, 0.5) for _ in range(9 * glen)]
+    s.temps = temps
+    s.flags = abi.FLAG_WINDOW_TEMPERATURE_SWAPS
+    gpu = sampler_cls(s, temps=temps, exchange_mode=abi.EXCHANGE_WINDOWS)
+    ref = oracle_mod.Oracle(s, temps=temps, exchange_mode=abi.EXCHANGE_WINDOWS)
+    gpu.set_windows(wins, grid, glen)
+    ref.set_windows(wins, grid, glen)
+    crossed = False
+    for _ in range(3):
+        gpu.run(2, 500)
+        ref.run(2, 500)
+        perm = gpu.permutation()
+        assert perm == ref.permutation()
+        crossed |= any(perm[q] // 3 != q // 3 for q in range(9))
+        for r in range(9):
+            assert_same(gpu, ref, r)
+            assert gpu.observables(r)["bias"] == ref.observables(r)["bias"], r
+    assert_counters(gpu, ref)
+    ag, bg = gpu.exchange_counts()
+    ar, br = ref.exchange_counts()
+    assert ag.tolist() == ar.tolist() and bg.tolist() == br.tolist()
+    # odd rounds pair ladders 1, 2 and even rounds 0, 1, window by window
+    assert ag[1].tolist() == [3, 3, 3, 3, 3, 3, 0, 0, 0]
+    assert bg[1].sum() > 0 and crossed
+    # the flag needs equal ladders: a different window in the last ladder
+    bad = list(wins)
+    bad[8] = abi.Window(wins[2].op, 0, wins[2].max_op + 1, 0, 0.0, 0.0, 1.0, 0.0, 1.0)
+    for x in (gpu, ref):
+        with pytest.raises(abi.LdoError):
+            x.set_windows(bad, grid, glen)
 
 
 def random_network(n_ident, seed, out_scale=3.0):

@@ -171,6 +171,33 @@ def exchange_system():
 
 
 PT_TEMPS = [342.0, 343.0, 344.0, 345.0]
+# staples of each type ([1, 2], [3, 4]) in the straight_configs
+STRAIGHT_TYPE_COUNTS = [(1, 1), (1, 1), (1, 0), (0, 0)]
+
+
+def reduced_chempot(system, length):
+    """mu / kT of a staple of `length` domains at staple_M molar:
+    ln(M) minus ln 6 per orientation of its (2 length - 1) free ends
+    and domains (molarity_to_chempots, origami_system.cpp:965-989)."""
+    return math.log(system.staple_M) - (2 * length - 1) * math.log(6)
+
+
+def pt_stationary(system, energies, temps, u_mults, type_counts=STRAIGHT_TYPE_COUNTS):
+    """Exact stationary distribution of the exchange chain over slot ->
+    replica permutations when slot q holds the grand-canonical ensemble at
+    temperature temps[q] and staple chemical potential u_mults[q] * mu:
+    prod_q exp(-H_perm[q] / T_q + u_q sum_k (mu_k / kT) N_k,perm[q])."""
+    import itertools
+    n = len(temps)
+    mu = [reduced_chempot(system, len(ids)) for ids in system.identities[1:]]
+
+    def lw_slot(r, q):
+        h = ptmc_enthalpy_K(system, *energies[r]) / temps[q]
+        return -h + u_mults[q] * sum(m * c for m, c in zip(mu, type_counts[r]))
+    lw = {p: sum(lw_slot(p[q], q) for q in range(n)) for p in itertools.permutations(range(n))}
+    m = max(lw.values())
+    z = sum(math.exp(v - m) for v in lw.values())
+    return {p: math.exp(v - m) / z for p, v in lw.items()}
 
 # window exchange: numfulldomains windows at one temperature; window 2
 # excludes 0 fully bound pairs, so the bare scaffold cannot enter it

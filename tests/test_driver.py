@@ -254,6 +254,32 @@ def expect_summary(ref, s):
     return "".join(log), "".join(moves)
 
 
+def expect_acceptance_freqs(ref, v1s, v2s=None):
+    """OneDPTGCMCSimulation / TwoDPTGCMCSimulation::write_acceptance_freqs
+    (ptmc_simulation.cpp:414-426, 562-592) from the oracle's per-pair
+    counts."""
+    att, acc = ref.exchange_counts()
+    out = ""
+    if v2s is None:
+        for i in range(len(v1s) - 1):
+            out += (f"{gdef(v1s[i])} {gdef(v1s[i + 1])} {acc[0][i]} {att[0][i]} "
+                    f"{ratio(acc[0][i], att[0][i])} \n")
+        return out + "\n"
+    n2 = len(v2s)
+    for i in range(len(v1s) - 1):
+        for j in range(n2):
+            q = i * n2 + j
+            out += (f"{gdef(v1s[i])} {gdef(v1s[i + 1])} {gdef(v2s[j])} {acc[0][q]} {att[0][q]} "
+                    f"{ratio(acc[0][q], att[0][q])} \n")
+    out += "\n"
+    for j in range(n2 - 1):
+        for i in range(len(v1s)):
+            q = i * n2 + j
+            out += (f"{gdef(v2s[j])} {gdef(v2s[j + 1])} {gdef(v1s[i])} {acc[1][q]} {att[1][q]} "
+                    f"{ratio(acc[1][q], att[1][q])} \n")
+    return out + "\n"
+
+
 def read(d, name):
     with open(os.path.join(d, name)) as f:
         return f.read().split("\n")
@@ -275,7 +301,16 @@ def test_driver_rejects_unsupported_inputs(tmp_path):
         pytest.skip("driver not built")
     s = systems.four_domain_system()
     for extra, msg in (({"domain_type": "Halfturn"}, "domain_type Halfturn"),
-                       ({"simulation_type": "annealing"}, "simulation_type annealing"),
+                       ({"simulation_type": "umbrella_sampling"},
+                        "simulation_type umbrella_sampling"),
+                       ({"simulation_type": "annealing", "max_temp": "340", "min_temp": "330",
+                         "temp_interval": "3"}, "Bad temperature interval"),
+                       ({"simulation_type": "2d_parallel_tempering", "num_reps": "5",
+                         "temps": "330 340", "stacking_mults": "1 1.1"},
+                        "num_reps must equal"),
+                       ({"simulation_type": "st_parallel_tempering", "num_reps": "2",
+                         "temps": "330 340", "stacking_mults": "1"},
+                        "stacking_mults must list"),
                        # biases outside the umbrella-sampling drivers: refused,
                        # not silently dropped
                        ({"bias_functions_file": "biases.json"}, "bias_functions_file"),
@@ -370,23 +405,137 @@ def test_driver_ptmc_matches_oracle(tmp_path, oracle_mod):
                               "exchange_interval": 500, "configs_output_freq": 1000,
                               "energies_output_freq": 1000, "order_params_output_freq": 1000,
                               "logging_freq": 1000})
-    run_driver(inp)
+    out = run_driver(inp).stdout
     ref = oracle_mod.Oracle(s, temps=temps, exchange_mode=abi.EXCHANGE_TEMPERATURE)
     swp = read(d, "out.swp")
     assert swp[0] == "".join(f"{g10(t)}/1/ " for t in temps)
     logs = [""] * 4
     for k in range(1, 5):
-        ref.run(2, 500)
-        assert swp[k] == "".join(f"{q} " for q in ref.permutation())
+        # each entry is written before the round's exchange
+        # (ptmc_simulation.cpp:140-147)
+        ref.run(1, 500)
+        before = ref.permutation()
+        ref.run(1, 500)
+        assert swp[k] == "".join(f"{q} " for q in before)
         for r in range(4):
             e, o, _, _ = expect_lines(ref, r, s, 1000 * k)
             assert read(d, f"out-{r}.ene")[k] == e
             assert read(d, f"out-{r}.ops")[k] == o
             logs[r] += expect_log_entry(ref, r, s, 1000 * k)
+    # the closing entry after the last exchange
+    assert swp[5:] == ["".join(f"{q} " for q in ref.permutation()), ""]
     # each rank's log (ptmc_simulation.cpp:48-56), no run summary
     for r in range(4):
         with open(os.path.join(d, f"out-{r}.out")) as f:
             assert f.read() == logs[r], r
+    assert out.endswith(expect_acceptance_freqs(ref, temps))
+
+
+PT_VARIANTS = {
+    # simulation_type: (exchange mode, exchanged control quantities)
+    "t_parallel_tempering": (abi.EXCHANGE_T, "t"),
+    "hut_parallel_tempering": (abi.EXCHANGE_HUT, "tub"),
+    "st_parallel_tempering": (abi.EXCHANGE_ST, "ts"),
+    "2d_parallel_tempering": (abi.EXCHANGE_2D, "tus"),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sim", sorted(PT_VARIANTS))
+def test_driver_pt_variants_match_oracle(tmp_path, oracle_mod, sim):
+    """t_, hut_, st_ and 2d_parallel_tempering (ptmc_simulation.cpp:428-680)
+    on the four-domain system: the swap file's header of exchanged control
+    quantities (91-104), its entries, every replica's energies and log, and
+    the acceptance frequencies on standard output, against the oracle."""
+    s = systems.four_domain_system()
+    d = str(tmp_path)
+    mode, qs = PT_VARIANTS[sim]
+    common = {"simulation_type": sim, "swaps": 8, "exchange_interval": 250,
+              "configs_output_freq": 500, "energies_output_freq": 500,
+              "logging_freq": 500, "staple_u_mult": "1"}
+    if mode == abi.EXCHANGE_2D:
+        v1s, v2s = [326.0, 334.0], [1.0, 1.15, 0.85]
+        temps = [t for t in v1s for _ in v2s]
+        ctrl = {"t": temps, "u": [1.0] * 6, "s": v2s * 2}
+        extra = {"num_reps": 6, "temps": "326 334", "stacking_mults": "1 1.15 0.85"}
+    else:
+        temps = [320.0, 326.0, 326.0, 332.0, 338.0, 344.0]
+        ctrl = {"t": temps, "u": [1.0, 1.1, 0.9, 1.05, 1.0, 0.95],
+                "b": [1.0, 0.9, 1.1, 1.0, 1.2, 1.0], "s": [1.0, 1.1, 0.9, 1.2, 1.0, 0.8]}
+        extra = {"num_reps": 6, "temps": " ".join(repr(t) for t in temps),
+                 "chem_pot_mults": " ".join(repr(x) for x in ctrl["u"]),
+                 "bias_mults": " ".join(repr(x) for x in ctrl["b"]),
+                 "stacking_mults": " ".join(repr(x) for x in ctrl["s"])}
+    out = run_driver(write_inputs(d, s, {**common, **extra})).stdout
+    ref = oracle_mod.Oracle(s, temps=temps, exchange_mode=mode)
+    if mode == abi.EXCHANGE_2D:
+        ref.set_ladder_2d(v1s, v2s, 1.0)
+    else:
+        ref.set_ladder(temps, ctrl["u"], ctrl["b"], ctrl["s"])
+    swp = read(d, "out.swp")
+    assert swp[0] == "".join("".join(f"{gdef(ctrl[k][q])}/" for k in qs) + " "
+                             for q in range(6))
+    logs = [""] * 6
+    for k in range(1, 5):
+        ref.run(1, 250)
+        before = ref.permutation()
+        ref.run(1, 250)
+        assert swp[k] == "".join(f"{q} " for q in before), k
+        for r in range(6):
+            e, o, _, _ = expect_lines(ref, r, s, 500 * k)
+            assert read(d, f"out-{r}.ene")[k] == e, (k, r)
+            logs[r] += expect_log_entry(ref, r, s, 500 * k)
+    assert swp[5:] == ["".join(f"{q} " for q in ref.permutation()), ""]
+    for r in range(6):
+        with open(os.path.join(d, f"out-{r}.out")) as f:
+            assert f.read() == logs[r], r
+    att, _ = ref.exchange_counts()
+    assert att.sum() > 0
+    if mode == abi.EXCHANGE_2D:
+        assert out.endswith(expect_acceptance_freqs(ref, v1s, v2s))
+    else:
+        assert out.endswith(expect_acceptance_freqs(ref, temps))
+
+
+@pytest.mark.gpu
+def test_driver_annealing_matches_oracle(tmp_path, oracle_mod):
+    """simulation_type annealing (AnnealingGCMCSimulation::run,
+    annealing_simulation.cpp:38-50): 340, 335 and 330 K for 1000 steps each,
+    each temperature starting at the step the previous simulate call
+    returned plus the steps before it (1..1000, 1002..2001, 3004..4003), the
+    log on standard output with a run summary after every temperature, and
+    the .moves file of the whole run."""
+    s = systems.four_domain_system()
+    d = str(tmp_path)
+    inp = write_inputs(d, s, {"simulation_type": "annealing", "max_temp": "340",
+                              "min_temp": "330", "temp_interval": "5", "steps_per_temp": 1000,
+                              "energies_output_freq": 500, "order_params_output_freq": 500,
+                              "configs_output_freq": 500, "logging_freq": 1000})
+    out = run_driver(inp).stdout
+    ref = oracle_mod.Oracle(s, temps=[340.0])
+    want_ene, want_trj, want_log = [], [], ""
+    step = 0
+    for temp in (340.0, 335.0, 330.0):
+        ref.set_ladder([temp])
+        ref.run(1, 0)
+        ref.set_step(step)
+        cur, end = step, step + 1000
+        for at in sorted(set(range((step // 500 + 1) * 500, end + 1, 500)) | {end}):
+            ref.sweep(at - cur)
+            cur = at
+            if at % 500 == 0:
+                e, _, _, trj = expect_lines(ref, 0, s, at)
+                want_ene.append(e)
+                want_trj += trj
+            if at % 1000 == 0:
+                want_log += expect_log_entry(ref, 0, s, at)
+        want_log += expect_summary(ref, s)[0]
+        step += step + 1000 + 1
+    assert read(d, "out.ene")[1:1 + len(want_ene)] == want_ene
+    assert read(d, "out.trj")[:len(want_trj)] == want_trj
+    assert out == want_log
+    with open(os.path.join(d, "out.moves")) as f:
+        assert f.read() == expect_summary(ref, s)[1]
 
 
 @pytest.mark.gpu

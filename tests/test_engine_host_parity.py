@@ -21,7 +21,7 @@ HOST_LIB = os.path.join(HOST_DIR, "_build", "libldo_engine_host.so")
 NAMES = ["create", "destroy", "last_error", "set_config", "get_config", "set_ladder", "set_tape",
          "set_windows", "sweep", "run", "get_observables", "get_counters", "get_permutation",
          "set_window_grid", "set_histogram", "get_histogram", "restart_swaps",
-         "get_move_tracking"]
+         "get_move_tracking", "set_ladder_2d", "get_exchange_counts", "set_step"]
 
 
 class HostLib:
@@ -166,3 +166,11 @@ def test_ctrg_jump_limits(sampler_cls, oracle_mod):
 
 def test_ctcb_jump_bit_exact(sampler_cls, oracle_mod):
     pc.case_ctcb_jump_bit_exact(sampler_cls, oracle_mod)
+
+
+def test_pt_variants_match_oracle(sampler_cls, oracle_mod):
+    pc.case_pt_variants_match_oracle(sampler_cls, oracle_mod)
+
+
+def test_window_temperature_swaps(sampler_cls, oracle_mod):
+    pc.case_window_temperature_swaps(sampler_cls, oracle_mod)

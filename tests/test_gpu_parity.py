@@ -33,6 +33,10 @@ def test_ptmc_exchange_matches_oracle(sampler_cls, oracle_mod):
     pc.case_ptmc_exchange_matches_oracle(sampler_cls, oracle_mod)
 
 
+def test_pt_variants_match_oracle(sampler_cls, oracle_mod):
+    pc.case_pt_variants_match_oracle(sampler_cls, oracle_mod)
+
+
 def test_windows_exchange_matches_oracle(sampler_cls, oracle_mod):
     pc.case_windows_exchange_matches_oracle(sampler_cls, oracle_mod)
 
@@ -122,3 +126,7 @@ def test_ctrg_jump_limits(sampler_cls, oracle_mod):
 
 def test_ctcb_jump_bit_exact(sampler_cls, oracle_mod):
     pc.case_ctcb_jump_bit_exact(sampler_cls, oracle_mod)
+
+
+def test_window_temperature_swaps(sampler_cls, oracle_mod):
+    pc.case_window_temperature_swaps(sampler_cls, oracle_mod)

@@ -109,6 +109,175 @@ def test_ptmc_exchange_stationary_distribution(oracle_mod):
         assert abs(m - q) <= 5 * se + 0.003, (p, m, q, se)
 
 
+def _straight_oracle(oracle_mod, mode, temps, replicas):
+    s = pc.exchange_system()
+    cfgs, _ = pc.straight_configs()
+    o = oracle_mod.Oracle(s, temps=temps, exchange_mode=mode)
+    for r, k in enumerate(replicas):
+        o.set_config(r, cfgs[k])
+    return o
+
+
+def test_pt_chemical_potential_stationary_distribution(oracle_mod):
+    """ut_parallel_tempering with per-slot chemical-potential multipliers
+    (UTPTGCMCSimulation, ptmc_simulation.cpp:628-663; the u term of
+    calc_acceptance_p, 292-300): over fixed configurations the permutation
+    visits follow the product of grand-canonical weights at each slot's
+    (T, u mu).  A dropped or sign-flipped u term, or the other slot's
+    multiplier, moves the law by far more than the bound."""
+    s = pc.exchange_system()
+    cfgs, energies = pc.straight_configs()
+    u = [1.0, 1.03, 0.97, 1.01]
+    o = _straight_oracle(oracle_mod, abi.EXCHANGE_TEMPERATURE, pc.PT_TEMPS, range(4))
+    o.set_ladder(pc.PT_TEMPS, u)
+    exact = pc.pt_stationary(s, energies, pc.PT_TEMPS, u)
+    plain = pc.pt_stationary(s, energies, pc.PT_TEMPS, [1.0] * 4)
+    assert max(abs(exact[p] - plain[p]) for p in exact) > 0.05
+    freq = _frequencies(lambda: o.run(1, 0), o.permutation, 60000)
+    assert set(freq) <= set(exact)
+    for p, q in exact.items():
+        m, se = freq.get(p, (0.0, 0.0))
+        assert abs(m - q) <= 5 * se + 0.003, (p, m, q, se)
+
+
+@pytest.mark.parametrize("mode,expect", [
+    (abi.EXCHANGE_TEMPERATURE, [[0, 2, 1], [2, 0, 1], [2, 1, 0], [2, 1, 0]]),
+    (abi.EXCHANGE_HUT, [[0, 2, 1], [2, 0, 1], [2, 1, 0], [2, 1, 0]]),
+    (abi.EXCHANGE_T, [[0, 2, 1], [2, 0, 1], [2, 1, 0], [1, 2, 0]]),
+    (abi.EXCHANGE_ST, [[0, 2, 1], [2, 0, 1], [2, 1, 0], [1, 2, 0]]),
+])
+def test_pt_chemical_potential_multiplier_known_answers(oracle_mod, mode, expect):
+    """Three slots at one temperature (only the u term acts) with slot
+    multipliers u = (1, 2, 1) and bias multipliers 1; replicas 0 and 1 hold no
+    staple, replica 2 both staples, mu / kT ~ -19.2 per staple.
+
+    Round 1 pairs slots 1, 2.  Every replica's first u is its bias
+    multiplier (initialize_control_qs writes bias_mults[i] to index m_bias_i
+    = m_staple_u_mult_i, ptmc_simulation.cpp:349-356, ptmc_simulation.hpp:74,
+    80), so the u term is 0 and the swap is certain; with the slots'
+    multipliers it would be exp(-38.4).  UT and HUT then give every replica
+    its slot's u (master_send, 212-226): round 2 (slots 0, 1) moves the
+    staples to u = 2 (accepted), round 3 swaps two empty replicas, round 4
+    would move them back to u = 1 at exp(-38.4) (rejected).  T and ST never
+    exchange u (m_exchange_q_is, 602-650), so every swap is certain."""
+    o = _straight_oracle(oracle_mod, mode, [345.0] * 3, [3, 3, 0])
+    o.set_ladder([345.0] * 3, [1.0, 2.0, 1.0])
+    got = []
+    for _ in range(4):
+        o.run(1, 0)
+        got.append(o.permutation())
+    assert got == expect
+
+
+def test_st_stacking_multiplier_known_answer(oracle_mod):
+    """st_parallel_tempering weighs the stacking energies, already scaled by
+    each slot's multiplier in the replicas' sweeps (update_temp(temp,
+    stacking_mult), ptmc_simulation.cpp:674-680; origami_potential.cpp:
+    1019-1030), by stacking_mult / T once more (calc_acceptance_p, 283-307).
+    Slots 1 and 2 at one temperature with multipliers 1 and 2 hold two copies
+    of the 3-stacked-pair configuration: the swap's log ratio is
+    (2 - 1) / T x (2 x 3 - 3) x stacking_ene = -260.9, so it is rejected,
+    while t_parallel_tempering (stacking multiplier 1 in the sweep) sees no
+    stacking difference and swaps.  Round 2 then pairs slots 0, 1 at equal
+    multipliers: certain."""
+    temps, smult = [345.0] * 3, [1.0, 1.0, 2.0]
+    for mode, expect in ((abi.EXCHANGE_ST, [[0, 1, 2], [1, 0, 2]]),
+                         (abi.EXCHANGE_T, [[0, 2, 1], [2, 0, 1]])):
+        o = _straight_oracle(oracle_mod, mode, temps, [3, 0, 0])
+        o.set_ladder(temps, s=smult)
+        got = []
+        for _ in range(2):
+            o.run(1, 0)
+            got.append(o.permutation())
+        assert got == expect, mode
+
+
+def test_2d_pairing_and_stationary_distribution(oracle_mod):
+    """2d_parallel_tempering (TwoDPTGCMCSimulation::attempt_exchange,
+    ptmc_simulation.cpp:495-560, sets of ptmc_simulation.hpp:158-164): on a
+    2 x 3 grid twelve rounds try each temperature pair of row 0 in set 0 and
+    each stacking pair of columns 0 and 1 in sets 1 and 3, three times each;
+    set 2 has no odd row to pair.  On a 2 x 2 grid with stacking multipliers
+    1 the stacking pairs (0, 1) and (2, 3) sit at one temperature and swap
+    with certainty, together, so the chain keeps to the permutations that
+    (0 2), (1 3) and (0 1)(2 3) generate (8 of the 24); its stationary law is
+    the product of the slots' Boltzmann weights conditioned on that set."""
+    s = pc.exchange_system()
+    cfgs, energies = pc.straight_configs()
+    o = oracle_mod.Oracle(s, temps=[345.0] * 6, exchange_mode=abi.EXCHANGE_2D)
+    o.set_ladder_2d([344.0, 345.0], [1.0, 1.1, 0.9])
+    o.run(12, 0)
+    att, acc = o.exchange_counts()
+    assert att.tolist() == [[3, 3, 3, 0, 0, 0], [3, 3, 0, 3, 3, 0]]
+    assert (acc <= att).all()
+    temps = [342.0, 342.0, 345.0, 345.0]
+    # two copies each of the two-staple configurations with 3 and 2 stacked
+    # pairs: the temperature swaps are accepted at odds of order 1
+    held = [0, 0, 1, 1]
+    o = _straight_oracle(oracle_mod, abi.EXCHANGE_2D, temps, held)
+    o.set_ladder_2d([342.0, 345.0], [1.0, 1.0])
+    gens = [(2, 1, 0, 3), (0, 3, 2, 1), (1, 0, 3, 2)]  # slot q takes slot g[q]'s replica
+    orbit, todo = {(0, 1, 2, 3)}, [(0, 1, 2, 3)]
+    while todo:
+        p = todo.pop()
+        for g in gens:
+            n = tuple(p[g[q]] for q in range(4))
+            if n not in orbit:
+                orbit.add(n)
+                todo.append(n)
+    assert len(orbit) == 8
+    full = pc.pt_stationary(s, [energies[k] for k in held], temps, [1.0] * 4,
+                            [pc.STRAIGHT_TYPE_COUNTS[k] for k in held])
+    z = sum(full[p] for p in orbit)
+    exact = {p: full[p] / z for p in orbit}
+    assert 0.02 < min(exact.values()) and max(exact.values()) < 0.5
+    freq = _frequencies(lambda: o.run(1, 0), o.permutation, 60000)
+    assert set(freq) <= set(exact)
+    for p, q in exact.items():
+        m, se = freq.get(p, (0.0, 0.0))
+        assert abs(m - q) <= 5 * se + 0.003, (p, m, q, se)
+
+
+def test_window_temperature_swaps_stationary_distribution(oracle_mod):
+    """LDO_FLAG_WINDOW_TEMPERATURE_SWAPS: two ladders (344 and 344.5 K) of the
+    same two numstaples windows over 0..4 staples, every slot with its own
+    grid G_q.  Over the four fixed configurations (two staples with 3 and 2
+    stacked pairs, one staple, none) the window swaps within a ladder and the swaps of a window
+    between the ladders form a chain over all 24 permutations whose
+    stationary law is prod_q exp(-H_perm[q] / T_q - G_q(op_perm[q])): a
+    dropped grid term or one with the slots transposed fails."""
+    import itertools
+    s = pc.exchange_system()
+    s.flags = abi.FLAG_WINDOW_TEMPERATURE_SWAPS
+    cfgs, energies = pc.straight_configs()
+    held = [0, 1, 2, 3]
+    temps = [344.0, 344.0, 344.5, 344.5]
+    grids = [[0.3, 0.4, -0.3, 0.0, 0.0], [-0.2, -0.5, 0.6, 0.0, 0.0],
+             [0.5, 0.9, 0.2, 0.0, 0.0], [0.0, 0.1, -0.8, 0.0, 0.0]]
+    wins = [abi.Window(abi.OP_NUMSTAPLES, 0, 4, 0, 0.0, 0.0, 1.0, 0.0, 1.0)] * 4
+    o = oracle_mod.Oracle(s, temps=temps, exchange_mode=abi.EXCHANGE_WINDOWS)
+    o.set_windows(wins, sum(grids, []), 5)
+    for r, k in enumerate(held):
+        o.set_config(r, cfgs[k])
+    ops = [energies[k][1] for k in held]
+    lw = {p: sum(-pc.ptmc_enthalpy_K(s, *energies[held[p[q]]]) / temps[q] - grids[q][ops[p[q]]]
+                 for q in range(4)) for p in itertools.permutations(range(4))}
+    m = max(lw.values())
+    z = sum(math.exp(v - m) for v in lw.values())
+    exact = {p: math.exp(v - m) / z for p, v in lw.items()}
+    assert 0.05 < max(exact.values()) < 0.5
+    freq = _frequencies(lambda: o.run(1, 0), o.permutation, 80000)
+    for p, q in exact.items():
+        m_, se = freq.get(p, (0.0, 0.0))
+        assert abs(m_ - q) <= 5 * se + 0.003, (p, m_, q, se)
+    # the slots' own grids matter: the law without the grid term is far off
+    plain = {p: sum(-pc.ptmc_enthalpy_K(s, *energies[held[p[q]]]) / temps[q]
+                    for q in range(4)) for p in itertools.permutations(range(4))}
+    mp = max(plain.values())
+    zp = sum(math.exp(v - mp) for v in plain.values())
+    assert max(abs(exact[p] - math.exp(plain[p] - mp) / zp) for p in exact) > 0.03
+
+
 def test_window_exchange_stationary_distribution(oracle_mod):
     """PTMWUSGCMCSimulation::attempt_exchange (us_simulation.cpp:770-864):
     adjacent windows swap configurations when each order parameter lies in
