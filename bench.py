@@ -486,7 +486,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed exchange intervals")
     ap.add_argument("--warmup", type=int, default=2, help="untimed exchange intervals")
-    ap.add_argument("--interval", type=int, default=1000, help="MC steps per replica per step")
+    ap.add_argument("--interval", type=int, default=None,
+                    help="MC steps per replica per step (default 1000; mwus3d: 500, its "
+                         "exchange interval)")
     ap.add_argument("--cpu-repeats", type=int, default=16,
                     help="raster: seeds per replica in the CPU baseline (each runs the GPU "
                          "line's warm-up and timed steps)")
@@ -498,8 +500,7 @@ def main():
                     choices=["raster64_ptmc32", "batch4096", "mwus3d", "learned", "hbm_copy"],
                     help="raster64_ptmc32: the 32-replica PTMC raster run; batch4096: 4096 "
                          "independent constant-T chains of 8-128 domains; mwus3d: the "
-                         "200-domain 3D scaffold, 64 umbrella windows x 4 T "
-                         "(use --interval 500)")
+                         "200-domain 3D scaffold, 64 umbrella windows x 4 T")
     ap.add_argument("--tune-steps", type=int, default=100,
                     help="batch4096: MC steps per chain in each autotuner trial")
     ap.add_argument("--batch-kernel", type=int, default=None, choices=[0, 1, 2],
@@ -525,6 +526,8 @@ def main():
     ap.add_argument("--learned-attempts", type=int, default=100,
                     help="learned: learned attempts per replica per evaluation round")
     args = ap.parse_args()
+    if args.interval is None:
+        args.interval = 500 if args.workload == "mwus3d" else 1000
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every replica of the run is resident on a single "

@@ -385,6 +385,40 @@ def case_window_temperature_swaps(sampler_cls, oracle_mod):
             x.set_windows(bad, grid, glen)
 
 
+def case_mwus3d_full_size_exchanges(sampler_cls, oracle_mod):
+    """The 3D umbrella workload at its full size: 64 windows x 4 temperatures
+    (256 replicas of the 200-domain scaffold), exchanges every 500 steps,
+    window swaps within each temperature and temperature-direction swaps.
+    The windows are widened to start at 0 fully bound pairs, so that the
+    replicas, still near the bare scaffold, lie inside their neighbours'
+    windows and the swaps are decided by seeded grid biases."""
+    import random
+    say("case: mwus3d full size, 256 replicas (64 windows x 4 T), 4 rounds x 500 steps")
+    s, wins, grid, glen = systems.mwus3d_system()
+    s.flags |= abi.FLAG_WINDOW_TEMPERATURE_SWAPS
+    glen = 8
+    wins = [abi.Window(w.op, 0, w.max_op, 0, 0.0, 0.0, 1.0, 0.0, 1.0) for w in wins]
+    rnd = random.Random(4711)
+    grid = [rnd.uniform(-1.0, 1.0) for _ in range(256 * glen)]
+    gpu = sampler_cls(s, temps=s.temps, exchange_mode=abi.EXCHANGE_WINDOWS)
+    ref = oracle_mod.Oracle(s, temps=s.temps, exchange_mode=abi.EXCHANGE_WINDOWS)
+    gpu.set_windows(wins, grid, glen)
+    ref.set_windows(wins, grid, glen)
+    for _ in range(2):
+        gpu.run(2, 500)
+        ref.run(2, 500)
+        assert gpu.permutation() == ref.permutation()
+    for r in range(256):
+        assert_same(gpu, ref, r)
+    assert_counters(gpu, ref)
+    ag, bg = gpu.exchange_counts()
+    ar, br = ref.exchange_counts()
+    assert ag.tolist() == ar.tolist() and bg.tolist() == br.tolist()
+    # both kinds of swap were tried and taken
+    assert bg[0].sum() > 0 and bg[1].sum() > 0
+    assert ag[1].sum() == 2 * 64 + 2 * 128
+
+
 def random_network(n_ident, seed, out_scale=3.0):
     """Seeded network parameters in the ldo_mlp.h layout; the output layer is
     scaled by `out_scale` (3: a proposal far from uniform)."""

@@ -174,3 +174,4 @@ def test_pt_variants_match_oracle(sampler_cls, oracle_mod):
 
 def test_window_temperature_swaps(sampler_cls, oracle_mod):
     pc.case_window_temperature_swaps(sampler_cls, oracle_mod)
+

@@ -130,3 +130,7 @@ def test_ctcb_jump_bit_exact(sampler_cls, oracle_mod):
 
 def test_window_temperature_swaps(sampler_cls, oracle_mod):
     pc.case_window_temperature_swaps(sampler_cls, oracle_mod)
+
+
+def test_mwus3d_full_size_exchanges(sampler_cls, oracle_mod):
+    pc.case_mwus3d_full_size_exchanges(sampler_cls, oracle_mod)
