@@ -6,8 +6,8 @@
 //   h1 = relu(z1); z2 = h1 W2 + b2; h2 = relu(z2); z3 = h2 W3 + b3;
 //   h3 = relu(z3); z4 = h3 W4 + b4 (36 logits), masked softmax.
 // The first layer is a gather (its input is one-hot); the products with W2,
-// W3, W4 and their backward products run on 16x16x16 bf16 MFMA tiles with
-// FP32 accumulation, one wavefront per 16x16 output tile.
+// W3, W4 and their backward products run on gfx950's 16x16x32 bf16 MFMA with
+// FP32 accumulation and LDS-staged bf16 operand tiles.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -27,7 +27,6 @@ constexpr int NO = LDO_MLP_OUT;
 constexpr int NS = LDO_MLP_SITES;
 constexpr int NE = LDO_MLP_EMBED;
 
-typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 // float -> bf16 bits, round to nearest even (finite inputs)
@@ -50,28 +49,47 @@ struct GemmArgs {
     int accumulate;
 };
 
-// v_mfma_f32_16x16x16bf16_1k operand layout: A(i, k) sits in lane
-// i + 16*(k/4), element k%4; B(k, j) in lane j + 16*(k/4), element k%4;
-// D(i, j) in lane j + 16*(i/4), element i%4.
-__global__ void __launch_bounds__(64) mlp_gemm_kernel(GemmArgs g) {
-    int lane = threadIdx.x;
-    int i0 = blockIdx.y * 16, j0 = blockIdx.x * 16;
-    int r = lane & 15, kq = (lane >> 4) * 4;
-    int i = i0 + r, j = j0 + r;
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// One 32 x 32 output tile per workgroup of four wavefronts, each owning a
+// 16 x 16 quarter computed with gfx950's v_mfma_f32_16x16x32_bf16 (32 k per
+// instruction).  Every k step stages the A tile (row i, k contiguous) and
+// the B tile (column j, k contiguous) in LDS, rounded to bf16 once, with rows
+// padded by 8 elements; a lane's operand is then one 16-byte LDS read.
+// Operand layout: A(i, k) in lane i + 16*(k/8), element k%8; B(k, j) in
+// lane j + 16*(k/8), element k%8; D(i, j) in lane j + 16*(i/4), element i%4.
+// (Only the row and column mappings matter: A and B share the k mapping, so
+// any order of k inside the instruction gives the same dot products.)
+constexpr int GT = 32;      // output tile edge
+constexpr int GK = 32;      // k per step
+constexpr int GP = GK + 8;  // padded LDS row (elements)
+
+__global__ void __launch_bounds__(256) mlp_gemm_kernel(GemmArgs g) {
+    __shared__ __attribute__((aligned(16))) unsigned short As[GT * GP];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[GT * GP];
+    int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int i0 = blockIdx.y * GT, j0 = blockIdx.x * GT;
+    int wi = (w >> 1) * 16, wj = (w & 1) * 16;
     v4f acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < g.K; k0 += 16) {
-        v4s a, b;
-        for (int t = 0; t < 4; t++) {
-            int k = k0 + kq + t;
-            float av = (i < g.M && k < g.K) ? g.A[i * g.as0 + k * g.as1] : 0.f;
-            float bv = (j < g.N && k < g.K) ? g.B[k * g.bs0 + j * g.bs1] : 0.f;
-            a[t] = to_bf16(av);
-            b[t] = to_bf16(bv);
+    for (int k0 = 0; k0 < g.K; k0 += GK) {
+        for (int e = tid; e < GT * GK; e += 256) {
+            int r = e / GK, k = e % GK;
+            int kk = k0 + k;
+            int i = i0 + r, j = j0 + r;
+            float av = (i < g.M && kk < g.K) ? g.A[i * g.as0 + kk * g.as1] : 0.f;
+            float bv = (j < g.N && kk < g.K) ? g.B[kk * g.bs0 + j * g.bs1] : 0.f;
+            As[r * GP + k] = (unsigned short)to_bf16(av);
+            Bs[r * GP + k] = (unsigned short)to_bf16(bv);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        __syncthreads();
+        v8bf a, b;
+        __builtin_memcpy(&a, &As[(wi + (lane & 15)) * GP + (lane >> 4) * 8], sizeof a);
+        __builtin_memcpy(&b, &Bs[(wj + (lane & 15)) * GP + (lane >> 4) * 8], sizeof b);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+        __syncthreads();
     }
     for (int t = 0; t < 4; t++) {
-        int ii = i0 + (lane >> 4) * 4 + t, jj = j0 + (lane & 15);
+        int ii = i0 + wi + (lane >> 4) * 4 + t, jj = j0 + wj + (lane & 15);
         if (ii < g.M && jj < g.N) {
             float* c = g.C + ii * g.cs0 + jj;
             *c = g.accumulate ? *c + acc[t] : acc[t];
@@ -302,7 +320,7 @@ void fill(MlpCtx& c, float* x, int64_t n, float v) {
 void gemm(MlpCtx& c, int M, int N, int K, const float* A, int64_t as0, int64_t as1, const float* B,
           int64_t bs0, int64_t bs1, float* C, int64_t cs0, int accumulate) {
     GemmArgs g {M, N, K, A, as0, as1, B, bs0, bs1, C, cs0, accumulate};
-    hipLaunchKernelGGL(mlp_gemm_kernel, dim3((N + 15) / 16, (M + 15) / 16), dim3(64), 0,
+    hipLaunchKernelGGL(mlp_gemm_kernel, dim3((N + GT - 1) / GT, (M + GT - 1) / GT), dim3(256), 0,
                        c.stream, g);
 }
 
