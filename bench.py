@@ -356,11 +356,20 @@ def gpu_leg_learned(args):
     """The learned proposal on the PTMC raster system: collect samples from
     the plain sampler, train the network on the device, then compare
     effective samples per second of the plain and the learned form."""
+    import dataclasses
     import numpy as np
     from latticednaorigami_amd import learned, systems
     from latticednaorigami_amd.sampler import Sampler
-    s = systems.raster_system()
-    temps = s.temps
+    if args.learned_system == "four_domain":
+        # 345 K: the enumerable system's melting region, where the staples
+        # bind and unbind (P(numfulldomains) = 0.29 / 0.53 / 0.17 at 0 / 2 / 4)
+        s = systems.four_domain_system(max_total_staples=2)
+        temps = [args.learned_eval_temp or 345.0] * 32
+        n_id = 4
+    else:
+        s = systems.raster_system()
+        temps = s.temps
+        n_id = 64
     t0 = time.perf_counter()
     data, n_cfg = learned.collect_samples(s, temps, args.learned_rounds, 500,
                                           seed_steps=args.learned_seed_steps)
@@ -374,19 +383,52 @@ def gpu_leg_learned(args):
     fit = perm[max(1, n // 10):]
     split = lambda idx: tuple(a[idx] for a in data)  # noqa: E731
     t0 = time.perf_counter()
-    params, hist = learned.train(split(fit), 64, args.learned_train_steps, batch=256, lr=1e-3,
+    params, hist = learned.train(split(fit), n_id, args.learned_train_steps, batch=256, lr=1e-3,
                                  log_every=500)
     t_train = time.perf_counter() - t0
-    held = learned.mean_loss(params, 64, split(hold))
+    held = learned.mean_loss(params, n_id, split(hold))
     uniform = float(np.mean(np.log(split(hold)[2].sum(axis=1))))
     keys = ["num_fully_bound_domain_pairs", "num_staples"]
     res = {}
-    eval_temps = [args.learned_eval_temp] * len(temps)
+    eval_temp = args.learned_eval_temp
+    if eval_temp is None:
+        eval_temp = 345.0 if args.learned_system == "four_domain" else 350.0
+    eval_temps = [eval_temp] * len(temps)
+    # the reference distribution of the accuracy protocol: the exact
+    # enumeration for the four-domain system (tests/golden), else a long
+    # plain run with its own seed
+    exact = None
+    if args.learned_system == "four_domain":
+        with open(os.path.join(ROOT, "tests", "golden", "four_domain_enumeration.json")) as f:
+            exact = json.load(f)["p_by_temperature"].get(f"{eval_temp:g}")
+    t_ref = 0.0
+    if exact is not None:
+        ref_series = None
+    else:
+        ref = Sampler(dataclasses.replace(s, seed=s.seed + 7919), temps=eval_temps)
+        ref.sweep(args.learned_seed_steps)
+        t0 = time.perf_counter()
+        ref_series, _ = learned.run_series(ref, args.learned_ref_rounds, args.interval, 0,
+                                           keys[:1])
+        t_ref = time.perf_counter() - t0
+        ref.close()
+
+    def f_error(vals):
+        if exact is None:
+            return learned.free_energy_gap(vals, ref_series[keys[0]], floor=0.01)
+        v = np.concatenate([np.asarray(x) for x in vals]).astype(int)
+        p = np.bincount(v, minlength=len(exact))[:len(exact)] / len(v)
+        pe = np.asarray(exact)
+        ok = pe >= 0.01
+        if (p[ok] <= 0).any():
+            return None
+        fa, fb = -np.log(p[ok]), -np.log(pe[ok])
+        return float(np.abs((fa - fa.min()) - (fb - fb.min())).max())
     for form in ("plain", "learned"):
         smp = Sampler(s, temps=eval_temps)
         smp.sweep(args.learned_seed_steps)
         if form == "learned":
-            smp.learned_set_network(64, params, 0.1)
+            smp.learned_set_network(n_id, params, 0.1)
             la = args.learned_attempts
         else:
             la = 0
@@ -395,8 +437,22 @@ def gpu_leg_learned(args):
         series, wall = learned.run_series(smp, args.learned_eval_rounds, args.interval, la, keys)
         c = smp.counters()
         print(f"learned: {form} form evaluated in {wall:.1f} s", file=sys.stderr, flush=True)
+        # free-energy error of numfulldomains against the reference after
+        # each twentieth of the run; the time to 0.1 kT is the first
+        # checkpoint after which the error stays at or below it
+        rounds = args.learned_eval_rounds
+        marks = sorted({max(1, rounds * q // 20) for q in range(1, 21)})
+        errs = [f_error([x[:k] for x in series[keys[0]]]) for k in marks]
+        t_tol = None
+        for q, k in enumerate(marks):
+            if all(e is not None and e <= 0.1 for e in errs[q:]):
+                t_tol = wall * k / rounds
+                break
         res[form] = {
             "seconds": wall,
+            "free_energy_error_kT": errs[-1],
+            "error_curve_kT": errs,
+            "seconds_to_0.1_kT": t_tol,
             "ess_per_s": {k: learned.ess(series[k]) / wall for k in keys},
             "series": series,
             "learned_regrowth": [c.accepts[7], c.attempts[7]],
@@ -413,6 +469,9 @@ def gpu_leg_learned(args):
     for f in res.values():
         f.pop("series")
     return res, ratio, gap, {"samples": n, "configurations": n_cfg,
+                             "reference": ("exact enumeration" if exact is not None
+                                           else f"plain run of {args.learned_ref_rounds} rounds"),
+                             "reference_s": t_ref,
                              "collect_s": t_collect, "train_s": t_train,
                              "train_steps": args.learned_train_steps,
                              "first_loss": float(np.mean(hist[:20])),
@@ -515,8 +574,13 @@ def main():
                     help="learned: sample-collection rounds (500 steps each)")
     ap.add_argument("--learned-seed-steps", type=int, default=100000,
                     help="learned: plain steps before collection and before each evaluation")
-    ap.add_argument("--learned-eval-temp", type=float, default=350.0,
-                    help="learned: temperature of every replica in the evaluation")
+    ap.add_argument("--learned-eval-temp", type=float, default=None,
+                    help="learned: temperature of every replica in the evaluation (default: "
+                         "350 K on the raster, 345 K on the four-domain system)")
+    ap.add_argument("--learned-system", default="raster", choices=["raster", "four_domain"],
+                    help="learned: the 64-domain raster or the enumerable four-domain system")
+    ap.add_argument("--learned-ref-rounds", type=int, default=800,
+                    help="learned: rounds of the plain reference run of the accuracy protocol")
     ap.add_argument("--learned-train-steps", type=int, default=3000)
     ap.add_argument("--mwus3d-learned", type=int, default=0,
                     help="mwus3d: learned attempts per replica before each exchange interval "
@@ -695,9 +759,19 @@ def main_learned(args):
         "dtype": "f64 chain, bf16/fp32 network",
         "data": "synthetic",
         "config": {
-            "workload": "learned_raster64",
-            "system": (f"64-domain raster, NN; samples from 32 replicas at 300-360 K, "
-                       f"evaluation with 32 replicas at {args.learned_eval_temp} K, no exchange"),
+            "workload": ("learned_four_domain" if args.learned_system == "four_domain"
+                         else "learned_raster64"),
+            "system": ("four-domain system (Uniform), samples from and evaluation with 32 "
+                       f"replicas at {args.learned_eval_temp or 345.0} K, no exchange"
+                       if args.learned_system == "four_domain" else
+                       "64-domain raster, NN; samples from 32 replicas at 300-360 K, "
+                       f"evaluation with 32 replicas at {args.learned_eval_temp or 350.0} K, "
+                       "no exchange"),
+            "protocol": ("accuracy: free-energy error of numfulldomains (max |dF| over values "
+                         "with P >= 0.01) against the exact enumeration (four-domain system) "
+                         f"or a {args.learned_ref_rounds}-round plain run with its own seed "
+                         "(raster); time until the error stays within 0.1 kT; ESS/s over the "
+                         "whole evaluation"),
             "network": "7x7x7x13 one-hot + 16-dim identity embedding, 3 x 256 ReLU, 36 outputs",
             "fallback_cb": 0.1,
             "round": f"{args.learned_attempts} learned attempts + {args.interval} plain steps "

@@ -575,3 +575,19 @@ def test_learned_exchange_matches_enumeration(oracle_mod):
             continue
         m, se = _mean_se(counts, key, n)
         assert abs(m - p) <= 4 * se + 0.005, (key, m, p, se)
+
+
+def test_four_domain_enumeration_fixture_is_current(oracle_mod):
+    """tests/golden/four_domain_enumeration.json (the exact reference of the
+    learned proposal's accuracy protocol in bench.py) equals the oracle's
+    enumeration."""
+    import json
+    import os
+    import sys
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, golden)
+    import make_four_domain_enumeration as mk
+    with open(os.path.join(golden, "four_domain_enumeration.json")) as f:
+        table = json.load(f)["p_by_temperature"]
+    for t in mk.TEMPS:
+        assert table[f"{t:g}"] == pytest.approx(mk.exact_numfulldomains(t), rel=1e-12, abs=1e-15)
