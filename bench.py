@@ -370,8 +370,12 @@ def gpu_leg_learned(args):
         s = systems.raster_system()
         temps = s.temps
         n_id = 64
+    # collection may run more replicas than the evaluation (each takes the
+    # temperatures of the evaluation ladder in turn)
+    n_col = max(len(temps), args.learned_collect_replicas)
+    col_temps = [temps[k % len(temps)] for k in range(n_col)]
     t0 = time.perf_counter()
-    data, n_cfg = learned.collect_samples(s, temps, args.learned_rounds, 500,
+    data, n_cfg = learned.collect_samples(s, col_temps, args.learned_rounds, 500,
                                           seed_steps=args.learned_seed_steps)
     t_collect = time.perf_counter() - t0
     n = len(data[1])
@@ -572,6 +576,9 @@ def main():
                     help="mwus3d: exchange intervals in each CPU-baseline process")
     ap.add_argument("--learned-rounds", type=int, default=200,
                     help="learned: sample-collection rounds (500 steps each)")
+    ap.add_argument("--learned-collect-replicas", type=int, default=32,
+                    help="learned: replicas of the sample collection (configurations = "
+                         "replicas x rounds)")
     ap.add_argument("--learned-seed-steps", type=int, default=100000,
                     help="learned: plain steps before collection and before each evaluation")
     ap.add_argument("--learned-eval-temp", type=float, default=None,
