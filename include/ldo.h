@@ -400,6 +400,44 @@ int ldo_learned_samples(void* ctx, int32_t cap, uint8_t* channels, int32_t* iden
  * 770-864): adjacent windows swap configurations. */
 int ldo_run(void* ctx, int64_t n_swaps, int64_t exchange_interval);
 
+/* One exchange round on the replicas' current configurations: a round of
+ * PTGCMCSimulation::run (ptmc_simulation.cpp:106-150) or of
+ * PTMWUSGCMCSimulation::run_swaps (us_simulation.cpp:672-697) with an
+ * interval of 0 steps, i.e. every replica takes its slot's control values
+ * (update_control_qs), its dependent quantities are recomputed
+ * (update_dependent_qs), then attempt_exchange and master_send run with pair
+ * set swap_index (its parity; swap_index % 4 for the 2D grid) instead of the
+ * run's own round count.  Draws from the exchange stream at its next
+ * counter. */
+int ldo_exchange(void* ctx, int64_t swap_index);
+
+/* Site evaluation: domain `domain` of the chain with unique index
+ * `chain_index` in `replica` placed at pos with orientation ore, on the
+ * replica's current configuration and tables.  The domain is first taken
+ * off its site if it is placed (OrigamiSystem::unassign_domain,
+ * origami_system.cpp:375-385), then checked at the new site
+ * (check_domain_constraints, 343-355), as the CB and CTCB trial loops score
+ * a trial (cb_movetypes.cpp:58-102).  Every query sees the configuration as
+ * it was before the call; the replica is left unchanged. */
+typedef struct ldo_site_query {
+    int32_t replica;
+    int32_t chain_index; /* ldo_config.chain_index of the chain */
+    int32_t domain;      /* domain index within the chain */
+    int32_t pos[3];
+    int32_t ore[3];      /* a unit lattice vector */
+    int32_t pad;
+} ldo_site_query;
+
+typedef struct ldo_site_result {
+    double unassign_delta; /* energy change of taking the domain off (0 if unplaced) */
+    double delta;          /* energy change of the placement (kT) */
+    int32_t violated;      /* 1: the placement breaks a constraint (the reference's
+                              OrigamiMisuse), delta is then not meaningful */
+    int32_t pad;
+} ldo_site_result;
+
+int ldo_site_eval(void* ctx, const ldo_site_query* queries, int32_t n, ldo_site_result* results);
+
 /* The next exchange round pairs as the first of a run (swap_i = 1, odd
  * pairs first), as PTMWUSGCMCSimulation::run_swaps restarts swap_i every
  * iteration (us_simulation.cpp:662-706).  The exchange's random stream

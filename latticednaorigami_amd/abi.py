@@ -159,6 +159,17 @@ class Window(C.Structure):
     ]
 
 
+class SiteQuery(C.Structure):
+    """ldo_site_query: a trial placement of one domain of one replica."""
+    _fields_ = [("replica", C.c_int32), ("chain_index", C.c_int32), ("domain", C.c_int32),
+                ("pos", C.c_int32 * 3), ("ore", C.c_int32 * 3), ("pad", C.c_int32)]
+
+
+class SiteResult(C.Structure):
+    _fields_ = [("unassign_delta", C.c_double), ("delta", C.c_double),
+                ("violated", C.c_int32), ("pad", C.c_int32)]
+
+
 class Counters(C.Structure):
     _fields_ = [
         ("attempts", C.c_int64 * MAX_MOVETYPES),
@@ -203,6 +214,16 @@ def split_tracking(values, n_types, n_scaffold, max_staples):
     return out
 
 
+def site_queries(queries):
+    """(replica, chain_index, domain, pos, ore) tuples as an ldo_site_query array."""
+    arr = (SiteQuery * max(1, len(queries)))()
+    for k, (r, c, d, pos, ore) in enumerate(queries):
+        arr[k].replica, arr[k].chain_index, arr[k].domain = r, c, d
+        arr[k].pos[:] = list(pos)
+        arr[k].ore[:] = list(ore)
+    return arr
+
+
 def declare_abi(lib, prefix):
     """Attach argtypes/restype for the ldo_* (or oracle_*) entry points."""
     v = C.c_void_p
@@ -212,6 +233,8 @@ def declare_abi(lib, prefix):
         "set_ladder": (C.c_int, [v, C.c_int32, _f64p, _f64p, _f64p, _f64p]),
         "set_ladder_2d": (C.c_int, [v, C.c_int32, _f64p, C.c_int32, _f64p, C.c_double]),
         "set_step": (C.c_int, [v, C.c_int64]),
+        "exchange": (C.c_int, [v, C.c_int64]),
+        "site_eval": (C.c_int, [v, C.POINTER(SiteQuery), C.c_int32, C.POINTER(SiteResult)]),
         "get_exchange_counts": (C.c_int, [v, C.c_int32, C.POINTER(C.c_int64),
                                           C.POINTER(C.c_int64)]),
         "sweep": (C.c_int, [v, C.c_int64]),

@@ -119,6 +119,38 @@ DEV void sweep_replica(const Params& P, int r, int64_t steps, const int32_t* rep
     sweep_replica_on(e, P, r, P.blocks + (size_t)r * P.L.bytes, steps, rep_slot, par);
 }
 
+// ldo_site_eval on a replica block (the caller's copy): the domain is taken
+// off its site if placed (unassign_domain, origami_system.cpp:375-385) and
+// checked at the query's site (check_domain_constraints, :343-355).  Returns
+// false when the chain or the domain does not exist.
+DEV bool site_eval_on(const Params& P, char* blk, const ldo_site_query& q, ldo_site_result& res) {
+    Eng e;
+    e.attach(&P, blk, q.replica);
+    e.select_tables();
+    res.unassign_delta = 0;
+    res.delta = 0;
+    res.violated = 0;
+    res.pad = 0;
+    int cs = -1;
+    for (int w = 0; w < e.S->n_chains; w++) {
+        if (e.ch[e.order[w]].uid == q.chain_index) {
+            cs = e.order[w];
+        }
+    }
+    if (cs < 0 || q.domain < 0 || q.domain >= e.ch[cs].len) {
+        return false;
+    }
+    int s = e.slot(cs, q.domain);
+    if (e.dom[s].state != OCC_UNASSIGNED) {
+        res.unassign_delta = e.unassign_domain(s);
+    }
+    e.S->violated = 0;
+    res.delta = e.check_domain_constraints(s, i3(q.pos[0], q.pos[1], q.pos[2]),
+                                           unit_index(i3(q.ore[0], q.ore[1], q.ore[2])));
+    res.violated = e.S->violated ? 1 : 0;
+    return true;
+}
+
 // OrigamiSystem::update_enthalpy_and_entropy (origami_system.cpp:204-246)
 DEV void dependent_replica(const Params& P, int r) {
     Eng e;

@@ -134,6 +134,16 @@ extern "C" __global__ void ldo_exchange_kernel(Params P, XState X, int64_t swap_
     }
 }
 
+// one site query on a copy of its replica's block (ldo_site_eval)
+extern "C" __global__ void ldo_site_eval_kernel(Params P, char* blk, ldo_site_query q,
+                                                ldo_site_result* out, int32_t* found) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ldo_site_result r;
+        *found = site_eval_on(P, blk, q, r) ? 1 : 0;
+        *out = r;
+    }
+}
+
 extern "C" __global__ void ldo_status_kernel(Params P, int32_t* out) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < P.n_replicas) {
@@ -389,6 +399,8 @@ struct Ctx {
     double* d_xdbl = nullptr;       // ladder_t | ladder_u | ladder_s | reduced_u | rep_u
     int64_t* d_xcnt = nullptr;      // exchange attempts, accepts, per-pair counts [4R]
     int n_classes = 1;              // table key classes of the ladder
+    char* d_site_blk = nullptr;     // block copy of a site evaluation
+    ldo_site_result* d_site_res = nullptr;
     int32_t* d_cfg = nullptr;
     size_t cfg_cap = 0;
     int32_t* scratch = nullptr;       // this device's `scratch` (ldo_set_config_noalloc)
@@ -667,7 +679,8 @@ void free_ctx(Ctx* c) {
     }
     void* bufs[] = {c->d_blocks, c->d_type_len, c->d_type_ident, c->d_tab, c->d_init,
                     c->d_tapes, c->d_tape_meta, c->d_xint, c->d_xdbl, c->d_xcnt, c->d_cfg,
-                    c->d_windows, c->d_grid, c->d_win_stored, c->d_net, c->d_hist};
+                    c->d_windows, c->d_grid, c->d_win_stored, c->d_net, c->d_hist,
+                    c->d_site_blk, c->d_site_res};
     for (void* b: bufs) {
         if (b) {
             (void)hipFree(b);
@@ -1131,6 +1144,74 @@ int ldo_run(void* ctx, int64_t n_swaps, int64_t interval) {
             hip_check(hipGetLastError(), "launch exchange");
         }
         check_statuses(*c);
+    });
+}
+
+int ldo_exchange(void* ctx, int64_t swap_index) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        if (swap_index < 0) {
+            throw AbiError {LDO_E_ARG, "negative swap index"};
+        }
+        XState X;
+        X.q_to_rep = c->d_xint;
+        X.rep_slot = c->d_xint + c->R;
+        X.ladder_t = c->d_xdbl;
+        X.ladder_u = c->d_xdbl + c->R;
+        X.ladder_s = c->d_xdbl + 2 * c->R;
+        X.reduced_u = c->d_xdbl + 3 * c->R;
+        X.rep_u = c->d_xdbl + 3 * c->R + c->plan.reduced_u.size();
+        X.counts = c->d_xcnt;
+        int64_t rng_ctr = ++c->swap_counter;
+        // update_control_qs and update_dependent_qs: an interval of 0 steps
+        launch_sweep(*c, 0, X.rep_slot);
+        launch_timed(*c, 1, [&]() {
+            hipLaunchKernelGGL(ldo_exchange_kernel, dim3(1), dim3(64), 0, c->stream, c->P, X,
+                               swap_index, rng_ctr);
+        });
+        hip_check(hipGetLastError(), "launch exchange");
+        check_statuses(*c);
+    });
+}
+
+int ldo_site_eval(void* ctx, const ldo_site_query* q, int32_t n, ldo_site_result* res) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return guarded(c, [&]() {
+        if (n < 0 || (n > 0 && (!q || !res))) {
+            throw AbiError {LDO_E_ARG, "null queries or results"};
+        }
+        for (int32_t k = 0; k < n; k++) {
+            int a = std::abs(q[k].ore[0]) + std::abs(q[k].ore[1]) + std::abs(q[k].ore[2]);
+            if (q[k].replica < 0 || q[k].replica >= c->R || a != 1) {
+                throw AbiError {LDO_E_ARG, "query " + std::to_string(k) +
+                                               ": replica out of range or ore not a unit vector"};
+            }
+        }
+        if (n == 0) {
+            return;
+        }
+        if (c->d_site_blk == nullptr) {
+            hip_check(hipMalloc(&c->d_site_blk, c->P.L.bytes), "hipMalloc site block");
+            hip_check(hipMalloc(&c->d_site_res, sizeof(ldo_site_result) + sizeof(int32_t)),
+                      "hipMalloc site result");
+        }
+        int32_t* d_found = (int32_t*)(c->d_site_res + 1);
+        for (int32_t k = 0; k < n; k++) {
+            hip_check(hipMemcpyAsync(c->d_site_blk,
+                                     c->d_blocks + (size_t)q[k].replica * c->P.L.bytes,
+                                     c->P.L.bytes, hipMemcpyDeviceToDevice, c->stream),
+                      "copy block");
+            hipLaunchKernelGGL(ldo_site_eval_kernel, dim3(1), dim3(64), 0, c->stream, c->P,
+                               c->d_site_blk, q[k], c->d_site_res, d_found);
+            hip_check(hipGetLastError(), "launch site eval");
+            int32_t found = 0;
+            download(*c, &res[k], c->d_site_res, sizeof(ldo_site_result), "copy site result");
+            download(*c, &found, d_found, sizeof found, "copy site flag");
+            if (!found) {
+                throw AbiError {LDO_E_ARG, "query " + std::to_string(k) +
+                                               ": no such chain or domain in the replica"};
+            }
+        }
     });
 }
 

@@ -44,6 +44,8 @@ int oracle_set_ladder_2d(void* o, int32_t n_temps, const double* temps, int32_t 
                          const double* stacking_mults, double staple_u_mult);
 int oracle_get_exchange_counts(void* o, int32_t n, int64_t* attempts, int64_t* accepts);
 int oracle_set_step(void* o, int64_t step);
+int oracle_exchange(void* o, int64_t swap_index);
+int oracle_site_eval(void* o, const ldo_site_query* q, int32_t n, ldo_site_result* res);
 /* ldo_set_window_grid, ldo_set_histogram, ldo_get_histogram */
 int oracle_set_window_grid(void* o, const double* grid);
 int oracle_set_histogram(void* o, int32_t on);

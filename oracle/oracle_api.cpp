@@ -424,124 +424,138 @@ int oracle_sweep(void* o, int64_t steps) {
 // lies inside the other window, with the grid biases of both windows.  A
 // window keeps its stored bias values across a swap, as each rank of the
 // reference keeps its own SystemBiases.
-static void run_windows(Oracle* O, int64_t n_swaps, int64_t interval) {
+// The exchange step of PTMWUSGCMCSimulation::run_swaps (us_simulation.cpp:
+// 672-697, 770-864): window swaps within each temperature's ladder, then,
+// with LDO_FLAG_WINDOW_TEMPERATURE_SWAPS, each window between neighbouring
+// ladders.  The replicas' dependent quantities must be current.
+static void windows_exchange(Oracle* O, int64_t swap_i, int64_t rng_ctr) {
     int W = (int)O->reps.size();
     bool t_swaps = (O->lim.flags & LDO_FLAG_WINDOW_TEMPERATURE_SWAPS) != 0;
-    for (int64_t s = 0; s < n_swaps; s++) {
-        int64_t rng_ctr = ++O->swap_counter;
-        int64_t swap_i = rng_ctr - O->swap_base;
-        for (int q = 0; q < W; q++) {
-            Replica& rep = *O->reps[O->q_to_rep[q]];
-            // a replica moved to another temperature by the previous round
-            // switches to it (update_temp recomputes its energy)
-            if (rep.sys.temp != O->ladder_t[q]) {
-                rep.sys.update_temp(O->ladder_t[q], O->ladder_s[q]);
-                rep.sys.update_staple_us(O->ladder_t[q], 1.0);
-            }
+    Rng& xr = (O->rng_kind == 1) ? O->reps[0]->rng : O->exchange_rng;
+    if (O->rng_kind != 1) {
+        xr.begin_step(rng_ctr);
+    }
+    // each run of consecutive slots at one temperature is its own window
+    // ladder (one reference PTMWUS run), paired from its first slot
+    std::vector<int> pairs;
+    for (int g0 = 0, g1; g0 < W; g0 = g1) {
+        for (g1 = g0 + 1; g1 < W && O->ladder_t[g1] == O->ladder_t[g0]; g1++) {
         }
-        for (auto& rep: O->reps) {
-            rep->simulate(interval);
-            if (t_swaps) {
-                rep->sys.update_enthalpy_and_entropy();
-            }
+        for (int i = g0 + (int)(swap_i % 2); i < g1 - 1; i += 2) {
+            pairs.push_back(i);
         }
-        Rng& xr = (O->rng_kind == 1) ? O->reps[0]->rng : O->exchange_rng;
-        if (O->rng_kind != 1) {
-            xr.begin_step(rng_ctr);
+    }
+    for (int i: pairs) {
+        O->counters.exchange_attempts++;
+        O->pair_att[i]++;
+        const WindowBias& w1 = O->windows[i];
+        const WindowBias& w2 = O->windows[i + 1];
+        int r1 = O->q_to_rep[i], r2 = O->q_to_rep[i + 1];
+        int p1 = O->reps[r1]->window_op(0);
+        int p2 = O->reps[r2]->window_op(0);
+        if (p2 < w1.min_op || p2 > w1.max_op || p1 < w2.min_op || p1 > w2.max_op) {
+            continue;
         }
-        // each run of consecutive slots at one temperature is its own window
-        // ladder (one reference PTMWUS run), paired from its first slot
-        std::vector<int> pairs;
-        for (int g0 = 0, g1; g0 < W; g0 = g1) {
-            for (g1 = g0 + 1; g1 < W && O->ladder_t[g1] == O->ladder_t[g0]; g1++) {
-            }
-            for (int i = g0 + (int)(swap_i % 2); i < g1 - 1; i += 2) {
-                pairs.push_back(i);
-            }
+        bool accepted = true;
+        if (p1 != p2) {
+            // GridFloats lookups with operator[]: points without a grid
+            // value read 0
+            auto g = [](const WindowBias& w, int p) {
+                return (p >= 0 && p < (int)w.grid.size()) ? w.grid[p] : 0.0;
+            };
+            double d1 = g(w1, p1) - g(w2, p1);
+            double d2 = g(w2, p2) - g(w1, p2);
+            double p = std::min(1.0, std::exp(d1 + d2));
+            accepted = (p == 1) ? true : (p > xr.uniform_real());
         }
-        for (int i: pairs) {
-            O->counters.exchange_attempts++;
-            O->pair_att[i]++;
-            const WindowBias& w1 = O->windows[i];
-            const WindowBias& w2 = O->windows[i + 1];
-            int r1 = O->q_to_rep[i], r2 = O->q_to_rep[i + 1];
-            int p1 = O->reps[r1]->window_op(0);
-            int p2 = O->reps[r2]->window_op(0);
-            if (p2 < w1.min_op || p2 > w1.max_op || p1 < w2.min_op || p1 > w2.max_op) {
-                continue;
-            }
-            bool accepted = true;
-            if (p1 != p2) {
-                // GridFloats lookups with operator[]: points without a grid
-                // value read 0
-                auto g = [](const WindowBias& w, int p) {
-                    return (p >= 0 && p < (int)w.grid.size()) ? w.grid[p] : 0.0;
+        if (accepted) {
+            O->counters.exchange_accepts++;
+            O->pair_acc[i]++;
+            O->q_to_rep[i] = r2;
+            O->q_to_rep[i + 1] = r1;
+        }
+    }
+    if (t_swaps) {
+        // window w of ladders g and g + 1 (g = swap_i mod 2, + 2, ...):
+        // the PTMC acceptance of the two slots plus the windows' grid
+        // biases at both order parameters
+        int L = 1;
+        while (L < W && O->ladder_t[L] == O->ladder_t[0]) {
+            L++;
+        }
+        for (int g = (int)(swap_i % 2); (g + 2) * L <= W; g += 2) {
+            for (int w = 0; w < L; w++) {
+                int a = g * L + w, b = (g + 1) * L + w;
+                O->counters.exchange_attempts++;
+                O->pair_att[(size_t)W + a]++;
+                int r1 = O->q_to_rep[a], r2 = O->q_to_rep[b];
+                System& s1 = O->reps[r1]->sys;
+                System& s2 = O->reps[r2]->sys;
+                double t1 = O->ladder_t[a], t2 = O->ladder_t[b];
+                double sm1 = O->ladder_s[a], sm2 = O->ladder_s[b];
+                std::vector<int> n1 = s1.staple_counts(), n2 = s2.staple_counts();
+                double dbu_dn = 0;
+                for (size_t k = 0; k < n1.size(); k++) {
+                    dbu_dn += (s2.staple_us[k] / t2 - s1.staple_us[k] / t1) *
+                              ((double)n2[k] - (double)n1[k]);
+                }
+                double DB = 1 / t2 - 1 / t1;
+                double DH = s2.hyb_enthalpy * t2 - s1.hyb_enthalpy * t1;
+                double Dst = s2.stacking_energy * t2 - s1.stacking_energy * t1;
+                double DBM = sm2 / t2 - sm1 / t1;
+                const WindowBias& wa = O->windows[a];
+                const WindowBias& wb = O->windows[b];
+                // each replica's order parameter as its new slot's window reads it
+                auto op_of = [](const System& sy, const WindowBias& x) {
+                    return x.op == LDO_OP_NUMSTAPLES ? sy.num_staples
+                                                     : sy.num_fully_bound_pairs;
                 };
-                double d1 = g(w1, p1) - g(w2, p1);
-                double d2 = g(w2, p2) - g(w1, p2);
-                double p = std::min(1.0, std::exp(d1 + d2));
-                accepted = (p == 1) ? true : (p > xr.uniform_real());
-            }
-            if (accepted) {
-                O->counters.exchange_accepts++;
-                O->pair_acc[i]++;
-                O->q_to_rep[i] = r2;
-                O->q_to_rep[i + 1] = r1;
-            }
-        }
-        if (t_swaps) {
-            // window w of ladders g and g + 1 (g = swap_i mod 2, + 2, ...):
-            // the PTMC acceptance of the two slots plus the windows' grid
-            // biases at both order parameters
-            int L = 1;
-            while (L < W && O->ladder_t[L] == O->ladder_t[0]) {
-                L++;
-            }
-            for (int g = (int)(swap_i % 2); (g + 2) * L <= W; g += 2) {
-                for (int w = 0; w < L; w++) {
-                    int a = g * L + w, b = (g + 1) * L + w;
-                    O->counters.exchange_attempts++;
-                    O->pair_att[(size_t)W + a]++;
-                    int r1 = O->q_to_rep[a], r2 = O->q_to_rep[b];
-                    System& s1 = O->reps[r1]->sys;
-                    System& s2 = O->reps[r2]->sys;
-                    double t1 = O->ladder_t[a], t2 = O->ladder_t[b];
-                    double sm1 = O->ladder_s[a], sm2 = O->ladder_s[b];
-                    std::vector<int> n1 = s1.staple_counts(), n2 = s2.staple_counts();
-                    double dbu_dn = 0;
-                    for (size_t k = 0; k < n1.size(); k++) {
-                        dbu_dn += (s2.staple_us[k] / t2 - s1.staple_us[k] / t1) *
-                                  ((double)n2[k] - (double)n1[k]);
-                    }
-                    double DB = 1 / t2 - 1 / t1;
-                    double DH = s2.hyb_enthalpy * t2 - s1.hyb_enthalpy * t1;
-                    double Dst = s2.stacking_energy * t2 - s1.stacking_energy * t1;
-                    double DBM = sm2 / t2 - sm1 / t1;
-                    const WindowBias& wa = O->windows[a];
-                    const WindowBias& wb = O->windows[b];
-                    // each replica's order parameter as its new slot's window reads it
-                    auto op_of = [](const System& sy, const WindowBias& x) {
-                        return x.op == LDO_OP_NUMSTAPLES ? sy.num_staples
-                                                         : sy.num_fully_bound_pairs;
-                    };
-                    int p1 = op_of(s1, wa);
-                    int p2 = op_of(s2, wb);
-                    auto g0 = [](const WindowBias& x, int p) {
-                        return (p >= 0 && p < (int)x.grid.size()) ? x.grid[p] : 0.0;
-                    };
-                    double dg = (g0(wa, p1) - g0(wb, p1)) + (g0(wb, p2) - g0(wa, p2));
-                    double p = std::min(1.0, std::exp(DB * DH + DBM * Dst - dbu_dn + wa.mult * dg));
-                    bool accepted = (p == 1) ? true : (p > xr.uniform_real());
-                    if (accepted) {
-                        O->counters.exchange_accepts++;
-                        O->pair_acc[(size_t)W + a]++;
-                        O->q_to_rep[a] = r2;
-                        O->q_to_rep[b] = r1;
-                    }
+                int p1 = op_of(s1, wa);
+                int p2 = op_of(s2, wb);
+                auto g0 = [](const WindowBias& x, int p) {
+                    return (p >= 0 && p < (int)x.grid.size()) ? x.grid[p] : 0.0;
+                };
+                double dg = (g0(wa, p1) - g0(wb, p1)) + (g0(wb, p2) - g0(wa, p2));
+                double p = std::min(1.0, std::exp(DB * DH + DBM * Dst - dbu_dn + wa.mult * dg));
+                bool accepted = (p == 1) ? true : (p > xr.uniform_real());
+                if (accepted) {
+                    O->counters.exchange_accepts++;
+                    O->pair_acc[(size_t)W + a]++;
+                    O->q_to_rep[a] = r2;
+                    O->q_to_rep[b] = r1;
                 }
             }
         }
-        assign_windows(O);
+    }
+    assign_windows(O);
+}
+
+// one round of PTMWUSGCMCSimulation::run_swaps: every replica's interval
+// (a replica moved to another temperature by the previous round first
+// switches to it, update_temp recomputing its energy), then the exchange
+static void windows_round(Oracle* O, int64_t interval, int64_t swap_i, int64_t rng_ctr) {
+    int W = (int)O->reps.size();
+    bool t_swaps = (O->lim.flags & LDO_FLAG_WINDOW_TEMPERATURE_SWAPS) != 0;
+    for (int q = 0; q < W; q++) {
+        Replica& rep = *O->reps[O->q_to_rep[q]];
+        if (rep.sys.temp != O->ladder_t[q]) {
+            rep.sys.update_temp(O->ladder_t[q], O->ladder_s[q]);
+            rep.sys.update_staple_us(O->ladder_t[q], 1.0);
+        }
+    }
+    for (auto& rep: O->reps) {
+        rep->simulate(interval);
+        if (t_swaps) {
+            rep->sys.update_enthalpy_and_entropy();
+        }
+    }
+    windows_exchange(O, swap_i, rng_ctr);
+}
+
+static void run_windows(Oracle* O, int64_t n_swaps, int64_t interval) {
+    for (int64_t s = 0; s < n_swaps; s++) {
+        int64_t rng_ctr = ++O->swap_counter;
+        windows_round(O, interval, rng_ctr - O->swap_base, rng_ctr);
     }
 }
 
@@ -648,99 +662,162 @@ int oracle_get_histogram(void* o, int64_t* counts, int32_t reset) {
 // update_control_qs, simulate(exchange_interval), update_dependent_qs, then
 // the exchange of OneDPTGCMCSimulation (360-412) or TwoDPTGCMCSimulation
 // (495-560) and master_send (212-226).
+// The exchange step of the PT drivers: OneDPTGCMCSimulation (ptmc_simulation
+// .cpp:360-412) or TwoDPTGCMCSimulation (495-560) attempt_exchange, then
+// master_send (212-226).  The replicas' dependent quantities must be current.
+static void pt_exchange(Oracle* O, int64_t swap_i, int64_t rng_ctr) {
+    int R = (int)O->reps.size();
+    int mode = O->exchange_mode;
+    std::vector<int> q_is = exchange_q_is(mode);
+    Rng& xr = (O->rng_kind == 1) ? O->reps[0]->rng : O->exchange_rng;
+    if (O->rng_kind != 1) {
+        xr.begin_step(rng_ctr);
+    }
+    // calc_acceptance_p (275-313) and test_acceptance (255-271) for
+    // slots (q1, q2); d: direction of the pair for the counts
+    auto attempt = [&](int q1, int q2, int d) {
+        O->counters.exchange_attempts++;
+        O->pair_att[(size_t)d * R + q1]++;
+        int r1 = O->q_to_rep[q1];
+        int r2 = O->q_to_rep[q2];
+        double t1 = control_q(O, 0, q1), t2 = control_q(O, 0, q2);
+        double sm1 = control_q(O, 3, q1), sm2 = control_q(O, 3, q2);
+        std::vector<int> n1 = O->reps[r1]->sys.staple_counts();
+        std::vector<int> n2 = O->reps[r2]->sys.staple_counts();
+        double dbu_dn = 0;
+        for (size_t k = 0; k < n1.size(); k++) {
+            double u1 = O->reps[r1]->sys.staple_us[k];
+            double u2 = O->reps[r2]->sys.staple_us[k];
+            dbu_dn += (u2 / t2 - u1 / t1) * ((double)n2[k] - (double)n1[k]);
+        }
+        double DB = 1 / t2 - 1 / t1;
+        double DH = O->reps[r2]->sys.hyb_enthalpy * t2 - O->reps[r1]->sys.hyb_enthalpy * t1;
+        double Dst = O->reps[r2]->sys.stacking_energy * t2 - O->reps[r1]->sys.stacking_energy * t1;
+        double DBM = sm2 / t2 - sm1 / t1;
+        double DBias = 0.0 * t2 - 0.0 * t1;
+        double p = std::min(1.0, std::exp(DB * (DH + DBias) + DBM * Dst - dbu_dn));
+        bool accept = (p == 1) ? true : (p > xr.uniform_real());
+        if (accept) {
+            O->counters.exchange_accepts++;
+            O->pair_acc[(size_t)d * R + q1]++;
+            O->q_to_rep[q1] = r2;
+            O->q_to_rep[q2] = r1;
+        }
+    };
+    if (mode == LDO_EXCHANGE_2D) {
+        // m_i_starts .. m_rep_incrs (ptmc_simulation.hpp:158-164)
+        int v1 = R / O->dim2, v2 = O->dim2;
+        const int i_starts[4] = {0, 0, 1, 0};
+        const int j_starts[4] = {0, 0, 0, 1};
+        const int i_incrs[4] = {2, 1, 2, 1};
+        const int j_incrs[4] = {1, 2, 1, 2};
+        const int i_ends[4] = {v1 - 1, v1, v1 - 1, v1};
+        const int j_ends[4] = {v2, v2 - 1, v2, v2 - 1};
+        const int rep_incrs[4] = {v2, 1, v2, 1};
+        int set = (int)(swap_i % 4);
+        int swap_v = (int)(swap_i % 2);
+        for (int i = i_starts[set]; i < i_ends[set]; i += i_incrs[set]) {
+            for (int j = j_starts[set]; j < j_ends[set]; j += j_incrs[set]) {
+                int rep_i = i * v2 + j;
+                attempt(rep_i, rep_i + rep_incrs[set], swap_v);
+            }
+        }
+    }
+    else {
+        for (int i = (int)(swap_i % 2); i < R - 1; i += 2) {
+            attempt(i, i + 1, 0);
+        }
+    }
+    // master_send: each replica takes its slot's exchanged quantities
+    for (int q = 0; q < R; q++) {
+        for (int i: q_is) {
+            O->ctrl[O->q_to_rep[q]][i] = control_q(O, i, q);
+        }
+    }
+}
+
+// one round of PTGCMCSimulation::run: update_control_qs (595-600, 651-680:
+// T, UT and HUT call update_temp(temp), whose stacking multiplier defaults to
+// 1, origami_system.hpp:163; HUT's update_bias_mult does nothing to the
+// system, origami_system.hpp:165), simulate, update_dependent_qs, exchange
+static void pt_round(Oracle* O, int64_t interval, int64_t swap_i, int64_t rng_ctr) {
+    int mode = O->exchange_mode;
+    bool stack = mode == LDO_EXCHANGE_ST || mode == LDO_EXCHANGE_2D;
+    for (size_t r = 0; r < O->reps.size(); r++) {
+        Replica& rep = *O->reps[r];
+        const std::array<double, 4>& c = O->ctrl[r];
+        rep.sys.update_temp(c[0], stack ? c[3] : 1.0);
+        rep.sys.update_staple_us(c[0], c[1]);
+        rep.simulate(interval);
+        rep.sys.update_enthalpy_and_entropy();
+    }
+    pt_exchange(O, swap_i, rng_ctr);
+}
+
+// PTGCMCSimulation::run (ptmc_simulation.cpp:106-150) for the PT drivers, or
+// PTMWUSGCMCSimulation::run_swaps for umbrella windows
 int oracle_run(void* o, int64_t n_swaps, int64_t interval) {
     ORACLE_TRY(o, {
-        if (O->exchange_mode == LDO_EXCHANGE_WINDOWS) {
-            run_windows(O, n_swaps, interval);
-            return LDO_OK;
-        }
-        int R = (int)O->reps.size();
-        int mode = O->exchange_mode;
-        std::vector<int> q_is = exchange_q_is(mode);
         for (int64_t s = 0; s < n_swaps; s++) {
             int64_t rng_ctr = ++O->swap_counter;
             int64_t swap_i = rng_ctr - O->swap_base;
-            std::vector<double> dh(R), dst(R);
-            for (int r = 0; r < R; r++) {
-                Replica& rep = *O->reps[r];
-                const std::array<double, 4>& c = O->ctrl[r];
-                // update_control_qs (595-600, 651-680): T, UT and HUT call
-                // update_temp(temp), whose stacking multiplier defaults to 1
-                // (origami_system.hpp:163); HUT's update_bias_mult does
-                // nothing to the system (origami_system.hpp:165)
-                bool stack = mode == LDO_EXCHANGE_ST || mode == LDO_EXCHANGE_2D;
-                rep.sys.update_temp(c[0], stack ? c[3] : 1.0);
-                rep.sys.update_staple_us(c[0], c[1]);
-                rep.simulate(interval);
-                rep.sys.update_enthalpy_and_entropy();
-                dh[r] = rep.sys.hyb_enthalpy;
-                dst[r] = rep.sys.stacking_energy;
-            }
-            Rng& xr = (O->rng_kind == 1) ? O->reps[0]->rng : O->exchange_rng;
-            if (O->rng_kind != 1) {
-                xr.begin_step(rng_ctr);
-            }
-            // calc_acceptance_p (275-313) and test_acceptance (255-271) for
-            // slots (q1, q2); d: direction of the pair for the counts
-            auto attempt = [&](int q1, int q2, int d) {
-                O->counters.exchange_attempts++;
-                O->pair_att[(size_t)d * R + q1]++;
-                int r1 = O->q_to_rep[q1];
-                int r2 = O->q_to_rep[q2];
-                double t1 = control_q(O, 0, q1), t2 = control_q(O, 0, q2);
-                double sm1 = control_q(O, 3, q1), sm2 = control_q(O, 3, q2);
-                std::vector<int> n1 = O->reps[r1]->sys.staple_counts();
-                std::vector<int> n2 = O->reps[r2]->sys.staple_counts();
-                double dbu_dn = 0;
-                for (size_t k = 0; k < n1.size(); k++) {
-                    double u1 = O->reps[r1]->sys.staple_us[k];
-                    double u2 = O->reps[r2]->sys.staple_us[k];
-                    dbu_dn += (u2 / t2 - u1 / t1) * ((double)n2[k] - (double)n1[k]);
-                }
-                double DB = 1 / t2 - 1 / t1;
-                double DH = dh[r2] * t2 - dh[r1] * t1;
-                double Dst = dst[r2] * t2 - dst[r1] * t1;
-                double DBM = sm2 / t2 - sm1 / t1;
-                double DBias = 0.0 * t2 - 0.0 * t1;
-                double p = std::min(1.0, std::exp(DB * (DH + DBias) + DBM * Dst - dbu_dn));
-                bool accept = (p == 1) ? true : (p > xr.uniform_real());
-                if (accept) {
-                    O->counters.exchange_accepts++;
-                    O->pair_acc[(size_t)d * R + q1]++;
-                    O->q_to_rep[q1] = r2;
-                    O->q_to_rep[q2] = r1;
-                }
-            };
-            if (mode == LDO_EXCHANGE_2D) {
-                // m_i_starts .. m_rep_incrs (ptmc_simulation.hpp:158-164)
-                int v1 = R / O->dim2, v2 = O->dim2;
-                const int i_starts[4] = {0, 0, 1, 0};
-                const int j_starts[4] = {0, 0, 0, 1};
-                const int i_incrs[4] = {2, 1, 2, 1};
-                const int j_incrs[4] = {1, 2, 1, 2};
-                const int i_ends[4] = {v1 - 1, v1, v1 - 1, v1};
-                const int j_ends[4] = {v2, v2 - 1, v2, v2 - 1};
-                const int rep_incrs[4] = {v2, 1, v2, 1};
-                int set = (int)(swap_i % 4);
-                int swap_v = (int)(swap_i % 2);
-                for (int i = i_starts[set]; i < i_ends[set]; i += i_incrs[set]) {
-                    for (int j = j_starts[set]; j < j_ends[set]; j += j_incrs[set]) {
-                        int rep_i = i * v2 + j;
-                        attempt(rep_i, rep_i + rep_incrs[set], swap_v);
-                    }
-                }
+            if (O->exchange_mode == LDO_EXCHANGE_WINDOWS) {
+                windows_round(O, interval, swap_i, rng_ctr);
             }
             else {
-                for (int i = (int)(swap_i % 2); i < R - 1; i += 2) {
-                    attempt(i, i + 1, 0);
+                pt_round(O, interval, swap_i, rng_ctr);
+            }
+        }
+    })
+}
+
+// one exchange step on the current configurations (ldo_exchange)
+int oracle_exchange(void* o, int64_t swap_index) {
+    ORACLE_TRY(o, {
+        if (swap_index < 0) {
+            throw OrigamiError(LDO_E_ARG, "negative swap index");
+        }
+        // update_control_qs and update_dependent_qs: an interval of 0 steps
+        int64_t rng_ctr = ++O->swap_counter;
+        if (O->exchange_mode == LDO_EXCHANGE_WINDOWS) {
+            windows_round(O, 0, swap_index, rng_ctr);
+        }
+        else {
+            pt_round(O, 0, swap_index, rng_ctr);
+        }
+    })
+}
+
+// site evaluation (ldo_site_eval) on a system rebuilt from the replica's
+// configuration at its temperature, so the replica is untouched: the domain
+// is taken off its site if placed, then checked at the query's site
+int oracle_site_eval(void* o, const ldo_site_query* q, int32_t n, ldo_site_result* res) {
+    ORACLE_TRY(o, {
+        for (int32_t k = 0; k < n; k++) {
+            if (q[k].replica < 0 || q[k].replica >= (int)O->reps.size()) {
+                throw OrigamiError(LDO_E_ARG, "replica out of range");
+            }
+            Replica& rep = *O->reps[q[k].replica];
+            System tmp(O->sys, O->idents, O->seqs, rep.sys.snapshot(), rep.sys.temp);
+            Dom* d = nullptr;
+            for (size_t w = 0; w < tmp.chains.size(); w++) {
+                if (tmp.chain_indices[w] == q[k].chain_index && q[k].domain >= 0 &&
+                    q[k].domain < (int)tmp.chains[w].size()) {
+                    d = tmp.chains[w][q[k].domain];
                 }
             }
-            // master_send: each replica takes its slot's exchanged quantities
-            for (int q = 0; q < R; q++) {
-                for (int i: q_is) {
-                    O->ctrl[O->q_to_rep[q]][i] = control_q(O, i, q);
-                }
+            if (d == nullptr) {
+                throw OrigamiError(LDO_E_ARG, "no such chain or domain in the replica");
             }
+            ldo_site_result r {};
+            if (d->st != Occ::unassigned) {
+                r.unassign_delta = tmp.unassign_domain(*d);
+            }
+            tmp.violated = false;
+            r.delta = tmp.check_domain_constraints(*d, V3 {q[k].pos[0], q[k].pos[1], q[k].pos[2]},
+                                                   V3 {q[k].ore[0], q[k].ore[1], q[k].ore[2]});
+            r.violated = tmp.violated ? 1 : 0;
+            res[k] = r;
         }
     })
 }

@@ -170,6 +170,19 @@ class Oracle:
         arrs = [(C.c_double * n)(*v) for v in (temps, u or one, b or one, s or one)]
         _check(self.L, self.h, self.L.oracle_set_ladder(self.h, n, *arrs))
 
+    def exchange(self, swap_index):
+        """One exchange round with pair set swap_index on the current configurations."""
+        _check(self.L, self.h, self.L.oracle_exchange(self.h, swap_index))
+
+    def site_eval(self, queries):
+        """Site evaluations of (replica, chain_index, domain, pos, ore) tuples:
+        a list of (unassign_delta, delta, violated)."""
+        arr = abi.site_queries(queries)
+        out = (abi.SiteResult * max(1, len(queries)))()
+        _check(self.L, self.h, self.L.oracle_site_eval(self.h, arr, len(queries), out))
+        return [(out[k].unassign_delta, out[k].delta, out[k].violated)
+                for k in range(len(queries))]
+
     def set_step(self, step):
         _check(self.L, self.h, self.L.oracle_set_step(self.h, step))
 

@@ -10,6 +10,7 @@
 // is not a fallback of the product, which never loads it.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
@@ -458,6 +459,54 @@ int eh_run(void* ctx, int64_t n_swaps, int64_t interval) {
             }
         }
         check_statuses(*c);
+    });
+}
+
+int eh_exchange(void* ctx, int64_t swap_index) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        if (swap_index < 0) {
+            throw ldo::AbiError {LDO_E_ARG, "negative swap index"};
+        }
+        XState X;
+        X.q_to_rep = c->xint.data();
+        X.rep_slot = c->xint.data() + c->R;
+        X.ladder_t = c->xdbl.data();
+        X.ladder_u = c->xdbl.data() + c->R;
+        X.ladder_s = c->xdbl.data() + 2 * c->R;
+        X.reduced_u = c->xdbl.data() + 3 * c->R;
+        X.rep_u = c->xdbl.data() + 3 * c->R + c->plan.reduced_u.size();
+        X.counts = c->xcnt.data();
+        int64_t rng_ctr = ++c->swap_counter;
+        // update_control_qs and update_dependent_qs: an interval of 0 steps
+        for (int r = 0; r < c->R; r++) {
+            sweep_replica(c->P, r, 0, X.rep_slot);
+        }
+        if (c->P.exchange_mode == LDO_EXCHANGE_WINDOWS) {
+            exchange_windows_round(c->P, X, swap_index, rng_ctr);
+        }
+        else {
+            exchange_round(c->P, X, swap_index, rng_ctr);
+        }
+        check_statuses(*c);
+    });
+}
+
+int eh_site_eval(void* ctx, const ldo_site_query* q, int32_t n, ldo_site_result* res) {
+    HostCtx* c = static_cast<HostCtx*>(ctx);
+    return guarded(c, [&]() {
+        std::vector<char> copy(c->P.L.bytes);
+        for (int32_t k = 0; k < n; k++) {
+            int a = std::abs(q[k].ore[0]) + std::abs(q[k].ore[1]) + std::abs(q[k].ore[2]);
+            if (q[k].replica < 0 || q[k].replica >= c->R || a != 1) {
+                throw ldo::AbiError {LDO_E_ARG, "replica out of range or ore not a unit vector"};
+            }
+            std::memcpy(copy.data(), c->blocks + (size_t)q[k].replica * c->P.L.bytes,
+                        c->P.L.bytes);
+            if (!site_eval_on(c->P, copy.data(), q[k], res[k])) {
+                throw ldo::AbiError {LDO_E_ARG, "no such chain or domain in the replica"};
+            }
+        }
     });
 }
 

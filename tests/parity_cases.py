@@ -419,6 +419,82 @@ def case_mwus3d_full_size_exchanges(sampler_cls, oracle_mod):
     assert ag[1].sum() == 2 * 64 + 2 * 128
 
 
+def case_exchange_and_site_eval(sampler_cls, oracle_mod):
+    """ldo_exchange between ldo_run intervals with chosen pair sets, and
+    ldo_site_eval: trial placements of every domain of a four-domain
+    configuration with staples bound, at the sites around its chain
+    neighbour in every orientation, at its own site and at an empty one."""
+    say("case: explicit exchange rounds between PTMC intervals, 6-replica raster ladder")
+    s = systems.raster_system(replicas=6)
+    gpu = sampler_cls(s, temps=s.temps, exchange_mode=abi.EXCHANGE_TEMPERATURE)
+    ref = oracle_mod.Oracle(s, temps=s.temps, exchange_mode=abi.EXCHANGE_TEMPERATURE)
+    for k, sw in enumerate([2, 2, 1, 4, 3, 3]):
+        for x in (gpu, ref):
+            x.run(1, 200)
+            x.exchange(sw)
+        assert gpu.permutation() == ref.permutation(), k
+    for r in range(6):
+        assert_same(gpu, ref, r)
+    cg, cr = gpu.counters(), ref.counters()
+    assert (cg.exchange_attempts, cg.exchange_accepts) == (cr.exchange_attempts,
+                                                           cr.exchange_accepts)
+    # the runs' own rounds are 1, 3, ..., 11 (odd: pairs (1,2), (3,4)); the
+    # explicit ones pair 3, 3, 2, 3, 2, 2 neighbours
+    assert cg.exchange_attempts == 6 * 2 + (3 + 3 + 2 + 3 + 2 + 2)
+    with pytest.raises(abi.LdoError):
+        gpu.exchange(-1)
+
+    say("case: site evaluations on the four-domain system at 345 K")
+    s = systems.four_domain_system(max_total_staples=2)
+    gpu = sampler_cls(s, temps=[345.0] * 2)
+    ref = oracle_mod.Oracle(s, temps=[345.0] * 2)
+    for x in (gpu, ref):
+        x.sweep(3000)
+    for r in range(2):
+        assert_same(gpu, ref, r)
+    units = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    queries = []
+    for r in range(2):
+        cfg = gpu.config(r)
+        for c in cfg:
+            n = len(c.positions)
+            for d in range(n):
+                queries.append((r, c.index, d, c.positions[d], c.orientations[d]))
+                queries.append((r, c.index, d, (40 + d, 40, 40), (0, 0, 1)))
+                a = c.positions[d - 1] if d > 0 else c.positions[(d + 1) % n]
+                for u in units:
+                    site = tuple(a[i] + u[i] for i in range(3))
+                    for o in units:
+                        queries.append((r, c.index, d, site, o))
+    before = [(gpu.observables(r)["energy"], gpu.config(r)) for r in range(2)]
+    got = gpu.site_eval(queries)
+    want = ref.site_eval(queries)
+    assert len(got) == len(want) == len(queries)
+    for k, (g, w) in enumerate(zip(got, want)):
+        assert g[2] == w[2], (k, queries[k], g, w)
+        if not g[2]:
+            assert g[1] == pytest.approx(w[1], rel=ENERGY_RTOL, abs=1e-12), (k, queries[k])
+        assert g[0] == pytest.approx(w[0], rel=ENERGY_RTOL, abs=1e-12), (k, queries[k])
+    # a domain put back where it was restores the energy; an empty far site
+    # costs nothing
+    for k in range(0, len(queries), 38):
+        own, far = got[k], got[k + 1]
+        assert not own[2] and own[0] + own[1] == pytest.approx(0.0, abs=1e-9), queries[k]
+        assert far == (own[0], 0.0, 0), queries[k + 1]
+    assert any(g[2] for g in got) and any(not g[2] and g[1] < 0 for g in got)
+    # the replicas are unchanged and keep replaying the oracle
+    for r in range(2):
+        assert gpu.observables(r)["energy"] == before[r][0] and gpu.config(r) == before[r][1]
+    for x in (gpu, ref):
+        x.sweep(2000)
+    for r in range(2):
+        assert_same(gpu, ref, r)
+    with pytest.raises(abi.LdoError):
+        gpu.site_eval([(0, 99, 0, (0, 0, 0), (1, 0, 0))])
+    with pytest.raises(abi.LdoError):
+        gpu.site_eval([(0, 0, 0, (0, 0, 0), (1, 1, 0))])
+
+
 def random_network(n_ident, seed, out_scale=3.0):
     """Seeded network parameters in the ldo_mlp.h layout; the output layer is
     scaled by `out_scale` (3: a proposal far from uniform)."""

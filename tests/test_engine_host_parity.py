@@ -21,7 +21,8 @@ HOST_LIB = os.path.join(HOST_DIR, "_build", "libldo_engine_host.so")
 NAMES = ["create", "destroy", "last_error", "set_config", "get_config", "set_ladder", "set_tape",
          "set_windows", "sweep", "run", "get_observables", "get_counters", "get_permutation",
          "set_window_grid", "set_histogram", "get_histogram", "restart_swaps",
-         "get_move_tracking", "set_ladder_2d", "get_exchange_counts", "set_step"]
+         "get_move_tracking", "set_ladder_2d", "get_exchange_counts", "set_step", "exchange",
+         "site_eval"]
 
 
 class HostLib:
@@ -175,3 +176,6 @@ def test_pt_variants_match_oracle(sampler_cls, oracle_mod):
 def test_window_temperature_swaps(sampler_cls, oracle_mod):
     pc.case_window_temperature_swaps(sampler_cls, oracle_mod)
 
+
+def test_exchange_and_site_eval(sampler_cls, oracle_mod):
+    pc.case_exchange_and_site_eval(sampler_cls, oracle_mod)

@@ -134,3 +134,7 @@ def test_window_temperature_swaps(sampler_cls, oracle_mod):
 
 def test_mwus3d_full_size_exchanges(sampler_cls, oracle_mod):
     pc.case_mwus3d_full_size_exchanges(sampler_cls, oracle_mod)
+
+
+def test_exchange_and_site_eval(sampler_cls, oracle_mod):
+    pc.case_exchange_and_site_eval(sampler_cls, oracle_mod)
