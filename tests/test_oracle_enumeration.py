@@ -340,13 +340,17 @@ def test_ctrg_scaffold_regrowth_matches_enumeration(oracle_mod, misbinding_h, at
         assert abs(m - p) <= 4 * se + 0.002, (key, m, p, se)
 
 
-def test_ctrg_jump_matches_enumeration_with_hairpin_staples(oracle_mod):
-    """Non-contiguous CTRG (segment pairs grown from the stems that the
-    crossover staples of a four-domain hairpin offer) with CB staple regrowth
-    and orientation rotations samples the enumerated ensemble restricted to
-    two bound staples at 345 K.  Without staple exchange the staple count
-    stays 2, so the target is the exact distribution conditioned on it."""
+@pytest.mark.parametrize("kind", ["ctrg_jump", "ctcb_jump"])
+def test_jump_matches_enumeration_with_hairpin_staples(oracle_mod, kind):
+    """Non-contiguous CTRG and CTCB (segment pairs grown from the stems that
+    the crossover staples of a four-domain hairpin offer) with CB staple
+    regrowth and orientation rotations sample the enumerated ensemble
+    restricted to two bound staples at 345 K.  Without staple exchange the
+    staple count stays 2, so the target is the exact distribution
+    conditioned on it."""
     from latticednaorigami_amd import abi
+    move = {"ctrg_jump": abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH,
+            "ctcb_jump": abi.MOVE_CTCB_JUMP_SCAFFOLD_REGROWTH}[kind]
     s = systems.hairpin_system(4, seed=7)
     s.moves = [systems.Move(abi.MOVE_ORIENTATION_ROTATION, 0.2),
                systems.Move(abi.MOVE_MET_STAPLE_EXCHANGE, 0.6),
@@ -357,7 +361,7 @@ def test_ctrg_jump_matches_enumeration_with_hairpin_staples(oracle_mod):
     s.chains = grow.config(0)
     s.moves = [systems.Move(abi.MOVE_ORIENTATION_ROTATION, 0.3),
                systems.Move(abi.MOVE_CB_STAPLE_REGROWTH, 0.2),
-               systems.Move(abi.MOVE_CTRG_JUMP_SCAFFOLD_REGROWTH, 0.5, 4, 1, 36, 2)]
+               systems.Move(move, 0.5, 4, 1, 36, 2)]
     o = oracle_mod.Oracle(s, temps=[345.0] * 8)
     exact, _, _ = o.enumerate()
     two = {k: v for k, v in exact.items() if k[0] == 2}
