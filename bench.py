@@ -40,6 +40,7 @@ N_CU = 256
 # launch from the summary's "traffic_bytes_per_launch" line
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "ldo_sweep_lds_kernel_raster64_ptmc32_pmc.md")
 BATCH_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "ldo_batch_sweep_kernel_batch4096.md")
+MWUS3D_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "ldo_sweep_lds_kernel_mwus3d.md")
 
 
 def profiled_traffic(path=TRAFFIC_PROFILE):
@@ -327,7 +328,10 @@ def gpu_leg_mwus3d(args):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS if achieved is not None else None,
-        "traffic": None,
+        # HBM bytes per launch from the committed counter summary of this
+        # workload (500-step intervals)
+        "traffic": (profiled_traffic(MWUS3D_TRAFFIC_PROFILE)
+                    if args.interval == 500 and n_learned == 0 else None),
         "kernel": smp.lib.ldo_kernel_name(smp.h).decode(),
         "launches": prof["sweep_launches"],
         "avg_launch_ms": prof["sweep_ms"] / max(1, prof["sweep_launches"]),

@@ -64,11 +64,24 @@ extern "C" __global__ void ldo_init_kernel(Params P, const int32_t* c_idx, const
     init_replica(P, r, c_idx, c_ident, c_pos, c_ore, n_chains, mode, rep_slot);
 }
 
+// The replica a workgroup of the one-replica-per-workgroup sweeps runs.
+// Workgroups are dealt to the 8 XCDs round-robin, so workgroup b runs on XCD
+// b % 8; when the ladder divides into 8 equal runs of slots, XCD x takes the
+// replicas of slots [x R/8, (x+1) R/8), so that each XCD's L2 holds the
+// energy tables of few temperatures (slots are temperature-major).  q_to_rep
+// (slot -> replica) is the exchange's permutation, or null for the identity.
+__device__ inline int replica_of_block(const Params& P, const int32_t* q_to_rep) {
+    int b = blockIdx.x, R = P.n_replicas;
+    int q = (R % 8 == 0) ? (b % 8) * (R / 8) + b / 8 : b;
+    return q_to_rep != nullptr ? q_to_rep[q] : q;
+}
+
 // With a wavefront per replica, its 64 lanes run the replica's chain
 // together (lane-parallel sections spread independent reads over them); the
 // engine object sits in LDS, so its section pointers are not re-read from
 // the lanes' stacks in device memory.
-extern "C" __global__ void ldo_sweep_kernel(Params P, int64_t steps, const int32_t* rep_slot) {
+extern "C" __global__ void ldo_sweep_kernel(Params P, int64_t steps, const int32_t* rep_slot,
+                                            const int32_t* q_to_rep) {
     __shared__ Eng lds_engine;
     if (P.lanes_per_replica == 1) {
         bool active;
@@ -78,8 +91,8 @@ extern "C" __global__ void ldo_sweep_kernel(Params P, int64_t steps, const int32
         }
         return;
     }
-    int r = blockIdx.x;
-    if (r < P.n_replicas) {
+    if ((int)blockIdx.x < P.n_replicas) {
+        int r = replica_of_block(P, q_to_rep);
         sweep_replica_on(lds_engine, P, r, P.blocks + (size_t)r * P.L.bytes, steps, rep_slot, 1);
     }
 }
@@ -90,14 +103,15 @@ extern "C" __global__ void ldo_sweep_kernel(Params P, int64_t steps, const int32
 // dynamic LDS holds the block (P.L.bytes, a multiple of 256); the host picks
 // this kernel only when that fits.
 extern "C" __global__ void __launch_bounds__(64)
-    ldo_sweep_lds_kernel(Params P, int64_t steps, const int32_t* rep_slot) {
+    ldo_sweep_lds_kernel(Params P, int64_t steps, const int32_t* rep_slot,
+                         const int32_t* q_to_rep) {
     extern __shared__ uint4 lds_block[];
     __shared__ Params lds_params;
     __shared__ Eng lds_engine;
-    int r = blockIdx.x;
-    if (r >= P.n_replicas) {
+    if ((int)blockIdx.x >= P.n_replicas) {
         return;
     }
+    int r = replica_of_block(P, q_to_rep);
     size_t n16 = (size_t)P.L.bytes / 16;
     uint4* g = (uint4*)(P.blocks + (size_t)r * P.L.bytes);
     for (size_t i = threadIdx.x; i < n16; i += blockDim.x) {
@@ -567,14 +581,16 @@ void choose_sweep_kernel(Ctx& c) {
 }
 
 void launch_sweep(Ctx& c, int64_t steps, const int32_t* rep_slot) {
+    // with a ladder, workgroups follow the slot order (XCD-aware placement)
+    const int32_t* q_to_rep = rep_slot != nullptr ? c.d_xint : nullptr;
     launch_timed(c, 0, [&]() {
         if (c.lds_sweep) {
             hipLaunchKernelGGL(ldo_sweep_lds_kernel, dim3(c.R), dim3(64), (size_t)c.P.L.bytes,
-                               c.stream, c.P, steps, rep_slot);
+                               c.stream, c.P, steps, rep_slot, q_to_rep);
         }
         else {
             hipLaunchKernelGGL(ldo_sweep_kernel, grid_of(c), dim3(64), 0, c.stream, c.P, steps,
-                               rep_slot);
+                               rep_slot, q_to_rep);
         }
     });
     hip_check(hipGetLastError(), "launch sweep");
