@@ -441,8 +441,10 @@ def gpu_leg_learned(args):
         else:
             la = 0
         # each form relaxes under its own moves before the measured window
-        learned.run_series(smp, args.learned_eval_rounds // 4, args.interval, la, keys)
-        series, wall = learned.run_series(smp, args.learned_eval_rounds, args.interval, la, keys)
+        learned.run_series(smp, args.learned_eval_rounds // 4, args.interval, la, keys,
+                           progress=f"learned: {form} form relaxing")
+        series, wall = learned.run_series(smp, args.learned_eval_rounds, args.interval, la, keys,
+                                          progress=f"learned: {form} form")
         c = smp.counters()
         print(f"learned: {form} form evaluated in {wall:.1f} s", file=sys.stderr, flush=True)
         # free-energy error of numfulldomains against the reference after

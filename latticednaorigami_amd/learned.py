@@ -92,15 +92,18 @@ def ess(series, c=5.0):
     return float(sum(len(s) / (2 * integrated_autocorr_time(s, c)) for s in series))
 
 
-def run_series(smp, rounds, plain_steps, learned_attempts, keys):
+def run_series(smp, rounds, plain_steps, learned_attempts, keys, progress=None):
     """Alternate `learned_attempts` learned moves and `plain_steps` plain
     steps per replica; record the observables `keys` of every replica after
     each round.  Returns ({key: [replica][round]}, seconds spent in the
-    sampler's moves, reads excluded)."""
+    sampler's moves, reads excluded).  With `progress` (a label) a line goes
+    to standard error every 100 rounds."""
     R = smp.n
     out = {k: [[] for _ in range(R)] for k in keys}
     wall = 0.0
-    for _ in range(rounds):
+    for i in range(rounds):
+        if progress and i % 100 == 0:
+            print(f"{progress}: round {i} of {rounds}", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         if learned_attempts:
             smp.learned_sweep(learned_attempts)
