@@ -483,7 +483,9 @@ int ldo_get_permutation(void* ctx, int32_t* q_to_rep);
 
 /* Set the permutation, as a restart from a swap file does (the file's last
  * line, ptmc_simulation.cpp:54-84); every replica takes its new slot's
- * control variables when the next ldo_run interval begins. */
+ * temperature and stacking multiplier when the next ldo_run interval begins
+ * (its own chemical-potential multiplier changes at the next exchange, as
+ * the exchange mode has it). */
 int ldo_set_permutation(void* ctx, const int32_t* q_to_rep);
 
 /* The HIP stream the sampler's kernels are launched on (hipStream_t). */

@@ -580,6 +580,14 @@ void choose_sweep_kernel(Ctx& c) {
     c.lds_sweep = true;
 }
 
+// The initiation-term bookkeeping of a replica keeps one bit per table key
+// class of the ladder (Eng::begin_interval).
+void check_classes(const Ctx& c) {
+    if (c.n_classes > 64) {
+        throw AbiError {LDO_E_ARG, "a ladder holds at most 64 (temperature, stacking) pairs"};
+    }
+}
+
 void launch_sweep(Ctx& c, int64_t steps, const int32_t* rep_slot) {
     // with a ladder, workgroups follow the slot order (XCD-aware placement)
     const int32_t* q_to_rep = rep_slot != nullptr ? c.d_xint : nullptr;
@@ -1137,9 +1145,7 @@ int ldo_run(void* ctx, int64_t n_swaps, int64_t interval) {
         if (n_swaps < 0 || interval < 0) {
             throw AbiError {LDO_E_ARG, "negative swap count or interval"};
         }
-        if (c->n_classes > 64) {
-            throw AbiError {LDO_E_ARG, "a ladder holds at most 64 (temperature, stacking) pairs"};
-        }
+        check_classes(*c);
         XState X;
         X.q_to_rep = c->d_xint;
         X.rep_slot = c->d_xint + c->R;
@@ -1169,6 +1175,7 @@ int ldo_exchange(void* ctx, int64_t swap_index) {
         if (swap_index < 0) {
             throw AbiError {LDO_E_ARG, "negative swap index"};
         }
+        check_classes(*c);
         XState X;
         X.q_to_rep = c->d_xint;
         X.rep_slot = c->d_xint + c->R;

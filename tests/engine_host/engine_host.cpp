@@ -468,6 +468,10 @@ int eh_exchange(void* ctx, int64_t swap_index) {
         if (swap_index < 0) {
             throw ldo::AbiError {LDO_E_ARG, "negative swap index"};
         }
+        if (c->n_classes > 64) {
+            throw ldo::AbiError {LDO_E_ARG,
+                                 "a ladder holds at most 64 (temperature, stacking) pairs"};
+        }
         XState X;
         X.q_to_rep = c->xint.data();
         X.rep_slot = c->xint.data() + c->R;
