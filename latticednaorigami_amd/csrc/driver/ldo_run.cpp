@@ -7,8 +7,8 @@
 // .staples .staplestates .times .ene .ops, and .swp for parallel tempering,
 // ptmc_simulation.cpp:91-104, 315-322) while every replica runs on the GPU
 // through ldo.h.  Supported: simulation_type constant_temp, annealing,
-// t_, ut_, hut_, st_ and 2d_parallel_tempering, mw_umbrella_sampling and
-// ptmw_umbrella_sampling
+// t_, ut_, hut_, st_ and 2d_parallel_tempering, umbrella_sampling,
+// mw_umbrella_sampling and ptmw_umbrella_sampling
 // (with the per-window,
 // per-iteration files of us_simulation.cpp:103-248: _iter-<n>.* outputs,
 // -inp.biases / .biases weights and the .out iteration summary) with the
@@ -824,7 +824,10 @@ struct MwusSetup {
     std::vector<std::string> postfix; // MWUSGCMCSimulation::setup_window_variables
 };
 
-MwusSetup parse_mwus(const Run& run) {
+// single: SimpleUSGCMCSimulation (simulation_type umbrella_sampling), one
+// window whose restraint, if any, is the bias file's own well; otherwise the
+// windows file of MWUSGCMCSimulation
+MwusSetup parse_mwus(const Run& run, bool single) {
     const Params& p = run.p;
     MwusSetup m;
     // order-parameter tags and types
@@ -835,26 +838,29 @@ MwusSetup parse_mwus(const Run& run) {
     }
     // windows file: bias tags, then "min, max" per window (parse_windows_file, 552-600)
     std::string wpath = p.str("windows_file", "");
-    std::ifstream wf(wpath);
-    if (!wf.good()) {
-        throw Fail(LDO_E_IO, "Windows file " + wpath + " does not exist");
-    }
+    std::ifstream wf;
     std::string line;
-    std::getline(wf, line);
     std::vector<std::string> wtags;
-    {
+    if (!single) {
+        wf.open(wpath);
+        if (!wf.good()) {
+            throw Fail(LDO_E_IO, "Windows file " + wpath + " does not exist");
+        }
+        std::getline(wf, line);
+    }
+    if (!single) {
         std::stringstream ss(line);
         std::string t;
         while (ss >> t) {
             wtags.push_back(t);
         }
     }
-    if (wtags.size() != 1) {
+    if (!single && wtags.size() != 1) {
         throw Fail(LDO_E_NOTIMPL, "windows over one bias function only (the file names " +
                                       std::to_string(wtags.size()) + ")");
     }
     std::vector<std::pair<int, int>> ranges;
-    while (std::getline(wf, line)) {
+    while (!single && std::getline(wf, line)) {
         if (trim(line).empty()) {
             continue;
         }
@@ -873,7 +879,7 @@ MwusSetup parse_mwus(const Run& run) {
         }
         ranges.push_back({lo, hi});
     }
-    if (ranges.empty()) {
+    if (!single && ranges.empty()) {
         throw Fail(LDO_E_IO, wpath + ": no windows");
     }
     // bias functions: the windows' restraint and the grid bias, on one order parameter
@@ -884,25 +890,25 @@ MwusSetup parse_mwus(const Run& run) {
     const Json* grid = nullptr;
     for (auto& b: bfs.arr) {
         const std::string& tag = b.at("tag").str;
-        if (tag == wtags[0]) {
-            well = &b;
-        }
-        else if (tag == grid_tag) {
+        if (tag == grid_tag) {
             grid = &b;
+        }
+        else if ((single && !well) || (!single && tag == wtags[0])) {
+            well = &b;
         }
         else {
             throw Fail(LDO_E_NOTIMPL, "bias function " + tag +
                                           ": only the window restraint and the grid bias");
         }
     }
-    if (!well || !grid) {
-        throw Fail(LDO_E_IO, "bias-function file lacks " + (well ? grid_tag : wtags[0]));
+    if ((!single && !well) || !grid) {
+        throw Fail(LDO_E_IO, "bias-function file lacks " + (grid ? wtags[0] : grid_tag));
     }
     if (grid->at("type").str != "Grid" || grid->at("ops").arr.size() != 1) {
         throw Fail(LDO_E_NOTIMPL, "us_grid_bias_tag must name a Grid bias over one order parameter");
     }
     const std::string& op_tag = grid->at("ops").arr[0].str;
-    if (well->at("ops").arr.size() != 1 || well->at("ops").arr[0].str != op_tag) {
+    if (well && (well->at("ops").arr.size() != 1 || well->at("ops").arr[0].str != op_tag)) {
         throw Fail(LDO_E_NOTIMPL, "the window restraint and the grid bias must share their order parameter");
     }
     auto t = op_type.find(op_tag);
@@ -920,14 +926,25 @@ MwusSetup parse_mwus(const Run& run) {
     }
     // every value either parameter can take lies on the grid
     m.grid_len = run.n_scaffold + run.max_total_staples * run.max_staple_size + 1;
-    const std::string& wtype = well->at("type").str;
+    if (single) {
+        // the well's own range (bias_functions.cpp:309-430), or no restraint
+        ranges.push_back(well ? std::make_pair((int)well->at("min_op").num,
+                                               (int)well->at("max_op").num)
+                              : std::make_pair(0, m.grid_len - 1));
+    }
+    const std::string wtype = well ? well->at("type").str : "";
     double mult = p.num("bias_functions_mult", 1);
     for (auto& r: ranges) {
         ldo_window w {};
         w.op = m.op;
         w.min_op = r.first;
         w.max_op = r.second;
-        if (wtype == "LinearStepWell") {
+        if (!well) {
+            w.well_bias = 0;
+            w.min_bias = 0;
+            w.slope = 0;
+        }
+        else if (wtype == "LinearStepWell") {
             w.well_bias = well->at("well_bias").num;
             w.min_bias = well->at("min_bias").num;
             w.slope = well->at("slope").num;
@@ -944,7 +961,8 @@ MwusSetup parse_mwus(const Run& run) {
         w.off_grid_bias = 0;
         w.bias_mult = mult;
         m.windows.push_back(w);
-        m.postfix.push_back("_win-" + std::to_string(r.first) + "--" + std::to_string(r.second));
+        m.postfix.push_back(single ? std::string() : "_win-" + std::to_string(r.first) + "--" +
+                                                         std::to_string(r.second));
     }
     for (const char* k: {"restart_us_iter"}) {
         if (p.flag(k, false)) {
@@ -970,9 +988,9 @@ void read_weights(const std::string& path, ldo_us::WindowState& u) {
 }
 
 int run_mwus(Run& run, const ldo_system_desc& sd, ldo_run_params rp, const ldo_config& init,
-             bool pt) {
+             bool pt, bool single = false) {
     const Params& p = run.p;
-    MwusSetup m = parse_mwus(run);
+    MwusSetup m = parse_mwus(run, single);
     // PTMWUS (us_simulation.cpp:602-919): windows swap configurations every
     // exchange_interval steps; the swap file records which configuration
     // each window holds whenever the iteration's step count is a multiple
@@ -996,15 +1014,35 @@ int run_mwus(Run& run, const ldo_system_desc& sd, ldo_run_params rp, const ldo_c
     int W = (int)m.windows.size();
     std::string base = p.str("output_filebase", "");
     std::vector<ldo_us::WindowState> us(W);
-    std::vector<std::unique_ptr<std::ofstream>> summary;
+    // each window's summary stream: its .out file (MWUSGCMCSimulation::
+    // update_internal, us_simulation.cpp:548-551), or standard output for a
+    // single window, whose constructor also reports its inputs there
+    // (USGCMCSimulation::USGCMCSimulation, 31-91)
+    std::vector<std::unique_ptr<std::ofstream>> summary_files;
+    std::vector<std::ostream*> summary;
     for (int w = 0; w < W; w++) {
         if (p.flag("read_biases", false)) {
-            read_weights(p.str("biases_filebase", "") + m.postfix[w] + ".biases", us[w]);
+            read_weights(single ? p.str("biases_file", "")
+                                : p.str("biases_filebase", "") + m.postfix[w] + ".biases",
+                         us[w]);
         }
-        summary.push_back(std::make_unique<std::ofstream>(base + m.postfix[w] + ".out"));
-        if (!summary.back()->good()) {
+        if (single) {
+            summary.push_back(&std::cout);
+            continue;
+        }
+        summary_files.push_back(std::make_unique<std::ofstream>(base + m.postfix[w] + ".out"));
+        if (!summary_files.back()->good()) {
             throw Fail(LDO_E_IO, base + m.postfix[w] + ".out: output path does not exist");
         }
+        summary.push_back(summary_files.back().get());
+    }
+    if (single) {
+        std::cout << (p.flag("read_biases", false) ? "Reading biases from file\n"
+                                                   : "No biases read in\n");
+        std::cout << (p.flag("restart_from_config", false)
+                          ? "Reading starting config from file\n"
+                          : "Starting from configuration in system file\n");
+        std::cout << "Starting new iteration\n";
     }
     run.R = W;
     run.temps.assign(W, p.num("temp", 300));
@@ -1017,8 +1055,10 @@ int run_mwus(Run& run, const ldo_system_desc& sd, ldo_run_params rp, const ldo_c
         std::string fb = p.str("restart_traj_filebase", "");
         std::string post = p.str("restart_traj_postfix", ".trj");
         for (int w = 0; w < W; w++) {
-            set_config_from(run, w, read_traj_frame(fb + m.postfix[w] + post,
-                                                    (int)p.integer("restart_step", 0)));
+            set_config_from(run, w,
+                            read_traj_frame(single ? p.str("restart_traj_file", "")
+                                                   : fb + m.postfix[w] + post,
+                                            (int)p.integer("restart_step", 0)));
         }
     }
     std::vector<double> grid((size_t)W * m.grid_len);
@@ -1302,6 +1342,9 @@ int main_run(int argc, char** argv) {
     if (sim == "mw_umbrella_sampling" || sim == "ptmw_umbrella_sampling") {
         return run_mwus(run, sd, rp, init, sim == "ptmw_umbrella_sampling");
     }
+    if (sim == "umbrella_sampling") {
+        return run_mwus(run, sd, rp, init, false, true);
+    }
     // the PT drivers of apps/main.cpp and their exchange modes
     static const std::map<std::string, int> pt_modes = {
         {"t_parallel_tempering", LDO_EXCHANGE_T},
@@ -1322,7 +1365,8 @@ int main_run(int argc, char** argv) {
     // rather than run unbiased.
     if (!p.str("bias_functions_file", "").empty()) {
         throw Fail(LDO_E_NOTIMPL, "bias_functions_file is supported only with "
-                                  "mw_umbrella_sampling and ptmw_umbrella_sampling");
+                                  "umbrella_sampling, mw_umbrella_sampling and "
+                                  "ptmw_umbrella_sampling");
     }
     // control quantities of every ladder slot: temperature, chemical
     // potential, bias and stacking multipliers (m_control_qs)

@@ -301,8 +301,7 @@ def test_driver_rejects_unsupported_inputs(tmp_path):
         pytest.skip("driver not built")
     s = systems.four_domain_system()
     for extra, msg in (({"domain_type": "Halfturn"}, "domain_type Halfturn"),
-                       ({"simulation_type": "umbrella_sampling"},
-                        "simulation_type umbrella_sampling"),
+                       ({"simulation_type": "enumerate"}, "simulation_type enumerate"),
                        ({"simulation_type": "annealing", "max_temp": "340", "min_temp": "330",
                          "temp_interval": "3"}, "Bad temperature interval"),
                        ({"simulation_type": "2d_parallel_tempering", "num_reps": "5",
@@ -682,6 +681,59 @@ def test_driver_mwus_iterations_match_oracle(tmp_path, oracle_mod):
         with open(os.path.join(d, post[w] + ".out")) as f:
             assert f.read() == summaries[w], w
     assert visited > 3 * W  # the windows sampled more than one point each
+
+
+@pytest.mark.gpu
+def test_driver_single_window_us_matches_oracle(tmp_path, oracle_mod):
+    """simulation_type umbrella_sampling (SimpleUSGCMCSimulation): one window
+    whose restraint is the bias file's own LinearStepWell (min_op 1, max_op
+    3), an equilibration run and two adaptive iterations. The iteration
+    energies and bias files equal the oracle's one-window run, and the
+    constructor's messages and the iteration summaries go to standard
+    output, as the reference's stream defaults to it for a single window
+    (us_simulation.hpp:78)."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "oracle"))
+    import us_ref
+    s = systems.raster_system(replicas=1)
+    d = str(tmp_path)
+    inp = write_mwus_inputs(d, s, [], {
+        "simulation_type": "umbrella_sampling", "equil_steps": 1000, "iter_steps": 2000,
+        "max_num_iters": 2, "max_D_bias": 1.0, "energies_output_freq": 500})
+    with open(os.path.join(d, "biases.json")) as f:
+        b = json.load(f)
+    b["origami"]["bias_functions"][0].update({"min_op": 1, "max_op": 3})
+    with open(os.path.join(d, "biases.json"), "w") as f:
+        json.dump(b, f)
+    p = run_driver(inp)
+    glen = 64 + s.max_total_staples * s.max_staple_size + 1
+    ref = oracle_mod.Oracle(s, temps=[330.0])
+    ref.set_windows([abi.Window(abi.OP_NUMSTAPLES, 1, 3, 0, 0.0, 2.0, 1.5, 0.0, 1.0)],
+                    [0.0] * glen, glen)
+    ref.set_histogram(True)
+    for k in (1, 2):
+        ref.sweep(500)
+        e, _, _, _ = expect_lines(ref, 0, s, 500 * k)
+        assert read(d, "out_iter-equil.ene")[k] == e, k
+    ref.histogram(reset=True)
+    us = us_ref.WindowState()
+    summary = ""
+    for n in range(2):
+        with open(os.path.join(d, f"out_iter-{n}-inp.biases")) as f:
+            assert f.read() == us_ref.weights_json(us), n
+        for k in range(1, 5):
+            ref.sweep(500)
+            e, _, _, _ = expect_lines(ref, 0, s, 500 * k)
+            assert read(d, f"out_iter-{n}.ene")[k] == e, (n, k)
+        h = ref.histogram(reset=True)
+        us_ref.process_iteration(us, [int(x) for x in h[0, :glen]], us.grid(glen), 2001, 1.0)
+        summary += us_ref.summary(us, n)
+        with open(os.path.join(d, f"out_iter-{n}.biases")) as f:
+            assert f.read() == us_ref.weights_json(us), n
+        ref.set_window_grid(list(us.grid(glen)))
+    assert p.stdout == ("No biases read in\nStarting from configuration in system file\n"
+                        "Starting new iteration\n" + summary), p.stdout
+    assert not os.path.exists(os.path.join(d, "out.out"))
 
 
 @pytest.mark.gpu
