@@ -83,6 +83,15 @@ int oracle_energy_tables(void* o, double temp, double stacking_mult,
  * reference's unassign-all / set-all sequence (origami_system.cpp:808-826) */
 int oracle_recompute_energy(void* o, int32_t replica, double* out_energy);
 
+/* Test hook: unassign every domain of replica r, then assign them again in
+ * `order` (indices into the chain-order list of all domains, each once),
+ * through the same set_domain_config as the chain-order rebuild. Reports the
+ * energy and stacked-pair count so built; the replica is left rebuilt in chain
+ * order. The energy is a function of the configuration, so every order must
+ * give the chain-order value. */
+int oracle_energy_in_order(void* o, int32_t replica, const int32_t* order, int32_t n,
+                           double* out_energy, int64_t* out_stacked);
+
 /* Nearest-neighbour thermodynamics (nearest_neighbour.cpp:34-177) */
 int oracle_nn_hybridization_thermo(const char* seq, double temp, double cation_M,
                                    double* out_h, double* out_s);

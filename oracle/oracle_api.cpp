@@ -966,6 +966,53 @@ int oracle_recompute_energy(void* o, int32_t r, double* out_energy) {
     })
 }
 
+int oracle_energy_in_order(void* o, int32_t r, const int32_t* order, int32_t n,
+                           double* out_energy, int64_t* out_stacked) {
+    ORACLE_TRY(o, {
+        System& s = O->reps.at(r)->sys;
+        std::vector<Dom*> all;
+        for (auto& ch: s.chains) {
+            for (Dom* d: ch) {
+                all.push_back(d);
+            }
+        }
+        std::vector<char> seen(all.size(), 0);
+        if (n != (int32_t)all.size()) {
+            throw OrigamiError(LDO_E_ARG, "the order must list every domain once");
+        }
+        for (int32_t k = 0; k < n; k++) {
+            if (order[k] < 0 || order[k] >= n || seen[order[k]]) {
+                throw OrigamiError(LDO_E_ARG, "the order must list every domain once");
+            }
+            seen[order[k]] = 1;
+        }
+        for (Dom* d: all) {
+            s.unassign_domain(*d);
+        }
+        s.energy = (s.chains.size() - 1) * s.pot.init_e();
+        bool violated = false;
+        for (int32_t k = 0; k < n && !violated; k++) {
+            Dom* d = all[order[k]];
+            s.set_domain_config(*d, d->pos, d->ore);
+            violated = s.violated;
+        }
+        *out_energy = s.energy;
+        *out_stacked = s.num_stacked;
+        // leave the replica as the chain-order rebuild leaves it
+        for (Dom* d: all) {
+            if (d->st != Occ::unassigned) {
+                s.unassign_domain(*d);
+            }
+        }
+        s.violated = false;
+        s.energy = (s.chains.size() - 1) * s.pot.init_e();
+        s.set_all_domains();
+        if (violated) {
+            throw OrigamiError(LDO_E_CONSTRAINT, "a partial configuration violated a constraint");
+        }
+    })
+}
+
 int oracle_nn_hybridization_thermo(const char* seq, double temp, double cation_M,
                                    double* out_h, double* out_s) {
     try {

@@ -6,6 +6,19 @@
 // std::set iteration orders) so that it reproduces the reference's
 // arithmetic and random-draw order, which the HIP sampler is then checked
 // against draw for draw.
+//
+// Parity unpinned (no golden vector, known answer or statistic fixes them;
+// the HIP sampler is checked against this restatement only):
+//   - the values of the FourBody junction terms in networks of several
+//     crossovers (Potential::fb_*_single_junction, origami_potential.cpp:
+//     654-934); which neighbours those walks read is pinned by the energy
+//     being a state function of folded networks (order and rotation
+//     invariance), single crossovers by hand-built configurations;
+//   - CTCB endpoint bookkeeping for networks of several staples bound to
+//     each other (cb_movetypes.cpp);
+//   - the mt19937_64 draws of the tape mode, which are libstdc++'s;
+//   - CTRG's feeler growth (rg_movetypes.cpp:426-480), pinned only loosely
+//     by the enumeration of the bare scaffold.
 #ifndef ORACLE_CORE_HPP
 #define ORACLE_CORE_HPP
 

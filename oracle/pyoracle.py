@@ -40,6 +40,9 @@ def lib():
         L.oracle_energy_tables.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32,
                                            C.c_int32, C.POINTER(C.c_double)]
         L.oracle_recompute_energy.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
+        L.oracle_energy_in_order.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
+                                             C.c_int32, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_int64)]
         L.oracle_nn_hybridization_thermo.argtypes = [C.c_char_p, C.c_double, C.c_double,
                                                      C.POINTER(C.c_double),
                                                      C.POINTER(C.c_double)]
@@ -284,6 +287,15 @@ class Oracle:
         e = C.c_double()
         _check(self.L, self.h, self.L.oracle_recompute_energy(self.h, r, C.byref(e)))
         return e.value
+
+    def energy_in_order(self, order, r=0):
+        """(energy, stacked pairs) rebuilt by assigning the domains in `order`
+        (indices into the chain-order list of all domains)."""
+        arr = (C.c_int32 * len(order))(*order)
+        e, st = C.c_double(), C.c_int64()
+        _check(self.L, self.h, self.L.oracle_energy_in_order(self.h, r, arr, len(order),
+                                                             C.byref(e), C.byref(st)))
+        return e.value, st.value
 
 
 def nn_thermo(seq, temp, cation):

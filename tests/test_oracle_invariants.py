@@ -62,3 +62,84 @@ def test_constraint_check_cadence_keeps_energy(oracle_mod):
     obs = o.observables()
     assert obs["status"] == 0
     assert o.recompute_energy() == pytest.approx(obs["energy"], rel=1e-12, abs=1e-9)
+
+
+def _folded_raster(s, keep):
+    """The raster's designed fold: every scaffold domain at its raster site,
+    orientations alternating +z/-z along the scaffold (the HalfTurn twist of
+    a straight helix, domain.cpp:33-40), and the
+    staple types for which keep(type) holds bound at their target sites with
+    the opposite orientations. Rows are joined by crossover staples, so the
+    FourBody junction walks see many crossovers at once."""
+    pos = s.chains[0].positions
+    ore = [(0, 0, (-1) ** i) for i in range(len(pos))]
+    chains = [systems.Chain(0, 0, list(pos), list(ore))]
+    for t, ids in enumerate(s.identities[1:], start=1):
+        if keep(t):
+            sc = [v - 1 for v in ids]
+            chains.append(systems.Chain(len(chains), t, [pos[i] for i in sc],
+                                        [tuple(-x for x in ore[i]) for i in sc]))
+    return chains
+
+
+def _rotate(chains, rot):
+    return [systems.Chain(c.index, c.identity, [rot(p) for p in c.positions],
+                          [rot(o) for o in c.orientations]) for c in chains]
+
+
+def test_fourbody_energy_is_a_state_function_of_folded_networks(oracle_mod):
+    """The FourBody energy and stacked-pair count of folded rasters with
+    many crossovers (the full fold: 64 bound pairs, 24 stacked; and seeded
+    random subsets of its staples) do not depend on the order in which the
+    domains are assigned, nor on a proper rotation of the whole lattice
+    configuration. Each binding adds only the junction and stacking terms it
+    can see from its own side (origami_potential.cpp:212-934), so a term
+    counted from one side only, a walk that reads the wrong neighbour or a
+    rule that names an axis breaks one of the two. Planted in a copy of the
+    oracle, each of these fails here: the central triplet walk reading
+    cd_j - 1 for cd_j + 1 (origami_potential.cpp:593), the backward triplet
+    walk reading the bound domain's backward neighbour for its forward one
+    (:500), and the central single-junction walk without its guard against
+    counting a doubly contiguous j3/j4 twice (:914-916). A wrong value applied
+    the same way in every order (a junction penalty of 1 removing a whole
+    stacking term) is invisible to this test; the hand-built crossovers of
+    the pins suite fix those values."""
+    import random
+    s = systems.raster_system(replicas=1)
+    o = oracle_mod.Oracle(s, temps=[330.0])
+    rng = random.Random(20261203)
+    rots = [lambda p: (-p[1], p[0], p[2]), lambda p: (p[0], -p[2], p[1]),
+            lambda p: (p[2], p[0], p[1])]
+    full = None
+    for case in range(24):
+        if case == 0:
+            keep = lambda t: True
+        else:
+            chosen = set(rng.sample(range(1, len(s.identities)), rng.randint(8, 28)))
+            keep = chosen.__contains__
+        if case < 12:
+            cfg = _folded_raster(s, keep)
+            o.set_config(0, cfg)
+        else:
+            # the fold evolved by the moveset: kinks, regrown scaffold
+            # stretches and rebound staples give other junction geometries
+            o.set_config(0, _folded_raster(s, keep))
+            o.sweep(3000)
+            cfg = o.config()
+        obs = o.observables()
+        e0, st0 = obs["energy"], obs["num_stacked_domain_pairs"]
+        if case == 0:
+            full = (obs["num_fully_bound_domain_pairs"], st0)
+        n = obs["num_domains"]
+        for _ in range(8):
+            order = list(range(n))
+            rng.shuffle(order)
+            e, st = o.energy_in_order(order)
+            assert st == st0, (case, order)
+            assert e == pytest.approx(e0, rel=1e-12, abs=1e-9), (case, order)
+        for rot in rots:
+            o.set_config(0, _rotate(cfg, rot))
+            r = o.observables()
+            assert r["num_stacked_domain_pairs"] == st0, case
+            assert r["energy"] == pytest.approx(e0, rel=1e-12, abs=1e-9), case
+    assert full == (64, 24)
