@@ -1127,6 +1127,8 @@ int run_mwus(Run& run, const ldo_system_desc& sd, ldo_run_params rp, const ldo_c
             swp << q_before[w] << " ";
         }
         swp << "\n";
+        std::vector<int64_t> att0(2 * run.R), acc0(2 * run.R), att1(2 * run.R), acc1(2 * run.R);
+        run.check(ldo_get_exchange_counts(run.ctx, 2 * run.R, att0.data(), acc0.data()));
         auto start = std::chrono::steady_clock::now();
         long long m_steps = 0;
         for (long long s = 1; s <= swaps; s++) {
@@ -1155,6 +1157,18 @@ int run_mwus(Run& run, const ldo_system_desc& sd, ldo_run_params rp, const ldo_c
             }
             swp << "\n";
         }
+        // write_acceptance_freqs (884-898): the iteration's swaps and
+        // attempts of each pair of neighbouring windows
+        run.check(ldo_get_exchange_counts(run.ctx, 2 * run.R, att1.data(), acc1.data()));
+        std::cout << "Iter " << tag << "\n";
+        std::cout << "Window 1, Window 2, Swaps, Attempts, Frequency\n";
+        for (int i = 0; i + 1 < W; i++) {
+            int64_t n_att = att1[i] - att0[i], n_acc = acc1[i] - acc0[i];
+            std::cout << i << " " << i + 1 << " " << n_acc << " " << n_att << " "
+                      << (static_cast<double>(n_acc) / n_att) << " \n";
+        }
+        std::cout << "\n";
+        std::cout.flush();
         return m_steps;
     };
     // run_equilibration (103-124): its visits are not used; PTMWUS skips it

@@ -753,7 +753,7 @@ def test_driver_ptmwus_iterations_match_oracle(tmp_path, oracle_mod):
     s = systems.raster_system(replicas=3)
     d = str(tmp_path)
     windows = [(0, 2), (0, 3), (1, 4)]
-    run_driver(write_mwus_inputs(d, s, windows, {
+    p = run_driver(write_mwus_inputs(d, s, windows, {
         "simulation_type": "ptmw_umbrella_sampling", "equil_steps": 1000, "iter_swaps": 3,
         "exchange_interval": 500, "max_num_iters": 2, "max_D_bias": 1.0,
         "energies_output_freq": 500, "configs_output_freq": 1000}))
@@ -771,8 +771,10 @@ def test_driver_ptmwus_iterations_match_oracle(tmp_path, oracle_mod):
     summaries = [""] * W
     swapped = False
     K = 3
+    freqs = ""
     for n in range(2):
         ref.restart_swaps()
+        att0, acc0 = ref.exchange_counts()
         swp = [" ".join(str(q) for q in ref.permutation()) + " "]
         for k in range(1, K + 1):
             before = ref.permutation()
@@ -788,6 +790,14 @@ def test_driver_ptmwus_iterations_match_oracle(tmp_path, oracle_mod):
         if (500 * K) % 1000 == 0:
             swp.append(" ".join(str(q) for q in ref.permutation()) + " ")
         assert read(d, f"out_iter-{n}.swp")[:-1] == swp, n
+        # write_acceptance_freqs (us_simulation.cpp:884-898) per iteration
+        att1, acc1 = ref.exchange_counts()
+        freqs += f"Iter {n}\nWindow 1, Window 2, Swaps, Attempts, Frequency\n"
+        for i in range(W - 1):
+            a, c = int(att1[0, i] - att0[0, i]), int(acc1[0, i] - acc0[0, i])
+            assert a > 0
+            freqs += f"{i} {i + 1} {c} {a} {c / a:g} \n"
+        freqs += "\n"
         h = ref.histogram(reset=True)
         for w in range(W):
             us_ref.process_iteration(us[w], [int(x) for x in h[w, :glen]], us[w].grid(glen),
@@ -799,4 +809,5 @@ def test_driver_ptmwus_iterations_match_oracle(tmp_path, oracle_mod):
     for w in range(W):
         with open(os.path.join(d, post[w] + ".out")) as f:
             assert f.read() == summaries[w], w
+    assert p.stdout == freqs, p.stdout
     assert swapped
