@@ -1054,11 +1054,26 @@ int run_mwus(Run& run, const ldo_system_desc& sd, ldo_run_params rp, const ldo_c
     if (p.flag("restart_from_config", false)) {
         std::string fb = p.str("restart_traj_filebase", "");
         std::string post = p.str("restart_traj_postfix", ".trj");
+        // restart_traj_files names each window's file in window order
+        // (MWUSGCMCSimulation::setup_window_sims, us_simulation.cpp:542-546);
+        // the reference never parses restart_steps into the list it reads
+        // there, so here restart_steps, when given, lists each window's step
+        // and restart_step serves every window otherwise
+        std::vector<std::string> files = p.words("restart_traj_files");
+        std::vector<double> steps = p.list("restart_steps");
+        if (!single && !files.empty() && (int)files.size() != W) {
+            throw Fail(LDO_E_ARG, "restart_traj_files must name one file per window");
+        }
+        if (!single && !steps.empty() && (int)steps.size() != W) {
+            throw Fail(LDO_E_ARG, "restart_steps must list one step per window");
+        }
         for (int w = 0; w < W; w++) {
-            set_config_from(run, w,
-                            read_traj_frame(single ? p.str("restart_traj_file", "")
-                                                   : fb + m.postfix[w] + post,
-                                            (int)p.integer("restart_step", 0)));
+            std::string file = single ? p.str("restart_traj_file", "")
+                               : files.empty() ? fb + m.postfix[w] + post
+                                               : files[w];
+            int step = (!single && !steps.empty()) ? (int)steps[w]
+                                                   : (int)p.integer("restart_step", 0);
+            set_config_from(run, w, read_traj_frame(file, step));
         }
     }
     std::vector<double> grid((size_t)W * m.grid_len);

@@ -684,6 +684,50 @@ def test_driver_mwus_iterations_match_oracle(tmp_path, oracle_mod):
 
 
 @pytest.mark.gpu
+def test_driver_mwus_restart_from_named_files(tmp_path, oracle_mod):
+    """restart_from_config with restart_traj_files (one file per window,
+    us_simulation.cpp:542-546) and restart_steps: each window starts from the
+    named frame of another window's equilibration trajectory and continues
+    as the oracle does from the same configurations."""
+    s = systems.raster_system(replicas=3)
+    d = str(tmp_path)
+    windows = [(0, 2), (1, 3), (2, 4)]
+    W = len(windows)
+    common = {"max_num_iters": 0, "configs_output_freq": 500, "energies_output_freq": 500}
+    run_driver(write_mwus_inputs(d, s, windows, {**common, "equil_steps": 1000}))
+    first = os.path.join(d, "first")
+    os.mkdir(first)
+    for name in os.listdir(d):
+        if name.startswith("out"):
+            os.rename(os.path.join(d, name), os.path.join(first, name))
+    post = [f"out_win-{lo}--{hi}" for lo, hi in windows]
+    src = [(2, 1), (0, 0), (1, 1)]  # window w starts from (window, frame)
+    files = " ".join(os.path.join(first, post[v] + "_iter-equil.trj") for v, _ in src)
+    run_driver(write_mwus_inputs(d, s, windows, {
+        **common, "equil_steps": 1000, "restart_from_config": "true",
+        "restart_traj_files": files, "restart_steps": " ".join(str(f) for _, f in src)}))
+    glen = 64 + s.max_total_staples * s.max_staple_size + 1
+    wins = [abi.Window(abi.OP_NUMSTAPLES, lo, hi, 0, 0.0, 2.0, 1.5, 0.0, 1.0)
+            for lo, hi in windows]
+    ref0 = oracle_mod.Oracle(s, temps=[330.0] * W)
+    ref0.set_windows(wins, [0.0] * (W * glen), glen)
+    frames = []
+    for _ in range(2):
+        ref0.sweep(500)
+        frames.append([ref0.config(w) for w in range(W)])
+    assert len({repr(frames[f][v]) for v, f in src}) == W
+    ref = oracle_mod.Oracle(s, temps=[330.0] * W)
+    ref.set_windows(wins, [0.0] * (W * glen), glen)
+    for w, (v, f) in enumerate(src):
+        ref.set_config(w, frames[f][v])
+    for k in (1, 2):
+        ref.sweep(500)
+        for w in range(W):
+            e, _, _, _ = expect_lines(ref, w, s, 500 * k)
+            assert read(d, post[w] + "_iter-equil.ene")[k] == e, (k, w)
+
+
+@pytest.mark.gpu
 def test_driver_single_window_us_matches_oracle(tmp_path, oracle_mod):
     """simulation_type umbrella_sampling (SimpleUSGCMCSimulation): one window
     whose restraint is the bias file's own LinearStepWell (min_op 1, max_op
