@@ -681,6 +681,38 @@ def case_hand_built_fourbody_configurations(sampler_cls, oracle_mod):
         assert gpu.observables(0)["num_stacked_domain_pairs"] == n_stack, name
 
 
+def case_folded_crossover_networks(sampler_cls, oracle_mod, steps=3000):
+    """The raster's designed fold (64 bound pairs joined by crossover
+    staples), seeded subsets of its staples and proper rotations of them:
+    the device's FourBody evaluation gives the oracle's observables for each
+    configuration, and the moveset started from the full fold replays the
+    oracle's chain bit for bit over dense crossover networks."""
+    import random
+    import pin_cases as pin
+    say("case: folded raster crossover networks (set_config, rotations, replay)")
+    s = systems.raster_system(replicas=1)
+    gpu = sampler_cls(s, temps=[330.0])
+    ref = oracle_mod.Oracle(s, temps=[330.0])
+    rng = random.Random(20261203)
+    rot = lambda p: (p[2], p[0], p[1])  # noqa: E731
+    for case in range(6):
+        keep = (lambda t: True) if case == 0 else set(
+            rng.sample(range(1, len(s.identities)), rng.randint(8, 28))).__contains__
+        cfg = pin.folded_raster(s, keep)
+        for c in (cfg, pin.rotate_chains(cfg, rot)):
+            gpu.set_config(0, c)
+            ref.set_config(0, c)
+            assert_same(gpu, ref, 0)
+    full = pin.folded_raster(s, lambda t: True)
+    gpu.set_config(0, full)
+    ref.set_config(0, full)
+    assert gpu.observables(0)["num_stacked_domain_pairs"] == 24
+    gpu.sweep(steps)
+    ref.sweep(steps)
+    assert_same(gpu, ref, 0)
+    assert gpu.observables(0)["num_fully_bound_domain_pairs"] > 32
+
+
 def case_fixed_configuration_exchange_chains(sampler_cls, oracle_mod, rounds=2000):
     """PTMC and window exchange over fixed configurations (0 MC steps per
     round): the device exchange kernel makes the oracle's decisions round by

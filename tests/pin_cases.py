@@ -309,3 +309,26 @@ def met_insert_tape(u):
 def met_delete_tape(u):
     # Met, delete (0.75), identity 1, its staple 0, acceptance
     return [bits(0.5), bits(0.75), 1, 0, bits(u)]
+
+
+def folded_raster(s, keep):
+    """The raster's designed fold: every scaffold domain at its raster site,
+    orientations alternating +z/-z along the scaffold (the HalfTurn twist of
+    a straight helix, domain.cpp:33-40), and the
+    staple types for which keep(type) holds bound at their target sites with
+    the opposite orientations. Rows are joined by crossover staples, so the
+    FourBody junction walks see many crossovers at once."""
+    pos = s.chains[0].positions
+    ore = [(0, 0, (-1) ** i) for i in range(len(pos))]
+    chains = [systems.Chain(0, 0, list(pos), list(ore))]
+    for t, ids in enumerate(s.identities[1:], start=1):
+        if keep(t):
+            sc = [v - 1 for v in ids]
+            chains.append(systems.Chain(len(chains), t, [pos[i] for i in sc],
+                                        [tuple(-x for x in ore[i]) for i in sc]))
+    return chains
+
+
+def rotate_chains(chains, rot):
+    return [systems.Chain(c.index, c.identity, [rot(p) for p in c.positions],
+                          [rot(o) for o in c.orientations]) for c in chains]

@@ -129,6 +129,10 @@ def test_hand_built_fourbody_configurations(sampler_cls, oracle_mod):
     pc.case_hand_built_fourbody_configurations(sampler_cls, oracle_mod)
 
 
+def test_folded_crossover_networks(sampler_cls, oracle_mod):
+    pc.case_folded_crossover_networks(sampler_cls, oracle_mod)
+
+
 def test_fixed_configuration_exchange_chains(sampler_cls, oracle_mod):
     pc.case_fixed_configuration_exchange_chains(sampler_cls, oracle_mod)
 

@@ -64,29 +64,6 @@ def test_constraint_check_cadence_keeps_energy(oracle_mod):
     assert o.recompute_energy() == pytest.approx(obs["energy"], rel=1e-12, abs=1e-9)
 
 
-def _folded_raster(s, keep):
-    """The raster's designed fold: every scaffold domain at its raster site,
-    orientations alternating +z/-z along the scaffold (the HalfTurn twist of
-    a straight helix, domain.cpp:33-40), and the
-    staple types for which keep(type) holds bound at their target sites with
-    the opposite orientations. Rows are joined by crossover staples, so the
-    FourBody junction walks see many crossovers at once."""
-    pos = s.chains[0].positions
-    ore = [(0, 0, (-1) ** i) for i in range(len(pos))]
-    chains = [systems.Chain(0, 0, list(pos), list(ore))]
-    for t, ids in enumerate(s.identities[1:], start=1):
-        if keep(t):
-            sc = [v - 1 for v in ids]
-            chains.append(systems.Chain(len(chains), t, [pos[i] for i in sc],
-                                        [tuple(-x for x in ore[i]) for i in sc]))
-    return chains
-
-
-def _rotate(chains, rot):
-    return [systems.Chain(c.index, c.identity, [rot(p) for p in c.positions],
-                          [rot(o) for o in c.orientations]) for c in chains]
-
-
 def test_fourbody_energy_is_a_state_function_of_folded_networks(oracle_mod):
     """The FourBody energy and stacked-pair count of folded rasters with
     many crossovers (the full fold: 64 bound pairs, 24 stacked; and seeded
@@ -105,6 +82,7 @@ def test_fourbody_energy_is_a_state_function_of_folded_networks(oracle_mod):
     stacking term) is invisible to this test; the hand-built crossovers of
     the pins suite fix those values."""
     import random
+    import pin_cases as pin
     s = systems.raster_system(replicas=1)
     o = oracle_mod.Oracle(s, temps=[330.0])
     rng = random.Random(20261203)
@@ -118,12 +96,12 @@ def test_fourbody_energy_is_a_state_function_of_folded_networks(oracle_mod):
             chosen = set(rng.sample(range(1, len(s.identities)), rng.randint(8, 28)))
             keep = chosen.__contains__
         if case < 12:
-            cfg = _folded_raster(s, keep)
+            cfg = pin.folded_raster(s, keep)
             o.set_config(0, cfg)
         else:
             # the fold evolved by the moveset: kinks, regrown scaffold
             # stretches and rebound staples give other junction geometries
-            o.set_config(0, _folded_raster(s, keep))
+            o.set_config(0, pin.folded_raster(s, keep))
             o.sweep(3000)
             cfg = o.config()
         obs = o.observables()
@@ -138,7 +116,7 @@ def test_fourbody_energy_is_a_state_function_of_folded_networks(oracle_mod):
             assert st == st0, (case, order)
             assert e == pytest.approx(e0, rel=1e-12, abs=1e-9), (case, order)
         for rot in rots:
-            o.set_config(0, _rotate(cfg, rot))
+            o.set_config(0, pin.rotate_chains(cfg, rot))
             r = o.observables()
             assert r["num_stacked_domain_pairs"] == st0, case
             assert r["energy"] == pytest.approx(e0, rel=1e-12, abs=1e-9), case
