@@ -151,13 +151,13 @@ def test_pt_chemical_potential_multiplier_known_answers(oracle_mod, mode, expect
     multipliers u = (1, 2, 1) and bias multipliers 1; replicas 0 and 1 hold no
     staple, replica 2 both staples, mu / kT ~ -19.2 per staple.
 
-    Round 1 pairs slots 1, 2.  Every replica's first u is its bias
+    Exchange 1 pairs slots 1, 2.  Every replica's first u is its bias
     multiplier (initialize_control_qs writes bias_mults[i] to index m_bias_i
     = m_staple_u_mult_i, ptmc_simulation.cpp:349-356, ptmc_simulation.hpp:74,
     80), so the u term is 0 and the swap is certain; with the slots'
     multipliers it would be exp(-38.4).  UT and HUT then give every replica
-    its slot's u (master_send, 212-226): round 2 (slots 0, 1) moves the
-    staples to u = 2 (accepted), round 3 swaps two empty replicas, round 4
+    its slot's u (master_send, 212-226): exchange 2 (slots 0, 1) moves the
+    staples to u = 2 (accepted), exchange 3 swaps two empty replicas, exchange 4
     would move them back to u = 1 at exp(-38.4) (rejected).  T and ST never
     exchange u (m_exchange_q_is, 602-650), so every swap is certain."""
     o = _straight_oracle(oracle_mod, mode, [345.0] * 3, [3, 3, 0])
@@ -178,7 +178,7 @@ def test_st_stacking_multiplier_known_answer(oracle_mod):
     of the 3-stacked-pair configuration: the swap's log ratio is
     (2 - 1) / T x (2 x 3 - 3) x stacking_ene = -260.9, so it is rejected,
     while t_parallel_tempering (stacking multiplier 1 in the sweep) sees no
-    stacking difference and swaps.  Round 2 then pairs slots 0, 1 at equal
+    stacking difference and swaps.  Exchange 2 then pairs slots 0, 1 at equal
     multipliers: certain."""
     temps, smult = [345.0] * 3, [1.0, 1.0, 2.0]
     for mode, expect in ((abi.EXCHANGE_ST, [[0, 1, 2], [1, 0, 2]]),
